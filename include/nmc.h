@@ -1,0 +1,212 @@
+/*
+ * nmc.h -- C ABI of the MI355X Metropolis engine (libnmc.so).
+ *
+ * The reference NanoMagMC has no FFI: its de-facto operator boundary is the
+ * C++ class `state` (Old/includes/state.hpp:12-52) driven by the sweep-and-
+ * sample loop in Old/lib/main.cpp:104-275.  Every entry point below names the
+ * reference member it replaces.  One nmc_ctx holds a batch of lattice
+ * instances (one per (lattice, temperature, field) chain) resident in HBM; the
+ * reference holds one `state` per chain in host memory.
+ *
+ * Conventions
+ *  - All functions return NMC_OK (0) or a negative NMC_E_* code; the text of
+ *    the last error on the calling thread is returned by nmc_last_error().
+ *  - The caller owns every host array passed in or out; nothing is retained
+ *    past the call.  The context owns all device memory.
+ *  - Work is ordered on the context's HIP stream.  Functions that return host
+ *    data synchronise that stream; the others only enqueue.
+ *  - One host thread per context.  Distinct contexts are independent.
+ *  - Coordinates follow the reference: a site of a d-dimensional box has
+ *    integer coordinates (x, y[, z]) and the last coordinate varies fastest in
+ *    the box scan (Old/lib/state.cpp:132-166).  "Site order" is the order of
+ *    the present sites in that scan, i.e. the reference's spin index.
+ *  - Spins cross the boundary as 3 doubles per site (Ising in component 0),
+ *    mirroring the reference's xtensorf<double,4> spin with the unused 4th
+ *    component dropped (Old/includes/field_type.hpp:17).
+ */
+#ifndef NMC_H
+#define NMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NMC_OK 0
+#define NMC_E_INVALID (-1)     /* bad argument (reference: exit 103/104/105/207) */
+#define NMC_E_HIP (-2)         /* a HIP runtime call failed */
+#define NMC_E_NOMEM (-3)       /* device allocation failed */
+#define NMC_E_UNSUPPORTED (-4) /* valid for the reference, not for this engine */
+
+/* Trial-spin generators.  SPHERE is the reference's draw (a fresh uniform
+ * direction on S^2, Old/lib/field_type.cpp:185-195 via gsl_ran_dir_3d; for
+ * Ising a fresh +-1).  CONE is a symmetric move uniform on the spherical cap
+ * of half-angle theta around the current spin (Heisenberg only). */
+#define NMC_TRIAL_SPHERE 0
+#define NMC_TRIAL_CONE 1
+
+/* Number of scalar observables per instance returned by nmc_measure. */
+#define NMC_NOBS 11
+/* Index of each scalar in one row of nmc_measure's obs output. */
+#define NMC_OBS_MX 0      /* calc_M   thermodynamics.cpp:99-111   */
+#define NMC_OBS_MY 1
+#define NMC_OBS_MZ 2
+#define NMC_OBS_SUBMX 3   /* calc_subM(0) thermodynamics.cpp:113-136 */
+#define NMC_OBS_SUBMY 4
+#define NMC_OBS_SUBMZ 5
+#define NMC_OBS_SUB4MX 6  /* calc_sub4M thermodynamics.cpp:138-163 */
+#define NMC_OBS_SUB4MY 7
+#define NMC_OBS_SUB4MZ 8
+#define NMC_OBS_ER 9      /* E_func as written (DMI term D x s_j), :71-96 */
+#define NMC_OBS_ES 10     /* energy whose differences dE_func returns (s_j x D) */
+
+/* Lattice-wide options: one INPUT.dat (Old/lib/param_read.cpp:35-82). */
+typedef struct nmc_config {
+    int dim;            /* 2 (LATTSHAPE s/w) or 3 (c/x)                      */
+    int ising;          /* ISISING: 1 = int8 +-1 spins, 0 = fp32 unit vectors */
+    int periodic;       /* ISPERIO                                            */
+    int use_j;          /* EXCHANGE != 0  (state.cpp:17)                      */
+    int use_d;          /* DMISTREN != 0                                      */
+    int trial;          /* NMC_TRIAL_*                                        */
+    uint64_t seed;      /* Philox4x32-10 key material                         */
+} nmc_config;
+
+/* One record of an INTERACTIONS file, already multiplied by EXCHANGE and
+ * DMISTREN (Old/lib/field_type.cpp:49-71).  d[2] is 0 for 2-D lattices. */
+typedef struct nmc_bond {
+    int d[3];
+    double J;
+    double D[3];
+} nmc_bond;
+
+/* One lattice instance: a box of n[0] x n[1] (x n[2]) sites placed at
+ * `origin` inside the reference's edge^d lattice box.  `mask` (optional,
+ * n[0]*n[1]*n[2] bytes in box-scan order, nonzero = site present) selects the
+ * sites a shape kept (shape::check, Old/lib/shape.cpp:40-58); NULL keeps all.
+ * `edge` is the reference's edgeSize, used by calc_sub4M and for the number of
+ * topological-charge slices in 3-D. */
+typedef struct nmc_instance_desc {
+    int n[3];
+    int origin[3];
+    int edge;
+    const uint8_t* mask;
+} nmc_instance_desc;
+
+typedef struct nmc_ctx nmc_ctx;
+
+/* Text of the last error on this thread ("" if none). */
+const char* nmc_last_error(void);
+
+/* Build a context: parse-free equivalent of state(stateOptions) ->
+ * init_points + field_type ctor + init_lattice (state.cpp:15-35,48-82,124-169)
+ * for `ninst` instances sharing one interaction table.  Spins start at the
+ * reference's default up spin (field_type.cpp:76-95); call nmc_randomize for
+ * all_rand (field_type.cpp:197-204). */
+int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
+               const nmc_instance_desc* inst, int ninst, nmc_ctx** out);
+void nmc_destroy(nmc_ctx* ctx);
+
+/* The hipStream_t (as void*) all work of this context is ordered on, for
+ * callers that order their own work (events, copies) against it.  No
+ * reference counterpart: the reference has no device. */
+int nmc_get_stream(nmc_ctx* ctx, void** stream);
+
+/* Number of present sites of an instance (state::num_spins, state.cpp:248). */
+int nmc_num_sites(nmc_ctx* ctx, int inst, int64_t* out);
+/* Number of topological-charge slices per instance (calc_TC's tcsize:
+ * edge in 3-D, 1 in 2-D, thermodynamics.cpp:170-175). */
+int nmc_num_slices(nmc_ctx* ctx, int* out);
+
+/* Per-instance inverse temperature and field strength: change_temp /
+ * change_field (state.cpp:263-276) with beta = 1/(k_B T) and H[h_ind] = h,
+ * h_ind = 0 for Ising, 2 for Heisenberg (state.cpp:22-28).  `beta` must be
+ * finite and > 0 (reference: exit 104 for T <= 0). */
+int nmc_set_params(nmc_ctx* ctx, const int* inst, int n, const double* beta,
+                   const double* h);
+/* Cone half-angle (radians, (0, pi]) per instance for NMC_TRIAL_CONE. */
+int nmc_set_cone(nmc_ctx* ctx, const int* inst, int n, const float* half_angle);
+
+/* field_type::all_rand (field_type.cpp:197-204): every present site gets a
+ * fresh trial spin drawn from the counter-based stream. */
+int nmc_randomize(nmc_ctx* ctx, const int* inst, int n);
+
+/* state::equil(nsweeps * N) (state.cpp:171-203) as `nsweeps` checkerboard
+ * sweeps: every present site gets one single-site Metropolis attempt per
+ * sweep.  inst == NULL means every instance. */
+int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps);
+/* Accepted-attempt counters since the last reset (one per instance). */
+int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out);
+int nmc_reset_accepts(nmc_ctx* ctx);
+
+/* `curr_state = base_state` (main.cpp:164, state.cpp:42-46): copy spins of
+ * src[i] into dst[i] (instances must have the same geometry). */
+int nmc_snapshot(nmc_ctx* ctx, const int* src, const int* dst, int n);
+
+/* Observables of each listed instance, written to obs[i*NMC_NOBS + k] and to
+ * q_ref/q_wrap[i*nslices + s]: magnetisation, submag(0), sub4mag, energy,
+ * tcharge (state.cpp:205-246; thermodynamics.cpp:71-221).  q_ref follows
+ * calc_TC exactly (no triangles across a periodic wrap); q_wrap also counts
+ * wrapped triangles.  q_ref / q_wrap may be NULL (Ising: zeros). */
+int nmc_measure(nmc_ctx* ctx, const int* inst, int n, double* obs,
+                double* q_ref, double* q_wrap);
+
+/* Sample series kept in HBM: nmc_series_reset sizes the buffer for
+ * `nsamples` samples of every instance (with_charge = 0 skips the per-slice
+ * topological charges, as the reference does for distributed runs,
+ * output.cpp:80-252); nmc_sample measures the listed instances into sample
+ * slot `slot` without a host round trip (the sample loop of
+ * main.cpp:182-240); nmc_get_series copies the series out with the layout of
+ * nmc_measure, one block of nsamples rows per instance. */
+int nmc_series_reset(nmc_ctx* ctx, int nsamples, int with_charge);
+int nmc_sample(nmc_ctx* ctx, const int* inst, int n, int slot);
+int nmc_get_series(nmc_ctx* ctx, const int* inst, int n, double* obs,
+                   double* q_ref, double* q_wrap);
+
+/* state::add_to_av (state.cpp:288-296): sum the current spins of src into
+ * the running lattice average of instance `dst_avg` slot; nmc_get_avg returns
+ * it in site order (3 doubles per site), nmc_zero_avg is field_type::all_zero
+ * on the summed field (main.cpp:176-179). */
+int nmc_accumulate_avg(nmc_ctx* ctx, int inst);
+int nmc_get_avg(nmc_ctx* ctx, int inst, double* out);
+int nmc_zero_avg(nmc_ctx* ctx, int inst);
+
+/* Spins of one instance in site order, 3 doubles per site (field_type::access
+ * / set_spin, field_type.hpp:80,188).  Heisenberg spins are stored as fp32;
+ * Ising spins must be -1, 0 or +1. */
+int nmc_get_spins(nmc_ctx* ctx, int inst, double* spins);
+int nmc_set_spins(nmc_ctx* ctx, int inst, const double* spins);
+
+/* ---- test mode (no counterpart on the reference's production path) ---- */
+
+/* dE_func (thermodynamics.cpp:51-69) on the current spins of `inst` for
+ * site `sites[i]` (site order) and trial spin trial[3i..3i+2], evaluated by
+ * the same device code the sweep uses.  Ising: the double the sweep's
+ * acceptance table holds for that neighbourhood.  Heisenberg: fp32. */
+int nmc_delta_e(nmc_ctx* ctx, int inst, const int* sites, const double* trial,
+                int n, double* dE);
+
+/* The reference's random-site loop (state.cpp:180-202) with the visit order,
+ * trial spins (+-1) and uniforms u[i] = k[i] * 2^-32 supplied by the caller;
+ * runs on the device with the sweep's acceptance rule and writes accept[i]
+ * (1 = the trial spin was stored).  Ising only. */
+int nmc_replay_ising(nmc_ctx* ctx, int inst, const int* sites,
+                     const int8_t* trial, const uint32_t* k, int n,
+                     uint8_t* accept);
+
+/* ---- lattice builder (state::init_lattice + shape::weibull::check) ---- */
+
+/* For each of `ngrains` grains, the edge^dim site mask of a Weibull grain
+ * of radius scale size[g]: site p (box-scan order) is kept iff
+ * U(g,p) < exp(-(|p-c|/size/r0)^wbeta) with c = (edge-1)/2 and
+ * r0 = 1/Gamma(1+1/wbeta) (Old/lib/shape.cpp:19-58); U comes from the
+ * counter-based stream keyed by `seed` and the grain index.  Computed on the
+ * device; mask_out is ngrains*edge^dim bytes. */
+int nmc_build_weibull(int dim, int edge, const double* size, int ngrains,
+                      double wbeta, uint64_t seed, uint8_t* mask_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NMC_H */

@@ -1,0 +1,867 @@
+// nmc_api.hip -- the C ABI of libnmc.so (include/nmc.h): context set-up,
+// parameter tables, launch sequencing and host<->device conversion between
+// the reference's site order and the colour-split layout in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nmc.h"
+#include "nmc_device.h"
+#include "nmc_launch.h"
+
+using namespace nmc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define NMC_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return fail(NMC_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+template <class T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(NMC_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    return NMC_OK;
+}
+
+constexpr int kSweepTile = 256;   // Heisenberg: one site per work-item
+constexpr int kIsingTile = 512;   // Ising: two sites per work-item
+
+// Host view of one instance (reference coordinates) next to its DevInst.
+struct HostInst {
+    int n[3];        // reference extents (x, y, z); z = 1 in 2-D
+    int origin[3];
+    std::vector<uint8_t> mask;   // box-scan order; empty = all present
+    int64_t nsites = 0;
+};
+
+// A cached instance list with its work decomposition in device memory.
+struct Set {
+    std::vector<int> list;
+    int* d_list = nullptr;
+    Tile* d_sweep = nullptr;
+    int n_sweep = 0;
+    Tile* d_init = nullptr;
+    int n_init = 0;
+    MTile* d_mt = nullptr;
+    int n_mt = 0;
+    int* d_tb = nullptr;
+    int* d_te = nullptr;
+};
+
+}  // namespace
+
+struct nmc_ctx {
+    nmc_config cfg{};
+    bool dim3 = false, ising = false, periodic = false, masked = false, use_d = false, cone = false;
+    int zslots = 4;          // stencil slots used (4 in 2-D, 6 in 3-D)
+    int tabsize = 0;         // Ising acceptance-table entries per instance
+    int nsl = 1;             // topological-charge slices
+    int a2max = 2;
+    std::vector<nmc_bond> bonds;
+    std::vector<int> bond_slot;   // stencil slot of each file bond
+    Stencil st{};
+    std::vector<HostInst> hin;
+    std::vector<DevInst> din;
+    DevInst* d_inst = nullptr;
+    void* pool = nullptr;
+    size_t pool_elems = 0;
+    uint32_t* mpool = nullptr;
+    uint32_t* d_tab = nullptr;
+    double* d_dtab = nullptr;
+    std::vector<uint32_t> h_tab;
+    std::vector<double> h_dtab;
+    uint64_t* d_sweeps = nullptr;
+    unsigned long long* d_acc = nullptr;
+    double* d_partials = nullptr;
+    size_t partials_cap = 0;
+    double* d_obs = nullptr;
+    size_t obs_cap = 0;
+    int series_n = 0;
+    bool series_q = false;
+    double* d_series = nullptr;
+    double* d_series_qr = nullptr;
+    double* d_series_qw = nullptr;
+    std::map<int, double*> avg;
+    uint64_t init_epoch = 0;
+    hipStream_t stream = nullptr;
+    std::map<std::vector<int>, std::unique_ptr<Set>> sets;
+    // Ising threshold rows keyed by (beta, h)
+    std::map<std::pair<double, double>, std::pair<std::vector<uint32_t>, std::vector<double>>> tab_cache;
+
+    ~nmc_ctx() {
+        for (auto& kv : sets) {
+            Set& s = *kv.second;
+            (void)hipFree(s.d_list); (void)hipFree(s.d_sweep); (void)hipFree(s.d_init);
+            (void)hipFree(s.d_mt); (void)hipFree(s.d_tb); (void)hipFree(s.d_te);
+        }
+        for (auto& kv : avg) (void)hipFree(kv.second);
+        (void)hipFree(d_inst); (void)hipFree(pool); (void)hipFree(mpool); (void)hipFree(d_tab);
+        (void)hipFree(d_dtab); (void)hipFree(d_sweeps); (void)hipFree(d_acc);
+        (void)hipFree(d_partials); (void)hipFree(d_obs); (void)hipFree(d_series);
+        (void)hipFree(d_series_qr); (void)hipFree(d_series_qw);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    size_t spin_bytes(const DevInst& d) const {
+        return (size_t)(ising ? 2 * d.Sp : 6 * d.Sp * sizeof(float));
+    }
+};
+
+namespace {
+
+// Internal coordinates (i0, i1, i2) of reference coordinates (x, y, z).
+inline void to_internal(bool dim3, int x, int y, int z, int& i0, int& i1, int& i2) {
+    if (dim3) { i0 = x; i1 = y; i2 = z; }
+    else { i0 = 0; i1 = x; i2 = y; }
+}
+
+// Walk the present sites of instance `i` in the reference's site order (box
+// scan, last coordinate fastest) and call f(site, colour, element).
+template <class F>
+void for_each_site(const nmc_ctx* c, int i, F f) {
+    const HostInst& h = c->hin[i];
+    const DevInst& d = c->din[i];
+    int64_t site = 0, box = 0;
+    const int nz = c->dim3 ? h.n[2] : 1;
+    for (int x = 0; x < h.n[0]; ++x)
+        for (int y = 0; y < h.n[1]; ++y)
+            for (int z = 0; z < nz; ++z, ++box) {
+                if (!h.mask.empty() && !h.mask[box]) continue;
+                int i0, i1, i2;
+                to_internal(c->dim3, x, y, z, i0, i1, i2);
+                const int col = (i0 + i1 + i2) & 1;
+                const int64_t e = ((int64_t)i0 * d.a1 + i1) * d.half + (i2 >> 1);
+                f(site++, col, e);
+            }
+}
+
+int check_inst(const nmc_ctx* c, int i) {
+    if (i < 0 || i >= (int)c->din.size()) return fail(NMC_E_INVALID, "instance index " + std::to_string(i) + " out of range");
+    return NMC_OK;
+}
+
+int upload_insts(nmc_ctx* c) {
+    NMC_HIP(hipMemcpyAsync(c->d_inst, c->din.data(), c->din.size() * sizeof(DevInst), hipMemcpyHostToDevice, c->stream));
+    NMC_HIP(hipStreamSynchronize(c->stream));
+    return NMC_OK;
+}
+
+template <class T>
+int upload(nmc_ctx* c, T** dptr, const std::vector<T>& v) {
+    int rc = dalloc(dptr, v.size());
+    if (rc) return rc;
+    if (!v.empty()) {
+        NMC_HIP(hipMemcpyAsync(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+        NMC_HIP(hipStreamSynchronize(c->stream));
+    }
+    return NMC_OK;
+}
+
+// Work decomposition for an instance list (cached per distinct list).
+int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
+    std::vector<int> list;
+    if (inst == nullptr) {
+        list.resize(c->din.size());
+        for (size_t i = 0; i < list.size(); ++i) list[i] = (int)i;
+    } else {
+        if (n < 0) return fail(NMC_E_INVALID, "negative instance count");
+        list.assign(inst, inst + n);
+        for (int i : list) if (int rc = check_inst(c, i)) return rc;
+    }
+    auto it = c->sets.find(list);
+    if (it != c->sets.end()) { *out = it->second.get(); return NMC_OK; }
+    auto s = std::make_unique<Set>();
+    s->list = list;
+    std::vector<Tile> sw, ini;
+    std::vector<MTile> mt;
+    std::vector<int> tb, te;
+    const int stile = c->ising ? kIsingTile : kSweepTile;
+    for (int i : list) {
+        const DevInst& d = c->din[i];
+        for (int64_t e = 0; e < d.S; e += stile) sw.push_back(Tile{i, (int)e});
+        for (int64_t e = 0; e < d.Sp; e += 256) ini.push_back(Tile{i, (int)e});
+        const int rows = d.a0 * d.a1;
+        int per = std::max(4, (int)std::min<int64_t>(rows, (4096 + d.half - 1) / std::max(1, d.half)));
+        tb.push_back((int)mt.size());
+        for (int r = 0; r < rows; r += per) mt.push_back(MTile{i, r, std::min(per, rows - r)});
+        te.push_back((int)mt.size());
+    }
+    int rc;
+    if ((rc = upload(c, &s->d_list, list))) return rc;
+    if ((rc = upload(c, &s->d_sweep, sw))) return rc;
+    if ((rc = upload(c, &s->d_init, ini))) return rc;
+    if ((rc = upload(c, &s->d_mt, mt))) return rc;
+    if ((rc = upload(c, &s->d_tb, tb))) return rc;
+    if ((rc = upload(c, &s->d_te, te))) return rc;
+    s->n_sweep = (int)sw.size();
+    s->n_init = (int)ini.size();
+    s->n_mt = (int)mt.size();
+    *out = s.get();
+    c->sets[list] = std::move(s);
+    return NMC_OK;
+}
+
+// Largest k in [0, 2^32) with -dE*beta > log(k * 2^-32); all k when dE <= 0.
+uint32_t accept_threshold(double dE, double beta) {
+    if (dE <= 0) return 0xFFFFFFFFu;
+    const double lhs = -dE * beta;
+    uint64_t lo = 0, hi = 0xFFFFFFFFull;
+    if (lhs > std::log((double)hi / 4294967296.0)) return 0xFFFFFFFFu;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (lhs > std::log((double)mid / 4294967296.0)) lo = mid; else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+// dE_func (thermodynamics.cpp:51-69) of flipping s_old -> -s_old for every
+// neighbourhood pattern, in file-bond order, then its acceptance threshold.
+void ising_rows(const nmc_ctx* c, double beta, double h, std::vector<uint32_t>& T, std::vector<double>& dEt) {
+    const int z = c->zslots;
+    const int g0 = c->dim3 ? 0 : 2;
+    T.assign(c->tabsize, 0);
+    dEt.assign(c->tabsize, 0.0);
+    std::vector<int> val(6, 0);
+    for (int idx = 0; idx < c->tabsize; ++idx) {
+        const int s_old = (idx & 1) ? 1 : -1;
+        int rest = idx >> 1;
+        for (int g = 0; g < z; ++g) { val[g0 + g] = rest % 3 - 1; rest /= 3; }
+        const double H[3] = {h, 0.0, 0.0};
+        const double sv[3] = {(double)s_old, 0.0, 0.0};
+        const double tv[3] = {(double)-s_old, 0.0, 0.0};
+        double Vdiff[3], Vsum[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; ++a) Vdiff[a] = sv[a] - tv[a];
+        for (size_t k = 0; k < c->bonds.size(); ++k) {
+            const int nv = val[c->bond_slot[k]];
+            if (nv == 0) continue;   // absent neighbour: not in the reference's list
+            const double sj[3] = {(double)nv, 0.0, 0.0};
+            if (c->cfg.use_j)
+                for (int a = 0; a < 3; ++a) Vsum[a] += c->bonds[k].J * sj[a];
+            if (c->cfg.use_d) {
+                const double* D = c->bonds[k].D;
+                const double t[3] = {sj[1] * D[2] - sj[2] * D[1], sj[2] * D[0] - sj[0] * D[2],
+                                     sj[0] * D[1] - sj[1] * D[0]};
+                for (int a = 0; a < 3; ++a) Vsum[a] += t[a];
+            }
+        }
+        double res[3];
+        for (int a = 0; a < 3; ++a) res[a] = (H[a] + Vsum[a]) * Vdiff[a];
+        const double dE = res[0] + res[1] + res[2];
+        dEt[idx] = dE;
+        T[idx] = accept_threshold(dE, beta);
+    }
+}
+
+int ensure_partials(nmc_ctx* c, size_t n) {
+    if (n <= c->partials_cap) return NMC_OK;
+    (void)hipFree(c->d_partials);
+    c->d_partials = nullptr;
+    c->partials_cap = 0;
+    if (int rc = dalloc(&c->d_partials, n)) return rc;
+    c->partials_cap = n;
+    return NMC_OK;
+}
+
+int pstride_of(const nmc_ctx* c) {
+    if (c->ising) return 15;
+    return c->dim3 ? 15 + 2 * c->a2max : 17;
+}
+
+int run_measure(nmc_ctx* c, Set* s) {
+    const int ps = pstride_of(c);
+    if (int rc = ensure_partials(c, (size_t)s->n_mt * ps)) return rc;
+    NMC_HIP(launch_measure(c->stream, c->dim3, c->periodic, c->masked, c->ising, c->d_inst, s->d_mt,
+                           s->n_mt, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
+    return NMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nmc_last_error(void) { return g_err.c_str(); }
+
+int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
+               const nmc_instance_desc* inst, int ninst, nmc_ctx** out) {
+    if (!cfg || !out || (nbonds > 0 && !bonds) || ninst <= 0 || !inst)
+        return fail(NMC_E_INVALID, "nmc_create: null argument or no instances");
+    *out = nullptr;
+    if (cfg->dim != 2 && cfg->dim != 3) return fail(NMC_E_INVALID, "dim must be 2 or 3");
+    if (cfg->trial != NMC_TRIAL_SPHERE && cfg->trial != NMC_TRIAL_CONE)
+        return fail(NMC_E_INVALID, "unknown trial generator");
+    if (cfg->ising && cfg->trial == NMC_TRIAL_CONE)
+        return fail(NMC_E_INVALID, "cone trials need Heisenberg spins");
+    auto c = std::make_unique<nmc_ctx>();
+    c->cfg = *cfg;
+    c->dim3 = cfg->dim == 3;
+    c->ising = cfg->ising != 0;
+    c->periodic = cfg->periodic != 0;
+    c->use_d = cfg->use_d != 0 && !c->ising;
+    c->cone = cfg->trial == NMC_TRIAL_CONE;
+    c->zslots = c->dim3 ? 6 : 4;
+    // interaction table -> stencil slots (nearest-neighbour bond sets)
+    bool used[6] = {false, false, false, false, false, false};
+    for (int k = 0; k < nbonds; ++k) {
+        const nmc_bond& b = bonds[k];
+        int axis = -1, nz = 0;
+        for (int a = 0; a < 3; ++a)
+            if (b.d[a] != 0) { ++nz; axis = a; }
+        if (nz != 1 || std::abs(b.d[axis]) != 1 || (!c->dim3 && axis == 2))
+            return fail(NMC_E_UNSUPPORTED, "bond " + std::to_string(k) +
+                        " is not a nearest-neighbour unit vector; only bipartite unit-bond tables are supported");
+        const int g = (c->dim3 ? 2 * axis : 2 * (axis + 1)) + (b.d[axis] > 0 ? 1 : 0);
+        if (used[g]) return fail(NMC_E_UNSUPPORTED, "bond direction repeated in the interaction table");
+        used[g] = true;
+        c->bonds.push_back(b);
+        c->bond_slot.push_back(g);
+        const double J = cfg->use_j ? b.J : 0.0;
+        c->st.J[g] = (float)J;
+        c->st.Jd[g] = J;
+        for (int a = 0; a < 3; ++a) {
+            const double D = cfg->use_d ? b.D[a] : 0.0;
+            c->st.D[g][a] = (float)D;
+            c->st.Dd[g][a] = D;
+        }
+    }
+    for (int g = 0, p = 1; g < 6; ++g) {
+        if (!c->dim3 && g < 2) { c->st.pow3[g] = 0; continue; }
+        c->st.pow3[g] = p;
+        p *= 3;
+    }
+    c->tabsize = c->ising ? 2 * (c->dim3 ? 729 : 81) : 0;
+    // instances
+    int64_t pool = 0, mwords = 0;
+    for (int i = 0; i < ninst; ++i) c->masked = c->masked || inst[i].mask != nullptr;
+    int nsl = 1;
+    for (int i = 0; i < ninst; ++i) {
+        const nmc_instance_desc& s = inst[i];
+        HostInst h;
+        const int nz = c->dim3 ? s.n[2] : 1;
+        h.n[0] = s.n[0]; h.n[1] = s.n[1]; h.n[2] = nz;
+        for (int a = 0; a < 3; ++a) h.origin[a] = (a < cfg->dim) ? s.origin[a] : 0;
+        if (s.n[0] < 1 || s.n[1] < 1 || nz < 1) return fail(NMC_E_INVALID, "instance extents must be >= 1");
+        if (s.edge < 1) return fail(NMC_E_INVALID, "edge must be >= 1");
+        for (int a = 0; a < cfg->dim; ++a) {
+            const int na = a < 2 ? s.n[a] : nz;
+            if (h.origin[a] < 0 || h.origin[a] + na > s.edge)
+                return fail(NMC_E_INVALID, "instance box lies outside the edge^d lattice box");
+            if (c->periodic && (na != s.edge || h.origin[a] != 0))
+                return fail(NMC_E_INVALID, "periodic instances must cover the whole edge^d box");
+        }
+        if (c->periodic && (s.edge % 2) != 0)
+            return fail(NMC_E_UNSUPPORTED, "periodic lattices need an even edge for a two-colour sweep");
+        DevInst d{};
+        int i0n, i1n, i2n;
+        to_internal(c->dim3, h.n[0], h.n[1], h.n[2], i0n, i1n, i2n);
+        if (!c->dim3) { i0n = 1; i1n = h.n[0]; i2n = h.n[1]; }
+        d.a0 = i0n; d.a1 = i1n; d.n2 = i2n; d.a2 = i2n + (i2n & 1); d.half = d.a2 / 2;
+        int o0, o1, o2;
+        to_internal(c->dim3, h.origin[0], h.origin[1], h.origin[2], o0, o1, o2);
+        d.o0 = o0; d.o1 = o1; d.o2 = o2;
+        d.edge = s.edge;
+        d.masked = c->masked ? 1 : 0;
+        d.div_half = make_fastdiv((uint32_t)d.half);
+        d.div_a1 = make_fastdiv((uint32_t)d.a1);
+        d.S = (int64_t)d.a0 * d.a1 * d.half;
+        if (d.S >= (int64_t)1 << 30) return fail(NMC_E_UNSUPPORTED, "instance too large (2^31 sites)");
+        d.Sp = (d.S + 63) / 64 * 64;
+        d.base = pool;
+        pool += c->ising ? 2 * d.Sp : 6 * d.Sp;
+        d.mask_stride = (d.Sp + 31) / 32;
+        d.mask_base = mwords;
+        if (c->masked) mwords += 2 * d.mask_stride;
+        d.tab_base = (int64_t)i * c->tabsize;
+        d.inst_id = (uint32_t)i;
+        d.beta = 1.f; d.beta_d = 1.0; d.h = 0.f; d.h_d = 0.0; d.cone_w = 1.f;
+        const int64_t box = (int64_t)h.n[0] * h.n[1] * h.n[2];
+        if (s.mask) h.mask.assign(s.mask, s.mask + box);
+        h.nsites = h.mask.empty() ? box : (int64_t)std::count_if(h.mask.begin(), h.mask.end(), [](uint8_t v) { return v != 0; });
+        c->a2max = std::max(c->a2max, d.a2);
+        if (c->dim3) nsl = std::max(nsl, s.edge);
+        c->hin.push_back(std::move(h));
+        c->din.push_back(d);
+    }
+    if (!c->ising && c->dim3 && c->a2max > 512)
+        return fail(NMC_E_UNSUPPORTED, "3-D extent along z above 512 is not supported by the charge reduction");
+    c->nsl = nsl;
+    c->pool_elems = (size_t)pool;
+    NMC_HIP(hipStreamCreate(&c->stream));
+    int rc;
+    if ((rc = dalloc(&c->d_inst, c->din.size()))) return rc;
+    if (c->ising) { int8_t* p; if ((rc = dalloc(&p, (size_t)pool))) return rc; c->pool = p; }
+    else { float* p; if ((rc = dalloc(&p, (size_t)pool))) return rc; c->pool = p; }
+    if (c->masked) {
+        std::vector<uint32_t> mw((size_t)mwords, 0u);
+        for (int i = 0; i < ninst; ++i) {
+            const DevInst& d = c->din[i];
+            for_each_site(c.get(), i, [&](int64_t, int col, int64_t e) {
+                mw[d.mask_base + col * d.mask_stride + (e >> 5)] |= 1u << (e & 31);
+            });
+        }
+        if ((rc = upload(c.get(), &c->mpool, mw))) return rc;
+    }
+    if (c->ising) {
+        if ((rc = dalloc(&c->d_tab, (size_t)ninst * c->tabsize))) return rc;
+        if ((rc = dalloc(&c->d_dtab, (size_t)ninst * c->tabsize))) return rc;
+        c->h_tab.assign((size_t)ninst * c->tabsize, 0u);
+        c->h_dtab.assign((size_t)ninst * c->tabsize, 0.0);
+    }
+    if ((rc = dalloc(&c->d_sweeps, c->din.size()))) return rc;
+    if ((rc = dalloc(&c->d_acc, c->din.size()))) return rc;
+    NMC_HIP(hipMemsetAsync(c->d_sweeps, 0, c->din.size() * sizeof(uint64_t), c->stream));
+    NMC_HIP(hipMemsetAsync(c->d_acc, 0, c->din.size() * sizeof(unsigned long long), c->stream));
+    if ((rc = upload_insts(c.get()))) return rc;
+    // default up spins (set_default_spins) with beta = 1, h = 0
+    std::vector<double> b1(ninst, 1.0), h0(ninst, 0.0);
+    nmc_ctx* raw = c.release();
+    if ((rc = nmc_set_params(raw, nullptr, ninst, b1.data(), h0.data()))) { delete raw; return rc; }
+    Set* s;
+    if ((rc = get_set(raw, nullptr, 0, &s))) { delete raw; return rc; }
+    for (int col = 0; col < 2; ++col) {
+        hipError_t e = launch_init_spins(raw->stream, raw->ising, raw->d_inst, s->d_init, s->n_init, raw->pool,
+                                         raw->mpool, col, false, 0, 0, 0, raw->dim3);
+        if (e != hipSuccess) { delete raw; return fail(NMC_E_HIP, std::string("init spins: ") + hipGetErrorString(e)); }
+    }
+    hipError_t e = hipStreamSynchronize(raw->stream);
+    if (e != hipSuccess) { delete raw; return fail(NMC_E_HIP, std::string("nmc_create: ") + hipGetErrorString(e)); }
+    *out = raw;
+    return NMC_OK;
+}
+
+void nmc_destroy(nmc_ctx* ctx) {
+    if (ctx) {
+        (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    }
+}
+
+int nmc_get_stream(nmc_ctx* ctx, void** stream) {
+    if (!ctx || !stream) return fail(NMC_E_INVALID, "null argument");
+    *stream = (void*)ctx->stream;
+    return NMC_OK;
+}
+
+int nmc_num_sites(nmc_ctx* ctx, int inst, int64_t* out) {
+    if (!ctx || !out) return fail(NMC_E_INVALID, "null argument");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    *out = ctx->hin[inst].nsites;
+    return NMC_OK;
+}
+
+int nmc_num_slices(nmc_ctx* ctx, int* out) {
+    if (!ctx || !out) return fail(NMC_E_INVALID, "null argument");
+    *out = ctx->nsl;
+    return NMC_OK;
+}
+
+int nmc_set_params(nmc_ctx* ctx, const int* inst, int n, const double* beta, const double* h) {
+    if (!ctx || !beta || !h) return fail(NMC_E_INVALID, "null argument");
+    const int cnt = inst ? n : (int)ctx->din.size();
+    if (!inst && n != cnt) return fail(NMC_E_INVALID, "with inst == NULL, n must equal the instance count");
+    for (int j = 0; j < cnt; ++j) {
+        const int i = inst ? inst[j] : j;
+        if (int rc = check_inst(ctx, i)) return rc;
+        if (!(beta[j] > 0) || !std::isfinite(beta[j]) || !std::isfinite(h[j]))
+            return fail(NMC_E_INVALID, "beta must be finite and > 0 (temperature <= 0 is invalid)");
+        DevInst& d = ctx->din[i];
+        d.beta_d = beta[j]; d.beta = (float)beta[j];
+        d.h_d = h[j]; d.h = (float)h[j];
+        if (ctx->ising) {
+            auto key = std::make_pair(beta[j], h[j]);
+            auto it = ctx->tab_cache.find(key);
+            if (it == ctx->tab_cache.end()) {
+                std::vector<uint32_t> T;
+                std::vector<double> dE;
+                ising_rows(ctx, beta[j], h[j], T, dE);
+                it = ctx->tab_cache.emplace(key, std::make_pair(std::move(T), std::move(dE))).first;
+            }
+            std::copy(it->second.first.begin(), it->second.first.end(), ctx->h_tab.begin() + d.tab_base);
+            std::copy(it->second.second.begin(), it->second.second.end(), ctx->h_dtab.begin() + d.tab_base);
+        }
+    }
+    if (ctx->ising) {
+        NMC_HIP(hipMemcpyAsync(ctx->d_tab, ctx->h_tab.data(), ctx->h_tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        NMC_HIP(hipMemcpyAsync(ctx->d_dtab, ctx->h_dtab.data(), ctx->h_dtab.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    return upload_insts(ctx);
+}
+
+int nmc_set_cone(nmc_ctx* ctx, const int* inst, int n, const float* half_angle) {
+    if (!ctx || !half_angle) return fail(NMC_E_INVALID, "null argument");
+    const int cnt = inst ? n : (int)ctx->din.size();
+    for (int j = 0; j < cnt; ++j) {
+        const int i = inst ? inst[j] : j;
+        if (int rc = check_inst(ctx, i)) return rc;
+        const float th = half_angle[j];
+        if (!(th > 0.f) || th > (float)M_PI + 1e-6f) return fail(NMC_E_INVALID, "cone half-angle must be in (0, pi]");
+        ctx->din[i].cone_w = (float)(1.0 - std::cos((double)th));
+    }
+    return upload_insts(ctx);
+}
+
+int nmc_randomize(nmc_ctx* ctx, const int* inst, int n) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    const uint64_t epoch = ++ctx->init_epoch;
+    for (int col = 0; col < 2; ++col)
+        NMC_HIP(launch_init_spins(ctx->stream, ctx->ising, ctx->d_inst, s->d_init, s->n_init, ctx->pool, ctx->mpool,
+                                  col, true, epoch, (uint32_t)ctx->cfg.seed, (uint32_t)(ctx->cfg.seed >> 32), ctx->dim3));
+    return NMC_OK;
+}
+
+int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (nsweeps < 0) return fail(NMC_E_INVALID, "negative sweep count");
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
+    for (int k = 0; k < nsweeps; ++k)
+        for (int col = 0; col < 2; ++col) {
+            if (ctx->ising)
+                NMC_HIP(launch_sweep_ising(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, true, ctx->d_inst,
+                                           s->d_sweep, s->n_sweep, (int8_t*)ctx->pool, ctx->mpool, ctx->d_tab,
+                                           ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, ctx->d_acc));
+            else
+                NMC_HIP(launch_sweep_heis(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d, ctx->cone,
+                                          true, ctx->d_inst, s->d_sweep, s->n_sweep, (float*)ctx->pool, ctx->mpool,
+                                          ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, ctx->d_acc));
+        }
+    if (nsweeps > 0)
+        NMC_HIP(launch_bump_sweeps(ctx->stream, ctx->d_sweeps, s->d_list, (int)s->list.size(), (uint64_t)nsweeps));
+    return NMC_OK;
+}
+
+int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out) {
+    if (!ctx || !out) return fail(NMC_E_INVALID, "null argument");
+    std::vector<uint64_t> all(ctx->din.size());
+    NMC_HIP(hipMemcpyAsync(all.data(), ctx->d_acc, all.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    const int cnt = inst ? n : (int)all.size();
+    for (int j = 0; j < cnt; ++j) {
+        const int i = inst ? inst[j] : j;
+        if (int rc = check_inst(ctx, i)) return rc;
+        out[j] = all[i];
+    }
+    return NMC_OK;
+}
+
+int nmc_reset_accepts(nmc_ctx* ctx) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    NMC_HIP(hipMemsetAsync(ctx->d_acc, 0, ctx->din.size() * sizeof(unsigned long long), ctx->stream));
+    return NMC_OK;
+}
+
+int nmc_snapshot(nmc_ctx* ctx, const int* src, const int* dst, int n) {
+    if (!ctx || (n > 0 && (!src || !dst))) return fail(NMC_E_INVALID, "null argument");
+    for (int j = 0; j < n; ++j) {
+        if (int rc = check_inst(ctx, src[j])) return rc;
+        if (int rc = check_inst(ctx, dst[j])) return rc;
+        const DevInst &a = ctx->din[src[j]], &b = ctx->din[dst[j]];
+        if (a.a0 != b.a0 || a.a1 != b.a1 || a.a2 != b.a2 || a.n2 != b.n2)
+            return fail(NMC_E_INVALID, "snapshot between instances of different geometry");
+        if (src[j] == dst[j]) continue;
+        const size_t esz = ctx->ising ? 1 : sizeof(float);
+        NMC_HIP(hipMemcpyAsync((char*)ctx->pool + b.base * esz, (char*)ctx->pool + a.base * esz, ctx->spin_bytes(a),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return NMC_OK;
+}
+
+int nmc_measure(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_ref, double* q_wrap) {
+    if (!ctx || !obs) return fail(NMC_E_INVALID, "null argument");
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    const int cnt = (int)s->list.size();
+    const size_t need = (size_t)cnt * (11 + 2 * ctx->nsl);
+    if (need > ctx->obs_cap) {
+        (void)hipFree(ctx->d_obs);
+        ctx->d_obs = nullptr;
+        ctx->obs_cap = 0;
+        if (int rc = dalloc(&ctx->d_obs, need)) return rc;
+        ctx->obs_cap = need;
+    }
+    double* dq_r = ctx->d_obs + (size_t)cnt * 11;
+    double* dq_w = dq_r + (size_t)cnt * ctx->nsl;
+    if (int rc = run_measure(ctx, s)) return rc;
+    NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, cnt, s->d_tb, s->d_te, ctx->d_partials,
+                                    pstride_of(ctx), ctx->a2max, ctx->dim3, !ctx->ising, ctx->nsl, ctx->d_obs,
+                                    dq_r, dq_w, false, 1, 0, ctx->ising ? 0 : 2));
+    std::vector<double> h(need);
+    NMC_HIP(hipMemcpyAsync(h.data(), ctx->d_obs, need * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    std::copy(h.begin(), h.begin() + (size_t)cnt * 11, obs);
+    const size_t qn = (size_t)cnt * ctx->nsl;
+    if (q_ref) { if (ctx->ising) std::fill(q_ref, q_ref + qn, 0.0); else std::copy(h.begin() + cnt * 11, h.begin() + cnt * 11 + qn, q_ref); }
+    if (q_wrap) { if (ctx->ising) std::fill(q_wrap, q_wrap + qn, 0.0); else std::copy(h.begin() + cnt * 11 + qn, h.begin() + cnt * 11 + 2 * qn, q_wrap); }
+    return NMC_OK;
+}
+
+int nmc_series_reset(nmc_ctx* ctx, int nsamples, int with_charge) {
+    if (!ctx || nsamples < 0) return fail(NMC_E_INVALID, "bad argument");
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(ctx->d_series); (void)hipFree(ctx->d_series_qr); (void)hipFree(ctx->d_series_qw);
+    ctx->d_series = ctx->d_series_qr = ctx->d_series_qw = nullptr;
+    ctx->series_n = 0;
+    const size_t rows = ctx->din.size() * (size_t)nsamples;
+    if (int rc = dalloc(&ctx->d_series, rows * 11)) return rc;
+    NMC_HIP(hipMemsetAsync(ctx->d_series, 0, rows * 11 * sizeof(double), ctx->stream));
+    ctx->series_q = with_charge && !ctx->ising;
+    if (ctx->series_q) {
+        if (int rc = dalloc(&ctx->d_series_qr, rows * ctx->nsl)) return rc;
+        if (int rc = dalloc(&ctx->d_series_qw, rows * ctx->nsl)) return rc;
+        NMC_HIP(hipMemsetAsync(ctx->d_series_qr, 0, rows * ctx->nsl * sizeof(double), ctx->stream));
+        NMC_HIP(hipMemsetAsync(ctx->d_series_qw, 0, rows * ctx->nsl * sizeof(double), ctx->stream));
+    }
+    ctx->series_n = nsamples;
+    return NMC_OK;
+}
+
+int nmc_sample(nmc_ctx* ctx, const int* inst, int n, int slot) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (slot < 0 || slot >= ctx->series_n) return fail(NMC_E_INVALID, "sample slot outside the series buffer");
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    if (int rc = run_measure(ctx, s)) return rc;
+    NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, (int)s->list.size(), s->d_tb, s->d_te,
+                                    ctx->d_partials, pstride_of(ctx), ctx->a2max, ctx->dim3, ctx->series_q, ctx->nsl,
+                                    ctx->d_series, ctx->d_series_qr, ctx->d_series_qw, true, ctx->series_n, slot,
+                                    ctx->ising ? 0 : 2));
+    return NMC_OK;
+}
+
+int nmc_get_series(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_ref, double* q_wrap) {
+    if (!ctx || !obs) return fail(NMC_E_INVALID, "null argument");
+    const int cnt = inst ? n : (int)ctx->din.size();
+    const size_t ns = (size_t)ctx->series_n;
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    for (int j = 0; j < cnt; ++j) {
+        const int i = inst ? inst[j] : j;
+        if (int rc = check_inst(ctx, i)) return rc;
+        NMC_HIP(hipMemcpy(obs + (size_t)j * ns * 11, ctx->d_series + (size_t)i * ns * 11, ns * 11 * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        const size_t qn = ns * ctx->nsl;
+        for (int w = 0; w < 2; ++w) {
+            double* dst = w ? q_wrap : q_ref;
+            if (!dst) continue;
+            double* src = w ? ctx->d_series_qw : ctx->d_series_qr;
+            if (!ctx->series_q) std::fill(dst + j * qn, dst + (j + 1) * qn, 0.0);
+            else NMC_HIP(hipMemcpy(dst + j * qn, src + (size_t)i * qn, qn * sizeof(double), hipMemcpyDeviceToHost));
+        }
+    }
+    return NMC_OK;
+}
+
+int nmc_accumulate_avg(nmc_ctx* ctx, int inst) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    const DevInst& d = ctx->din[inst];
+    double*& a = ctx->avg[inst];
+    if (!a) {
+        if (int rc = dalloc(&a, (size_t)6 * d.Sp)) return rc;
+        NMC_HIP(hipMemsetAsync(a, 0, (size_t)6 * d.Sp * sizeof(double), ctx->stream));
+    }
+    NMC_HIP(launch_accum_avg(ctx->stream, ctx->ising, ctx->d_inst, inst, d.Sp, ctx->pool, a));
+    return NMC_OK;
+}
+
+int nmc_zero_avg(nmc_ctx* ctx, int inst) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    auto it = ctx->avg.find(inst);
+    if (it != ctx->avg.end())
+        NMC_HIP(hipMemsetAsync(it->second, 0, (size_t)6 * ctx->din[inst].Sp * sizeof(double), ctx->stream));
+    return NMC_OK;
+}
+
+int nmc_get_avg(nmc_ctx* ctx, int inst, double* out) {
+    if (!ctx || !out) return fail(NMC_E_INVALID, "null argument");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    const DevInst& d = ctx->din[inst];
+    std::vector<double> h((size_t)6 * d.Sp, 0.0);
+    auto it = ctx->avg.find(inst);
+    if (it != ctx->avg.end()) {
+        NMC_HIP(hipMemcpyAsync(h.data(), it->second, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        NMC_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    for_each_site(ctx, inst, [&](int64_t site, int col, int64_t e) {
+        for (int a = 0; a < 3; ++a) out[3 * site + a] = h[(size_t)(col * 3 + a) * d.Sp + e];
+    });
+    return NMC_OK;
+}
+
+int nmc_get_spins(nmc_ctx* ctx, int inst, double* spins) {
+    if (!ctx || !spins) return fail(NMC_E_INVALID, "null argument");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    const DevInst& d = ctx->din[inst];
+    const size_t esz = ctx->ising ? 1 : sizeof(float);
+    std::vector<char> h(ctx->spin_bytes(d));
+    NMC_HIP(hipMemcpyAsync(h.data(), (char*)ctx->pool + d.base * esz, h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->ising) {
+        const int8_t* p = (const int8_t*)h.data();
+        for_each_site(ctx, inst, [&](int64_t site, int col, int64_t e) {
+            spins[3 * site] = (double)p[col * d.Sp + e];
+            spins[3 * site + 1] = 0.0;
+            spins[3 * site + 2] = 0.0;
+        });
+    } else {
+        const float* p = (const float*)h.data();
+        for_each_site(ctx, inst, [&](int64_t site, int col, int64_t e) {
+            for (int a = 0; a < 3; ++a) spins[3 * site + a] = (double)p[(col * 3 + a) * d.Sp + e];
+        });
+    }
+    return NMC_OK;
+}
+
+int nmc_set_spins(nmc_ctx* ctx, int inst, const double* spins) {
+    if (!ctx || !spins) return fail(NMC_E_INVALID, "null argument");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    const DevInst& d = ctx->din[inst];
+    const size_t esz = ctx->ising ? 1 : sizeof(float);
+    std::vector<char> h(ctx->spin_bytes(d), 0);
+    int bad = 0;
+    if (ctx->ising) {
+        int8_t* p = (int8_t*)h.data();
+        for_each_site(ctx, inst, [&](int64_t site, int col, int64_t e) {
+            const double v = spins[3 * site];
+            if (v != 1.0 && v != -1.0 && v != 0.0) bad = 1;
+            p[col * d.Sp + e] = (int8_t)v;
+        });
+    } else {
+        float* p = (float*)h.data();
+        for_each_site(ctx, inst, [&](int64_t site, int col, int64_t e) {
+            for (int a = 0; a < 3; ++a) p[(col * 3 + a) * d.Sp + e] = (float)spins[3 * site + a];
+        });
+    }
+    if (bad) return fail(NMC_E_INVALID, "Ising spins must be -1, 0 or +1");
+    NMC_HIP(hipMemcpyAsync((char*)ctx->pool + d.base * esz, h.data(), h.size(), hipMemcpyHostToDevice, ctx->stream));
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    return NMC_OK;
+}
+
+namespace {
+// (colour, element) of the listed site indices (site order).
+int site_map(nmc_ctx* ctx, int inst, const int* sites, int n, std::vector<int>& cs, std::vector<int>& es) {
+    const int64_t N = ctx->hin[inst].nsites;
+    std::vector<int> col((size_t)N);
+    std::vector<int> el((size_t)N);
+    for_each_site(ctx, inst, [&](int64_t site, int c, int64_t e) { col[site] = c; el[site] = (int)e; });
+    cs.resize(n); es.resize(n);
+    for (int i = 0; i < n; ++i) {
+        if (sites[i] < 0 || sites[i] >= N) return fail(NMC_E_INVALID, "site index out of range");
+        cs[i] = col[sites[i]];
+        es[i] = el[sites[i]];
+    }
+    return NMC_OK;
+}
+}  // namespace
+
+int nmc_delta_e(nmc_ctx* ctx, int inst, const int* sites, const double* trial, int n, double* dE) {
+    if (!ctx || (n > 0 && (!sites || !trial || !dE))) return fail(NMC_E_INVALID, "null argument");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    if (n <= 0) return NMC_OK;
+    std::vector<int> cs, es;
+    if (int rc = site_map(ctx, inst, sites, n, cs, es)) return rc;
+    std::vector<float> tf((size_t)3 * n);
+    for (int i = 0; i < 3 * n; ++i) tf[i] = (float)trial[i];
+    int *dc = nullptr, *de = nullptr;
+    float* dt = nullptr;
+    double* dout = nullptr;
+    int rc = upload(ctx, &dc, cs);
+    if (!rc) rc = upload(ctx, &de, es);
+    if (!rc) rc = upload(ctx, &dt, tf);
+    if (!rc) rc = dalloc(&dout, (size_t)n);
+    if (!rc) {
+        hipError_t e = launch_delta_e(ctx->stream, ctx->dim3, ctx->periodic, ctx->ising, ctx->d_inst, inst, ctx->pool,
+                                      ctx->st, dc, de, dt, ctx->d_dtab, n, dout);
+        if (e == hipSuccess) e = hipMemcpyAsync(dE, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(NMC_E_HIP, std::string("nmc_delta_e: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(dc); (void)hipFree(de); (void)hipFree(dt); (void)hipFree(dout);
+    return rc;
+}
+
+int nmc_replay_ising(nmc_ctx* ctx, int inst, const int* sites, const int8_t* trial, const uint32_t* k, int n,
+                     uint8_t* accept) {
+    if (!ctx || (n > 0 && (!sites || !trial || !k || !accept))) return fail(NMC_E_INVALID, "null argument");
+    if (!ctx->ising) return fail(NMC_E_INVALID, "nmc_replay_ising needs an Ising context");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    if (n <= 0) return NMC_OK;
+    for (int i = 0; i < n; ++i)
+        if (trial[i] != 1 && trial[i] != -1) return fail(NMC_E_INVALID, "Ising trial spins must be +-1");
+    std::vector<int> cs, es;
+    if (int rc = site_map(ctx, inst, sites, n, cs, es)) return rc;
+    int *dc = nullptr, *de = nullptr;
+    int8_t* dt = nullptr;
+    uint32_t* dk = nullptr;
+    uint8_t* da = nullptr;
+    int rc = upload(ctx, &dc, cs);
+    if (!rc) rc = upload(ctx, &de, es);
+    if (!rc) rc = upload(ctx, &dt, std::vector<int8_t>(trial, trial + n));
+    if (!rc) rc = upload(ctx, &dk, std::vector<uint32_t>(k, k + n));
+    if (!rc) rc = dalloc(&da, (size_t)n);
+    if (!rc) {
+        hipError_t e = launch_replay_ising(ctx->stream, ctx->dim3, ctx->periodic, ctx->d_inst, inst, (int8_t*)ctx->pool,
+                                           ctx->d_tab, ctx->st, dc, de, dt, dk, n, da);
+        if (e == hipSuccess) e = hipMemcpyAsync(accept, da, (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(NMC_E_HIP, std::string("nmc_replay_ising: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(dc); (void)hipFree(de); (void)hipFree(dt); (void)hipFree(dk); (void)hipFree(da);
+    return rc;
+}
+
+int nmc_build_weibull(int dim, int edge, const double* size, int ngrains, double wbeta, uint64_t seed,
+                      uint8_t* mask_out) {
+    if ((dim != 2 && dim != 3) || edge < 1 || ngrains < 0 || !(wbeta > 0) || (ngrains > 0 && (!size || !mask_out)))
+        return fail(NMC_E_INVALID, "nmc_build_weibull: bad argument");
+    if (ngrains == 0) return NMC_OK;
+    const int64_t box = (int64_t)edge * edge * (dim == 3 ? edge : 1);
+    if ((box + 3) / 4 * (int64_t)ngrains >= ((int64_t)1 << 31) * 256)
+        return fail(NMC_E_UNSUPPORTED, "nmc_build_weibull: batch too large for one launch");
+    const double r0 = 1 / std::tgamma(1. + 1. / wbeta);
+    hipStream_t st;
+    NMC_HIP(hipStreamCreate(&st));
+    double* ds = nullptr;
+    uint8_t* dm = nullptr;
+    int rc = dalloc(&ds, (size_t)ngrains);
+    if (!rc) rc = dalloc(&dm, (size_t)box * ngrains);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(ds, size, (size_t)ngrains * sizeof(double), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = launch_build_weibull(st, dim, edge, ds, ngrains, wbeta, r0, (uint32_t)seed, (uint32_t)(seed >> 32), dm);
+        if (e == hipSuccess) e = hipMemcpyAsync(mask_out, dm, (size_t)box * ngrains, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = fail(NMC_E_HIP, std::string("nmc_build_weibull: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(ds); (void)hipFree(dm);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+}  // extern "C"

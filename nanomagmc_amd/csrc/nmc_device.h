@@ -1,0 +1,176 @@
+// nmc_device.h -- data layout, counter-based RNG and stencil helpers shared by
+// the HIP kernels of libnmc.so (gfx950).
+//
+// Layout in HBM (one pool per context):
+//   Every instance is a box with internal extents (a0, a1, a2): 3-D lattices
+//   map reference (x, y, z) -> (a0, a1, a2); 2-D lattices map (x, y) ->
+//   (1, a1, a2).  a2 is the fastest axis and is even (padded with absent
+//   sites when the real extent n2 is odd).  Sites are split by colour
+//   c = (i0 + i1 + i2) & 1 ("checkerboard") and each colour is stored as its
+//   own array of S = a0 * a1 * a2 / 2 elements, element
+//   e = (i0 * a1 + i1) * half + (i2 >> 1), half = a2 / 2.  A colour pass then
+//   reads its own colour with unit stride and the other colour's neighbours
+//   at the same e (axes 0/1) or e -/+ 1 (axis 2): fully coalesced.
+//   Heisenberg spins: fp32, structure of arrays [colour][component][Sp].
+//   Ising spins: int8 [colour][Sp].  Sp = S rounded up to 64 elements.
+//   Absent sites (outside a shape mask or padding) hold the zero spin and are
+//   never updated, so stencil sums need no presence test.
+//
+// RNG: Philox4x32-10 (Salmon et al., SC'11), key {seed_lo, seed_hi + inst},
+// counter {index, colour | purpose << 8, epoch_lo, epoch_hi}.  The stream a
+// site sees depends only on (seed, instance, sweep, colour, element), never on
+// the launch shape.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nmc {
+
+constexpr int kPurposeSweep = 0;
+constexpr int kPurposeInit = 1;
+constexpr int kPurposeBuild = 2;
+constexpr int kPurposeBlock = 3;
+
+// Unsigned division by a runtime constant d >= 1, exact for every 32-bit n
+// (Granlund & Montgomery, PLDI'94, round-up variant with the add fix-up).
+struct FastDiv {
+    uint32_t m, s1, s2, d;
+};
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv& f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t t = __umulhi(f.m, n);
+#else
+    uint32_t t = (uint32_t)(((uint64_t)f.m * n) >> 32);
+#endif
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.m = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    f.s1 = l < 1 ? l : 1;
+    f.s2 = l > 0 ? l - 1 : 0;
+    return f;
+}
+
+// Per-instance descriptor, resident in device memory.
+struct DevInst {
+    int a0, a1, a2, half;   // internal extents; half = a2 / 2
+    int n2;                 // real extent along axis 2 (<= a2)
+    int o0, o1, o2;         // box origin in the reference lattice, internal order
+    int edge;               // reference edgeSize
+    int masked;             // 1 if mask words are present
+    FastDiv div_half, div_a1;
+    int64_t S, Sp;          // colour-array length and padded stride
+    int64_t base;           // element offset of colour 0 in the spin pool
+    int64_t mask_base;      // word offset of colour 0 in the mask pool
+    int64_t mask_stride;    // words per colour
+    int64_t tab_base;       // Ising: offset of the acceptance table
+    uint32_t inst_id;       // RNG key component
+    float beta, h, cone_w;  // beta = 1/kT ; field on h_ind ; 1 - cos(theta)
+    double beta_d, h_d;
+};
+
+// Interaction table in stencil order: slot g = 2*axis + (sign > 0) over the
+// reference axes x, y(, z); J and D are zero for slots absent from the file.
+struct Stencil {
+    float J[6];
+    float D[6][3];
+    double Jd[6];
+    double Dd[6][3];
+    int pow3[6];
+};
+
+// Sweep work item: `e0` is the first colour-array element of a tile of one
+// instance.  Measurement work item: a range of rows of one instance.
+struct Tile {
+    int inst;
+    int e0;
+};
+
+struct MTile {
+    int inst, row0, nrows;
+};
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Uniform in (0, 1) with 24 random bits, centred on the 2^-24 grid.
+__device__ __forceinline__ float u24_open(uint32_t w) {
+    return ((float)(w >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+// Uniform in [0, 1) with 24 random bits.
+__device__ __forceinline__ float u24(uint32_t w) {
+    return (float)(w >> 8) * (1.0f / 16777216.0f);
+}
+
+// Position of colour-array element e of colour c in instance `in`.
+struct SitePos {
+    int i0, i1, i2, k, q;
+    uint32_t row;
+};
+
+__device__ __forceinline__ SitePos site_of(const DevInst& in, uint32_t e, int c, bool dim3) {
+    SitePos p;
+    p.row = fdiv(e, in.div_half);
+    p.k = (int)(e - p.row * (uint32_t)in.half);
+    if (dim3) {
+        p.i0 = (int)fdiv(p.row, in.div_a1);
+        p.i1 = (int)p.row - p.i0 * in.a1;
+    } else {
+        p.i0 = 0;
+        p.i1 = (int)p.row;
+    }
+    p.q = (c + p.i0 + p.i1) & 1;
+    p.i2 = 2 * p.k + p.q;
+    return p;
+}
+
+// Element index (in the other colour) of the neighbour in stencil slot g, or
+// -1 if it lies outside the box (open boundaries).  Slots: 0/1 = -/+ axis0
+// (3-D only), 2/3 = -/+ axis1, 4/5 = -/+ axis2; 2-D uses slots 2..5 only.
+template <bool PERIODIC>
+__device__ __forceinline__ int nbr_elem(const DevInst& in, const SitePos& p, int g) {
+    int i0 = p.i0, i1 = p.i1, k = p.k;
+    if (g < 2) {
+        i0 += (g & 1) ? 1 : -1;
+        if (PERIODIC) { i0 = (i0 < 0) ? i0 + in.a0 : (i0 >= in.a0 ? i0 - in.a0 : i0); }
+        else if (i0 < 0 || i0 >= in.a0) return -1;
+    } else if (g < 4) {
+        i1 += (g & 1) ? 1 : -1;
+        if (PERIODIC) { i1 = (i1 < 0) ? i1 + in.a1 : (i1 >= in.a1 ? i1 - in.a1 : i1); }
+        else if (i1 < 0 || i1 >= in.a1) return -1;
+    } else {
+        if (g & 1) {
+            if (!PERIODIC && p.i2 + 1 >= in.n2) return -1;
+            k = k + p.q;
+            if (PERIODIC && k == in.half) k = 0;
+        } else {
+            if (!PERIODIC && p.i2 == 0) return -1;
+            k = k - 1 + p.q;
+            if (PERIODIC && k < 0) k = in.half - 1;
+        }
+    }
+    return (i0 * in.a1 + i1) * in.half + k;
+}
+
+}  // namespace nmc

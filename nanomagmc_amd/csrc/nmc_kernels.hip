@@ -1,0 +1,778 @@
+// nmc_kernels.hip -- HIP kernels of the Metropolis engine for gfx950.
+//
+// Reference hot path (NanoMagMC, Old/): state::equil (lib/state.cpp:171-203)
+// calling dE_func (lib/thermodynamics.cpp:51-69) and the trial draw
+// field_type::gen_rand (lib/field_type.cpp:185-195); the observables E_func,
+// calc_M, calc_subM, calc_sub4M, calc_TC, solid_angle
+// (lib/thermodynamics.cpp:71-221).
+//
+// Each sweep is two colour passes.  In a pass every work-item owns one site
+// of the active colour (Ising: two), reads its neighbours from the other
+// colour array (coalesced, L2-resident across neighbouring tiles), draws one
+// Philox4x32-10 block for its trial spin and acceptance uniform, and writes
+// the site back only if the move is accepted.
+#include "nmc_device.h"
+#include "nmc_launch.h"
+
+namespace nmc {
+
+// Block index -> tile, keeping consecutive tiles on one XCD.  Hardware places
+// workgroup b on XCD b % 8; tiles t and t +- (rows of a plane) share halo rows,
+// so giving each XCD one contiguous range of tiles keeps those rows in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+    const int per = ntiles >> 3, rem = ntiles & 7;
+    const int x = b & 7, j = b >> 3;
+    return x * per + (x < rem ? x : rem) + j;
+}
+
+__device__ __forceinline__ bool mask_bit(const uint32_t* __restrict__ mp, const DevInst& in, int c, int64_t e) {
+    return (mp[in.mask_base + c * in.mask_stride + (e >> 5)] >> (e & 31)) & 1u;
+}
+
+// ---------------------------------------------------------------------------
+// Heisenberg colour pass.
+// ---------------------------------------------------------------------------
+template <bool DIM3, bool PERIODIC, bool MASKED, bool USE_D, bool CONE, bool COUNT>
+__global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ insts,
+                                                    const Tile* __restrict__ tiles, int ntiles,
+                                                    float* __restrict__ pool,
+                                                    const uint32_t* __restrict__ mpool,
+                                                    const uint64_t* __restrict__ sweeps,
+                                                    Stencil st, int colour, uint32_t s_off,
+                                                    uint32_t seed_lo, uint32_t seed_hi,
+                                                    unsigned long long* __restrict__ acc) {
+    const Tile tl = tiles[xcd_tile(blockIdx.x, ntiles)];
+    const DevInst& in = insts[tl.inst];
+    const uint32_t e = (uint32_t)tl.e0 + threadIdx.x;
+    const int64_t Sp = in.Sp;
+    float* __restrict__ own = pool + in.base + (int64_t)colour * 3 * Sp;
+    const float* __restrict__ oth = pool + in.base + (int64_t)(colour ^ 1) * 3 * Sp;
+
+    bool accepted = false;
+    if (e < (uint32_t)in.S) {
+        const SitePos p = site_of(in, e, colour, DIM3);
+        bool present = p.i2 < in.n2;
+        if (MASKED && present) present = mask_bit(mpool, in, colour, e);
+        if (present) {
+            const float sx = own[e], sy = own[Sp + e], sz = own[2 * Sp + e];
+            float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+            for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+                const int ne = nbr_elem<PERIODIC>(in, p, g);
+                float nx = 0.f, ny = 0.f, nz = 0.f;
+                if (PERIODIC || ne >= 0) { nx = oth[ne]; ny = oth[Sp + ne]; nz = oth[2 * Sp + ne]; }
+                vx += st.J[g] * nx;
+                vy += st.J[g] * ny;
+                vz += st.J[g] * nz;
+                if (USE_D) {   // s_j x D_j, the orientation dE_func uses (thermodynamics.cpp:45-46)
+                    vx += ny * st.D[g][2] - nz * st.D[g][1];
+                    vy += nz * st.D[g][0] - nx * st.D[g][2];
+                    vz += nx * st.D[g][1] - ny * st.D[g][0];
+                }
+            }
+            const uint64_t sweep = sweeps[tl.inst] + s_off;
+            uint32_t w[4];
+            philox4x32_10(e, (uint32_t)colour, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                          seed_lo, seed_hi + in.inst_id, w);
+            const float ua = u24_open(w[0]);
+            const float uphi = u24(w[1]);
+            const float cphi = __builtin_amdgcn_cosf(uphi);   // cos(2 pi u)
+            const float sphi = __builtin_amdgcn_sinf(uphi);
+            float tx, ty, tz;
+            if (!CONE) {
+                // uniform direction on S^2: z uniform in (-1, 1), azimuth uniform
+                tz = 2.f * ua - 1.f;
+                const float r = __builtin_sqrtf(fmaxf(0.f, 1.f - tz * tz));
+                tx = r * cphi;
+                ty = r * sphi;
+            } else {
+                // uniform on the cap of half-angle theta around s
+                // (orthonormal frame: Duff et al., JCGT 6(1), 2017)
+                const float ca = 1.f - ua * in.cone_w;
+                const float sa = __builtin_sqrtf(fmaxf(0.f, 1.f - ca * ca));
+                const float sgn = sz >= 0.f ? 1.f : -1.f;
+                const float a = -1.f / (sgn + sz);
+                const float b = sx * sy * a;
+                const float e1x = 1.f + sgn * sx * sx * a, e1y = sgn * b, e1z = -sgn * sx;
+                const float e2x = b, e2y = sgn + sy * sy * a, e2z = -sy;
+                const float px = cphi * e1x + sphi * e2x;
+                const float py = cphi * e1y + sphi * e2y;
+                const float pz = cphi * e1z + sphi * e2z;
+                tx = ca * sx + sa * px;
+                ty = ca * sy + sa * py;
+                tz = ca * sz + sa * pz;
+                const float inv = __builtin_amdgcn_rsqf(tx * tx + ty * ty + tz * tz);
+                tx *= inv; ty *= inv; tz *= inv;
+            }
+            // dE_func: (H + Vsum) . (s_old - s_trial), H = (0, 0, h)
+            const float dE = vx * (sx - tx) + vy * (sy - ty) + (in.h + vz) * (sz - tz);
+            accepted = (dE <= 0.f) || (u24(w[2]) < __expf(-in.beta * dE));
+            if (accepted) {
+                own[e] = tx;
+                own[Sp + e] = ty;
+                own[2 * Sp + e] = tz;
+            }
+        }
+    }
+    if (COUNT) {
+        const unsigned long long m = __ballot(accepted);
+        if ((threadIdx.x & 63) == 0 && m) atomicAdd(acc + tl.inst, (unsigned long long)__popcll(m));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ising colour pass: two sites (one Philox block) per work-item.  The move is
+// the reference's fresh +-1 trial; a trial equal to the current spin is the
+// dE = 0 no-op the reference accepts.  A flip is accepted iff the 32-bit
+// uniform k satisfies k <= T[idx], where T is the host-built threshold of
+// -dE*beta > log(k * 2^-32) for the neighbourhood idx -- the reference's
+// double-precision decision, exactly (state.cpp:190-201).
+// ---------------------------------------------------------------------------
+template <bool DIM3, bool PERIODIC, bool MASKED, bool COUNT>
+__global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__ insts,
+                                                     const Tile* __restrict__ tiles, int ntiles,
+                                                     int8_t* __restrict__ pool,
+                                                     const uint32_t* __restrict__ mpool,
+                                                     const uint32_t* __restrict__ tab,
+                                                     const uint64_t* __restrict__ sweeps,
+                                                     Stencil st, int colour, uint32_t s_off,
+                                                     uint32_t seed_lo, uint32_t seed_hi,
+                                                     unsigned long long* __restrict__ acc) {
+    const Tile tl = tiles[xcd_tile(blockIdx.x, ntiles)];
+    const DevInst& in = insts[tl.inst];
+    const uint32_t P = ((uint32_t)tl.e0 >> 1) + threadIdx.x;
+    int8_t* __restrict__ own = pool + in.base + (int64_t)colour * in.Sp;
+    const int8_t* __restrict__ oth = pool + in.base + (int64_t)(colour ^ 1) * in.Sp;
+    int nacc = 0;
+    if (2 * P < (uint32_t)in.S) {
+        const uint64_t sweep = sweeps[tl.inst] + s_off;
+        uint32_t w[4];
+        philox4x32_10(P, (uint32_t)colour, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                      seed_lo, seed_hi + in.inst_id, w);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t e = 2 * P + j;
+            if (e >= (uint32_t)in.S) break;
+            const SitePos p = site_of(in, e, colour, DIM3);
+            bool present = p.i2 < in.n2;
+            if (MASKED && present) present = mask_bit(mpool, in, colour, e);
+            if (!present) continue;
+            const int s = own[e];
+            const int trial = ((w[2] >> j) & 1u) ? 1 : -1;
+            if (trial == s) { ++nacc; continue; }
+            int idx = 0;
+#pragma unroll
+            for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+                const int ne = nbr_elem<PERIODIC>(in, p, g);
+                const int sn = (PERIODIC || ne >= 0) ? (int)oth[ne] : 0;
+                idx += (sn + 1) * st.pow3[g];
+            }
+            idx = 2 * idx + (s > 0 ? 1 : 0);
+            if (w[j] <= tab[in.tab_base + idx]) {
+                own[e] = (int8_t)trial;
+                ++nacc;
+            }
+        }
+    }
+    if (COUNT) {
+        // per-wave sum of accepted attempts (0..2 per lane)
+        int v = nacc;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc + tl.inst, (unsigned long long)v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Observables.  One workgroup = 4 waves over a range of rows of one
+// instance; lane l of a wave owns the columns k = l, l + 64, ... of each row
+// (sites i2 = 2k and 2k + 1), so every per-slice accumulator in LDS has a
+// single writer and the reduction order is fixed: results are bitwise
+// reproducible from run to run.  Accumulators are fp64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double solid_angle_d(double s1x, double s1y, double s1z, double s2x,
+                                                double s2y, double s2z, double s3x, double s3y,
+                                                double s3z) {
+#pragma clang fp contract(off)
+    // solid_angle (thermodynamics.cpp:200-221), same expression order
+    const double s1s2 = s1x * s2x + s1y * s2y + s1z * s2z;
+    const double s1s3 = s1x * s3x + s1y * s3y + s1z * s3z;
+    const double s2s3 = s2x * s3x + s2y * s3y + s2z * s3z;
+    const double rho = 2 * (1 + s1s2) * (1 + s1s3) * (1 + s2s3);
+    const double dotsum = 1 + s1s2 + s1s3 + s2s3;
+    const double cx = s1y * s3z - s1z * s3y;
+    const double cy = s1z * s3x - s1x * s3z;
+    const double cz = s1x * s3y - s1y * s3x;
+    const double crosssum = cx * s2x + cy * s2y + cz * s2z;
+    return -atan2(crosssum / rho, dotsum / rho);
+}
+
+template <bool ISING>
+__device__ __forceinline__ void load_spin(const void* pool, const DevInst& in, int c, int64_t e,
+                                          double& x, double& y, double& z) {
+    if (ISING) {
+        const int8_t* p = (const int8_t*)pool + in.base + (int64_t)c * in.Sp;
+        x = (double)p[e]; y = 0.0; z = 0.0;
+    } else {
+        const float* p = (const float*)pool + in.base + (int64_t)c * 3 * in.Sp;
+        x = (double)p[e]; y = (double)p[in.Sp + e]; z = (double)p[2 * in.Sp + e];
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// C remainder semantics (truncating), as the reference's % on int.
+__device__ __forceinline__ int crem(int a, int b) { return a % b; }
+
+template <bool DIM3, bool PERIODIC, bool MASKED, bool ISING>
+__global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ insts,
+                                                 const MTile* __restrict__ tiles,
+                                                 const void* __restrict__ pool,
+                                                 const uint32_t* __restrict__ mpool, Stencil st,
+                                                 double* __restrict__ partials, int pstride,
+                                                 int a2max) {
+#pragma clang fp contract(off)
+    extern __shared__ double lds[];   // [4 waves][2][a2max] (3-D Heisenberg) + [4][15]
+    const MTile tl = tiles[blockIdx.x];
+    const DevInst& in = insts[tl.inst];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr bool TC = !ISING;
+    constexpr bool TCS = TC && DIM3;   // per-slice charges in LDS
+    double* qs = lds;                  // [4][2][a2max]
+    double* red = lds + (TCS ? 8 * a2max : 0);
+    if (TCS) {
+        for (int i = threadIdx.x; i < 8 * a2max; i += 256) qs[i] = 0.0;
+        __syncthreads();
+    }
+    double acc[15];
+#pragma unroll
+    for (int i = 0; i < 15; ++i) acc[i] = 0.0;
+    double qr = 0.0, qw = 0.0;   // 2-D charges
+
+    const int sx_slot = DIM3 ? 0 : 2;   // first stencil slot of the reference x axis
+    const int sy_slot = DIM3 ? 2 : 4;   // ... of the reference y axis
+
+    for (int row = tl.row0 + wv; row < tl.row0 + tl.nrows; row += 4) {
+        const int i0 = DIM3 ? row / in.a1 : 0;
+        const int i1 = DIM3 ? row - i0 * in.a1 : row;
+        for (int k = lane; k < in.half; k += 64) {
+            for (int q2 = 0; q2 < 2; ++q2) {
+                const int i2 = 2 * k + q2;
+                if (i2 >= in.n2) continue;
+                const int c = (i0 + i1 + i2) & 1;
+                const int64_t e = (int64_t)row * in.half + k;
+                if (MASKED && !mask_bit(mpool, in, c, e)) continue;
+                SitePos p;
+                p.i0 = i0; p.i1 = i1; p.i2 = i2; p.k = k; p.q = q2; p.row = row;
+                double s0, s1, s2;
+                load_spin<ISING>(pool, in, c, e, s0, s1, s2);
+                acc[0] += s0; acc[1] += s1; acc[2] += s2;
+                // reference coordinates of the site
+                const int rx = DIM3 ? i0 + in.o0 : i1 + in.o1;
+                const int ry = DIM3 ? i1 + in.o1 : i2 + in.o2;
+                const int rz = DIM3 ? i2 + in.o2 : 0;
+                if (((i0 + in.o0 + i1 + in.o1 + i2 + in.o2) % 2) == 0) {   // calc_subM(0)
+                    acc[3] += s0; acc[4] += s1; acc[5] += s2;
+                }
+                {   // calc_sub4M
+                    const int possum = rx + ry;
+                    const int posdiff = -rx + ry + (in.edge - in.edge % 4);
+                    const int kz = rz;
+                    if ((crem(possum, 4) == 0 && crem(posdiff, 4) == 0 && crem(kz, 4) == 0) ||
+                        (crem(possum, 4) == 2 && crem(posdiff, 4) == 2 && crem(kz, 4) == 2)) {
+                        acc[6] += s0; acc[7] += s1; acc[8] += s2;
+                    }
+                }
+                // local fields: X = sum J s_j, Y = sum D x s_j
+                double X0 = 0, X1 = 0, X2 = 0, Y0 = 0, Y1 = 0, Y2 = 0;
+                for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+                    const int ne = nbr_elem<PERIODIC>(in, p, g);
+                    if (ne < 0) continue;
+                    double n0, n1, n2;
+                    load_spin<ISING>(pool, in, c ^ 1, ne, n0, n1, n2);
+                    X0 += st.Jd[g] * n0; X1 += st.Jd[g] * n1; X2 += st.Jd[g] * n2;
+                    const double* D = st.Dd[g];
+                    Y0 += D[1] * n2 - D[2] * n1;
+                    Y1 += D[2] * n0 - D[0] * n2;
+                    Y2 += D[0] * n1 - D[1] * n0;
+                }
+                acc[9] += s0 * (X0 + Y0); acc[10] += s1 * (X1 + Y1); acc[11] += s2 * (X2 + Y2);
+                acc[12] += s0 * (X0 - Y0); acc[13] += s1 * (X1 - Y1); acc[14] += s2 * (X2 - Y2);
+                if (TC) {
+                    // calc_TC triangles (i, -x, +y) and (i, +x, -y); q_ref without
+                    // periodic wrap (the reference's adj), q_wrap with it.
+                    double t_ref = 0.0, t_wrap = 0.0;
+                    for (int tri = 0; tri < 2; ++tri) {
+                        const int ga = tri == 0 ? sx_slot : sx_slot + 1;
+                        const int gb = tri == 0 ? sy_slot + 1 : sy_slot;
+                        for (int wrap = 0; wrap < (PERIODIC ? 2 : 1); ++wrap) {
+                            const int na = wrap ? nbr_elem<true>(in, p, ga) : nbr_elem<false>(in, p, ga);
+                            const int nb = wrap ? nbr_elem<true>(in, p, gb) : nbr_elem<false>(in, p, gb);
+                            if (na < 0 || nb < 0) continue;
+                            if (MASKED && (!mask_bit(mpool, in, c ^ 1, na) || !mask_bit(mpool, in, c ^ 1, nb)))
+                                continue;
+                            double a0, a1, a2, b0, b1, b2;
+                            load_spin<false>(pool, in, c ^ 1, na, a0, a1, a2);
+                            load_spin<false>(pool, in, c ^ 1, nb, b0, b1, b2);
+                            const double ang = solid_angle_d(s0, s1, s2, a0, a1, a2, b0, b1, b2);
+                            if (wrap) t_wrap += ang; else t_ref += ang;
+                        }
+                    }
+                    if (!PERIODIC) t_wrap = t_ref;
+                    if (TCS) {
+                        qs[(wv * 2 + 0) * a2max + i2] += t_ref;
+                        qs[(wv * 2 + 1) * a2max + i2] += t_wrap;
+                    } else {
+                        qr += t_ref;
+                        qw += t_wrap;
+                    }
+                }
+            }
+        }
+    }
+    // fixed-order block reduction of the 15 scalars (+ 2-D charges)
+    const int nsc = 15 + ((TC && !DIM3) ? 2 : 0);
+    if (TC && !DIM3) { qr = wave_sum(qr); qw = wave_sum(qw); }
+#pragma unroll
+    for (int i = 0; i < 15; ++i) acc[i] = wave_sum(acc[i]);
+    if (lane == 0) {
+        for (int i = 0; i < 15; ++i) red[wv * 17 + i] = acc[i];
+        red[wv * 17 + 15] = qr;
+        red[wv * 17 + 16] = qw;
+    }
+    __syncthreads();
+    double* out = partials + (int64_t)blockIdx.x * pstride;
+    if (threadIdx.x < nsc) {
+        const int i = threadIdx.x;
+        out[i] = ((red[i] + red[17 + i]) + red[34 + i]) + red[51 + i];
+    }
+    if (TCS) {
+        for (int i = threadIdx.x; i < 2 * in.a2; i += 256) {
+            const int which = i / in.a2, z = i - which * in.a2;
+            double v = 0.0;
+            for (int w = 0; w < 4; ++w) v += qs[(w * 2 + which) * a2max + z];
+            out[15 + which * a2max + z] = v;
+        }
+    }
+}
+
+// Sum the partials of each instance's tiles (in tile order) and write one
+// observable row: M, subM, sub4M, E_r, E_s, q_ref[nsl], q_wrap[nsl].
+__global__ __launch_bounds__(256) void k_measure_finalize(
+    const DevInst* __restrict__ insts, const int* __restrict__ list, const int* __restrict__ t_begin,
+    const int* __restrict__ t_end, const double* __restrict__ partials, int pstride, int a2max,
+    int dim3, int tc, int nsl, double* __restrict__ obs, double* __restrict__ qref,
+    double* __restrict__ qwrap, int row_by_inst, int64_t row_mul, int64_t row_add, int hidx) {
+#pragma clang fp contract(off)
+    __shared__ double sum[15];
+    const int li = blockIdx.x;
+    const int inst = list[li];
+    const DevInst& in = insts[inst];
+    const int64_t row = (row_by_inst ? (int64_t)inst : (int64_t)li) * row_mul + row_add;
+    const int tb = t_begin[li], te = t_end[li];
+    const int nq = tc ? (dim3 ? 2 * a2max : 2) : 0;
+    for (int i = threadIdx.x; i < 15 + nq; i += 256) {
+        double v = 0.0;
+        for (int t = tb; t < te; ++t) v += partials[(int64_t)t * pstride + i];
+        if (i < 15) sum[i] = v;
+        else if (tc) {
+            const int j = i - 15;
+            if (dim3) {
+                const int which = j / a2max, z = j - which * a2max;
+                if (z < in.a2 && in.o2 + z < nsl) {
+                    double* dst = which ? qwrap : qref;
+                    if (dst) dst[row * nsl + in.o2 + z] = v / (2 * M_PI);
+                }
+            } else {
+                double* dst = j ? qwrap : qref;
+                if (dst) dst[row * nsl] = v / (2 * M_PI);
+            }
+        }
+    }
+    if (tc && dim3) {
+        // slices outside this instance's box carry no triangles
+        for (int z = threadIdx.x; z < nsl; z += 256) {
+            if (z < in.o2 || z >= in.o2 + in.a2) {
+                if (qref) qref[row * nsl + z] = 0.0;
+                if (qwrap) qwrap[row * nsl + z] = 0.0;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* o = obs + row * 11;
+        for (int i = 0; i < 9; ++i) o[i] = sum[i];
+        double H[3] = {0.0, 0.0, 0.0};
+        H[hidx] = in.h_d;
+        // E_func: res[a] = -H[a] M[a] - 0.5 Vsum[a]; E = res[0] + res[1] + res[2]
+        double r[3], q[3];
+        for (int a = 0; a < 3; ++a) {
+            r[a] = -H[a] * sum[a] - 0.5 * sum[9 + a];
+            q[a] = -H[a] * sum[a] - 0.5 * sum[12 + a];
+        }
+        o[9] = r[0] + r[1] + r[2];
+        o[10] = q[0] + q[1] + q[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Initial spins: up spin (set_default_spins, field_type.cpp:76-95) or a fresh
+// draw per site (all_rand, field_type.cpp:197-204); absent sites get zero.
+// ---------------------------------------------------------------------------
+template <bool ISING>
+__global__ __launch_bounds__(256) void k_init_spins(const DevInst* __restrict__ insts,
+                                                    const Tile* __restrict__ tiles, int ntiles,
+                                                    void* __restrict__ pool,
+                                                    const uint32_t* __restrict__ mpool, int colour,
+                                                    int random, uint64_t epoch, uint32_t seed_lo,
+                                                    uint32_t seed_hi, int dim3) {
+    const Tile tl = tiles[blockIdx.x];
+    const DevInst& in = insts[tl.inst];
+    const uint32_t e = (uint32_t)tl.e0 + threadIdx.x;
+    if (e >= (uint32_t)in.Sp) return;
+    bool present = false;
+    if (e < (uint32_t)in.S) {
+        const SitePos p = site_of(in, e, colour, dim3 != 0);
+        present = p.i2 < in.n2;
+        if (in.masked && present) present = mask_bit(mpool, in, colour, e);
+    }
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (random)
+        philox4x32_10(e, (uint32_t)colour | (kPurposeInit << 8), (uint32_t)epoch,
+                      (uint32_t)(epoch >> 32), seed_lo, seed_hi + in.inst_id, w);
+    if (ISING) {
+        int8_t* own = (int8_t*)pool + in.base + (int64_t)colour * in.Sp;
+        int8_t v = 0;
+        if (present) v = random ? ((w[0] & 1u) ? 1 : -1) : 1;
+        own[e] = v;
+    } else {
+        float* own = (float*)pool + in.base + (int64_t)colour * 3 * in.Sp;
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (present) {
+            if (random) {
+                z = 2.f * u24_open(w[0]) - 1.f;
+                const float r = __builtin_sqrtf(fmaxf(0.f, 1.f - z * z));
+                const float ph = u24(w[1]);
+                x = r * __builtin_amdgcn_cosf(ph);
+                y = r * __builtin_amdgcn_sinf(ph);
+            } else {
+                z = 1.f;
+            }
+        }
+        own[e] = x;
+        own[in.Sp + e] = y;
+        own[2 * in.Sp + e] = z;
+    }
+}
+
+__global__ void k_bump_sweeps(uint64_t* __restrict__ sweeps, const int* __restrict__ list, int n,
+                              uint64_t by) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sweeps[list[i]] += by;
+}
+
+// running lattice sum (state::add_to_av, state.cpp:288-296), fp64, both colours
+template <bool ISING>
+__global__ void k_accum_avg(const DevInst* __restrict__ insts, int inst, const void* __restrict__ pool,
+                            double* __restrict__ avg) {
+    const DevInst& in = insts[inst];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // over 2 * Sp
+    if (i >= 2 * in.Sp) return;
+    const int c = (int)(i / in.Sp);
+    const int64_t e = i - (int64_t)c * in.Sp;
+    double x, y, z;
+    load_spin<ISING>(pool, in, c, e, x, y, z);
+    avg[(c * 3 + 0) * in.Sp + e] += x;
+    avg[(c * 3 + 1) * in.Sp + e] += y;
+    avg[(c * 3 + 2) * in.Sp + e] += z;
+}
+
+// ---------------------------------------------------------------------------
+// Test mode.
+// ---------------------------------------------------------------------------
+// dE of trial spins at given (colour, element) sites, computed by the sweep's
+// own code: Heisenberg fp32 local field; Ising the acceptance-table double.
+template <bool DIM3, bool PERIODIC, bool ISING>
+__global__ void k_delta_e(const DevInst* __restrict__ insts, int inst, const void* __restrict__ pool,
+                          Stencil st, const int* __restrict__ site_c, const int* __restrict__ site_e,
+                          const float* __restrict__ trial, const double* __restrict__ dtab, int n,
+                          double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevInst& in = insts[inst];
+    const int c = site_c[i];
+    const uint32_t e = (uint32_t)site_e[i];
+    const SitePos p = site_of(in, e, c, DIM3);
+    if (ISING) {
+        const int8_t* own = (const int8_t*)pool + in.base + (int64_t)c * in.Sp;
+        const int8_t* oth = (const int8_t*)pool + in.base + (int64_t)(c ^ 1) * in.Sp;
+        const int s = own[e];
+        const int t = (int)trial[3 * i];
+        if (t == s) { out[i] = 0.0; return; }
+        int idx = 0;
+        for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+            const int ne = nbr_elem<PERIODIC>(in, p, g);
+            const int sn = ne >= 0 ? (int)oth[ne] : 0;
+            idx += (sn + 1) * st.pow3[g];
+        }
+        idx = 2 * idx + (s > 0 ? 1 : 0);
+        out[i] = dtab[in.tab_base + idx];
+    } else {
+        const float* own = (const float*)pool + in.base + (int64_t)c * 3 * in.Sp;
+        const float* oth = (const float*)pool + in.base + (int64_t)(c ^ 1) * 3 * in.Sp;
+        const int64_t Sp = in.Sp;
+        const float sx = own[e], sy = own[Sp + e], sz = own[2 * Sp + e];
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+        for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+            const int ne = nbr_elem<PERIODIC>(in, p, g);
+            float nx = 0.f, ny = 0.f, nz = 0.f;
+            if (ne >= 0) { nx = oth[ne]; ny = oth[Sp + ne]; nz = oth[2 * Sp + ne]; }
+            vx += st.J[g] * nx;
+            vy += st.J[g] * ny;
+            vz += st.J[g] * nz;
+            vx += ny * st.D[g][2] - nz * st.D[g][1];
+            vy += nz * st.D[g][0] - nx * st.D[g][2];
+            vz += nx * st.D[g][1] - ny * st.D[g][0];
+        }
+        const float tx = trial[3 * i], ty = trial[3 * i + 1], tz = trial[3 * i + 2];
+        out[i] = (double)(vx * (sx - tx) + vy * (sy - ty) + (in.h + vz) * (sz - tz));
+    }
+}
+
+// The reference's random-site loop (state.cpp:180-202) on one lane, with
+// supplied sites, trials and 32-bit uniforms, using the sweep's threshold
+// table.  Sequential by nature: each step sees every earlier update.
+template <bool DIM3, bool PERIODIC>
+__global__ void k_replay_ising(const DevInst* __restrict__ insts, int inst, int8_t* __restrict__ pool,
+                               const uint32_t* __restrict__ tab, Stencil st,
+                               const int* __restrict__ site_c, const int* __restrict__ site_e,
+                               const int8_t* __restrict__ trial, const uint32_t* __restrict__ ku,
+                               int n, uint8_t* __restrict__ accept) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const DevInst& in = insts[inst];
+    for (int i = 0; i < n; ++i) {
+        const int c = site_c[i];
+        const uint32_t e = (uint32_t)site_e[i];
+        int8_t* own = pool + in.base + (int64_t)c * in.Sp;
+        const int8_t* oth = pool + in.base + (int64_t)(c ^ 1) * in.Sp;
+        const SitePos p = site_of(in, e, c, DIM3);
+        const int s = own[e];
+        const int t = trial[i];
+        uint8_t a = 1;
+        if (t != s) {
+            int idx = 0;
+            for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+                const int ne = nbr_elem<PERIODIC>(in, p, g);
+                const int sn = ne >= 0 ? (int)oth[ne] : 0;
+                idx += (sn + 1) * st.pow3[g];
+            }
+            idx = 2 * idx + (s > 0 ? 1 : 0);
+            a = ku[i] <= tab[in.tab_base + idx] ? 1 : 0;
+        }
+        if (a) own[e] = (int8_t)t;
+        accept[i] = a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weibull grain builder (shape::weibull::check, Old/lib/shape.cpp:40-58) for
+// a batch of grains: one work-item per box site; U = k * 2^-32 from Philox
+// keyed by the grain; kept iff U < exp(-(dist/r0)^wbeta).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_build_weibull(int dim, int edge, const double* __restrict__ size,
+                                                       int ngrains, double wbeta, double r0,
+                                                       uint32_t seed_lo, uint32_t seed_hi,
+                                                       uint8_t* __restrict__ mask) {
+    const int64_t box = (int64_t)edge * edge * (dim == 3 ? edge : 1);
+    const int64_t quads = (box + 3) / 4;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (int)(gid / quads);
+    if (g >= ngrains) return;
+    const int64_t qd = gid - (int64_t)g * quads;
+    uint32_t w[4];
+    philox4x32_10((uint32_t)qd, (uint32_t)(kPurposeBuild << 8), 0u, 0u, seed_lo, seed_hi + (uint32_t)g, w);
+    const double a = size[g];
+    const double centre = double(edge - 1) / 2.;
+    for (int j = 0; j < 4; ++j) {
+        const int64_t site = qd * 4 + j;
+        if (site >= box) break;
+        int pos[3];
+        int64_t r = site;
+        for (int d = dim - 1; d >= 0; --d) { pos[d] = (int)(r % edge); r /= edge; }
+        double dist2 = 0;
+        for (int d = 0; d < dim; ++d) {
+            const double t = (pos[d] - centre) / a;
+            dist2 += t * t;
+        }
+        const double dist = sqrt(dist2);
+        const double test = exp(-pow(dist / r0, wbeta));
+        const double U = (double)w[j] / 4294967296.0;
+        mask[(int64_t)g * box + site] = U < test ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (declared in nmc_launch.h).
+// ---------------------------------------------------------------------------
+#define NMC_HEIS_CASE(D3, PER, MSK, UD, CN, CT)                                                     \
+    if (dim3 == D3 && periodic == PER && masked == MSK && use_d == UD && cone == CN && count == CT) \
+        k_sweep_heis<D3, PER, MSK, UD, CN, CT><<<ntiles, 256, 0, stream>>>(                         \
+            insts, tiles, ntiles, pool, mpool, sweeps, st, colour, s_off, seed_lo, seed_hi, acc);
+
+hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, bool use_d,
+                             bool cone, bool count, const DevInst* insts, const Tile* tiles,
+                             int ntiles, float* pool, const uint32_t* mpool, const uint64_t* sweeps,
+                             const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo,
+                             uint32_t seed_hi, unsigned long long* acc) {
+    if (ntiles <= 0) return hipSuccess;
+#define NMC_HEIS_B(D3, PER, MSK)                   \
+    NMC_HEIS_CASE(D3, PER, MSK, false, false, false) \
+    NMC_HEIS_CASE(D3, PER, MSK, false, false, true)  \
+    NMC_HEIS_CASE(D3, PER, MSK, false, true, false)  \
+    NMC_HEIS_CASE(D3, PER, MSK, false, true, true)   \
+    NMC_HEIS_CASE(D3, PER, MSK, true, false, false)  \
+    NMC_HEIS_CASE(D3, PER, MSK, true, false, true)   \
+    NMC_HEIS_CASE(D3, PER, MSK, true, true, false)   \
+    NMC_HEIS_CASE(D3, PER, MSK, true, true, true)
+    NMC_HEIS_B(true, true, false)
+    NMC_HEIS_B(true, false, false)
+    NMC_HEIS_B(true, false, true)
+    NMC_HEIS_B(false, true, false)
+    NMC_HEIS_B(false, false, false)
+    NMC_HEIS_B(false, false, true)
+    NMC_HEIS_B(true, true, true)
+    NMC_HEIS_B(false, true, true)
+#undef NMC_HEIS_B
+    return hipGetLastError();
+}
+
+#define NMC_ISING_CASE(D3, PER, MSK, CT)                                                 \
+    if (dim3 == D3 && periodic == PER && masked == MSK && count == CT)                   \
+        k_sweep_ising<D3, PER, MSK, CT><<<ntiles, 256, 0, stream>>>(                     \
+            insts, tiles, ntiles, pool, mpool, tab, sweeps, st, colour, s_off, seed_lo, \
+            seed_hi, acc);
+
+hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool masked,
+                              bool count, const DevInst* insts, const Tile* tiles, int ntiles,
+                              int8_t* pool, const uint32_t* mpool, const uint32_t* tab,
+                              const uint64_t* sweeps, const Stencil& st, int colour,
+                              uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                              unsigned long long* acc) {
+    if (ntiles <= 0) return hipSuccess;
+#define NMC_ISING_B(D3, PER, MSK) NMC_ISING_CASE(D3, PER, MSK, false) NMC_ISING_CASE(D3, PER, MSK, true)
+    NMC_ISING_B(true, true, false)
+    NMC_ISING_B(true, true, true)
+    NMC_ISING_B(true, false, false)
+    NMC_ISING_B(true, false, true)
+    NMC_ISING_B(false, true, false)
+    NMC_ISING_B(false, true, true)
+    NMC_ISING_B(false, false, false)
+    NMC_ISING_B(false, false, true)
+#undef NMC_ISING_B
+    return hipGetLastError();
+}
+
+hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising,
+                          const DevInst* insts, const MTile* t, int ntiles, const void* pool,
+                          const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
+                          int a2max) {
+    if (ntiles <= 0) return hipSuccess;
+    const size_t shm = (size_t)(((!ising && dim3) ? 8 * a2max : 0) + 4 * 17) * sizeof(double);
+#define NMC_MEAS(D3, PER, MSK, IS)                                                           \
+    if (dim3 == D3 && periodic == PER && masked == MSK && ising == IS)                       \
+        k_measure<D3, PER, MSK, IS><<<ntiles, 256, shm, stream>>>(insts, t, pool, mpool, st, \
+                                                                  partials, pstride, a2max);
+#define NMC_MEAS_B(D3, PER) \
+    NMC_MEAS(D3, PER, false, false) NMC_MEAS(D3, PER, true, false) NMC_MEAS(D3, PER, false, true) NMC_MEAS(D3, PER, true, true)
+    NMC_MEAS_B(true, true)
+    NMC_MEAS_B(true, false)
+    NMC_MEAS_B(false, true)
+    NMC_MEAS_B(false, false)
+#undef NMC_MEAS_B
+#undef NMC_MEAS
+    return hipGetLastError();
+}
+
+hipError_t launch_measure_finalize(hipStream_t stream, const DevInst* insts, const int* list, int n,
+                                   const int* t_begin, const int* t_end, const double* partials,
+                                   int pstride, int a2max, bool dim3, bool tc, int nsl, double* obs,
+                                   double* qref, double* qwrap, bool row_by_inst, int64_t row_mul,
+                                   int64_t row_add, int hidx) {
+    if (n <= 0) return hipSuccess;
+    k_measure_finalize<<<n, 256, 0, stream>>>(insts, list, t_begin, t_end, partials, pstride, a2max,
+                                              dim3 ? 1 : 0, tc ? 1 : 0, nsl, obs, qref, qwrap,
+                                              row_by_inst ? 1 : 0, row_mul, row_add, hidx);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_spins(hipStream_t stream, bool ising, const DevInst* insts, const Tile* tiles,
+                             int ntiles, void* pool, const uint32_t* mpool, int colour, bool random,
+                             uint64_t epoch, uint32_t seed_lo, uint32_t seed_hi, bool dim3) {
+    if (ntiles <= 0) return hipSuccess;
+    if (ising)
+        k_init_spins<true><<<ntiles, 256, 0, stream>>>(insts, tiles, ntiles, pool, mpool, colour,
+                                                       random ? 1 : 0, epoch, seed_lo, seed_hi, dim3 ? 1 : 0);
+    else
+        k_init_spins<false><<<ntiles, 256, 0, stream>>>(insts, tiles, ntiles, pool, mpool, colour,
+                                                        random ? 1 : 0, epoch, seed_lo, seed_hi, dim3 ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_bump_sweeps(hipStream_t stream, uint64_t* sweeps, const int* list, int n, uint64_t by) {
+    if (n <= 0) return hipSuccess;
+    k_bump_sweeps<<<(n + 255) / 256, 256, 0, stream>>>(sweeps, list, n, by);
+    return hipGetLastError();
+}
+
+hipError_t launch_accum_avg(hipStream_t stream, bool ising, const DevInst* insts, int inst,
+                            int64_t Sp, const void* pool, double* avg) {
+    const int64_t n = 2 * Sp;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (ising) k_accum_avg<true><<<nb, 256, 0, stream>>>(insts, inst, pool, avg);
+    else k_accum_avg<false><<<nb, 256, 0, stream>>>(insts, inst, pool, avg);
+    return hipGetLastError();
+}
+
+hipError_t launch_delta_e(hipStream_t stream, bool dim3, bool periodic, bool ising,
+                          const DevInst* insts, int inst, const void* pool, const Stencil& st,
+                          const int* site_c, const int* site_e, const float* trial,
+                          const double* dtab, int n, double* out) {
+    if (n <= 0) return hipSuccess;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+#define NMC_DE(D3, PER, IS)                          \
+    if (dim3 == D3 && periodic == PER && ising == IS) \
+        k_delta_e<D3, PER, IS><<<nb, 256, 0, stream>>>(insts, inst, pool, st, site_c, site_e, trial, dtab, n, out);
+    NMC_DE(true, true, true) NMC_DE(true, true, false) NMC_DE(true, false, true) NMC_DE(true, false, false)
+    NMC_DE(false, true, true) NMC_DE(false, true, false) NMC_DE(false, false, true) NMC_DE(false, false, false)
+#undef NMC_DE
+    return hipGetLastError();
+}
+
+hipError_t launch_replay_ising(hipStream_t stream, bool dim3, bool periodic, const DevInst* insts,
+                               int inst, int8_t* pool, const uint32_t* tab, const Stencil& st,
+                               const int* site_c, const int* site_e, const int8_t* trial,
+                               const uint32_t* ku, int n, uint8_t* accept) {
+    if (n <= 0) return hipSuccess;
+#define NMC_RP(D3, PER)                  \
+    if (dim3 == D3 && periodic == PER) \
+        k_replay_ising<D3, PER><<<1, 64, 0, stream>>>(insts, inst, pool, tab, st, site_c, site_e, trial, ku, n, accept);
+    NMC_RP(true, true) NMC_RP(true, false) NMC_RP(false, true) NMC_RP(false, false)
+#undef NMC_RP
+    return hipGetLastError();
+}
+
+hipError_t launch_build_weibull(hipStream_t stream, int dim, int edge, const double* size, int ngrains,
+                                double wbeta, double r0, uint32_t seed_lo, uint32_t seed_hi,
+                                uint8_t* mask) {
+    const int64_t box = (int64_t)edge * edge * (dim == 3 ? edge : 1);
+    const int64_t quads = (box + 3) / 4;
+    const int64_t total = quads * ngrains;
+    if (total <= 0) return hipSuccess;
+    k_build_weibull<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(dim, edge, size, ngrains, wbeta,
+                                                                       r0, seed_lo, seed_hi, mask);
+    return hipGetLastError();
+}
+
+}  // namespace nmc

@@ -1,0 +1,54 @@
+// nmc_launch.h -- host-callable launchers for the kernels in nmc_kernels.hip.
+#pragma once
+#include "nmc_device.h"
+
+namespace nmc {
+
+hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, bool use_d,
+                             bool cone, bool count, const DevInst* insts, const Tile* tiles,
+                             int ntiles, float* pool, const uint32_t* mpool, const uint64_t* sweeps,
+                             const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo,
+                             uint32_t seed_hi, unsigned long long* acc);
+
+hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool masked,
+                              bool count, const DevInst* insts, const Tile* tiles, int ntiles,
+                              int8_t* pool, const uint32_t* mpool, const uint32_t* tab,
+                              const uint64_t* sweeps, const Stencil& st, int colour,
+                              uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                              unsigned long long* acc);
+
+hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising,
+                          const DevInst* insts, const MTile* tiles, int ntiles, const void* pool,
+                          const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
+                          int a2max);
+
+hipError_t launch_measure_finalize(hipStream_t stream, const DevInst* insts, const int* list, int n,
+                                   const int* t_begin, const int* t_end, const double* partials,
+                                   int pstride, int a2max, bool dim3, bool tc, int nsl, double* obs,
+                                   double* qref, double* qwrap, bool row_by_inst, int64_t row_mul,
+                                   int64_t row_add, int hidx);
+
+hipError_t launch_init_spins(hipStream_t stream, bool ising, const DevInst* insts, const Tile* tiles,
+                             int ntiles, void* pool, const uint32_t* mpool, int colour, bool random,
+                             uint64_t epoch, uint32_t seed_lo, uint32_t seed_hi, bool dim3);
+
+hipError_t launch_bump_sweeps(hipStream_t stream, uint64_t* sweeps, const int* list, int n, uint64_t by);
+
+hipError_t launch_accum_avg(hipStream_t stream, bool ising, const DevInst* insts, int inst,
+                            int64_t Sp, const void* pool, double* avg);
+
+hipError_t launch_delta_e(hipStream_t stream, bool dim3, bool periodic, bool ising,
+                          const DevInst* insts, int inst, const void* pool, const Stencil& st,
+                          const int* site_c, const int* site_e, const float* trial,
+                          const double* dtab, int n, double* out);
+
+hipError_t launch_replay_ising(hipStream_t stream, bool dim3, bool periodic, const DevInst* insts,
+                               int inst, int8_t* pool, const uint32_t* tab, const Stencil& st,
+                               const int* site_c, const int* site_e, const int8_t* trial,
+                               const uint32_t* ku, int n, uint8_t* accept);
+
+hipError_t launch_build_weibull(hipStream_t stream, int dim, int edge, const double* size, int ngrains,
+                                double wbeta, double r0, uint32_t seed_lo, uint32_t seed_hi,
+                                uint8_t* mask);
+
+}  // namespace nmc

@@ -1,0 +1,535 @@
+// oracle.cpp -- FP64 CPU restatement of NanoMagMC's Metropolis hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+// tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+// and by nothing on the product path (nanomagmc_amd/ never links or loads it).
+//
+// Each function restates one piece of the reference (paths relative to the
+// reference checkout) in plain C++17 with IEEE double arithmetic in the
+// reference's expression order; it is compiled with -ffp-contract=off so no
+// multiply-add is fused.  Parity status of each function is listed in the
+// oracle's README block at the end of this file.
+//
+// Spins are passed as n x 3 doubles (the reference's xtensorf<double,4> with
+// the always-zero 4th component dropped, Old/includes/field_type.hpp:17).
+// Site coordinates are n x 3 ints (x, y, z; z = 0 in 2-D).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <random>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+#include <chrono>
+#include <array>
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11).  Multipliers and Weyl key increments are the published
+// constants of the paper.  Used by the checkerboard replay below, which
+// follows the counter layout of the device sweep.
+// ---------------------------------------------------------------------------
+static inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k[0];
+    uint32_t n2 = hi0 ^ c[3] ^ k[1];
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+}
+
+void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    uint32_t k[2] = {key[0], key[1]};
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k[0] += 0x9E3779B9u; k[1] += 0xBB67AE85u; }
+        philox_round(c, k);
+    }
+    out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = c[3];
+}
+
+// ---------------------------------------------------------------------------
+// Interaction file parser: field_type::field_type (Old/lib/field_type.cpp:39-71).
+// Reads records "d ints, J, Dx Dy Dz" with operator>> semantics (so a file
+// with the wrong number of columns is misparsed exactly as the reference
+// misparses it).  J and D are multiplied by J_mod / D_mod.
+// ---------------------------------------------------------------------------
+int orc_parse_bonds(const char* path, int dim, double J_mod, double D_mod,
+                    int* d_out, double* J_out, double* D_out, int max_bonds) {
+    std::ifstream in(path);
+    if (!in.is_open()) return -1;
+    int icurr; double dcurr; int nb = 0;
+    while (in >> icurr) {
+        int loc[3] = {0, 0, 0};
+        loc[0] = icurr;
+        for (int i = 1; i < dim; ++i) { in >> icurr; loc[i] = icurr; }
+        in >> dcurr;
+        double J = J_mod * dcurr;
+        double D[3];
+        for (int i = 0; i < 3; ++i) { in >> dcurr; D[i] = D_mod * dcurr; }
+        if (nb < max_bonds) {
+            for (int i = 0; i < 3; ++i) { d_out[3 * nb + i] = loc[i]; D_out[3 * nb + i] = D[i]; }
+            J_out[nb] = J;
+        }
+        ++nb;
+    }
+    return nb;
+}
+
+// ---------------------------------------------------------------------------
+// Lattice builder: state::init_lattice (Old/lib/state.cpp:124-169) with the
+// shape test shape::weibull::check (Old/lib/shape.cpp:19-58); square/cube keep
+// every site (shape.hpp:63-71).  The uniforms the Weibull test consumes (one
+// per box site, scan order) are injected.  faithful_carry=1 reproduces the
+// reference's coordinate carry, which for dim=3 tests coordinate 1 before
+// coordinate 2 (state.cpp:156-165) and therefore emits (x, edge, 0) in place
+// of (x+1, 0, 0); faithful_carry=0 is the intended row-major scan (the one the
+// reference's test fixtures use, Old/tests/test_functions.hpp:25-37).
+// Returns the number of sites written to locs_out (n x 3), or -1 on overflow.
+// ---------------------------------------------------------------------------
+double orc_weibull_r0(double wbeta) { return 1 / tgamma(1. + 1. / wbeta); }
+
+int orc_weibull_check(const int* pos, int dim, int edge, double a, double wbeta, double u) {
+    double r0 = 1 / tgamma(1. + 1. / wbeta);
+    double centre = double(edge - 1) / 2.;
+    double dist2 = 0;
+    for (int i = 0; i < dim; ++i) dist2 += pow((pos[i] - centre) / (a), 2);
+    double dist = pow(dist2, 0.5);
+    double test = exp(-pow((dist / r0), wbeta));
+    return u < test ? 1 : 0;
+}
+
+int orc_build(int dim, int edge, char shape, double size, double wbeta,
+              const double* uniforms, int64_t nuni, int faithful_carry,
+              int* locs_out, int64_t max_sites) {
+    const bool weib = (shape == 'w' || shape == 'W' || shape == 'x' || shape == 'X');
+    int pos[3] = {0, 0, 0};
+    int64_t n = 0, iu = 0;
+    int64_t total = 1;
+    for (int i = 0; i < dim; ++i) total *= edge;
+    for (int64_t i = 0; i < total; ++i) {
+        bool fill = true;
+        if (weib) {
+            if (iu >= nuni) return -1;
+            fill = orc_weibull_check(pos, dim, edge, size, wbeta, uniforms[iu++]);
+        }
+        if (fill) {
+            if (n >= max_sites) return -1;
+            locs_out[3 * n + 0] = pos[0];
+            locs_out[3 * n + 1] = dim > 1 ? pos[1] : 0;
+            locs_out[3 * n + 2] = dim > 2 ? pos[2] : 0;
+            ++n;
+        }
+        pos[dim - 1]++;
+        if (faithful_carry) {
+            for (int j = dim - 1; j > 0; --j)
+                if (pos[dim - j] == edge) { pos[dim - j] = 0; pos[dim - j - 1]++; }
+        } else {
+            for (int j = dim - 1; j > 0; --j)
+                if (pos[j] == edge) { pos[j] = 0; pos[j - 1]++; }
+        }
+    }
+    return (int)n;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour tables: field_type::set_neigh (Old/lib/field_type.cpp:104-183),
+// with the O(N^2) search replaced by a coordinate hash that returns the same
+// (unique) match.  For every bond k in file order the neighbour at
+// loc + d_k (wrapped into [0, edge) when periodic, :127-130) is appended
+// unless it is the site itself (:133).  adj[i] = sites at +x, -x, +y, -y
+// with no wrap (:143-181); -1 if absent.  wrap_adj=1 builds the same four
+// links with periodic wrap (used for the wrapped topological charge, which
+// the reference does not compute).
+// neigh / nchoice are n x nb (padded with -1), ncount is n.
+// ---------------------------------------------------------------------------
+static inline int64_t key3(int a, int b, int c) {
+    return ((int64_t)(a + 1048576) << 42) | ((int64_t)(b + 1048576) << 21) | (int64_t)(c + 1048576);
+}
+
+void orc_set_neigh(int n, const int* locs, int dim, int edge, int periodic, int nb,
+                   const int* bd, int* neigh, int* nchoice, int* ncount, int* adj,
+                   int wrap_adj) {
+    std::unordered_map<int64_t, int> where;
+    where.reserve((size_t)n * 2);
+    for (int i = 0; i < n; ++i) {
+        int64_t k = key3(locs[3 * i], locs[3 * i + 1], locs[3 * i + 2]);
+        if (where.find(k) == where.end()) where[k] = i;   // first index wins (j loop order)
+    }
+    for (int i = 0; i < n; ++i) {
+        int c = 0;
+        for (int k = 0; k < nb; ++k) {
+            int t[3];
+            for (int a = 0; a < 3; ++a) t[a] = locs[3 * i + a] + bd[3 * k + a];
+            for (int a = 0; a < dim && periodic; ++a) t[a] = (edge + (t[a] % edge)) % edge;
+            auto it = where.find(key3(t[0], t[1], t[2]));
+            if (it != where.end() && it->second != i) {
+                neigh[(int64_t)i * nb + c] = it->second;
+                nchoice[(int64_t)i * nb + c] = k;
+                ++c;
+            }
+        }
+        ncount[i] = c;
+        for (int k = c; k < nb; ++k) { neigh[(int64_t)i * nb + k] = -1; nchoice[(int64_t)i * nb + k] = -1; }
+        static const int dd[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+        for (int q = 0; q < 4; ++q) {
+            int t[3] = {locs[3 * i] + dd[q][0], locs[3 * i + 1] + dd[q][1], locs[3 * i + 2]};
+            if (wrap_adj && periodic)
+                for (int a = 0; a < 2 && a < dim; ++a) t[a] = (edge + (t[a] % edge)) % edge;
+            auto it = where.find(key3(t[0], t[1], t[2]));
+            int j = (it != where.end()) ? it->second : -1;
+            if (j == i) j = -1;
+            adj[4 * i + q] = j;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Energy: functionObject::setup -> E_func (Old/lib/thermodynamics.cpp:10-49,
+// 71-96).  convention 0 is E_func as written, whose DMI term is D_ij x s_j
+// (c_prod(get_D_vec, s_j), :36-37); convention 1 uses s_j x D_ij, the term
+// dE_func uses (:45-46), i.e. the energy the sampler's differences belong to.
+// ---------------------------------------------------------------------------
+static inline void c_prod(const double* a, const double* b, double* out) {
+    // Old/lib/functions.cpp:70-78
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+double orc_energy(int n, const double* s, const int* neigh, const int* nchoice,
+                  const int* ncount, int nb, const double* J, const double* D,
+                  const double* H, int use_j, int use_d, int convention) {
+    double Vsum[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        double Vtemp[3] = {0, 0, 0};
+        for (int q = 0; q < ncount[i]; ++q) {
+            const int j = neigh[(int64_t)i * nb + q];
+            const int k = nchoice[(int64_t)i * nb + q];
+            const double* sj = s + 3 * (int64_t)j;
+            if (use_j)
+                for (int a = 0; a < 3; ++a) Vtemp[a] += J[k] * sj[a];
+            if (use_d) {
+                double t2[3];
+                if (convention == 0) c_prod(D + 3 * k, sj, t2);
+                else c_prod(sj, D + 3 * k, t2);
+                for (int a = 0; a < 3; ++a) Vtemp[a] += t2[a];
+            }
+        }
+        for (int a = 0; a < 3; ++a) Vsum[a] += Vtemp[a] * s[3 * (int64_t)i + a];
+    }
+    double M[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) M[a] += s[3 * (int64_t)i + a];
+    double res[3];
+    for (int a = 0; a < 3; ++a) res[a] = -H[a] * M[a] - 0.5 * Vsum[a];
+    return res[0] + res[1] + res[2];
+}
+
+// dE_func (Old/lib/thermodynamics.cpp:51-69): (H + sum_j J s_j + s_j x D_j) .
+// (s_pos - trial), summed res[0] + res[1] + res[2].
+double orc_delta_e(int pos, const double* trial, const double* s, const int* neigh,
+                   const int* nchoice, const int* ncount, int nb, const double* J,
+                   const double* D, const double* H, int use_j, int use_d) {
+    double Vdiff[3], Vsum[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) Vdiff[a] = s[3 * (int64_t)pos + a] - trial[a];
+    for (int q = 0; q < ncount[pos]; ++q) {
+        const int j = neigh[(int64_t)pos * nb + q];
+        const int k = nchoice[(int64_t)pos * nb + q];
+        const double* sj = s + 3 * (int64_t)j;
+        if (use_j)
+            for (int a = 0; a < 3; ++a) Vsum[a] += J[k] * sj[a];
+        if (use_d) {
+            double t[3];
+            c_prod(sj, D + 3 * k, t);
+            for (int a = 0; a < 3; ++a) Vsum[a] += t[a];
+        }
+    }
+    double res[3];
+    for (int a = 0; a < 3; ++a) res[a] = (H[a] + Vsum[a]) * Vdiff[a];
+    return res[0] + res[1] + res[2];
+}
+
+// calc_M / calc_subM / calc_sub4M (Old/lib/thermodynamics.cpp:99-163).
+void orc_mag(int n, const double* s, double* out) {
+    out[0] = out[1] = out[2] = 0;
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) out[a] += s[3 * (int64_t)i + a];
+}
+
+void orc_submag(int n, const double* s, const int* locs, int dim, int subnumber, double* out) {
+    out[0] = out[1] = out[2] = 0;
+    for (int i = 0; i < n; ++i) {
+        int possum = 0;
+        for (int j = 0; j < dim; ++j) possum += locs[3 * i + j];
+        if (possum % 2 == subnumber)
+            for (int a = 0; a < 3; ++a) out[a] += s[3 * (int64_t)i + a];
+    }
+}
+
+void orc_sub4mag(int n, const double* s, const int* locs, int dim, int edge, double* out) {
+    out[0] = out[1] = out[2] = 0;
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+        if (dim == 3) k = locs[3 * i + 2];
+        int possum = locs[3 * i] + locs[3 * i + 1];
+        int posdiff = -locs[3 * i] + locs[3 * i + 1] + (edge - edge % 4);
+        if ((possum % 4 == 0 && posdiff % 4 == 0 && k % 4 == 0) ||
+            (possum % 4 == 2 && posdiff % 4 == 2 && k % 4 == 2))
+            for (int a = 0; a < 3; ++a) out[a] += s[3 * (int64_t)i + a];
+    }
+}
+
+// solid_angle (Old/lib/thermodynamics.cpp:200-221).
+double orc_solid_angle(const double* s1, const double* s2, const double* s3) {
+    double s1s2 = s1[0] * s2[0] + s1[1] * s2[1] + s1[2] * s2[2];
+    double s1s3 = s1[0] * s3[0] + s1[1] * s3[1] + s1[2] * s3[2];
+    double s2s3 = s2[0] * s3[0] + s2[1] * s3[1] + s2[2] * s3[2];
+    double rho = 2 * (1 + s1s2) * (1 + s1s3) * (1 + s2s3);
+    double dotsum = 1 + s1s2 + s1s3 + s2s3;
+    double v[3];
+    c_prod(s1, s3, v);
+    double crosssum = v[0] * s2[0] + v[1] * s2[1] + v[2] * s2[2];
+    double ang = atan2(crosssum / rho, dotsum / rho);
+    return -ang;
+}
+
+// calc_TC (Old/lib/thermodynamics.cpp:165-198): out has edge entries in 3-D
+// (slice = z), one in 2-D.
+void orc_tcharge(int n, const double* s, const int* locs, int dim, int edge,
+                 const int* adj, double* out) {
+    int tcsize = (dim == 3) ? edge : 1;
+    for (int q = 0; q < tcsize; ++q) out[q] = 0;
+    for (int i = 0; i < n; ++i) {
+        const int* a = adj + 4 * i;
+        if (a[1] >= 0 && a[2] >= 0)
+            out[locs[3 * i + 2]] += orc_solid_angle(s + 3 * (int64_t)i, s + 3 * (int64_t)a[1], s + 3 * (int64_t)a[2]);
+        if (a[0] >= 0 && a[3] >= 0)
+            out[locs[3 * i + 2]] += orc_solid_angle(s + 3 * (int64_t)i, s + 3 * (int64_t)a[0], s + 3 * (int64_t)a[3]);
+    }
+    for (int q = 0; q < tcsize; ++q) out[q] /= 2 * M_PI;
+}
+
+// AtoLn (Old/lib/functions.cpp:56-62).
+void orc_atoln(double amean, double asd, double* lmean, double* lsd) {
+    double av = pow(asd, 2);
+    double am2 = pow(amean, 2);
+    *lmean = log(am2 / pow(av + am2, 0.5));
+    *lsd = pow(log(1 + av / am2), 0.5);
+}
+
+// norm (Old/lib/functions.cpp:26-34) over the 4-entry magnetisation vector.
+double orc_norm(const double* v, int n) {
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += pow(v[i], 2);
+    return pow(s, 0.5);
+}
+
+// ---------------------------------------------------------------------------
+// The single-site update of state::equil (Old/lib/state.cpp:180-202) with the
+// site, trial spin and uniforms supplied: accept iff dE <= 0 or
+// -dE*beta > log(u).  u_acc[i] is only consulted when dE > 0 (as the
+// reference only draws it then).  accept[i] = 1 if the trial spin was stored.
+// ---------------------------------------------------------------------------
+void orc_equil_replay(int n, double* s, const int* neigh, const int* nchoice,
+                      const int* ncount, int nb, const double* J, const double* D,
+                      const double* H, int use_j, int use_d, double beta,
+                      int64_t nsteps, const int* sites, const double* trials,
+                      const double* u_acc, uint8_t* accept) {
+    for (int64_t it = 0; it < nsteps; ++it) {
+        const int choice = sites[it];
+        const double* t = trials + 3 * it;
+        double dE = orc_delta_e(choice, t, s, neigh, nchoice, ncount, nb, J, D, H, use_j, use_d);
+        bool acc;
+        if (dE <= 0) acc = true;
+        else {
+            double log_eta = log(u_acc[it]);
+            acc = ((-dE * beta) > log_eta);
+        }
+        if (acc) for (int a = 0; a < 3; ++a) s[3 * (int64_t)choice + a] = t[a];
+        accept[it] = acc ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ising checkerboard sweep with the device's random-number layout: the
+// reference single-site update (state.cpp:184-201) applied to every site of
+// colour c = (x+y+z) & 1, colour 0 first, with the trial spin and uniform
+// taken from Philox4x32-10:
+//   key     = {seed_lo, seed_hi + inst}
+//   counter = {e >> 1, c, sweep_lo, sweep_hi}   (e = colour-array index)
+//   u       = word[e & 1] * 2^-32,  trial = 2*bit(e & 1 of word[2]) - 1
+// with e = ((i0 * a1 + i1) * (a2 / 2) + (i2 >> 1)) in the instance box of
+// internal extents (a0, a1, a2) (2-D: (1, nx, ny), 3-D: (nx, ny, nz); a2
+// rounded up to even) and internal coordinates (i0, i1, i2).  site_e holds e
+// and site_c the colour of each site (computed by the caller).
+// ---------------------------------------------------------------------------
+void orc_checkerboard_ising(int n, double* s, const int* neigh, const int* nchoice,
+                            const int* ncount, int nb, const double* J, const double* D,
+                            const double* H, int use_j, int use_d, double beta,
+                            const int64_t* site_e, const int* site_c, uint64_t seed,
+                            uint32_t inst, uint64_t sweep0, int nsweeps) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32) + inst};
+    for (int sw = 0; sw < nsweeps; ++sw) {
+        const uint64_t sweep = sweep0 + (uint64_t)sw;
+        for (int c = 0; c < 2; ++c) {
+            for (int i = 0; i < n; ++i) {
+                if (site_c[i] != c) continue;
+                const int64_t e = site_e[i];
+                uint32_t ctr[4] = {(uint32_t)(e >> 1), (uint32_t)c, (uint32_t)sweep, (uint32_t)(sweep >> 32)};
+                uint32_t w[4];
+                orc_philox(ctr, key, w);
+                const int b = (int)((w[2] >> (e & 1)) & 1u);
+                const double trial[3] = {(double)(b * 2 - 1), 0, 0};
+                const double u = (double)w[e & 1] / 4294967296.0;
+                double dE = orc_delta_e(i, trial, s, neigh, nchoice, ncount, nb, J, D, H, use_j, use_d);
+                bool acc;
+                if (dE <= 0) acc = true;
+                else acc = ((-dE * beta) > log(u));
+                if (acc) s[3 * (int64_t)i] = trial[0];
+            }
+        }
+    }
+}
+
+// Largest k in [0, 2^32) with -dE*beta > log(k * 2^-32) (k = 0 always
+// qualifies since log(0) = -inf); 2^32-1 when dE <= 0.  The acceptance
+// predicate of state.cpp:190-201 on GSL's u = k / 2^32 grid, as a threshold.
+uint32_t orc_accept_threshold(double dE, double beta) {
+    if (dE <= 0) return 0xFFFFFFFFu;
+    const double lhs = -dE * beta;
+    uint64_t lo = 0, hi = 0xFFFFFFFFull;        // invariant: pred(lo) true
+    if (lhs > log((double)hi / 4294967296.0)) return 0xFFFFFFFFu;
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) / 2;
+        if (lhs > log((double)mid / 4294967296.0)) lo = mid; else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+// ---------------------------------------------------------------------------
+// CPU baseline: the reference's own algorithm and data layout, timed.
+// AoS 4-double spins, vector<vector<int>> neighbour and bond-index lists in
+// set_neigh order, dE through nested std::function closures exactly shaped as
+// functionObject::setup (thermodynamics.cpp:10-69), a random site per attempt,
+// fresh +-1 / uniform-sphere trials, acceptance on log(u) (state.cpp:171-203).
+// The RNG is std::mt19937 (the recurrence GSL's mt19937 implements) with
+// u = k / 2^32 as gsl_rng_uniform; the sphere draw is the rejection method
+// documented for gsl_ran_dir_3d.  One independent chain per thread.
+// Returns attempted updates per second over all threads.
+// ---------------------------------------------------------------------------
+struct RefLattice {
+    std::vector<std::array<double, 4>> spins;
+    std::vector<std::vector<int>> neighbours, choice;
+    std::vector<double> J;
+    std::vector<std::array<double, 4>> D;
+};
+
+static void ref_chain(RefLattice lat, bool ising, bool use_j, bool use_d, double h, double beta,
+                      int64_t attempts, uint32_t seed, int64_t* acc_out) {
+    std::mt19937 mt(seed);
+    auto uni = [&]() { return (double)mt() / 4294967296.0; };
+    std::array<double, 4> H = {0, 0, 0, 0};
+    H[ising ? 0 : 2] = h;
+    std::array<double, 4> Vsum, Vdiff, Vtemp, testspin = {0, 0, 0, 0};
+    std::function<void(int, int)> inner = [](int, int) {};
+    if (use_j) inner = [&](int p, int j) {
+        const double Jv = lat.J[lat.choice[p][j]];
+        const auto& sj = lat.spins[lat.neighbours[p][j]];
+        for (int a = 0; a < 4; ++a) Vsum[a] += Jv * sj[a];
+    };
+    if (use_d) inner = [inner, &lat, &Vsum, &Vtemp](int p, int j) {
+        inner(p, j);
+        const auto& sj = lat.spins[lat.neighbours[p][j]];
+        const auto& Dv = lat.D[lat.choice[p][j]];
+        Vtemp[0] = sj[1] * Dv[2] - sj[2] * Dv[1];
+        Vtemp[1] = sj[2] * Dv[0] - sj[0] * Dv[2];
+        Vtemp[2] = sj[0] * Dv[1] - sj[1] * Dv[0];
+        for (int a = 0; a < 4; ++a) Vsum[a] += Vtemp[a];
+    };
+    std::function<double(int)> dE_func = [&](int p) {
+        for (int a = 0; a < 4; ++a) Vdiff[a] = lat.spins[p][a] - testspin[a];
+        Vsum = {0, 0, 0, 0};
+        for (size_t j = 0; j < lat.neighbours[p].size(); ++j) inner(p, (int)j);
+        double res[4];
+        for (int a = 0; a < 4; ++a) res[a] = (H[a] + Vsum[a]) * Vdiff[a];
+        return res[0] + res[1] + res[2];
+    };
+    const int size = (int)lat.spins.size();
+    int64_t acc = 0;
+    for (int64_t i = 0; i < attempts; ++i) {
+        int c = int(uni() * size);
+        if (ising) testspin[0] = (double)(mt() & 1u) * 2 - 1;
+        else {
+            double x, y, s2;
+            do { x = -1 + 2 * uni(); y = -1 + 2 * uni(); s2 = x * x + y * y; } while (s2 > 1.0);
+            double z = -1 + 2 * s2, a = 2 * sqrt(1 - s2);
+            testspin[0] = x * a; testspin[1] = y * a; testspin[2] = z;
+        }
+        double dE = dE_func(c);
+        if (dE <= 0) { lat.spins[c] = testspin; ++acc; }
+        else if ((-dE * beta) > log(uni())) { lat.spins[c] = testspin; ++acc; }
+    }
+    *acc_out = acc;
+}
+
+double orc_bench_reference(int n, const int* neigh, const int* nchoice, const int* ncount,
+                           int nb, const double* J, const double* D, int ising, int use_j,
+                           int use_d, double h, double beta, int nthreads,
+                           int64_t attempts_per_thread, uint32_t seed, double* seconds_out) {
+    RefLattice lat;
+    lat.spins.assign(n, {0, 0, 0, 0});
+    std::mt19937 init(seed ^ 0xA5A5A5u);
+    for (int i = 0; i < n; ++i) {
+        if (ising) lat.spins[i][0] = (init() & 1u) ? 1.0 : -1.0;
+        else {
+            double z = -1 + 2 * ((double)init() / 4294967296.0);
+            double p = 2 * M_PI * ((double)init() / 4294967296.0);
+            double r = sqrt(1 - z * z);
+            lat.spins[i] = {r * cos(p), r * sin(p), z, 0};
+        }
+    }
+    lat.neighbours.resize(n); lat.choice.resize(n);
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < ncount[i]; ++q) {
+            lat.neighbours[i].push_back(neigh[(int64_t)i * nb + q]);
+            lat.choice[i].push_back(nchoice[(int64_t)i * nb + q]);
+        }
+    lat.J.assign(J, J + nb);
+    for (int k = 0; k < nb; ++k) lat.D.push_back({D[3 * k], D[3 * k + 1], D[3 * k + 2], 0});
+    std::vector<std::thread> th;
+    std::vector<int64_t> accs(nthreads);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int t = 0; t < nthreads; ++t)
+        th.emplace_back(ref_chain, lat, ising != 0, use_j != 0, use_d != 0, h, beta,
+                        attempts_per_thread, seed + 7919u * (uint32_t)t, &accs[t]);
+    for (auto& x : th) x.join();
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (seconds_out) *seconds_out = secs;
+    return (double)attempts_per_thread * nthreads / secs;
+}
+
+}  // extern "C"
+
+// ORACLE PARITY STATUS (each function's pin is a test under tests/):
+//  orc_philox            Random123 known-answer vectors + rocRAND's philox4x32_10
+//  orc_parse_bonds       Old/Js/3d_nn_bloch.dat contents
+//  orc_build             site counts; carry bug (edge-1 displaced sites) ;
+//                        Weibull keep rate vs closed form exp(-(d/r0)^b)
+//  orc_set_neigh         bond counts (open 3L^2(L-1), periodic 3L^3) ; brute O(N^2)
+//  orc_energy / mag /    reference gtest known answers (-2700, -3000, -2800,
+//  submag / sub4mag      -3100, +2700, M=1000, subM=500, sub4M=63) ; 2-D
+//                        closed forms (-180, -190, +180) ; DMI closed form
+//  orc_delta_e           E_s(new) - E_s(old) consistency (sampler energy)
+//  orc_tcharge           FM -> 0 ; 30^3 skyrmion fixture -> -1 +- 0.01 ;
+//                        single-triangle closed form (octant = 1/8 of sphere)
+//  orc_equil_replay      exact enumeration of 2x2 / 4x4 Ising (stationary law)
+//  orc_checkerboard_ising detailed balance by exact enumeration (3x... small)
+//  orc_accept_threshold  brute force over k near the threshold

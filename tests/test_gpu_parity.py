@@ -1,0 +1,348 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the
+reference's known answers.  Needs an MI355X."""
+import math
+
+import numpy as np
+import pytest
+
+from tests import exact
+from tests.fixtures import afm_spins, box_locs, neel2d, nn2d, random_unit, skyrm_spins, up_spins
+from tests.gpu_util import make_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def f32(s):
+    return np.asarray(s, dtype=np.float32).astype(np.float64)
+
+
+# ---------------------------------------------------- reference known answers
+@pytest.mark.parametrize("ising", [True, False])
+def test_known_answers_3d(orc, bloch3d, ising):
+    d, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.0)
+    for periodic in (0, 1):
+        lat, f = make_pair(orc, 3, 10, periodic, d, J, D, ising)
+        H = 0.1
+        for spins, want_E0, want_Eh in (
+                (up_spins(1000, ising), -3000 if periodic else -2700, -3100 if periodic else -2800),
+                (afm_spins(f.locs, 3, ising), None, None)):
+            lat.set_spins(0, spins)
+            for h, want in ((0.0, want_E0), (H, want_Eh)):
+                lat.set_params(1.0, h)
+                obs, _, _ = lat.measure()
+                if want is not None:
+                    assert obs[0, 9] == want
+                Hv = [h, 0, 0] if ising else [0, 0, h]
+                assert obs[0, 9] == f.energy(spins, Hv)
+                a = 0 if ising else 2
+                assert obs[0, a] == f.mag(spins)[a]
+                assert obs[0, 3 + a] == f.submag(spins)[a]
+                assert obs[0, 6 + a] == f.sub4mag(spins)[a]
+        if not periodic:
+            lat.set_spins(0, afm_spins(f.locs, 3, ising))
+            lat.set_params(1.0, 0.1)
+            obs, _, _ = lat.measure()
+            assert obs[0, 9] == 2700 and obs[0, 3 + (0 if ising else 2)] == 500
+            if not ising:
+                assert obs[0, 8] == 63
+
+
+# ------------------------------------------------- random configurations ----
+CASES = [
+    # dim, edge, periodic, ising, use_d, masked
+    (2, 16, 1, True, False, False),
+    (2, 15, 0, True, True, True),
+    (3, 8, 1, True, False, False),
+    (3, 7, 0, True, False, True),
+    (2, 16, 1, False, True, False),
+    (2, 13, 0, False, True, True),
+    (3, 8, 1, False, True, False),
+    (3, 9, 0, False, True, True),
+]
+
+
+def _case(orc, bloch3d, dim, edge, periodic, ising, use_d, masked, rng, trial="sphere"):
+    if dim == 3:
+        bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.3)
+    else:
+        bd, J, D = neel2d(0.3)
+    locs = None
+    if masked:
+        locs = box_locs(dim, edge)
+        locs = locs[rng.random(len(locs)) < 0.75]
+    lat, f = make_pair(orc, dim, edge, periodic, bd, J, D, ising, use_d=use_d, locs=locs, trial=trial)
+    if ising:
+        s = np.zeros((f.n, 3))
+        s[:, 0] = rng.choice([-1.0, 1.0], f.n)
+    else:
+        s = f32(random_unit(f.n, rng))
+    lat.set_spins(0, s)
+    assert np.array_equal(lat.get_spins(0), s)
+    return lat, f, s
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_measure_matches_oracle(orc, bloch3d, case):
+    dim, edge, periodic, ising, use_d, masked = case
+    rng = np.random.default_rng(hash(case) % 2 ** 32)
+    lat, f, s = _case(orc, bloch3d, *case, rng)
+    h = 0.1
+    lat.set_params(0.7, h)
+    obs, qr, qw = lat.measure()
+    H = [h, 0, 0] if ising else [0, 0, h]
+    want = np.concatenate([f.mag(s), f.submag(s), f.sub4mag(s),
+                           [f.energy(s, H, use_d=use_d, convention=0), f.energy(s, H, use_d=use_d, convention=1)]])
+    if ising:
+        assert np.array_equal(obs[0], want)       # integer-valued: bit-exact
+    else:
+        assert np.allclose(obs[0], want, rtol=1e-12, atol=1e-9)
+        assert np.allclose(qr[0], f.tcharge(s, wrap=False), rtol=1e-10, atol=1e-10)
+        assert np.allclose(qw[0], f.tcharge(s, wrap=True), rtol=1e-10, atol=1e-10)
+
+
+def test_tcharge_skyrmion_fixture(orc, bloch3d):
+    # Old/tests/heis_test.hpp:378-393 on the device
+    d, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.75)
+    locs, s = skyrm_spins(30, 3)
+    lat, f = make_pair(orc, 3, 30, 0, d, J, D, False)
+    s = f32(s)
+    lat.set_spins(0, s)
+    _, qr, _ = lat.measure()
+    assert qr.shape == (1, 30)
+    assert np.all(np.abs(qr[0] + 1) < 0.01)
+    assert np.allclose(qr[0], f.tcharge(s), atol=1e-12)
+    lat.set_spins(0, up_spins(27000, False))
+    _, qr, _ = lat.measure()
+    assert np.all(qr[0] == 0)
+
+
+# ------------------------------------------------------------------ dE ------
+@pytest.mark.parametrize("case", CASES)
+def test_delta_e_matches_oracle(orc, bloch3d, case):
+    dim, edge, periodic, ising, use_d, masked = case
+    rng = np.random.default_rng(1 + hash(case) % 2 ** 31)
+    lat, f, s = _case(orc, bloch3d, *case, rng)
+    h = 0.1
+    lat.set_params(0.7, h)
+    H = [h, 0, 0] if ising else [0, 0, h]
+    n = 256
+    sites = rng.integers(0, f.n, n)
+    if ising:
+        trials = np.zeros((n, 3))
+        trials[:, 0] = rng.choice([-1.0, 1.0], n)
+    else:
+        trials = f32(random_unit(n, rng))
+    got = lat.delta_e(0, sites, trials)
+    want = np.array([f.delta_e(s, sites[i], trials[i], H, use_d=use_d) for i in range(n)])
+    if ising:
+        assert np.array_equal(got, want)          # bit-exact
+    else:
+        # fp32 local field: 1e-5 relative to the local scale |H+V| |s - t|
+        for i in range(n):
+            j = sites[i]
+            scale = (abs(h) + 6 * 1.5) * np.linalg.norm(s[j] - trials[i])
+            assert abs(got[i] - want[i]) <= 1e-5 * max(scale, 1e-30)
+
+
+# ------------------------------------------------ accept decisions (Ising) ---
+@pytest.mark.parametrize("dim,edge,periodic", [(2, 16, 1), (3, 6, 0), (2, 9, 0)])
+def test_replay_accept_decisions_identical(orc, bloch3d, dim, edge, periodic):
+    rng = np.random.default_rng(dim * 100 + edge)
+    bd, J, D = (orc.parse_bonds(bloch3d, 3, 1.0, 0.0) if dim == 3 else nn2d())
+    lat, f = make_pair(orc, dim, edge, periodic, bd, J, D, True)
+    s = np.zeros((f.n, 3))
+    s[:, 0] = rng.choice([-1.0, 1.0], f.n)
+    lat.set_spins(0, s)
+    T, h = 2.3, 0.1
+    lat.set_params(1 / T, h)
+    n = 20000
+    sites = rng.integers(0, f.n, n).astype(np.int32)
+    tr = rng.choice([-1, 1], n).astype(np.int8)
+    k = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    got = lat.replay_ising(0, sites, tr, k)
+    trials = np.zeros((n, 3))
+    trials[:, 0] = tr
+    acc = f.equil_replay(s, [h, 0, 0], 1 / T, sites, trials, k / 4294967296.0)
+    assert np.array_equal(got, acc)
+    assert np.array_equal(lat.get_spins(0), s)
+
+
+def test_replay_boundary_uniforms(orc):
+    # uniforms placed on k = T and T + 1 for each flip's threshold
+    d, J, D = nn2d()
+    lat, f = make_pair(orc, 2, 8, 1, d, J, D, True)
+    rng = np.random.default_rng(4)
+    T, h = 1.7, 0.05
+    lat.set_params(1 / T, h)
+    s = up_spins(64, True)
+    lat.set_spins(0, s)
+    sites, trials, ks = [], [], []
+    cur = s.copy()
+    for _ in range(4000):
+        i = int(rng.integers(64))
+        t = -cur[i, 0]
+        dE = f.delta_e(cur, i, [t, 0, 0], [h, 0, 0])
+        thr = orc.accept_threshold(dE, 1 / T)
+        k = min(thr + int(rng.integers(0, 2)), 2 ** 32 - 1)
+        acc = dE <= 0 or ((-dE / T) > (math.log(k / 4294967296.0) if k else -math.inf))
+        if acc:
+            cur[i, 0] = t
+        sites.append(i); trials.append(int(t)); ks.append(k)
+    got = lat.replay_ising(0, np.array(sites), np.array(trials, dtype=np.int8), np.array(ks, dtype=np.uint32))
+    tr = np.zeros((len(sites), 3))
+    tr[:, 0] = trials
+    want = f.equil_replay(s, [h, 0, 0], 1 / T, np.array(sites), tr, np.array(ks) / 4294967296.0)
+    assert np.array_equal(got, want)
+    assert 0 < want.sum() < len(want)
+
+
+# ------------------------------------ whole checkerboard sweeps (Ising) ------
+@pytest.mark.parametrize("dim,edge,periodic,masked", [(2, 16, 1, False), (2, 11, 0, True), (3, 8, 1, False),
+                                                      (3, 7, 0, True)])
+def test_checkerboard_trajectory_identical(orc, bloch3d, dim, edge, periodic, masked):
+    rng = np.random.default_rng(edge + 17 * dim)
+    bd, J, D = (orc.parse_bonds(bloch3d, 3, 1.0, 0.0) if dim == 3 else nn2d())
+    locs = None
+    if masked:
+        locs = box_locs(dim, edge)
+        locs = locs[rng.random(len(locs)) < 0.8]
+    seed = 0x4E4D4301
+    lat, f = make_pair(orc, dim, edge, periodic, bd, J, D, True, locs=locs, seed=seed, ninst=3)
+    T, h = 2.4, 0.07
+    lat.set_params([1 / T, 1 / 2.0, 1 / 3.0], [h, 0.0, -0.1])
+    params = [(1 / T, h), (1 / 2.0, 0.0), (1 / 3.0, -0.1)]
+    e, c = f.colour_index((edge,) * dim)
+    starts = []
+    for i in range(3):
+        s = np.zeros((f.n, 3))
+        s[:, 0] = rng.choice([-1.0, 1.0], f.n)
+        lat.set_spins(i, s)
+        starts.append(s)
+    lat.equil(4)
+    lat.equil(3, inst=[1, 2])
+    for i in range(3):
+        beta, hh = params[i]
+        s = starts[i]
+        nsw = 4 if i == 0 else 7
+        f.checkerboard_ising(s, [hh, 0, 0], beta, e, c, seed=seed, inst=i, sweep0=0, nsweeps=nsw)
+        assert np.array_equal(lat.get_spins(i), s), "instance %d" % i
+
+
+# ------------------------------------------------------------ statistics ----
+def test_ising_2d_energy_matches_kaufman(orc):
+    # configs[0]-style lattice: 2-D square L=64 periodic, J=1, h=0
+    L = 64
+    d, J, D = nn2d()
+    Ts = [1.5, 2.0, 2.5, 3.5]
+    lat, f = make_pair(orc, 2, L, 1, d, J, D, True, ninst=len(Ts))
+    lat.set_params([1 / T for T in Ts], 0.0)
+    lat.randomize()
+    lat.equil(1000)
+    nsamp = 1000
+    lat.series_reset(nsamp, with_charge=False)
+    for k in range(nsamp):
+        lat.equil(10)
+        lat.sample(k)
+    obs, _, _ = lat.get_series()
+    for i, T in enumerate(Ts):
+        m, err, tau = exact.mean_err(obs[i, :, 9])
+        want = exact.kaufman_energy(L, T)
+        assert abs(m - want) < 4 * err + 1e-9, (T, m, want, err, tau)
+
+
+@pytest.mark.parametrize("trial", ["sphere", "cone"])
+def test_heisenberg_single_spin_and_dimer_langevin(orc, trial):
+    # one spin in a field: <s_z> = L(beta h); open dimer: <E> = -J L(beta J)
+    from nanomagmc_amd import Bond, Lattices
+    bonds = [Bond((1, 0), 1.0), Bond((-1, 0), 1.0), Bond((0, 1), 1.0), Bond((0, -1), 1.0)]
+    insts = [{"n": (1, 1), "edge": 2}] * 3 + [{"n": (2, 1), "edge": 2}] * 3
+    lat = Lattices(2, False, False, bonds, insts, trial=trial, seed=11)
+    betas = [0.5, 1.0, 2.0, 0.5, 1.0, 2.0]
+    hs = [1.0, 1.0, 1.0, 0.0, 0.0, 0.0]
+    lat.set_params(betas, hs)
+    if trial == "cone":
+        lat.set_cone(1.2)
+    lat.randomize()
+    lat.equil(200)
+    nsamp = 20000
+    lat.series_reset(nsamp, with_charge=False)
+    for k in range(nsamp):
+        lat.equil(2)
+        lat.sample(k)
+    obs, _, _ = lat.get_series()
+    for i in range(3):
+        m, err, _ = exact.mean_err(obs[i, :, 2])
+        assert abs(m - exact.langevin(betas[i] * 1.0)) < 4 * err
+    for i in range(3, 6):
+        m, err, _ = exact.mean_err(obs[i, :, 9])
+        assert abs(m - (-exact.langevin(betas[i]))) < 4 * err
+
+
+def test_randomize_draws_uniform_directions(orc):
+    from nanomagmc_amd import Bond, Lattices
+    bonds = [Bond((1, 0, 0), 1.0), Bond((-1, 0, 0), 1.0)]
+    lat = Lattices(3, False, True, bonds, [{"n": (32, 32, 32), "edge": 32}], seed=5)
+    lat.randomize()
+    s = lat.get_spins(0)
+    nrm = np.linalg.norm(s, axis=1)
+    assert np.all(np.abs(nrm - 1) < 2e-6)
+    n = len(s)
+    for a in range(3):
+        assert abs(s[:, a].mean()) < 5 / math.sqrt(3 * n)
+        assert abs((s[:, a] ** 2).mean() - 1 / 3) < 5 * math.sqrt(4 / 45 / n)
+    # z uniform in (-1, 1): 10-bin chi^2
+    hist, _ = np.histogram(s[:, 2], bins=10, range=(-1, 1))
+    chi2 = ((hist - n / 10) ** 2 / (n / 10)).sum()
+    assert chi2 < 40
+
+
+def test_build_weibull_matches_oracle(orc):
+    from nanomagmc_amd import build_weibull
+    sizes = np.array([5.0, 7.5, 10.0])
+    edge, wb, seed = 30, 8.0, 0x4E4D4304
+    masks = build_weibull(3, edge, sizes, wb, seed)
+    for g, a in enumerate(sizes):
+        u = np.zeros(edge ** 3)
+        for q in range((edge ** 3 + 3) // 4):
+            w = orc.philox([q, 2 << 8, 0, 0], [seed & 0xFFFFFFFF, ((seed >> 32) + g) & 0xFFFFFFFF])
+            u[4 * q:4 * q + 4] = (w / 4294967296.0)[: min(4, edge ** 3 - 4 * q)]
+        locs = orc.build_sites(3, edge, "x", a, wb, u)
+        want = np.zeros(edge ** 3, dtype=np.uint8)
+        want[(locs[:, 0] * edge + locs[:, 1]) * edge + locs[:, 2]] = 1
+        assert np.array_equal(masks[g], want)
+
+
+def test_snapshot_series_and_average(orc):
+    d, J, D = nn2d()
+    lat, f = make_pair(orc, 2, 8, 1, d, J, D, False, ninst=2)
+    lat.randomize()
+    lat.equil(3, inst=[0])
+    lat.snapshot([0], [1])
+    assert np.array_equal(lat.get_spins(0), lat.get_spins(1))
+    lat.zero_avg(0)
+    acc = np.zeros((64, 3))
+    for _ in range(3):
+        lat.equil(1, inst=[0])
+        lat.accumulate_avg(0)
+        acc += lat.get_spins(0)
+    assert np.allclose(lat.get_avg(0), acc, atol=1e-12)
+    lat.series_reset(4)
+    for k in range(4):
+        lat.equil(1)
+        lat.sample(k)
+    obs, qr, qw = lat.get_series()
+    o2, q2, w2 = lat.measure()
+    assert np.array_equal(obs[:, 3, :], o2) and np.array_equal(qr[:, 3, :], q2)
+
+
+def test_invalid_arguments_raise(orc):
+    from nanomagmc_amd import Bond, Lattices
+    from nanomagmc_amd._lib import NmcError
+    bonds = [Bond((1, 0), 1.0), Bond((-1, 0), 1.0)]
+    lat = Lattices(2, True, True, bonds, [{"n": (4, 4), "edge": 4}])
+    with pytest.raises(NmcError):
+        lat.set_params(-1.0, 0.0)          # T <= 0 (reference exit 104)
+    with pytest.raises(NmcError):
+        Lattices(2, True, True, bonds, [{"n": (5, 5), "edge": 5}])   # odd periodic edge
+    with pytest.raises(NmcError):
+        Lattices(2, True, False, [Bond((2, 0), 1.0)], [{"n": (4, 4), "edge": 4}])
