@@ -114,6 +114,19 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// Element access through a wave-uniform base pointer and a 32-bit byte
+// offset, which maps to the scalar-base + vector-offset addressing mode
+// (no per-lane 64-bit address arithmetic).  Every instance is < 2^30
+// elements, so the offset cannot wrap.
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, uint32_t idx) {
+    return *(const T*)((const char*)base + idx * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* base, uint32_t idx, T v) {
+    *(T*)((char*)base + idx * (uint32_t)sizeof(T)) = v;
+}
+
 // Uniform in (0, 1) with 24 random bits, centred on the 2^-24 grid.
 __device__ __forceinline__ float u24_open(uint32_t w) {
     return ((float)(w >> 8) + 0.5f) * (1.0f / 16777216.0f);

@@ -45,8 +45,14 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
     const DevInst& in = insts[tl.inst];
     const uint32_t e = (uint32_t)tl.e0 + threadIdx.x;
     const int64_t Sp = in.Sp;
-    float* __restrict__ own = pool + in.base + (int64_t)colour * 3 * Sp;
-    const float* __restrict__ oth = pool + in.base + (int64_t)(colour ^ 1) * 3 * Sp;
+    // per-component base pointers are wave-uniform (scalar registers); every
+    // access below is base + 32-bit element offset
+    float* __restrict__ own_x = pool + in.base + (int64_t)colour * 3 * Sp;
+    float* __restrict__ own_y = own_x + Sp;
+    float* __restrict__ own_z = own_y + Sp;
+    const float* __restrict__ oth_x = pool + in.base + (int64_t)(colour ^ 1) * 3 * Sp;
+    const float* __restrict__ oth_y = oth_x + Sp;
+    const float* __restrict__ oth_z = oth_y + Sp;
 
     bool accepted = false;
     if (e < (uint32_t)in.S) {
@@ -54,13 +60,16 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
         bool present = p.i2 < in.n2;
         if (MASKED && present) present = mask_bit(mpool, in, colour, e);
         if (present) {
-            const float sx = own[e], sy = own[Sp + e], sz = own[2 * Sp + e];
+            const float sx = ld32(own_x, e), sy = ld32(own_y, e), sz = ld32(own_z, e);
             float vx = 0.f, vy = 0.f, vz = 0.f;
 #pragma unroll
             for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
                 const int ne = nbr_elem<PERIODIC>(in, p, g);
                 float nx = 0.f, ny = 0.f, nz = 0.f;
-                if (PERIODIC || ne >= 0) { nx = oth[ne]; ny = oth[Sp + ne]; nz = oth[2 * Sp + ne]; }
+                if (PERIODIC || ne >= 0) {
+                    const uint32_t u = (uint32_t)ne;
+                    nx = ld32(oth_x, u); ny = ld32(oth_y, u); nz = ld32(oth_z, u);
+                }
                 vx += st.J[g] * nx;
                 vy += st.J[g] * ny;
                 vz += st.J[g] * nz;
@@ -91,7 +100,7 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
                 const float ca = 1.f - ua * in.cone_w;
                 const float sa = __builtin_sqrtf(fmaxf(0.f, 1.f - ca * ca));
                 const float sgn = sz >= 0.f ? 1.f : -1.f;
-                const float a = -1.f / (sgn + sz);
+                const float a = -__builtin_amdgcn_rcpf(sgn + sz);   // |sgn + sz| >= 1
                 const float b = sx * sy * a;
                 const float e1x = 1.f + sgn * sx * sx * a, e1y = sgn * b, e1z = -sgn * sx;
                 const float e2x = b, e2y = sgn + sy * sy * a, e2z = -sy;
@@ -108,9 +117,9 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
             const float dE = vx * (sx - tx) + vy * (sy - ty) + (in.h + vz) * (sz - tz);
             accepted = (dE <= 0.f) || (u24(w[2]) < __expf(-in.beta * dE));
             if (accepted) {
-                own[e] = tx;
-                own[Sp + e] = ty;
-                own[2 * Sp + e] = tz;
+                st32(own_x, e, tx);
+                st32(own_y, e, ty);
+                st32(own_z, e, tz);
             }
         }
     }

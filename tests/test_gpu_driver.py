@@ -1,0 +1,124 @@
+"""End-to-end runs of the C++ host driver nmc_run on tiny inputs: INPUT.dat
+keys, protocol path order, HDF5 layout (Old/lib/output.cpp:80-458) and the
+/Complete resume flags.  Needs the GPU and the HDF5 command-line tools."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+RUNNER = os.path.join(ROOT, "nanomagmc_amd", "nmc_run")
+H5DUMP = "/opt/conda/bin/h5dump"
+H5LS = "/opt/conda/bin/h5ls"
+
+
+def run(case_dir, timeout=120):
+    r = subprocess.run([RUNNER, "INPUT.dat"], cwd=case_dir, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def h5_shapes(path):
+    out = subprocess.run([H5LS, "-r", path], capture_output=True, text=True, check=True).stdout
+    shapes = {}
+    for line in out.splitlines():
+        m = re.match(r"^(\S+)\s+Dataset \{([^}]*)\}", line)
+        if m:
+            shapes[m.group(1)] = tuple(int(x.split("/")[0]) for x in m.group(2).split(","))
+    return shapes
+
+
+def h5_read(path, name):
+    out = subprocess.run([H5DUMP, "-d", name, "-y", "-w", "0", "-o", "/dev/stdout", path], capture_output=True,
+                         text=True, check=True).stdout
+    body = out.split("DATA {", 1)[1]
+    nums = re.findall(r"-?\d+(?:\.\d+)?(?:e[-+]?\d+)?|nan|inf", body.rsplit("}", 2)[0])
+    return np.array([float(x) for x in nums])
+
+
+def make_case(tmp_path, **keys):
+    from nanomagmc_amd.inputs import write_case
+    base = dict(ISISING=1, EXCHANGE=1, DMISTREN=0, ISDISTRIB=0, SIZE=8, LATTSHAPE="s", NLATTS=2, BOLTZMANN=1,
+                ISPERIO=1, SAMPSWEEPS=2, EQSWEEPS=10, NSAMPS=5, PROTOCOL=1, INTERACTIONS="2d_nn.dat")
+    temps = keys.pop("temps", [3.0, 2.0, 1.5])
+    fields = keys.pop("fields", [0.0, 0.1])
+    base.update(keys)
+    write_case(str(tmp_path), base, temps, fields)
+    return str(tmp_path)
+
+
+@pytest.mark.parametrize("protocol", [1, 2, 4])
+def test_ising_layout(tmp_path, protocol):
+    d = make_case(tmp_path, PROTOCOL=protocol)
+    r = run(d)
+    assert "3 temperatures" in r.stdout and "2 field strengths" in r.stdout
+    path = os.path.join(d, "Output", "J_1-s_8-k_1-K_0-sh_s-per_1-pr_%d.h5" % protocol)
+    assert os.path.exists(path)
+    sh = h5_shapes(path)
+    for n in ("mags", "energies", "sub_mags", "sub4_mags"):
+        assert sh["/Fulldat/" + n] == (2, 3, 10)
+    assert "/Fulldat/mag_xs" not in sh and "/Fulldat/top_chars" not in sh
+    assert sh["/Fulldat/nums"] == (5,)
+    assert sh["/Ts"] == (3,) and sh["/Hs"] == (2,)
+    v1 = 2 if protocol == 1 else 3
+    assert sh["/Complete"] == (v1, 2)
+    assert sh["/Av_Latt/T_0-H_0"] == (8, 8, 1) and sh["/Sing_Latt/T_2-H_1"] == (8, 8, 1)
+    E = h5_read(path, "/Fulldat/energies").reshape(2, 3, 10)
+    assert np.all(np.abs(E) <= 2 * 64) and np.all(E % 4 == 0)       # h = 0 rows: -sum s_i s_j
+    M = h5_read(path, "/Fulldat/mags").reshape(2, 3, 10)
+    assert np.all(M <= 64) and np.all(M >= 0)
+    # every (H, T, lattice) slot was written: the ordered phase at T = 1.5
+    # has |M| close to N
+    assert M[0, 2].mean() > 40
+    comp = h5_read(path, "/Complete").reshape(v1, 2)
+    if protocol != 4:
+        assert np.all(comp == 1)
+    Ts = h5_read(path, "/Ts")
+    assert np.allclose(Ts, [3.0, 2.0, 1.5])
+    sing = h5_read(path, "/Sing_Latt/T_2-H_0")
+    assert set(np.unique(sing)) <= {-1.0, 1.0}
+
+
+def test_resume_skips_completed_chains(tmp_path):
+    d = make_case(tmp_path, PROTOCOL=2)
+    run(d)
+    path = os.path.join(d, "Output", "J_1-s_8-k_1-K_0-sh_s-per_1-pr_2.h5")
+    before = h5_read(path, "/Fulldat/energies")
+    r = run(d)   # everything complete: the v2 walks are skipped, data unchanged
+    assert "Printing" not in r.stdout
+    assert np.array_equal(before, h5_read(path, "/Fulldat/energies"))
+
+
+def test_heisenberg_charges_and_components(tmp_path):
+    d = make_case(tmp_path, ISISING=0, DMISTREN=0.3, INTERACTIONS="2d_nn_neel.dat", SIZE=12, NLATTS=1,
+                  temps=[1.0, 0.5], fields=[0.0])
+    run(d)
+    path = os.path.join(d, "Output", "J_1-s_12-k_1-K_0.3-sh_s-per_1-pr_1.h5")
+    sh = h5_shapes(path)
+    assert sh["/Fulldat/top_chars"] == (1, 2, 12, 5)
+    assert sh["/Fulldat/mag_zs"] == (1, 2, 5)
+    assert sh["/Av_Latt/T_1-H_0"] == (12, 12, 3)
+    mx, my, mz = (h5_read(path, "/Fulldat/mag_%ss" % c) for c in "xyz")
+    m = h5_read(path, "/Fulldat/mags")
+    assert np.allclose(np.sqrt(mx ** 2 + my ** 2 + mz ** 2), m, rtol=1e-5, atol=1e-4)
+    qw = h5_read(path, "/Fulldat/top_chars_wrap").reshape(1, 2, 12, 5)
+    assert np.allclose(qw[:, :, 0, :], np.round(qw[:, :, 0, :]), atol=1e-3)   # integer on the torus
+    assert np.all(qw[:, :, 1:, :] == 0)
+
+
+def test_weibull_grains_distributed(tmp_path):
+    d = make_case(tmp_path, ISISING=0, DMISTREN=0.2, INTERACTIONS="3d_nn_bloch.dat", LATTSHAPE="x",
+                  ISDISTRIB=1, MEANSIZE=3, SIZEDEV=0.5, WEIBULLFACT=8, ISPERIO=0, NLATTS=3,
+                  temps=[1.0], fields=[0.0, 0.05])
+    run(d)
+    path = os.path.join(d, "Output", "J_1-am_3-asd_0.5-k_1-K_0.2-sh_x-per_0-pr_1.h5")
+    sh = h5_shapes(path)
+    assert sh["/Fulldat/energies"] == (2, 1, 15)
+    assert "/Fulldat/top_chars" not in sh and "/Av_Latt/T_0-H_0" not in sh
+    m = h5_read(path, "/Fulldat/mags")
+    assert np.all(m > 0)
