@@ -135,6 +135,15 @@ int nmc_randomize(nmc_ctx* ctx, const int* inst, int n);
  * sweeps: every present site gets one single-site Metropolis attempt per
  * sweep.  inst == NULL means every instance. */
 int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps);
+/* Kernel path for nmc_sweep.  AUTO keeps every instance whose two colour
+ * arrays fit in one workgroup's LDS resident there for whole sweeps (one
+ * persistent workgroup per instance) and streams the others from HBM with
+ * one launch per colour pass; STREAM forces streaming for all.  Both paths
+ * draw the same random numbers, so they give the same trajectories. */
+#define NMC_PATH_AUTO 0
+#define NMC_PATH_STREAM 1
+int nmc_set_path(nmc_ctx* ctx, int path);
+
 /* Accepted-attempt counters since the last reset (one per instance). */
 int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out);
 int nmc_reset_accepts(nmc_ctx* ctx);
@@ -162,6 +171,14 @@ int nmc_series_reset(nmc_ctx* ctx, int nsamples, int with_charge);
 int nmc_sample(nmc_ctx* ctx, const int* inst, int n, int slot);
 int nmc_get_series(nmc_ctx* ctx, const int* inst, int n, double* obs,
                    double* q_ref, double* q_wrap);
+
+/* The whole sample loop of one path point (main.cpp:182-240): for slot
+ * 0..nsamples-1, `samp_sweeps` sweeps then a measurement into that slot,
+ * for the listed instances.  Recorded once per (instance list, nsamples,
+ * samp_sweeps) as a HIP graph and replayed as one launch; the random stream
+ * is the one nmc_sweep + nmc_sample would draw.  nsamples must not exceed
+ * the series size of nmc_series_reset. */
+int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp_sweeps);
 
 /* state::add_to_av (state.cpp:288-296): sum the current spins of src into
  * the running lattice average of instance `dst_avg` slot; nmc_get_avg returns

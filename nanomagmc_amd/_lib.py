@@ -23,9 +23,9 @@ OBS_NAMES = ["Mx", "My", "Mz", "subMx", "subMy", "subMz", "sub4Mx", "sub4My", "s
 # Every symbol include/nmc.h declares, in header order.
 EXPORTS = [
     "nmc_last_error", "nmc_create", "nmc_destroy", "nmc_get_stream", "nmc_num_sites", "nmc_num_slices",
-    "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_get_accepts",
+    "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_set_path", "nmc_get_accepts",
     "nmc_reset_accepts", "nmc_snapshot", "nmc_measure", "nmc_series_reset", "nmc_sample",
-    "nmc_get_series", "nmc_accumulate_avg", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
+    "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
     "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_build_weibull",
 ]
 
@@ -73,6 +73,7 @@ def lib():
         "nmc_set_cone": (i, [vp, P(i), i, P(C.c_float)]),
         "nmc_randomize": (i, [vp, P(i), i]),
         "nmc_sweep": (i, [vp, P(i), i, i]),
+        "nmc_set_path": (i, [vp, i]),
         "nmc_get_accepts": (i, [vp, P(i), i, P(u64)]),
         "nmc_reset_accepts": (i, [vp]),
         "nmc_snapshot": (i, [vp, P(i), P(i), i]),
@@ -80,6 +81,7 @@ def lib():
         "nmc_series_reset": (i, [vp, i, i]),
         "nmc_sample": (i, [vp, P(i), i, i]),
         "nmc_get_series": (i, [vp, P(i), i, P(d), P(d), P(d)]),
+        "nmc_run_samples": (i, [vp, P(i), i, i, i]),
         "nmc_accumulate_avg": (i, [vp, i]),
         "nmc_get_avg": (i, [vp, i, P(d)]),
         "nmc_zero_avg": (i, [vp, i]),

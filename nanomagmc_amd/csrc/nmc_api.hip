@@ -70,7 +70,20 @@ struct Set {
     int n_mt = 0;
     int* d_tb = nullptr;
     int* d_te = nullptr;
+    int* d_lds = nullptr;    // instances swept LDS-resident (the rest are in d_sweep)
+    int n_lds = 0;
+    size_t lds_bytes = 0;    // dynamic LDS of the largest of them
+    int lds_block = 256;
+
+    void release() {
+        (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
+        (void)hipFree(d_mt); (void)hipFree(d_tb); (void)hipFree(d_te); (void)hipFree(d_lds);
+        d_list = nullptr; d_sweep = nullptr; d_init = nullptr; d_mt = nullptr;
+        d_tb = d_te = d_lds = nullptr;
+    }
 };
+
+constexpr int kLdsSweepChunk = 256;   // sweeps per LDS-resident launch
 
 }  // namespace
 
@@ -112,12 +125,20 @@ struct nmc_ctx {
     // Ising threshold rows keyed by (beta, h)
     std::map<std::pair<double, double>, std::pair<std::vector<uint32_t>, std::vector<double>>> tab_cache;
 
+    int path = NMC_PATH_AUTO;
+    size_t max_lds = 65536;   // dynamic LDS one workgroup may use on this device
+    // instantiated sample-loop graphs keyed by (instance list, nsamples, sweeps)
+    std::map<std::tuple<std::vector<int>, int, int>, hipGraphExec_t> graphs;
+
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+
     ~nmc_ctx() {
-        for (auto& kv : sets) {
-            Set& s = *kv.second;
-            (void)hipFree(s.d_list); (void)hipFree(s.d_sweep); (void)hipFree(s.d_init);
-            (void)hipFree(s.d_mt); (void)hipFree(s.d_tb); (void)hipFree(s.d_te);
-        }
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        for (auto& kv : sets) kv.second->release();
         for (auto& kv : avg) (void)hipFree(kv.second);
         (void)hipFree(d_inst); (void)hipFree(pool); (void)hipFree(mpool); (void)hipFree(d_tab);
         (void)hipFree(d_dtab); (void)hipFree(d_sweeps); (void)hipFree(d_acc);
@@ -200,9 +221,18 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     std::vector<MTile> mt;
     std::vector<int> tb, te;
     const int stile = c->ising ? kIsingTile : kSweepTile;
+    std::vector<int> lds;
+    int64_t lds_work = 0;
     for (int i : list) {
         const DevInst& d = c->din[i];
-        for (int64_t e = 0; e < d.S; e += stile) sw.push_back(Tile{i, (int)e});
+        const size_t bytes = (size_t)(c->ising ? 2 * d.Sp : 6 * d.Sp * (int64_t)sizeof(float));
+        if (c->path == NMC_PATH_AUTO && bytes <= c->max_lds) {
+            lds.push_back(i);
+            s->lds_bytes = std::max(s->lds_bytes, bytes);
+            lds_work = std::max<int64_t>(lds_work, c->ising ? (d.S + 1) / 2 : d.S);
+        } else {
+            for (int64_t e = 0; e < d.S; e += stile) sw.push_back(Tile{i, (int)e});
+        }
         for (int64_t e = 0; e < d.Sp; e += 256) ini.push_back(Tile{i, (int)e});
         const int rows = d.a0 * d.a1;
         int per = std::max(4, (int)std::min<int64_t>(rows, (4096 + d.half - 1) / std::max(1, d.half)));
@@ -217,6 +247,9 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     if ((rc = upload(c, &s->d_mt, mt))) return rc;
     if ((rc = upload(c, &s->d_tb, tb))) return rc;
     if ((rc = upload(c, &s->d_te, te))) return rc;
+    if ((rc = upload(c, &s->d_lds, lds))) return rc;
+    s->n_lds = (int)lds.size();
+    s->lds_block = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (lds_work + 63) / 64 * 64));
     s->n_sweep = (int)sw.size();
     s->n_init = (int)ini.size();
     s->n_mt = (int)mt.size();
@@ -278,6 +311,8 @@ void ising_rows(const nmc_ctx* c, double beta, double h, std::vector<uint32_t>& 
 
 int ensure_partials(nmc_ctx* c, size_t n) {
     if (n <= c->partials_cap) return NMC_OK;
+    NMC_HIP(hipStreamSynchronize(c->stream));
+    c->drop_graphs();   // recorded graphs hold the old buffer
     (void)hipFree(c->d_partials);
     c->d_partials = nullptr;
     c->partials_cap = 0;
@@ -409,6 +444,12 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
         return fail(NMC_E_UNSUPPORTED, "3-D extent along z above 512 is not supported by the charge reduction");
     c->nsl = nsl;
     c->pool_elems = (size_t)pool;
+    {
+        int dev = 0, v = 0;
+        NMC_HIP(hipGetDevice(&dev));
+        NMC_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        c->max_lds = (size_t)std::max(0, v);
+    }
     NMC_HIP(hipStreamCreate(&c->stream));
     int rc;
     if ((rc = dalloc(&c->d_inst, c->din.size()))) return rc;
@@ -534,13 +575,19 @@ int nmc_randomize(nmc_ctx* ctx, const int* inst, int n) {
     return NMC_OK;
 }
 
-int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
-    if (!ctx) return fail(NMC_E_INVALID, "null context");
-    if (nsweeps < 0) return fail(NMC_E_INVALID, "negative sweep count");
-    Set* s;
-    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+namespace {
+// Enqueue `nsweeps` sweeps of a prepared set (launches only: no allocation,
+// no synchronisation, so it can be recorded into a graph).
+int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
-    for (int k = 0; k < nsweeps; ++k)
+    for (int k = 0; k < nsweeps && s->n_lds > 0; k += kLdsSweepChunk) {
+        const int chunk = std::min(kLdsSweepChunk, nsweeps - k);
+        NMC_HIP(launch_sweep_lds(ctx->stream, ctx->ising, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d,
+                                 ctx->cone, std::min(s->n_lds, 65535), s->lds_block, s->lds_bytes, ctx->d_inst,
+                                 s->d_lds, s->n_lds, ctx->pool, ctx->mpool, ctx->d_tab, ctx->d_sweeps, ctx->st,
+                                 chunk, (uint32_t)k, slo, shi, ctx->d_acc));
+    }
+    for (int k = 0; k < nsweeps && s->n_sweep > 0; ++k)
         for (int col = 0; col < 2; ++col) {
             if (ctx->ising)
                 NMC_HIP(launch_sweep_ising(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, true, ctx->d_inst,
@@ -553,6 +600,27 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
         }
     if (nsweeps > 0)
         NMC_HIP(launch_bump_sweeps(ctx->stream, ctx->d_sweeps, s->d_list, (int)s->list.size(), (uint64_t)nsweeps));
+    return NMC_OK;
+}
+}  // namespace
+
+int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (nsweeps < 0) return fail(NMC_E_INVALID, "negative sweep count");
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    return sweep_launches(ctx, s, nsweeps);
+}
+
+int nmc_set_path(nmc_ctx* ctx, int path) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (path != NMC_PATH_AUTO && path != NMC_PATH_STREAM) return fail(NMC_E_INVALID, "unknown sweep path");
+    if (path == ctx->path) return NMC_OK;
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graphs();
+    for (auto& kv : ctx->sets) kv.second->release();
+    ctx->sets.clear();
+    ctx->path = path;
     return NMC_OK;
 }
 
@@ -624,6 +692,7 @@ int nmc_measure(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_ref
 int nmc_series_reset(nmc_ctx* ctx, int nsamples, int with_charge) {
     if (!ctx || nsamples < 0) return fail(NMC_E_INVALID, "bad argument");
     NMC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graphs();   // recorded graphs hold the old series buffers
     (void)hipFree(ctx->d_series); (void)hipFree(ctx->d_series_qr); (void)hipFree(ctx->d_series_qw);
     ctx->d_series = ctx->d_series_qr = ctx->d_series_qw = nullptr;
     ctx->series_n = 0;
@@ -641,16 +710,57 @@ int nmc_series_reset(nmc_ctx* ctx, int nsamples, int with_charge) {
     return NMC_OK;
 }
 
+namespace {
+// Enqueue one measurement of a prepared set into series slot `slot`
+// (launches only; the partials buffer must already be large enough).
+int sample_launches(nmc_ctx* ctx, Set* s, int slot) {
+    NMC_HIP(launch_measure(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, ctx->ising, ctx->d_inst, s->d_mt,
+                           s->n_mt, ctx->pool, ctx->mpool, ctx->st, ctx->d_partials, pstride_of(ctx), ctx->a2max));
+    NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, (int)s->list.size(), s->d_tb, s->d_te,
+                                    ctx->d_partials, pstride_of(ctx), ctx->a2max, ctx->dim3, ctx->series_q, ctx->nsl,
+                                    ctx->d_series, ctx->d_series_qr, ctx->d_series_qw, true, ctx->series_n, slot,
+                                    ctx->ising ? 0 : 2));
+    return NMC_OK;
+}
+}  // namespace
+
 int nmc_sample(nmc_ctx* ctx, const int* inst, int n, int slot) {
     if (!ctx) return fail(NMC_E_INVALID, "null context");
     if (slot < 0 || slot >= ctx->series_n) return fail(NMC_E_INVALID, "sample slot outside the series buffer");
     Set* s;
     if (int rc = get_set(ctx, inst, n, &s)) return rc;
-    if (int rc = run_measure(ctx, s)) return rc;
-    NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, (int)s->list.size(), s->d_tb, s->d_te,
-                                    ctx->d_partials, pstride_of(ctx), ctx->a2max, ctx->dim3, ctx->series_q, ctx->nsl,
-                                    ctx->d_series, ctx->d_series_qr, ctx->d_series_qw, true, ctx->series_n, slot,
-                                    ctx->ising ? 0 : 2));
+    if (int rc = ensure_partials(ctx, (size_t)s->n_mt * pstride_of(ctx))) return rc;
+    return sample_launches(ctx, s, slot);
+}
+
+int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp_sweeps) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (nsamples < 0 || nsamples > ctx->series_n) return fail(NMC_E_INVALID, "nsamples exceeds the series buffer");
+    if (samp_sweeps < 0) return fail(NMC_E_INVALID, "negative sweep count");
+    if (nsamples == 0) return NMC_OK;
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    if (int rc = ensure_partials(ctx, (size_t)s->n_mt * pstride_of(ctx))) return rc;
+    auto key = std::make_tuple(s->list, nsamples, samp_sweeps);
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        NMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = NMC_OK;
+        for (int slot = 0; slot < nsamples && rc == NMC_OK; ++slot) {
+            rc = sweep_launches(ctx, s, samp_sweeps);
+            if (rc == NMC_OK) rc = sample_launches(ctx, s, slot);
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != NMC_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(NMC_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(NMC_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        it = ctx->graphs.emplace(key, x).first;
+    }
+    NMC_HIP(hipGraphLaunch(it->second, ctx->stream));
     return NMC_OK;
 }
 

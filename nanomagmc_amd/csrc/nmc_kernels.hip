@@ -1,194 +1,18 @@
-// nmc_kernels.hip -- HIP kernels of the Metropolis engine for gfx950.
+// nmc_kernels.hip -- observable, initialisation, test-mode and builder
+// kernels of the Metropolis engine for gfx950 (the sweeps are in
+// nmc_sweep.hip).
 //
-// Reference hot path (NanoMagMC, Old/): state::equil (lib/state.cpp:171-203)
-// calling dE_func (lib/thermodynamics.cpp:51-69) and the trial draw
-// field_type::gen_rand (lib/field_type.cpp:185-195); the observables E_func,
-// calc_M, calc_subM, calc_sub4M, calc_TC, solid_angle
-// (lib/thermodynamics.cpp:71-221).
-//
-// Each sweep is two colour passes.  In a pass every work-item owns one site
-// of the active colour (Ising: two), reads its neighbours from the other
-// colour array (coalesced, L2-resident across neighbouring tiles), draws one
-// Philox4x32-10 block for its trial spin and acceptance uniform, and writes
-// the site back only if the move is accepted.
+// Reference: the observables E_func, calc_M, calc_subM, calc_sub4M, calc_TC
+// and solid_angle (Old/lib/thermodynamics.cpp:71-221), field_type::all_rand
+// (Old/lib/field_type.cpp:197-204), state::add_to_av (Old/lib/state.cpp:
+// 288-296) and shape::weibull::check (Old/lib/shape.cpp:40-58).
 #include "nmc_device.h"
 #include "nmc_launch.h"
 
 namespace nmc {
 
-// Block index -> tile, keeping consecutive tiles on one XCD.  Hardware places
-// workgroup b on XCD b % 8; tiles t and t +- (rows of a plane) share halo rows,
-// so giving each XCD one contiguous range of tiles keeps those rows in one L2.
-__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
-    const int per = ntiles >> 3, rem = ntiles & 7;
-    const int x = b & 7, j = b >> 3;
-    return x * per + (x < rem ? x : rem) + j;
-}
-
 __device__ __forceinline__ bool mask_bit(const uint32_t* __restrict__ mp, const DevInst& in, int c, int64_t e) {
     return (mp[in.mask_base + c * in.mask_stride + (e >> 5)] >> (e & 31)) & 1u;
-}
-
-// ---------------------------------------------------------------------------
-// Heisenberg colour pass.
-// ---------------------------------------------------------------------------
-template <bool DIM3, bool PERIODIC, bool MASKED, bool USE_D, bool CONE, bool COUNT>
-__global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ insts,
-                                                    const Tile* __restrict__ tiles, int ntiles,
-                                                    float* __restrict__ pool,
-                                                    const uint32_t* __restrict__ mpool,
-                                                    const uint64_t* __restrict__ sweeps,
-                                                    Stencil st, int colour, uint32_t s_off,
-                                                    uint32_t seed_lo, uint32_t seed_hi,
-                                                    unsigned long long* __restrict__ acc) {
-    const Tile tl = tiles[xcd_tile(blockIdx.x, ntiles)];
-    const DevInst& in = insts[tl.inst];
-    const uint32_t e = (uint32_t)tl.e0 + threadIdx.x;
-    const int64_t Sp = in.Sp;
-    // per-component base pointers are wave-uniform (scalar registers); every
-    // access below is base + 32-bit element offset
-    float* __restrict__ own_x = pool + in.base + (int64_t)colour * 3 * Sp;
-    float* __restrict__ own_y = own_x + Sp;
-    float* __restrict__ own_z = own_y + Sp;
-    const float* __restrict__ oth_x = pool + in.base + (int64_t)(colour ^ 1) * 3 * Sp;
-    const float* __restrict__ oth_y = oth_x + Sp;
-    const float* __restrict__ oth_z = oth_y + Sp;
-
-    bool accepted = false;
-    if (e < (uint32_t)in.S) {
-        const SitePos p = site_of(in, e, colour, DIM3);
-        bool present = p.i2 < in.n2;
-        if (MASKED && present) present = mask_bit(mpool, in, colour, e);
-        if (present) {
-            const float sx = ld32(own_x, e), sy = ld32(own_y, e), sz = ld32(own_z, e);
-            float vx = 0.f, vy = 0.f, vz = 0.f;
-#pragma unroll
-            for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-                const int ne = nbr_elem<PERIODIC>(in, p, g);
-                float nx = 0.f, ny = 0.f, nz = 0.f;
-                if (PERIODIC || ne >= 0) {
-                    const uint32_t u = (uint32_t)ne;
-                    nx = ld32(oth_x, u); ny = ld32(oth_y, u); nz = ld32(oth_z, u);
-                }
-                vx += st.J[g] * nx;
-                vy += st.J[g] * ny;
-                vz += st.J[g] * nz;
-                if (USE_D) {   // s_j x D_j, the orientation dE_func uses (thermodynamics.cpp:45-46)
-                    vx += ny * st.D[g][2] - nz * st.D[g][1];
-                    vy += nz * st.D[g][0] - nx * st.D[g][2];
-                    vz += nx * st.D[g][1] - ny * st.D[g][0];
-                }
-            }
-            const uint64_t sweep = sweeps[tl.inst] + s_off;
-            uint32_t w[4];
-            philox4x32_10(e, (uint32_t)colour, (uint32_t)sweep, (uint32_t)(sweep >> 32),
-                          seed_lo, seed_hi + in.inst_id, w);
-            const float ua = u24_open(w[0]);
-            const float uphi = u24(w[1]);
-            const float cphi = __builtin_amdgcn_cosf(uphi);   // cos(2 pi u)
-            const float sphi = __builtin_amdgcn_sinf(uphi);
-            float tx, ty, tz;
-            if (!CONE) {
-                // uniform direction on S^2: z uniform in (-1, 1), azimuth uniform
-                tz = 2.f * ua - 1.f;
-                const float r = __builtin_sqrtf(fmaxf(0.f, 1.f - tz * tz));
-                tx = r * cphi;
-                ty = r * sphi;
-            } else {
-                // uniform on the cap of half-angle theta around s
-                // (orthonormal frame: Duff et al., JCGT 6(1), 2017)
-                const float ca = 1.f - ua * in.cone_w;
-                const float sa = __builtin_sqrtf(fmaxf(0.f, 1.f - ca * ca));
-                const float sgn = sz >= 0.f ? 1.f : -1.f;
-                const float a = -__builtin_amdgcn_rcpf(sgn + sz);   // |sgn + sz| >= 1
-                const float b = sx * sy * a;
-                const float e1x = 1.f + sgn * sx * sx * a, e1y = sgn * b, e1z = -sgn * sx;
-                const float e2x = b, e2y = sgn + sy * sy * a, e2z = -sy;
-                const float px = cphi * e1x + sphi * e2x;
-                const float py = cphi * e1y + sphi * e2y;
-                const float pz = cphi * e1z + sphi * e2z;
-                tx = ca * sx + sa * px;
-                ty = ca * sy + sa * py;
-                tz = ca * sz + sa * pz;
-                const float inv = __builtin_amdgcn_rsqf(tx * tx + ty * ty + tz * tz);
-                tx *= inv; ty *= inv; tz *= inv;
-            }
-            // dE_func: (H + Vsum) . (s_old - s_trial), H = (0, 0, h)
-            const float dE = vx * (sx - tx) + vy * (sy - ty) + (in.h + vz) * (sz - tz);
-            accepted = (dE <= 0.f) || (u24(w[2]) < __expf(-in.beta * dE));
-            if (accepted) {
-                st32(own_x, e, tx);
-                st32(own_y, e, ty);
-                st32(own_z, e, tz);
-            }
-        }
-    }
-    if (COUNT) {
-        const unsigned long long m = __ballot(accepted);
-        if ((threadIdx.x & 63) == 0 && m) atomicAdd(acc + tl.inst, (unsigned long long)__popcll(m));
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Ising colour pass: two sites (one Philox block) per work-item.  The move is
-// the reference's fresh +-1 trial; a trial equal to the current spin is the
-// dE = 0 no-op the reference accepts.  A flip is accepted iff the 32-bit
-// uniform k satisfies k <= T[idx], where T is the host-built threshold of
-// -dE*beta > log(k * 2^-32) for the neighbourhood idx -- the reference's
-// double-precision decision, exactly (state.cpp:190-201).
-// ---------------------------------------------------------------------------
-template <bool DIM3, bool PERIODIC, bool MASKED, bool COUNT>
-__global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__ insts,
-                                                     const Tile* __restrict__ tiles, int ntiles,
-                                                     int8_t* __restrict__ pool,
-                                                     const uint32_t* __restrict__ mpool,
-                                                     const uint32_t* __restrict__ tab,
-                                                     const uint64_t* __restrict__ sweeps,
-                                                     Stencil st, int colour, uint32_t s_off,
-                                                     uint32_t seed_lo, uint32_t seed_hi,
-                                                     unsigned long long* __restrict__ acc) {
-    const Tile tl = tiles[xcd_tile(blockIdx.x, ntiles)];
-    const DevInst& in = insts[tl.inst];
-    const uint32_t P = ((uint32_t)tl.e0 >> 1) + threadIdx.x;
-    int8_t* __restrict__ own = pool + in.base + (int64_t)colour * in.Sp;
-    const int8_t* __restrict__ oth = pool + in.base + (int64_t)(colour ^ 1) * in.Sp;
-    int nacc = 0;
-    if (2 * P < (uint32_t)in.S) {
-        const uint64_t sweep = sweeps[tl.inst] + s_off;
-        uint32_t w[4];
-        philox4x32_10(P, (uint32_t)colour, (uint32_t)sweep, (uint32_t)(sweep >> 32),
-                      seed_lo, seed_hi + in.inst_id, w);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const uint32_t e = 2 * P + j;
-            if (e >= (uint32_t)in.S) break;
-            const SitePos p = site_of(in, e, colour, DIM3);
-            bool present = p.i2 < in.n2;
-            if (MASKED && present) present = mask_bit(mpool, in, colour, e);
-            if (!present) continue;
-            const int s = own[e];
-            const int trial = ((w[2] >> j) & 1u) ? 1 : -1;
-            if (trial == s) { ++nacc; continue; }
-            int idx = 0;
-#pragma unroll
-            for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-                const int ne = nbr_elem<PERIODIC>(in, p, g);
-                const int sn = (PERIODIC || ne >= 0) ? (int)oth[ne] : 0;
-                idx += (sn + 1) * st.pow3[g];
-            }
-            idx = 2 * idx + (s > 0 ? 1 : 0);
-            if (w[j] <= tab[in.tab_base + idx]) {
-                own[e] = (int8_t)trial;
-                ++nacc;
-            }
-        }
-    }
-    if (COUNT) {
-        // per-wave sum of accepted attempts (0..2 per lane)
-        int v = nacc;
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc + tl.inst, (unsigned long long)v);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -625,64 +449,6 @@ __global__ __launch_bounds__(256) void k_build_weibull(int dim, int edge, const 
 // ---------------------------------------------------------------------------
 // Host-side launchers (declared in nmc_launch.h).
 // ---------------------------------------------------------------------------
-#define NMC_HEIS_CASE(D3, PER, MSK, UD, CN, CT)                                                     \
-    if (dim3 == D3 && periodic == PER && masked == MSK && use_d == UD && cone == CN && count == CT) \
-        k_sweep_heis<D3, PER, MSK, UD, CN, CT><<<ntiles, 256, 0, stream>>>(                         \
-            insts, tiles, ntiles, pool, mpool, sweeps, st, colour, s_off, seed_lo, seed_hi, acc);
-
-hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, bool use_d,
-                             bool cone, bool count, const DevInst* insts, const Tile* tiles,
-                             int ntiles, float* pool, const uint32_t* mpool, const uint64_t* sweeps,
-                             const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo,
-                             uint32_t seed_hi, unsigned long long* acc) {
-    if (ntiles <= 0) return hipSuccess;
-#define NMC_HEIS_B(D3, PER, MSK)                   \
-    NMC_HEIS_CASE(D3, PER, MSK, false, false, false) \
-    NMC_HEIS_CASE(D3, PER, MSK, false, false, true)  \
-    NMC_HEIS_CASE(D3, PER, MSK, false, true, false)  \
-    NMC_HEIS_CASE(D3, PER, MSK, false, true, true)   \
-    NMC_HEIS_CASE(D3, PER, MSK, true, false, false)  \
-    NMC_HEIS_CASE(D3, PER, MSK, true, false, true)   \
-    NMC_HEIS_CASE(D3, PER, MSK, true, true, false)   \
-    NMC_HEIS_CASE(D3, PER, MSK, true, true, true)
-    NMC_HEIS_B(true, true, false)
-    NMC_HEIS_B(true, false, false)
-    NMC_HEIS_B(true, false, true)
-    NMC_HEIS_B(false, true, false)
-    NMC_HEIS_B(false, false, false)
-    NMC_HEIS_B(false, false, true)
-    NMC_HEIS_B(true, true, true)
-    NMC_HEIS_B(false, true, true)
-#undef NMC_HEIS_B
-    return hipGetLastError();
-}
-
-#define NMC_ISING_CASE(D3, PER, MSK, CT)                                                 \
-    if (dim3 == D3 && periodic == PER && masked == MSK && count == CT)                   \
-        k_sweep_ising<D3, PER, MSK, CT><<<ntiles, 256, 0, stream>>>(                     \
-            insts, tiles, ntiles, pool, mpool, tab, sweeps, st, colour, s_off, seed_lo, \
-            seed_hi, acc);
-
-hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool masked,
-                              bool count, const DevInst* insts, const Tile* tiles, int ntiles,
-                              int8_t* pool, const uint32_t* mpool, const uint32_t* tab,
-                              const uint64_t* sweeps, const Stencil& st, int colour,
-                              uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
-                              unsigned long long* acc) {
-    if (ntiles <= 0) return hipSuccess;
-#define NMC_ISING_B(D3, PER, MSK) NMC_ISING_CASE(D3, PER, MSK, false) NMC_ISING_CASE(D3, PER, MSK, true)
-    NMC_ISING_B(true, true, false)
-    NMC_ISING_B(true, true, true)
-    NMC_ISING_B(true, false, false)
-    NMC_ISING_B(true, false, true)
-    NMC_ISING_B(false, true, false)
-    NMC_ISING_B(false, true, true)
-    NMC_ISING_B(false, false, false)
-    NMC_ISING_B(false, false, true)
-#undef NMC_ISING_B
-    return hipGetLastError();
-}
-
 hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising,
                           const DevInst* insts, const MTile* t, int ntiles, const void* pool,
                           const uint32_t* mpool, const Stencil& st, double* partials, int pstride,

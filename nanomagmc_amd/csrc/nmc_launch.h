@@ -17,6 +17,12 @@ hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool
                               uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                               unsigned long long* acc);
 
+hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool periodic, bool masked, bool use_d,
+                            bool cone, int grid, int block, size_t shm, const DevInst* insts, const int* list,
+                            int n, void* pool, const uint32_t* mpool, const uint32_t* tab, const uint64_t* sweeps,
+                            const Stencil& st, int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                            unsigned long long* acc);
+
 hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising,
                           const DevInst* insts, const MTile* tiles, int ntiles, const void* pool,
                           const uint32_t* mpool, const Stencil& st, double* partials, int pstride,

@@ -158,6 +158,10 @@ class Lattices:
         a, n = _ilist(inst)
         check(lib().nmc_sweep(self._h, _iptr(a), n, int(sweeps)))
 
+    def set_path(self, path):
+        """'auto' (LDS-resident sweeps where the lattice fits) or 'stream'."""
+        check(lib().nmc_set_path(self._h, {"auto": 0, "stream": 1}[path]))
+
     def accepts(self, inst=None):
         a, n = _ilist(inst)
         cnt = n if a is not None else self.ninst
@@ -191,6 +195,11 @@ class Lattices:
     def sample(self, slot, inst=None):
         a, n = _ilist(inst)
         check(lib().nmc_sample(self._h, _iptr(a), n, int(slot)))
+
+    def run_samples(self, nsamples, samp_sweeps, inst=None):
+        """nsamples x (samp_sweeps sweeps + sample) as one replayed HIP graph."""
+        a, n = _ilist(inst)
+        check(lib().nmc_run_samples(self._h, _iptr(a), n, int(nsamples), int(samp_sweeps)))
 
     def get_series(self, inst=None):
         a, n = _ilist(inst)

@@ -228,6 +228,62 @@ def test_checkerboard_trajectory_identical(orc, bloch3d, dim, edge, periodic, ma
         assert np.array_equal(lat.get_spins(i), s), "instance %d" % i
 
 
+@pytest.mark.parametrize("ising,dim,periodic,masked,trial", [
+    (True, 2, 1, False, "sphere"), (True, 3, 0, True, "sphere"),
+    (False, 2, 1, False, "sphere"), (False, 3, 0, True, "cone"), (False, 3, 1, False, "cone")])
+def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, periodic, masked, trial):
+    # the LDS-resident launch (whole sweeps in one workgroup) and the
+    # streaming launches (one per colour pass) draw the same numbers
+    rng = np.random.default_rng(31 + dim)
+    edge = 10 if dim == 3 else 24
+    bd, J, D = (orc.parse_bonds(bloch3d, 3, 1.0, 0.2) if dim == 3 else neel2d(0.2))
+    locs = None
+    if masked:
+        locs = box_locs(dim, edge)
+        locs = locs[rng.random(len(locs)) < 0.8]
+    lats = []
+    for path in ("auto", "stream"):
+        lat, f = make_pair(orc, dim, edge, periodic, bd, J, D, ising, use_d=not ising, locs=locs, trial=trial,
+                           ninst=3, seed=77)
+        lat.set_path(path)
+        lat.set_params([0.4, 1.0, 2.5], [0.0, 0.05, -0.1])
+        if trial == "cone":
+            lat.set_cone([0.5, 1.0, 2.0])
+        lat.randomize()
+        lat.equil(7)
+        lat.equil(300, inst=[1])
+        lats.append(lat)
+    for i in range(3):
+        assert np.array_equal(lats[0].get_spins(i), lats[1].get_spins(i)), i
+    assert np.array_equal(lats[0].accepts(), lats[1].accepts())
+
+
+@pytest.mark.parametrize("ising", [True, False])
+def test_graph_replays_match_eager_loop(orc, ising):
+    # the captured sample loop, replayed several times with eager work in
+    # between, gives exactly what the eager sweep + sample calls give
+    d, J, D = nn2d() if ising else neel2d(0.3)
+    series = []
+    for mode in ("eager", "graph"):
+        lat, f = make_pair(orc, 2, 32, 1, d, J, D, ising, use_d=not ising, ninst=2, seed=5)
+        lat.set_params([0.5, 1.2], [0.0, 0.05])
+        lat.randomize()
+        lat.series_reset(6)
+        out = []
+        for rep in range(3):
+            lat.equil(2)                     # eager work between replays
+            if mode == "eager":
+                for k in range(6):
+                    lat.equil(3)
+                    lat.sample(k)
+            else:
+                lat.run_samples(6, 3)
+            out.append(lat.get_series()[0].copy())
+        series.append(out)
+    for a, b in zip(*series):
+        assert np.array_equal(a, b)
+
+
 # ------------------------------------------------------------ statistics ----
 def test_ising_2d_energy_matches_kaufman(orc):
     # configs[0]-style lattice: 2-D square L=64 periodic, J=1, h=0
