@@ -1,15 +1,19 @@
 """Throughput of the Metropolis hot path on one MI355X.
 
-Workload (the lattice the headline metric is quoted on): 3-D simple-cubic
-classical Heisenberg, L = 64 periodic, J = 1, D = 0, fp32 unit-vector spins
-drawn uniformly on the sphere, h_z in {0, 0.05} x 32 temperatures 2.0 -> 0.8:
-64 lattice instances resident in HBM, swept together.  One step = one
-checkerboard sweep of every instance (64 * 64^3 = 16.8M attempted single-site
-updates) with cone-limited trial moves whose half-angles were adapted toward
-50% acceptance during warm-up and then frozen.
+Default workload (the lattice the headline metric is quoted on): 3-D
+simple-cubic classical Heisenberg, L = 64 periodic, J = 1, D = 0, fp32
+unit-vector spins drawn uniformly on the sphere, h_z in {0, 0.05} x 32
+temperatures 2.0 -> 0.8: 64 lattice instances resident in HBM, swept
+together.  One step = one checkerboard sweep of every instance
+(64 * 64^3 = 16.8M attempted single-site updates) with cone-limited trial
+moves whose half-angles are adapted toward 50% acceptance during warm-up and
+then frozen.
 
-Prints one JSON line (see the README of the measurement contract in this
-repository's documentation).
+Other workloads (--workload): `grains` (512 Weibull spheres x 64 (T, h)
+points = 32768 uneven instances, Heisenberg + Bloch DMI, open boundaries),
+`ising2d` (2-D Ising L = 64 periodic, 21 temperatures).
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -23,34 +27,83 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-L = 64
-TEMPS = list(np.linspace(2.0, 0.8, 32))
-FIELDS = [0.0, 0.05]
-BYTES_PER_UPDATE = 36          # fp32 x 3: read as target, read as neighbour, written once
 HBM_PEAK = 8.0e12
+NN3 = [(0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)]
 
 
-def make_lattices(trial="cone"):
+def heis3d(trial):
+    """64 instances of the 64^3 periodic Heisenberg lattice (32 T x 2 h)."""
     from nanomagmc_amd import Bond, Lattices
-    bonds = [Bond((0, 0, 1), 1.0), Bond((0, 0, -1), 1.0), Bond((0, 1, 0), 1.0),
-             Bond((0, -1, 0), 1.0), Bond((1, 0, 0), 1.0), Bond((-1, 0, 0), 1.0)]
-    insts = [{"n": (L, L, L), "edge": L} for _ in range(len(TEMPS) * len(FIELDS))]
+    L = 64
+    temps = list(np.linspace(2.0, 0.8, 32))
+    fields = [0.0, 0.05]
+    bonds = [Bond(d, 1.0) for d in NN3]
+    insts = [{"n": (L, L, L), "edge": L} for _ in range(len(temps) * len(fields))]
     lat = Lattices(3, False, True, bonds, insts, use_j=True, use_d=False, trial=trial, seed=0x4E4D4302)
-    betas, hs = [], []
-    for h in FIELDS:
-        for T in TEMPS:
-            betas.append(1.0 / T)
-            hs.append(h)
+    betas = [1.0 / T for h in fields for T in temps]
+    hs = [h for h in fields for T in temps]
     lat.set_params(betas, hs)
     lat.randomize()
-    return lat
+    sites = [L ** 3] * lat.ninst
+    desc = ("3-D cubic Heisenberg L=64 periodic, J=1, D=0, h_z in {0,0.05} x 32 T 2.0->0.8 = 64 instances, "
+            "one checkerboard sweep per step, %s trials" % trial)
+    return lat, sites, 36, desc
 
 
-def adapt_cone(lat, theta, sweeps):
+def grains(trial):
+    """512 Weibull spheres (LATTSHAPE x, open): radius ~ log-normal (mean 10,
+    sd 2.5, truncated to [4, 20]), Weibull factor 8, J=1, Bloch D=0.2,
+    h_z in {0,0.02,0.05,0.1} x 16 T 2.0->0.1 = 32768 instances."""
+    from nanomagmc_amd import Bond, Lattices, build_weibull, crop_to_box
+    ngr, edge, wb = 512, 45, 8.0
+    am, asd = 10.0, 2.5
+    lmean = math.log(am * am / math.sqrt(asd * asd + am * am))
+    lsd = math.sqrt(math.log(1 + asd * asd / (am * am)))
+    rng = np.random.default_rng(0x4E4D4304)
+    sizes = np.clip(rng.lognormal(lmean, lsd, ngr), 4.0, 20.0)
+    masks = build_weibull(3, edge, sizes, wb, seed=0x4E4D4304)
+    temps = list(np.linspace(2.0, 0.1, 16))
+    fields = [0.0, 0.02, 0.05, 0.1]
+    insts, sites, betas, hs = [], [], [], []
+    for g in range(ngr):
+        n, lo, sub = crop_to_box(masks[g], 3, edge)
+        nk = int(sub.sum())
+        for h in fields:
+            for T in temps:
+                insts.append({"n": tuple(n), "origin": tuple(lo), "edge": edge, "mask": sub})
+                sites.append(nk)
+                betas.append(1.0 / T)
+                hs.append(h)
+    bonds = [Bond(d, 1.0, tuple(0.2 * x for x in d)) for d in NN3]
+    lat = Lattices(3, False, False, bonds, insts, use_j=True, use_d=True, trial=trial, seed=0x4E4D4304)
+    lat.set_params(betas, hs)
+    lat.randomize()
+    desc = ("512 Weibull grains (radius log-normal 10+-2.5 in [4,20], factor 8, open) x 4 h x 16 T = "
+            "32768 instances, Heisenberg J=1 + Bloch D=0.2, one sweep of every instance per step, %s trials"
+            % trial)
+    return lat, sites, 36, desc
+
+
+def ising2d(trial):
+    from nanomagmc_amd import Bond, Lattices
+    L = 64
+    temps = list(np.linspace(3.5, 1.5, 21))
+    bonds = [Bond((0, 1), 1.0), Bond((0, -1), 1.0), Bond((1, 0), 1.0), Bond((-1, 0), 1.0)]
+    lat = Lattices(2, True, True, bonds, [{"n": (L, L), "edge": L}] * len(temps), seed=0x4E4D4301)
+    lat.set_params([1.0 / T for T in temps], 0.0)
+    lat.randomize()
+    desc = "2-D square Ising L=64 periodic, J=1, h=0, 21 T 3.5->1.5, one checkerboard sweep per step"
+    return lat, [L * L] * lat.ninst, 3, desc
+
+
+WORKLOADS = {"heis3d": heis3d, "grains": grains, "ising2d": ising2d}
+
+
+def adapt_cone(lat, theta, sweeps, sites):
     """One adaptation round: sweep, read acceptance, scale half-angles."""
     lat.reset_accepts()
     lat.equil(sweeps)
-    acc = lat.accepts().astype(np.float64) / (sweeps * L ** 3)
+    acc = lat.accepts().astype(np.float64) / (sweeps * np.asarray(sites, dtype=np.float64))
     theta = np.clip(theta * np.clip(acc / 0.5, 0.5, 2.0), 0.02, math.pi)
     lat.set_cone(theta.astype(np.float32))
     return theta, acc
@@ -64,6 +117,7 @@ def cpu_baseline(budget_s=12.0):
     from oracle import oracle as orc
     orc.build()
     from tests.fixtures import box_locs
+    L = 64
     d, J, D = orc.nn_bonds(3)
     f = orc.Field(box_locs(3, L), 3, L, 1, d, J, D)
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -80,7 +134,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--trial", default="cone", choices=["cone", "sphere"])
+    ap.add_argument("--workload", default="heis3d", choices=sorted(WORKLOADS))
+    ap.add_argument("--trial", default=None, choices=["cone", "sphere"])
+    ap.add_argument("--tune", action="store_true", help="autotune the sweep path before timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -88,17 +144,24 @@ def main():
               "build does not launch multi-GPU runs (use --gpus 1).", file=sys.stderr)
         sys.exit(2)
     import torch
-    lat = make_lattices(args.trial)
-    sites = lat.ninst * L ** 3
-    # warm-up: adapt cone half-angles toward 50% acceptance, then freeze them
+    trial = args.trial or ("cone" if args.workload == "heis3d" else "sphere")
+    t_setup = time.perf_counter()
+    lat, sites, bpu, desc = WORKLOADS[args.workload](trial)
+    setup_s = time.perf_counter() - t_setup
+    total_sites = int(np.sum(sites))
+    tuned = None
+    if args.tune:
+        from nanomagmc_amd.autotune import tune
+        tuned, _ = tune(lat, "%s/%s/%d" % (args.workload, trial, lat.ninst))
+    # warm-up; cone half-angles adapt toward 50% acceptance, then freeze
     theta = np.full(lat.ninst, 1.0)
-    if args.trial == "cone":
+    if trial == "cone":
         lat.set_cone(theta.astype(np.float32))
     done = 0
     while done < args.warmup:
         k = min(10, args.warmup - done)
-        if args.trial == "cone":
-            theta, acc = adapt_cone(lat, theta, k)
+        if trial == "cone":
+            theta, _ = adapt_cone(lat, theta, k, sites)
         else:
             lat.equil(k)
         done += k
@@ -108,18 +171,20 @@ def main():
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
-    lat.equil(args.steps)              # 2 * steps colour-pass launches + 1 counter bump
+    lat.equil(args.steps)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_s = ev0.elapsed_time(ev1) / 1e3
-    acc = lat.accepts().astype(np.float64).sum() / (args.steps * sites)
-    updates = args.steps * sites
+    acc = lat.accepts().astype(np.float64).sum() / (args.steps * total_sites)
+    updates = args.steps * total_sites
     value = updates / wall
-    launch_s = gpu_s / (2 * args.steps)
-    achieved = (sites / 2) * BYTES_PER_UPDATE / launch_s
+    # dominant kernel: one colour pass when streaming (2 launches per sweep)
+    launches = 2 * args.steps
+    launch_s = gpu_s / launches
+    achieved = (total_sites / 2) * bpu / launch_s
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_heis3d_L64x64.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     out = {
@@ -133,16 +198,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (spins uniform on S^2 from Philox, seed 0x4E4D4302)",
-        "config": {"workload": "3-D cubic Heisenberg L=64 periodic, J=1, D=0, h_z in {0,0.05} x 32 T "
-                               "2.0->0.8 = 64 instances, one checkerboard sweep per step, %s trials"
-                               % args.trial,
-                   "instances": lat.ninst, "sites": sites, "acceptance": acc},
+        "dtype": "f32" if bpu == 36 else "int8",
+        "data": "synthetic (spins from Philox, seeds 0x4E4D43xx)",
+        "config": {"workload": desc, "instances": lat.ninst, "sites": total_sites, "acceptance": acc,
+                   "setup_s": setup_s, "tuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k_sweep_heis (one colour pass: %d updates, %d B/update algorithmic)"
-                               % (sites // 2, BYTES_PER_UPDATE),
+                     "kernel": "sweep colour pass (%d updates, %d B/update algorithmic)" % (total_sites // 2, bpu),
                      "avg_launch_us": launch_s * 1e6},
     }
     if not args.no_cpu_baseline:

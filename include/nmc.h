@@ -112,6 +112,9 @@ void nmc_destroy(nmc_ctx* ctx);
  * reference counterpart: the reference has no device. */
 int nmc_get_stream(nmc_ctx* ctx, void** stream);
 
+/* Wait for all work enqueued on the context's stream. */
+int nmc_sync(nmc_ctx* ctx);
+
 /* Number of present sites of an instance (state::num_spins, state.cpp:248). */
 int nmc_num_sites(nmc_ctx* ctx, int inst, int64_t* out);
 /* Number of topological-charge slices per instance (calc_TC's tcsize:
@@ -143,6 +146,10 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps);
 #define NMC_PATH_AUTO 0
 #define NMC_PATH_STREAM 1
 int nmc_set_path(nmc_ctx* ctx, int path);
+/* Work-items per LDS-resident workgroup (64..1024, multiple of 64); 0 picks
+ * one from the largest instance.  Set by the autotuner; results do not
+ * depend on it. */
+int nmc_set_lds_block(nmc_ctx* ctx, int block);
 
 /* Accepted-attempt counters since the last reset (one per instance). */
 int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out);

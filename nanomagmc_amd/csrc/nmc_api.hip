@@ -64,6 +64,8 @@ struct Set {
     int* d_list = nullptr;
     Tile* d_sweep = nullptr;
     int n_sweep = 0;
+    std::vector<Tile> h_sweep;                // host copy (tile -> instance)
+    unsigned long long* d_tacc = nullptr;     // accepted attempts per streaming tile
     Tile* d_init = nullptr;
     int n_init = 0;
     MTile* d_mt = nullptr;
@@ -78,6 +80,8 @@ struct Set {
     void release() {
         (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
         (void)hipFree(d_mt); (void)hipFree(d_tb); (void)hipFree(d_te); (void)hipFree(d_lds);
+        (void)hipFree(d_tacc);
+        d_tacc = nullptr;
         d_list = nullptr; d_sweep = nullptr; d_init = nullptr; d_mt = nullptr;
         d_tb = d_te = d_lds = nullptr;
     }
@@ -109,6 +113,7 @@ struct nmc_ctx {
     std::vector<double> h_dtab;
     uint64_t* d_sweeps = nullptr;
     unsigned long long* d_acc = nullptr;
+    std::vector<uint64_t> acc_folded;   // counts of released work lists
     double* d_partials = nullptr;
     size_t partials_cap = 0;
     double* d_obs = nullptr;
@@ -126,6 +131,7 @@ struct nmc_ctx {
     std::map<std::pair<double, double>, std::pair<std::vector<uint32_t>, std::vector<double>>> tab_cache;
 
     int path = NMC_PATH_AUTO;
+    int lds_block = 0;        // 0 = derived from the largest LDS-resident instance
     size_t max_lds = 65536;   // dynamic LDS one workgroup may use on this device
     // instantiated sample-loop graphs keyed by (instance list, nsamples, sweeps)
     std::map<std::tuple<std::vector<int>, int, int>, hipGraphExec_t> graphs;
@@ -249,8 +255,13 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     if ((rc = upload(c, &s->d_te, te))) return rc;
     if ((rc = upload(c, &s->d_lds, lds))) return rc;
     s->n_lds = (int)lds.size();
-    s->lds_block = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (lds_work + 63) / 64 * 64));
+    s->lds_block = c->lds_block > 0 ? c->lds_block
+                                    : (int)std::min<int64_t>(1024, std::max<int64_t>(64, (lds_work + 63) / 64 * 64));
     s->n_sweep = (int)sw.size();
+    s->h_sweep = sw;
+    if ((rc = dalloc(&s->d_tacc, sw.size()))) return rc;
+    NMC_HIP(hipMemsetAsync(s->d_tacc, 0, std::max<size_t>(1, sw.size()) * sizeof(unsigned long long), c->stream));
+    NMC_HIP(hipStreamSynchronize(c->stream));
     s->n_init = (int)ini.size();
     s->n_mt = (int)mt.size();
     *out = s.get();
@@ -506,6 +517,12 @@ int nmc_get_stream(nmc_ctx* ctx, void** stream) {
     return NMC_OK;
 }
 
+int nmc_sync(nmc_ctx* ctx) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    return NMC_OK;
+}
+
 int nmc_num_sites(nmc_ctx* ctx, int inst, int64_t* out) {
     if (!ctx || !out) return fail(NMC_E_INVALID, "null argument");
     if (int rc = check_inst(ctx, inst)) return rc;
@@ -576,6 +593,25 @@ int nmc_randomize(nmc_ctx* ctx, const int* inst, int n) {
 }
 
 namespace {
+// Move the per-tile accept counters of every cached work list into
+// acc_folded, then drop the lists (their device memory is freed).
+int release_sets(nmc_ctx* ctx) {
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graphs();
+    if (ctx->acc_folded.size() != ctx->din.size()) ctx->acc_folded.assign(ctx->din.size(), 0);
+    for (auto& kv : ctx->sets) {
+        Set& s = *kv.second;
+        if (!s.h_sweep.empty()) {
+            std::vector<uint64_t> t(s.h_sweep.size());
+            NMC_HIP(hipMemcpy(t.data(), s.d_tacc, t.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < t.size(); ++k) ctx->acc_folded[s.h_sweep[k].inst] += t[k];
+        }
+        s.release();
+    }
+    ctx->sets.clear();
+    return NMC_OK;
+}
+
 // Enqueue `nsweeps` sweeps of a prepared set (launches only: no allocation,
 // no synchronisation, so it can be recorded into a graph).
 int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
@@ -592,11 +628,11 @@ int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
             if (ctx->ising)
                 NMC_HIP(launch_sweep_ising(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, true, ctx->d_inst,
                                            s->d_sweep, s->n_sweep, (int8_t*)ctx->pool, ctx->mpool, ctx->d_tab,
-                                           ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, ctx->d_acc));
+                                           ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc));
             else
                 NMC_HIP(launch_sweep_heis(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d, ctx->cone,
                                           true, ctx->d_inst, s->d_sweep, s->n_sweep, (float*)ctx->pool, ctx->mpool,
-                                          ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, ctx->d_acc));
+                                          ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc));
         }
     if (nsweeps > 0)
         NMC_HIP(launch_bump_sweeps(ctx->stream, ctx->d_sweeps, s->d_list, (int)s->list.size(), (uint64_t)nsweeps));
@@ -616,19 +652,35 @@ int nmc_set_path(nmc_ctx* ctx, int path) {
     if (!ctx) return fail(NMC_E_INVALID, "null context");
     if (path != NMC_PATH_AUTO && path != NMC_PATH_STREAM) return fail(NMC_E_INVALID, "unknown sweep path");
     if (path == ctx->path) return NMC_OK;
-    NMC_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->drop_graphs();
-    for (auto& kv : ctx->sets) kv.second->release();
-    ctx->sets.clear();
+    if (int rc = release_sets(ctx)) return rc;
     ctx->path = path;
+    return NMC_OK;
+}
+
+int nmc_set_lds_block(nmc_ctx* ctx, int block) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (block != 0 && (block < 64 || block > 1024 || block % 64 != 0))
+        return fail(NMC_E_INVALID, "LDS block must be 0 or a multiple of 64 in [64, 1024]");
+    if (block == ctx->lds_block) return NMC_OK;
+    if (int rc = release_sets(ctx)) return rc;
+    ctx->lds_block = block;
     return NMC_OK;
 }
 
 int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out) {
     if (!ctx || !out) return fail(NMC_E_INVALID, "null argument");
+    // LDS-resident sweeps count per instance, streaming sweeps per tile
     std::vector<uint64_t> all(ctx->din.size());
     NMC_HIP(hipMemcpyAsync(all.data(), ctx->d_acc, all.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     NMC_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& kv : ctx->sets) {
+        const Set& s = *kv.second;
+        if (s.h_sweep.empty()) continue;
+        std::vector<uint64_t> t(s.h_sweep.size());
+        NMC_HIP(hipMemcpy(t.data(), s.d_tacc, t.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < t.size(); ++k) all[s.h_sweep[k].inst] += t[k];
+    }
+    for (size_t i = 0; i < ctx->acc_folded.size(); ++i) all[i] += ctx->acc_folded[i];
     const int cnt = inst ? n : (int)all.size();
     for (int j = 0; j < cnt; ++j) {
         const int i = inst ? inst[j] : j;
@@ -641,6 +693,11 @@ int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out) {
 int nmc_reset_accepts(nmc_ctx* ctx) {
     if (!ctx) return fail(NMC_E_INVALID, "null context");
     NMC_HIP(hipMemsetAsync(ctx->d_acc, 0, ctx->din.size() * sizeof(unsigned long long), ctx->stream));
+    std::fill(ctx->acc_folded.begin(), ctx->acc_folded.end(), 0);
+    for (auto& kv : ctx->sets)
+        if (!kv.second->h_sweep.empty())
+            NMC_HIP(hipMemsetAsync(kv.second->d_tacc, 0, kv.second->h_sweep.size() * sizeof(unsigned long long),
+                                   ctx->stream));
     return NMC_OK;
 }
 

@@ -28,6 +28,21 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
     return x * per + (x < rem ? x : rem) + j;
 }
 
+// Accepted-attempt count of a 256-work-item tile, added to that tile's own
+// counter.  Only this workgroup writes slot t within a launch and launches
+// are stream-ordered, so a plain read-modify-write suffices: no device-scope
+// atomics, which on MI355X are resolved beyond the per-XCD L2 and serialise
+// when thousands of workgroups hit one address.
+__device__ __forceinline__ void tile_count(int wave_count, unsigned long long* __restrict__ tile_acc, int t) {
+    __shared__ int part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = wave_count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int s = part[0] + part[1] + part[2] + part[3];
+        if (s) tile_acc[t] += (unsigned long long)s;
+    }
+}
+
 __device__ __forceinline__ bool mask_bit_s(const uint32_t* __restrict__ mp, const DevInst& in, int c, uint32_t e) {
     return (mp[in.mask_base + c * in.mask_stride + (e >> 5)] >> (e & 31)) & 1u;
 }
@@ -157,7 +172,8 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
                                                     Stencil st, int colour, uint32_t s_off,
                                                     uint32_t seed_lo, uint32_t seed_hi,
                                                     unsigned long long* __restrict__ acc) {
-    const Tile tl = tiles[xcd_tile(blockIdx.x, ntiles)];
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const Tile tl = tiles[t];
     const DevInst& in = insts[tl.inst];
     const uint32_t e = (uint32_t)tl.e0 + threadIdx.x;
     const int64_t Sp = in.Sp;
@@ -175,10 +191,7 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
                 in, st, own_x, own_x + Sp, own_x + 2 * Sp, oth_x, oth_x + Sp, oth_x + 2 * Sp, e, colour, p,
                 sweeps[tl.inst] + s_off, seed_lo, seed_hi);
     }
-    if (COUNT) {
-        const unsigned long long m = __ballot(accepted);
-        if ((threadIdx.x & 63) == 0 && m) atomicAdd(acc + tl.inst, (unsigned long long)__popcll(m));
-    }
+    if (COUNT) tile_count(__popcll(__ballot(accepted)), acc, t);
 }
 
 template <bool DIM3, bool PERIODIC, bool MASKED, bool COUNT>
@@ -191,7 +204,8 @@ __global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__
                                                      Stencil st, int colour, uint32_t s_off,
                                                      uint32_t seed_lo, uint32_t seed_hi,
                                                      unsigned long long* __restrict__ acc) {
-    const Tile tl = tiles[xcd_tile(blockIdx.x, ntiles)];
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const Tile tl = tiles[t];
     const DevInst& in = insts[tl.inst];
     const uint32_t P = ((uint32_t)tl.e0 >> 1) + threadIdx.x;
     int8_t* own = pool + in.base + (int64_t)colour * in.Sp;
@@ -203,7 +217,7 @@ __global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__
     if (COUNT) {
         int v = nacc;   // per-wave sum of accepted attempts (0..2 per lane)
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc + tl.inst, (unsigned long long)v);
+        tile_count(v, acc, t);
     }
 }
 
@@ -262,8 +276,16 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
         uint4* gdst = (uint4*)((unsigned char*)pool + in.base * (ISING ? 1 : 4));
         const uint4* lsrc = (const uint4*)lds_raw;
         for (int64_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) gdst[i] = lsrc[i];
+        // accepted attempts: this workgroup is the only writer of acc[inst]
+        __shared__ int part[16];
         for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
-        if ((threadIdx.x & 63) == 0 && nacc) atomicAdd(acc + inst, (unsigned long long)nacc);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = nacc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += part[w];
+            if (s) acc[inst] += (unsigned long long)s;
+        }
         __syncthreads();
     }
 }

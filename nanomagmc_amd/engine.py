@@ -133,6 +133,12 @@ class Lattices:
         check(lib().nmc_get_stream(self._h, C.byref(p)))
         return p.value or 0
 
+    def sync(self):
+        check(lib().nmc_sync(self._h))
+
+    def set_lds_block(self, block):
+        check(lib().nmc_set_lds_block(self._h, int(block)))
+
     def num_sites(self, i):
         n = C.c_int64()
         check(lib().nmc_num_sites(self._h, int(i), C.byref(n)))

@@ -33,12 +33,16 @@ def h5_shapes(path):
     return shapes
 
 
-def h5_read(path, name):
-    out = subprocess.run([H5DUMP, "-d", name, "-y", "-w", "0", "-o", "/dev/stdout", path], capture_output=True,
-                         text=True, check=True).stdout
-    body = out.split("DATA {", 1)[1]
-    nums = re.findall(r"-?\d+(?:\.\d+)?(?:e[-+]?\d+)?|nan|inf", body.rsplit("}", 2)[0])
-    return np.array([float(x) for x in nums])
+def h5_read(path, name, dtype=None):
+    """Raw little-endian dump of one dataset (h5dump -b LE)."""
+    import tempfile
+    with tempfile.NamedTemporaryFile(suffix=".bin") as tmp:
+        subprocess.run([H5DUMP, "-d", name, "-b", "LE", "-o", tmp.name, path], capture_output=True, text=True,
+                       check=True)
+        raw = open(tmp.name, "rb").read()
+    if dtype is None:
+        dtype = np.int32 if name.endswith("Complete") else np.float32
+    return np.frombuffer(raw, dtype=dtype).astype(np.float64)
 
 
 def make_case(tmp_path, **keys):
