@@ -47,7 +47,7 @@ int dalloc(T** p, size_t count) {
     return NMC_OK;
 }
 
-constexpr int kSweepTile = 256;   // Heisenberg: one site per work-item
+constexpr int kSweepTile = 1024;  // Heisenberg: four sites (one quad) per work-item
 constexpr int kIsingTile = 512;   // Ising: two sites per work-item
 
 // Host view of one instance (reference coordinates) next to its DevInst.
@@ -235,7 +235,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         if (c->path == NMC_PATH_AUTO && bytes <= c->max_lds) {
             lds.push_back(i);
             s->lds_bytes = std::max(s->lds_bytes, bytes);
-            lds_work = std::max<int64_t>(lds_work, c->ising ? (d.S + 1) / 2 : d.S);
+            lds_work = std::max<int64_t>(lds_work, c->ising ? (d.S + 1) / 2 : (d.S + 3) / 4);
         } else {
             for (int64_t e = 0; e < d.S; e += stile) sw.push_back(Tile{i, (int)e});
         }

@@ -47,41 +47,37 @@ __device__ __forceinline__ bool mask_bit_s(const uint32_t* __restrict__ mp, cons
     return (mp[in.mask_base + c * in.mask_stride + (e >> 5)] >> (e & 31)) & 1u;
 }
 
-// One Heisenberg single-site update of element e (colour `colour`, position
-// p): local field of the six stencil neighbours (+ s_j x D_j), trial spin
-// from Philox, dE_func, Metropolis test, conditional store.  Returns whether
-// the trial spin was stored.
-template <bool DIM3, bool PERIODIC, bool USE_D, bool CONE>
-__device__ __forceinline__ bool heis_update(const DevInst& in, const Stencil& st, float* own_x, float* own_y,
-                                            float* own_z, const float* oth_x, const float* oth_y,
-                                            const float* oth_z, uint32_t e, int colour, const SitePos& p,
-                                            uint64_t sweep, uint32_t seed_lo, uint32_t seed_hi) {
-    const float sx = ld32(own_x, e), sy = ld32(own_y, e), sz = ld32(own_z, e);
-    float vx = 0.f, vy = 0.f, vz = 0.f;
-#pragma unroll
-    for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-        const int ne = nbr_elem<PERIODIC>(in, p, g);
-        float nx = 0.f, ny = 0.f, nz = 0.f;
-        if (PERIODIC || ne >= 0) {
-            const uint32_t u = (uint32_t)ne;
-            nx = ld32(oth_x, u); ny = ld32(oth_y, u); nz = ld32(oth_z, u);
-        }
-        vx += st.J[g] * nx;
-        vy += st.J[g] * ny;
-        vz += st.J[g] * nz;
-        if (USE_D) {   // s_j x D_j, the orientation dE_func uses (thermodynamics.cpp:45-46)
-            vx += ny * st.D[g][2] - nz * st.D[g][1];
-            vy += nz * st.D[g][0] - nx * st.D[g][2];
-            vz += nx * st.D[g][1] - ny * st.D[g][0];
-        }
+// Heisenberg updates are done four colour elements at a time (a "quad",
+// e0 = 4q .. 4q+3): three Philox4x32-10 blocks with counters 3q, 3q+1, 3q+2
+// give the 12 words the quad needs, element m taking words 3m, 3m+1
+// (trial) and 3m+2 (acceptance uniform).
+
+// Local field of stencil neighbour n added to v: J n (+ n x D, the
+// orientation dE_func uses, thermodynamics.cpp:45-46).
+template <bool USE_D>
+__device__ __forceinline__ void add_bond(const Stencil& st, int g, float nx, float ny, float nz, float& vx, float& vy,
+                                         float& vz) {
+    vx += st.J[g] * nx;
+    vy += st.J[g] * ny;
+    vz += st.J[g] * nz;
+    if (USE_D) {
+        vx += ny * st.D[g][2] - nz * st.D[g][1];
+        vy += nz * st.D[g][0] - nx * st.D[g][2];
+        vz += nx * st.D[g][1] - ny * st.D[g][0];
     }
-    uint32_t w[4];
-    philox4x32_10(e, (uint32_t)colour, (uint32_t)sweep, (uint32_t)(sweep >> 32), seed_lo, seed_hi + in.inst_id, w);
-    const float ua = u24_open(w[0]);
-    const float uphi = u24(w[1]);
+}
+
+// Trial spin, dE_func and the Metropolis test for one site with current spin
+// s and local field v, from Philox words wa (polar), wb (azimuth) and wc
+// (acceptance).  Writes the trial spin to t and returns the decision.
+template <bool CONE>
+__device__ __forceinline__ bool heis_decide(const DevInst& in, float sx, float sy, float sz, float vx, float vy,
+                                            float vz, uint32_t wa, uint32_t wb, uint32_t wc, float& tx, float& ty,
+                                            float& tz) {
+    const float ua = u24_open(wa);
+    const float uphi = u24(wb);
     const float cphi = __builtin_amdgcn_cosf(uphi);   // cos(2 pi u): v_cos_f32 takes revolutions
     const float sphi = __builtin_amdgcn_sinf(uphi);
-    float tx, ty, tz;
     if (!CONE) {
         // uniform direction on S^2: z uniform in (-1, 1), azimuth uniform
         tz = 2.f * ua - 1.f;
@@ -109,13 +105,135 @@ __device__ __forceinline__ bool heis_update(const DevInst& in, const Stencil& st
     }
     // dE_func: (H + Vsum) . (s_old - s_trial), H = (0, 0, h)
     const float dE = vx * (sx - tx) + vy * (sy - ty) + (in.h + vz) * (sz - tz);
-    const bool accepted = (dE <= 0.f) || (u24(w[2]) < __expf(-in.beta * dE));
-    if (accepted) {
-        st32(own_x, e, tx);
-        st32(own_y, e, ty);
-        st32(own_z, e, tz);
+    return (dE <= 0.f) || (u24(wc) < __expf(-in.beta * dE));
+}
+
+template <class T>
+__device__ __forceinline__ float4 ld4(const T* base, uint32_t idx) {
+    return *(const float4*)((const char*)base + idx * 4u);
+}
+__device__ __forceinline__ void st4(float* base, uint32_t idx, float4 v) {
+    *(float4*)((char*)base + idx * 4u) = v;
+}
+__device__ __forceinline__ float f4(const float4& v, int j) {
+    return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+}
+
+// Updates of the quad starting at colour element e0 (a multiple of 4).
+// Vector form when the quad lies in one row (half % 4 == 0): 16-byte loads
+// of the own quad, of the rows above/below and of the same row in the other
+// colour (plus one element past the quad's end for the +/-axis-2 bond), and
+// 16-byte stores of the quad.  Otherwise one site at a time.  Returns the
+// number of accepted attempts.
+template <bool DIM3, bool PERIODIC, bool MASKED, bool USE_D, bool CONE>
+__device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, const uint32_t* __restrict__ mpool,
+                                         float* own_x, float* own_y, float* own_z, const float* oth_x,
+                                         const float* oth_y, const float* oth_z, uint32_t e0, int colour,
+                                         uint64_t sweep, uint32_t seed_lo, uint32_t seed_hi) {
+    uint32_t w[12];
+    {
+        const uint32_t q3 = (e0 >> 2) * 3u;
+        const uint32_t slo = (uint32_t)sweep, shi = (uint32_t)(sweep >> 32), k1 = seed_hi + in.inst_id;
+        philox4x32_10(q3, (uint32_t)colour, slo, shi, seed_lo, k1, w);
+        philox4x32_10(q3 + 1, (uint32_t)colour, slo, shi, seed_lo, k1, w + 4);
+        philox4x32_10(q3 + 2, (uint32_t)colour, slo, shi, seed_lo, k1, w + 8);
     }
-    return accepted;
+    const uint32_t S = (uint32_t)in.S;
+    int nacc = 0;
+    if ((in.half & 3) == 0 && e0 + 3 < S) {
+        const SitePos p = site_of(in, e0, colour, DIM3);   // the whole quad shares its row
+        const float4 sx = ld4(own_x, e0), sy = ld4(own_y, e0), sz = ld4(own_z, e0);
+        float vx[4] = {0.f, 0.f, 0.f, 0.f}, vy[4] = {0.f, 0.f, 0.f, 0.f}, vz[4] = {0.f, 0.f, 0.f, 0.f};
+        // rows: slots 0/1 (3-D axis 0) and 2/3 (axis 1); same k range
+#pragma unroll
+        for (int g = DIM3 ? 0 : 2; g < 4; ++g) {
+            const int ne = nbr_elem<PERIODIC>(in, p, g);
+            float4 nx = make_float4(0.f, 0.f, 0.f, 0.f), ny = nx, nz = nx;
+            if (PERIODIC || ne >= 0) {
+                nx = ld4(oth_x, (uint32_t)ne); ny = ld4(oth_y, (uint32_t)ne); nz = ld4(oth_z, (uint32_t)ne);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) add_bond<USE_D>(st, g, f4(nx, j), f4(ny, j), f4(nz, j), vx[j], vy[j], vz[j]);
+        }
+        // axis 2 within the row: element k-1+q (minus) and k+q (plus)
+        {
+            const uint32_t rowbase = e0 - (uint32_t)p.k;   // element of k = 0 in this row
+            const float4 ax = ld4(oth_x, e0), ay = ld4(oth_y, e0), az = ld4(oth_z, e0);
+            float ex = 0.f, ey = 0.f, ez = 0.f;   // the element just outside the quad
+            if (p.q == 0) {
+                // minus neighbour of j = 0 is element k0 - 1
+                if (p.k > 0) { ex = ld32(oth_x, e0 - 1); ey = ld32(oth_y, e0 - 1); ez = ld32(oth_z, e0 - 1); }
+                else if (PERIODIC) {
+                    const uint32_t u = rowbase + (uint32_t)in.half - 1;
+                    ex = ld32(oth_x, u); ey = ld32(oth_y, u); ez = ld32(oth_z, u);
+                }
+            } else {
+                // plus neighbour of j = 3 is element k0 + 4
+                if (p.k + 4 < in.half) { ex = ld32(oth_x, e0 + 4); ey = ld32(oth_y, e0 + 4); ez = ld32(oth_z, e0 + 4); }
+                else if (PERIODIC) { ex = ld32(oth_x, rowbase); ey = ld32(oth_y, rowbase); ez = ld32(oth_z, rowbase); }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float mx, my, mz, qx, qy, qz;
+                if (p.q == 0) {
+                    mx = j ? f4(ax, j - 1) : ex; my = j ? f4(ay, j - 1) : ey; mz = j ? f4(az, j - 1) : ez;
+                    qx = f4(ax, j); qy = f4(ay, j); qz = f4(az, j);
+                } else {
+                    mx = f4(ax, j); my = f4(ay, j); mz = f4(az, j);
+                    qx = j < 3 ? f4(ax, j + 1) : ex; qy = j < 3 ? f4(ay, j + 1) : ey; qz = j < 3 ? f4(az, j + 1) : ez;
+                }
+                add_bond<USE_D>(st, 4, mx, my, mz, vx[j], vy[j], vz[j]);
+                add_bond<USE_D>(st, 5, qx, qy, qz, vx[j], vy[j], vz[j]);
+            }
+        }
+        float ox[4], oy[4], oz[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float cx = f4(sx, j), cy = f4(sy, j), cz = f4(sz, j);
+            ox[j] = cx; oy[j] = cy; oz[j] = cz;
+            const int i2 = p.i2 + 2 * j;
+            bool present = i2 < in.n2;
+            if (MASKED && present) present = mask_bit_s(mpool, in, colour, e0 + j);
+            if (!present) continue;
+            float tx, ty, tz;
+            if (heis_decide<CONE>(in, cx, cy, cz, vx[j], vy[j], vz[j], w[3 * j], w[3 * j + 1], w[3 * j + 2], tx, ty,
+                                  tz)) {
+                ox[j] = tx; oy[j] = ty; oz[j] = tz;
+                ++nacc;
+            }
+        }
+        st4(own_x, e0, make_float4(ox[0], ox[1], ox[2], ox[3]));
+        st4(own_y, e0, make_float4(oy[0], oy[1], oy[2], oy[3]));
+        st4(own_z, e0, make_float4(oz[0], oz[1], oz[2], oz[3]));
+        return nacc;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t e = e0 + j;
+        if (e >= S) break;
+        const SitePos p = site_of(in, e, colour, DIM3);
+        bool present = p.i2 < in.n2;
+        if (MASKED && present) present = mask_bit_s(mpool, in, colour, e);
+        if (!present) continue;
+        const float sx = ld32(own_x, e), sy = ld32(own_y, e), sz = ld32(own_z, e);
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+        for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+            const int ne = nbr_elem<PERIODIC>(in, p, g);
+            if (PERIODIC || ne >= 0) {
+                const uint32_t u = (uint32_t)ne;
+                add_bond<USE_D>(st, g, ld32(oth_x, u), ld32(oth_y, u), ld32(oth_z, u), vx, vy, vz);
+            }
+        }
+        float tx, ty, tz;
+        if (heis_decide<CONE>(in, sx, sy, sz, vx, vy, vz, w[3 * j], w[3 * j + 1], w[3 * j + 2], tx, ty, tz)) {
+            st32(own_x, e, tx);
+            st32(own_y, e, ty);
+            st32(own_z, e, tz);
+            ++nacc;
+        }
+    }
+    return nacc;
 }
 
 // Two Ising updates (elements 2P and 2P+1 of one colour) from one Philox
@@ -175,23 +293,21 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
     const int t = xcd_tile(blockIdx.x, ntiles);
     const Tile tl = tiles[t];
     const DevInst& in = insts[tl.inst];
-    const uint32_t e = (uint32_t)tl.e0 + threadIdx.x;
+    const uint32_t e0 = (uint32_t)tl.e0 + 4u * threadIdx.x;   // this work-item's quad
     const int64_t Sp = in.Sp;
     // per-component base pointers are wave-uniform (scalar registers); every
     // access is base + 32-bit element offset
     float* own_x = pool + in.base + (int64_t)colour * 3 * Sp;
     const float* oth_x = pool + in.base + (int64_t)(colour ^ 1) * 3 * Sp;
-    bool accepted = false;
-    if (e < (uint32_t)in.S) {
-        const SitePos p = site_of(in, e, colour, DIM3);
-        bool present = p.i2 < in.n2;
-        if (MASKED && present) present = mask_bit_s(mpool, in, colour, e);
-        if (present)
-            accepted = heis_update<DIM3, PERIODIC, USE_D, CONE>(
-                in, st, own_x, own_x + Sp, own_x + 2 * Sp, oth_x, oth_x + Sp, oth_x + 2 * Sp, e, colour, p,
-                sweeps[tl.inst] + s_off, seed_lo, seed_hi);
+    int nacc = 0;
+    if (e0 < (uint32_t)in.S)
+        nacc = heis_quad<DIM3, PERIODIC, MASKED, USE_D, CONE>(in, st, mpool, own_x, own_x + Sp, own_x + 2 * Sp,
+                                                              oth_x, oth_x + Sp, oth_x + 2 * Sp, e0, colour,
+                                                              sweeps[tl.inst] + s_off, seed_lo, seed_hi);
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        tile_count(nacc, acc, t);
     }
-    if (COUNT) tile_count(__popcll(__ballot(accepted)), acc, t);
 }
 
 template <bool DIM3, bool PERIODIC, bool MASKED, bool COUNT>
@@ -258,16 +374,11 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
                 } else {
                     float* own = (float*)lds_raw + c * 3 * Sp;
                     const float* oth = (const float*)lds_raw + (c ^ 1) * 3 * Sp;
-                    for (uint32_t e = threadIdx.x; e < (uint32_t)in.S; e += blockDim.x) {
-                        const SitePos p = site_of(in, e, c, DIM3);
-                        bool present = p.i2 < in.n2;
-                        if (MASKED && present) present = mask_bit_s(mpool, in, c, e);
-                        if (present)
-                            nacc += heis_update<DIM3, PERIODIC, USE_D, CONE>(in, st, own, own + Sp, own + 2 * Sp, oth,
-                                                                            oth + Sp, oth + 2 * Sp, e, c, p,
-                                                                            sweep0 + s, seed_lo, seed_hi)
-                                        ? 1 : 0;
-                    }
+                    for (uint32_t e0 = 4u * threadIdx.x; e0 < (uint32_t)in.S; e0 += 4u * blockDim.x)
+                        nacc += heis_quad<DIM3, PERIODIC, MASKED, USE_D, CONE>(in, st, mpool, own, own + Sp,
+                                                                               own + 2 * Sp, oth, oth + Sp,
+                                                                               oth + 2 * Sp, e0, c, sweep0 + s,
+                                                                               seed_lo, seed_hi);
                 }
                 __syncthreads();
             }
