@@ -138,7 +138,18 @@ def main():
     ap.add_argument("--trial", default=None, choices=["cone", "sphere"])
     ap.add_argument("--tune", action="store_true", help="autotune the sweep path before timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 256 MiB (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine runs independent chains per GPU and this "
               "build does not launch multi-GPU runs (use --gpus 1).", file=sys.stderr)
@@ -152,7 +163,8 @@ def main():
     tuned = None
     if args.tune:
         from nanomagmc_amd.autotune import tune
-        tuned, _ = tune(lat, "%s/%s/%d" % (args.workload, trial, lat.ninst))
+        choice, timings = tune(lat, "%s/%s/%d" % (args.workload, trial, lat.ninst))
+        tuned = {"choice": list(choice), "seconds_per_candidate": timings}
     # warm-up; cone half-angles adapt toward 50% acceptance, then freeze
     theta = np.full(lat.ninst, 1.0)
     if trial == "cone":

@@ -47,3 +47,16 @@ def test_crop_to_box():
     m[2:4, 1:5] = 1
     n, lo, sub = crop_to_box(m.reshape(-1), 2, 6)
     assert n == [2, 4] and lo == [2, 1] and sub.sum() == 8
+
+
+def test_device_index_math_on_host(tmp_path):
+    """Fast division and the XCD block->tile permutation (compiled for the
+    host from the device header)."""
+    import subprocess
+    exe = str(tmp_path / "index_check")
+    src = os.path.join(ROOT, "tests", "index_check.cpp")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", src, "-o", exe], check=True,
+                   capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    assert "fdiv_bad=0 xcd_bad=0" in r.stdout
