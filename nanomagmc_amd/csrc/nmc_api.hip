@@ -231,7 +231,9 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     int64_t lds_work = 0;
     for (int i : list) {
         const DevInst& d = c->din[i];
-        const size_t bytes = (size_t)(c->ising ? 2 * d.Sp : 6 * d.Sp * (int64_t)sizeof(float));
+        // both colour arrays, then (masked lattices) both colours' presence bits
+        const size_t bytes = (size_t)(c->ising ? 2 * d.Sp : 6 * d.Sp * (int64_t)sizeof(float)) +
+                             (c->masked ? (size_t)(2 * d.mask_stride * 4 + 15) / 16 * 16 : 0);
         if (c->path == NMC_PATH_AUTO && bytes <= c->max_lds) {
             lds.push_back(i);
             s->lds_bytes = std::max(s->lds_bytes, bytes);

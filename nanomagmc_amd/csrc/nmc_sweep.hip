@@ -6,9 +6,9 @@
 //
 // A sweep is two colour passes.  Two launch forms share one update routine:
 //  * streaming (k_sweep_heis / k_sweep_ising): one launch per colour pass over
-//    tiles of every instance; each work-item owns one site (Ising: two),
-//    reads its neighbours from the other colour array in HBM/L2 (coalesced),
-//    and stores the site only if the move is accepted;
+//    tiles of every instance; each work-item owns four consecutive sites of
+//    the colour (Ising: two), reads their neighbours from the other colour
+//    array in HBM/L2 with coalesced 16-byte loads, and writes them back;
 //  * LDS-resident (k_sweep_lds): one persistent workgroup per instance keeps
 //    both colour arrays in LDS for `nsweeps` sweeps, with a workgroup barrier
 //    between colour passes; HBM is touched only to load and store the lattice.
@@ -43,8 +43,10 @@ __device__ __forceinline__ void tile_count(int wave_count, unsigned long long* _
     }
 }
 
-__device__ __forceinline__ bool mask_bit_s(const uint32_t* __restrict__ mp, const DevInst& in, int c, uint32_t e) {
-    return (mp[in.mask_base + c * in.mask_stride + (e >> 5)] >> (e & 31)) & 1u;
+// Presence bit of colour element e; `mw` points at the instance's mask words
+// (colour 0 then colour 1, mask_stride words each), in HBM or in LDS.
+__device__ __forceinline__ bool mask_bit_s(const uint32_t* mw, const DevInst& in, int c, uint32_t e) {
+    return (mw[c * (uint32_t)in.mask_stride + (e >> 5)] >> (e & 31)) & 1u;
 }
 
 // Heisenberg updates are done four colour elements at a time (a "quad",
@@ -126,10 +128,15 @@ __device__ __forceinline__ float f4(const float4& v, int j) {
 // 16-byte stores of the quad.  Otherwise one site at a time.  Returns the
 // number of accepted attempts.
 template <bool DIM3, bool PERIODIC, bool MASKED, bool USE_D, bool CONE>
-__device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, const uint32_t* __restrict__ mpool,
+__device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, const uint32_t* mw,
                                          float* own_x, float* own_y, float* own_z, const float* oth_x,
                                          const float* oth_y, const float* oth_z, uint32_t e0, int colour,
                                          uint64_t sweep, uint32_t seed_lo, uint32_t seed_hi) {
+    if (MASKED) {
+        // e0 % 4 == 0, so the quad's four presence bits share one mask word;
+        // a quad with no present site (outside a grain) costs nothing
+        if (((mw[colour * (uint32_t)in.mask_stride + (e0 >> 5)] >> (e0 & 31)) & 0xFu) == 0) return 0;
+    }
     uint32_t w[12];
     {
         const uint32_t q3 = (e0 >> 2) * 3u;
@@ -193,7 +200,7 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
             ox[j] = cx; oy[j] = cy; oz[j] = cz;
             const int i2 = p.i2 + 2 * j;
             bool present = i2 < in.n2;
-            if (MASKED && present) present = mask_bit_s(mpool, in, colour, e0 + j);
+            if (MASKED && present) present = mask_bit_s(mw, in, colour, e0 + j);
             if (!present) continue;
             float tx, ty, tz;
             if (heis_decide<CONE>(in, cx, cy, cz, vx[j], vy[j], vz[j], w[3 * j], w[3 * j + 1], w[3 * j + 2], tx, ty,
@@ -213,7 +220,7 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
         if (e >= S) break;
         const SitePos p = site_of(in, e, colour, DIM3);
         bool present = p.i2 < in.n2;
-        if (MASKED && present) present = mask_bit_s(mpool, in, colour, e);
+        if (MASKED && present) present = mask_bit_s(mw, in, colour, e);
         if (!present) continue;
         const float sx = ld32(own_x, e), sy = ld32(own_y, e), sz = ld32(own_z, e);
         float vx = 0.f, vy = 0.f, vz = 0.f;
@@ -245,7 +252,7 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
 // (state.cpp:190-201).  Returns the number of accepted attempts.
 template <bool DIM3, bool PERIODIC, bool MASKED>
 __device__ __forceinline__ int ising_update2(const DevInst& in, const Stencil& st, int8_t* own, const int8_t* oth,
-                                             const uint32_t* __restrict__ tab, const uint32_t* __restrict__ mpool,
+                                             const uint32_t* __restrict__ tab, const uint32_t* mw,
                                              uint32_t P, int colour, uint64_t sweep, uint32_t seed_lo,
                                              uint32_t seed_hi) {
     uint32_t w[4];
@@ -257,7 +264,7 @@ __device__ __forceinline__ int ising_update2(const DevInst& in, const Stencil& s
         if (e >= (uint32_t)in.S) break;
         const SitePos p = site_of(in, e, colour, DIM3);
         bool present = p.i2 < in.n2;
-        if (MASKED && present) present = mask_bit_s(mpool, in, colour, e);
+        if (MASKED && present) present = mask_bit_s(mw, in, colour, e);
         if (!present) continue;
         const int s = ld32(own, e);
         const int trial = ((w[2] >> j) & 1u) ? 1 : -1;
@@ -301,7 +308,7 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
     const float* oth_x = pool + in.base + (int64_t)(colour ^ 1) * 3 * Sp;
     int nacc = 0;
     if (e0 < (uint32_t)in.S)
-        nacc = heis_quad<DIM3, PERIODIC, MASKED, USE_D, CONE>(in, st, mpool, own_x, own_x + Sp, own_x + 2 * Sp,
+        nacc = heis_quad<DIM3, PERIODIC, MASKED, USE_D, CONE>(in, st, mpool + in.mask_base, own_x, own_x + Sp, own_x + 2 * Sp,
                                                               oth_x, oth_x + Sp, oth_x + 2 * Sp, e0, colour,
                                                               sweeps[tl.inst] + s_off, seed_lo, seed_hi);
     if (COUNT) {
@@ -328,7 +335,7 @@ __global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__
     const int8_t* oth = pool + in.base + (int64_t)(colour ^ 1) * in.Sp;
     int nacc = 0;
     if (2 * P < (uint32_t)in.S)
-        nacc = ising_update2<DIM3, PERIODIC, MASKED>(in, st, own, oth, tab, mpool, P, colour,
+        nacc = ising_update2<DIM3, PERIODIC, MASKED>(in, st, own, oth, tab, mpool + in.mask_base, P, colour,
                                                      sweeps[tl.inst] + s_off, seed_lo, seed_hi);
     if (COUNT) {
         int v = nacc;   // per-wave sum of accepted attempts (0..2 per lane)
@@ -360,6 +367,10 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
         const uint4* src = (const uint4*)((const unsigned char*)pool + in.base * (ISING ? 1 : 4));
         uint4* dst = (uint4*)lds_raw;
         for (int64_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) dst[i] = src[i];
+        // the presence bits of both colours follow the spins in LDS
+        uint32_t* lmask = (uint32_t*)(lds_raw + bytes);
+        if (MASKED)
+            for (int64_t i = threadIdx.x; i < 2 * in.mask_stride; i += blockDim.x) lmask[i] = mpool[in.mask_base + i];
         __syncthreads();
         const uint64_t sweep0 = sweeps[inst] + s_off;
         int nacc = 0;
@@ -369,13 +380,13 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
                     int8_t* own = (int8_t*)lds_raw + c * Sp;
                     const int8_t* oth = (const int8_t*)lds_raw + (c ^ 1) * Sp;
                     for (uint32_t P = threadIdx.x; 2 * P < (uint32_t)in.S; P += blockDim.x)
-                        nacc += ising_update2<DIM3, PERIODIC, MASKED>(in, st, own, oth, tab, mpool, P, c,
+                        nacc += ising_update2<DIM3, PERIODIC, MASKED>(in, st, own, oth, tab, lmask, P, c,
                                                                       sweep0 + s, seed_lo, seed_hi);
                 } else {
                     float* own = (float*)lds_raw + c * 3 * Sp;
                     const float* oth = (const float*)lds_raw + (c ^ 1) * 3 * Sp;
                     for (uint32_t e0 = 4u * threadIdx.x; e0 < (uint32_t)in.S; e0 += 4u * blockDim.x)
-                        nacc += heis_quad<DIM3, PERIODIC, MASKED, USE_D, CONE>(in, st, mpool, own, own + Sp,
+                        nacc += heis_quad<DIM3, PERIODIC, MASKED, USE_D, CONE>(in, st, lmask, own, own + Sp,
                                                                                own + 2 * Sp, oth, oth + Sp,
                                                                                oth + 2 * Sp, e0, c, sweep0 + s,
                                                                                seed_lo, seed_hi);
