@@ -426,7 +426,13 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
         int i0n, i1n, i2n;
         to_internal(c->dim3, h.n[0], h.n[1], h.n[2], i0n, i1n, i2n);
         if (!c->dim3) { i0n = 1; i1n = h.n[0]; i2n = h.n[1]; }
-        d.a0 = i0n; d.a1 = i1n; d.n2 = i2n; d.a2 = i2n + (i2n & 1); d.half = d.a2 / 2;
+        // the fast axis is padded (absent sites) to a multiple of 8 on open
+        // lattices, so every row holds whole 4-element quads of each colour
+        // and the sweep always takes its 16-byte vector path; periodic
+        // lattices keep their extent (the wrap must stay at the edge)
+        d.a0 = i0n; d.a1 = i1n; d.n2 = i2n;
+        d.a2 = c->periodic ? i2n : (i2n + 7) / 8 * 8;
+        d.half = d.a2 / 2;
         int o0, o1, o2;
         to_internal(c->dim3, h.origin[0], h.origin[1], h.origin[2], o0, o1, o2);
         d.o0 = o0; d.o1 = o1; d.o2 = o2;

@@ -4,8 +4,10 @@
 // Layout in HBM (one pool per context):
 //   Every instance is a box with internal extents (a0, a1, a2): 3-D lattices
 //   map reference (x, y, z) -> (a0, a1, a2); 2-D lattices map (x, y) ->
-//   (1, a1, a2).  a2 is the fastest axis and is even (padded with absent
-//   sites when the real extent n2 is odd).  Sites are split by colour
+//   (1, a1, a2).  a2 is the fastest axis: the (even) edge on periodic
+//   lattices, the real extent n2 padded with absent sites to a multiple of 8
+//   on open ones (so rows hold whole quads of 4 elements per colour).  Sites
+//   are split by colour
 //   c = (i0 + i1 + i2) & 1 ("checkerboard") and each colour is stored as its
 //   own array of S = a0 * a1 * a2 / 2 elements, element
 //   e = (i0 * a1 + i1) * half + (i2 >> 1), half = a2 / 2.  A colour pass then

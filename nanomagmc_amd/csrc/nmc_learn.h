@@ -1,0 +1,36 @@
+// nmc_learn.h -- launchers of the learned block-proposal kernels (nmc_learn.hip).
+#pragma once
+#include "nmc_device.h"
+
+namespace nmc {
+
+// Network shape: 24 x 24 patches (16 x 16 block + 4-site halo), four valid
+// 3x3 convolutions 16(4 used) -> 32 -> 32 -> 32 -> 32(4 used) channels.
+constexpr int kPatch = 24;
+constexpr int kBlock = 16;
+constexpr int kHalo = 4;
+constexpr int kCin0 = 16;
+constexpr int kNetLayers = 4;
+
+hipError_t launch_conv_fwd(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* W, const float* bias,
+                           int npatch, int relu, __bf16* Ybf, float* Yf);
+hipError_t launch_conv_dgrad(hipStream_t s, const __bf16* dY, int Hout, const __bf16* Wd, int Cin, int npatch,
+                             const __bf16* relu_in, __bf16* dX);
+hipError_t launch_conv_wgrad(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* dY, int npatch,
+                             int chunk, float* Pw, int* nchunks_out);
+hipError_t launch_grad_reduce(hipStream_t s, const float* Pw, int nchunks, int K, const __bf16* dY, int M, float* gW,
+                              float* gB);
+hipError_t launch_adam(hipStream_t s, float* w, float* mw, float* vw, const float* g, int K, int Cin, int nout,
+                       float lr, float b1, float b2, float eps, float bc1, float bc2, float* b, float* mb, float* vb,
+                       const float* gb, __bf16* W16, __bf16* Wd16);
+hipError_t launch_gather_patches(hipStream_t s, const float* pool, int64_t base, int64_t Sp, int nx, int ny,
+                                 const int* bxy, int npatch, __bf16* X, float* target);
+hipError_t launch_head_loss(hipStream_t s, const float* O, const float* target, int npatch, float scale, __bf16* dO,
+                            float* loss);
+hipError_t launch_block_mh(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
+                           int nblocks, const float* O, const Stencil& st, float beta, float h, uint32_t seed_lo,
+                           uint32_t seed_hi, uint32_t inst, uint64_t round, int* accepted, float* stats);
+hipError_t launch_copy_rows(hipStream_t s, const void* src, void* dst, const int* idx, int n, int64_t row_bytes);
+hipError_t launch_put_patches(hipStream_t s, const float* in, int n, __bf16* X);
+
+}  // namespace nmc

@@ -1,0 +1,508 @@
+// nmc_learn.hip -- learned block proposals for 2-D Heisenberg lattices.
+//
+// A 4-layer 3x3 convolutional network (32 channels, bf16 weights and
+// activations, fp32 accumulation on the gfx950 matrix cores) reads the
+// 24 x 24 patch around a 16 x 16 block -- the block's own spins masked out,
+// so the network sees only the block's frozen surroundings -- and outputs,
+// for every block site, a von Mises-Fisher distribution (mean direction mu,
+// concentration kappa) on S^2.  A proposal draws the whole block from the
+// product of these distributions; because it does not depend on the block's
+// current spins, the forward and reverse proposal densities come from the
+// same network evaluation, and the move is accepted by exact
+// Metropolis-Hastings on the sampler's energy E_s (the energy whose
+// differences Old/lib/thermodynamics.cpp:51-69 computes):
+//     A = min(1, exp(-beta dE) q(x_B | halo) / q(x'_B | halo)).
+// The stationary distribution is therefore the Boltzmann one whatever the
+// network has learned; training (maximum likelihood of blocks sampled by
+// the plain sweep) only changes how often moves are accepted.
+//
+// Convolutions are implicit GEMMs: one wave computes a 32 x 32 output tile
+// with v_mfma_f32_32x32x16_bf16, gathering its A and B operands (8 bf16 per
+// lane) straight from NHWC activations and pre-transposed weights.
+#include "nmc_device.h"
+#include "nmc_learn.h"
+
+namespace nmc {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) z[i] = (__bf16)0.f;
+    return z;
+}
+
+// ---------------------------------------------------------------------------
+// Convolution GEMMs.
+//   fwd  : Y[m, n]  = sum_k X(m, k) W[n][k]          m = (p, oy, ox), k = (dy, dx, ci)
+//   dgrad: dX[m, ci] = sum_k dY(m, k) Wd[ci][k]       m = (p, iy, ix), k = (dy, dx, co)
+//   wgrad: dW[k, n] = sum_m X(m, k) dY[m, n]          reduction over m, split in chunks
+// Tiles: blockDim = 256 (4 waves); each wave owns one 32 x 32 tile.
+// MFMA 32x32x16 operand layout: lane l holds A[row = l % 32][k0 + 8 (l / 32) .. +7],
+// B[k0 + 8 (l / 32) .. +7][col = l % 32]; result element v of lane l is
+// C[8 (v / 4) + 4 (l / 32) + (v % 4)][l % 32].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv_fwd(const __bf16* __restrict__ X, int Hin, int Cin,
+                                                  const __bf16* __restrict__ W, const float* __restrict__ bias,
+                                                  int npatch, int relu, __bf16* __restrict__ Ybf,
+                                                  float* __restrict__ Yf) {
+    const int Hout = Hin - 2;
+    const int M = npatch * Hout * Hout;
+    const int K = 9 * Cin;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + wave;
+    const int m0 = tile * 32;
+    if (m0 >= M) return;
+    const int row = lane & 31, kh = (lane >> 5) * 8;
+    const int m = m0 + row;
+    const bool mv = m < M;
+    int p = 0, oy = 0, ox = 0;
+    if (mv) { p = m / (Hout * Hout); const int r = m - p * Hout * Hout; oy = r / Hout; ox = r - oy * Hout; }
+    const int n = lane & 31;   // output channel of this lane's B column
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const int k = k0 + kh;
+        const int tap = k / Cin, ci = k - tap * Cin;
+        const int dy = tap / 3, dx = tap - dy * 3;
+        bf16x8 a = zero8();
+        if (mv) a = *(const bf16x8*)(X + (((size_t)p * Hin + oy + dy) * Hin + ox + dx) * Cin + ci);
+        const bf16x8 b = *(const bf16x8*)(W + (size_t)n * K + k);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int mr = m0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (mr >= M) continue;
+        float y = acc[v] + bias[n];
+        if (relu) y = fmaxf(y, 0.f);
+        if (Ybf) Ybf[(size_t)mr * 32 + n] = (__bf16)y;
+        if (Yf) Yf[(size_t)mr * 32 + n] = y;
+    }
+}
+
+// dX (pre-activation of the layer below when relu_in != null) of a valid 3x3
+// convolution with 32 output channels; Wd[ci][tap][co] layout.
+__global__ __launch_bounds__(256) void k_conv_dgrad(const __bf16* __restrict__ dY, int Hout,
+                                                    const __bf16* __restrict__ Wd, int Cin, int npatch,
+                                                    const __bf16* __restrict__ relu_in, __bf16* __restrict__ dX) {
+    const int Hin = Hout + 2;
+    const int M = npatch * Hin * Hin;
+    const int K = 9 * 32;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ntiles_n = (Cin + 31) / 32;
+    const int tile = blockIdx.x * 4 + wave;
+    const int tm = tile / ntiles_n, tn = tile - tm * ntiles_n;
+    const int m0 = tm * 32;
+    if (m0 >= M) return;
+    const int row = lane & 31, kh = (lane >> 5) * 8;
+    const int m = m0 + row;
+    const bool mv = m < M;
+    int p = 0, iy = 0, ix = 0;
+    if (mv) { p = m / (Hin * Hin); const int r = m - p * Hin * Hin; iy = r / Hin; ix = r - iy * Hin; }
+    const int ci = tn * 32 + (lane & 31);
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const int k = k0 + kh;
+        const int tap = k >> 5, co = k & 31;
+        const int dy = tap / 3, dx = tap - dy * 3;
+        const int oy = iy - dy, ox = ix - dx;
+        bf16x8 a = zero8();
+        if (mv && oy >= 0 && oy < Hout && ox >= 0 && ox < Hout)
+            a = *(const bf16x8*)(dY + (((size_t)p * Hout + oy) * Hout + ox) * 32 + co);
+        bf16x8 b = zero8();
+        if (ci < Cin) b = *(const bf16x8*)(Wd + ((size_t)ci * 9 + tap) * 32 + co);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (ci >= Cin) return;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int mr = m0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (mr >= M) continue;
+        float g = acc[v];
+        if (relu_in && (float)relu_in[(size_t)mr * Cin + ci] <= 0.f) g = 0.f;   // through the ReLU
+        dX[(size_t)mr * Cin + ci] = (__bf16)g;
+    }
+}
+
+// Partial weight gradients: chunk c of the reduction over output pixels m
+// writes Pw[c][k][n] for one 32-row slice of k.
+__global__ __launch_bounds__(256) void k_conv_wgrad(const __bf16* __restrict__ X, int Hin, int Cin,
+                                                    const __bf16* __restrict__ dY, int npatch, int chunk,
+                                                    float* __restrict__ Pw) {
+    const int Hout = Hin - 2;
+    const int M = npatch * Hout * Hout;
+    const int K = 9 * Cin;
+    const int ktiles = (K + 31) / 32;   // K = 144 (first layer) leaves a half tile
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + wave;
+    const int c = tile / ktiles, tk = tile - c * ktiles;
+    const int mbeg = c * chunk;
+    if (mbeg >= M) return;
+    const int mend = min(M, mbeg + chunk);
+    const int krow = min(tk * 32 + (lane & 31), K - 1);   // rows >= K are computed, not stored
+    const int tap = krow / Cin, ci = krow - tap * Cin;
+    const int dy = tap / 3, dx = tap - dy * 3;
+    const int n = lane & 31;
+    const int mh = (lane >> 5) * 8;
+    f32x16 acc = {};
+    for (int mk = mbeg; mk < mend; mk += 16) {
+        bf16x8 a = zero8(), b = zero8();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int m = mk + mh + j;
+            if (m < mend) {
+                const int p = m / (Hout * Hout), r = m - p * Hout * Hout;
+                const int oy = r / Hout, ox = r - oy * Hout;
+                a[j] = X[(((size_t)p * Hin + oy + dy) * Hin + ox + dx) * Cin + ci];
+                b[j] = dY[(size_t)m * 32 + n];
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    float* out = Pw + (size_t)c * K * 32;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int kr = tk * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (kr < K) out[(size_t)kr * 32 + n] = acc[v];
+    }
+}
+
+// Sum the weight-gradient partials and the bias gradient (sum of dY over m).
+__global__ void k_grad_reduce(const float* __restrict__ Pw, int nchunks, int K, const __bf16* __restrict__ dY,
+                              int M, float* __restrict__ gW, float* __restrict__ gB) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K * 32) {
+        float s = 0.f;
+        for (int c = 0; c < nchunks; ++c) s += Pw[(size_t)c * K * 32 + i];
+        gW[i] = s;   // [k][n]
+    }
+    if (i < 32) {
+        float s = 0.f;
+        for (int m = 0; m < M; ++m) s += (float)dY[(size_t)m * 32 + i];
+        gB[i] = s;
+    }
+}
+
+// Adam on the fp32 master weights ([k][n] layout, columns >= nout frozen at
+// zero), then the two bf16 copies the GEMMs read: W[n][k] and Wd[ci][tap][n].
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ mw, float* __restrict__ vw,
+                       const float* __restrict__ g, int K, int Cin, int nout, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2, float* __restrict__ b, float* __restrict__ mb,
+                       float* __restrict__ vb, const float* __restrict__ gb, __bf16* __restrict__ W16,
+                       __bf16* __restrict__ Wd16) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K * 32) {
+        const int k = i >> 5, n = i & 31;
+        float x = 0.f;
+        if (n < nout) {
+            const float gi = g[i];
+            mw[i] = b1 * mw[i] + (1.f - b1) * gi;
+            vw[i] = b2 * vw[i] + (1.f - b2) * gi * gi;
+            x = w[i] - lr * (mw[i] / bc1) / (sqrtf(vw[i] / bc2) + eps);
+            w[i] = x;
+        }
+        W16[(size_t)n * K + k] = (__bf16)x;
+        const int tap = k / Cin, ci = k - tap * Cin;
+        Wd16[((size_t)ci * 9 + tap) * 32 + n] = (__bf16)x;
+    }
+    if (i < 32 && i < nout) {
+        const float gi = gb[i];
+        mb[i] = b1 * mb[i] + (1.f - b1) * gi;
+        vb[i] = b2 * vb[i] + (1.f - b2) * gi * gi;
+        b[i] = b[i] - lr * (mb[i] / bc1) / (sqrtf(vb[i] / bc2) + eps);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Patches, the vMF head and the Metropolis-Hastings block move (2-D).
+// ---------------------------------------------------------------------------
+struct Lat2 {   // one 2-D Heisenberg instance in the colour-split layout
+    const float* c0;   // colour 0, components x|y|z at stride Sp
+    const float* c1;
+    int64_t Sp;
+    int nx, ny, half;
+};
+
+__device__ __forceinline__ void spin_at(const Lat2& L, int x, int y, float& sx, float& sy, float& sz) {
+    x = (x % L.nx + L.nx) % L.nx;
+    y = (y % L.ny + L.ny) % L.ny;
+    const float* c = ((x + y) & 1) ? L.c1 : L.c0;
+    const int64_t e = (int64_t)x * L.half + (y >> 1);
+    sx = c[e]; sy = c[L.Sp + e]; sz = c[2 * L.Sp + e];
+}
+
+// Patch p of block (bx, by): 24 x 24 x 16 bf16 NHWC; channels 0..2 spins of
+// the halo (block interior zeroed), channel 3 = 1 on the halo, rest 0.
+// Optional target: the block's 16 x 16 x 3 spins.
+__global__ void k_gather_patches(const float* __restrict__ pool, int64_t base, int64_t Sp, int nx, int ny,
+                                 const int* __restrict__ bxy, int npatch, __bf16* __restrict__ X,
+                                 float* __restrict__ target) {
+    const int p = blockIdx.x;
+    if (p >= npatch) return;
+    Lat2 L{pool + base, pool + base + 3 * Sp, Sp, nx, ny, ny / 2};
+    const int bx = bxy[2 * p], by = bxy[2 * p + 1];
+    for (int t = threadIdx.x; t < 24 * 24; t += blockDim.x) {
+        const int u = t / 24, v = t - u * 24;   // patch row (x) and column (y)
+        float sx, sy, sz;
+        spin_at(L, bx - 4 + u, by - 4 + v, sx, sy, sz);
+        const bool inner = u >= 4 && u < 20 && v >= 4 && v < 20;
+        __bf16* o = X + ((size_t)p * 576 + t) * 16;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) o[c] = (__bf16)0.f;
+        if (!inner) { o[0] = (__bf16)sx; o[1] = (__bf16)sy; o[2] = (__bf16)sz; o[3] = (__bf16)1.f; }
+        else if (target) {
+            float* tg = target + ((size_t)p * 256 + (u - 4) * 16 + (v - 4)) * 3;
+            tg[0] = sx; tg[1] = sy; tg[2] = sz;
+        }
+    }
+}
+
+struct VMF {
+    float mx, my, mz, kappa, r;
+};
+
+__device__ __forceinline__ VMF vmf_of(const float* o) {
+    VMF f;
+    const float r = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-6f;
+    f.r = r;
+    f.mx = o[0] / r; f.my = o[1] / r; f.mz = o[2] / r;
+    const float z = o[3];
+    f.kappa = (z > 20.f ? z : log1pf(expf(z))) + 0.05f;   // softplus + floor
+    return f;
+}
+
+// log density of the vMF distribution on S^2: log(kappa / (4 pi sinh kappa)) + kappa mu.x
+__device__ __forceinline__ float vmf_logpdf(const VMF& f, float x, float y, float z) {
+    const float k = f.kappa;
+    const float lsinh = k + log1pf(-expf(-2.f * k)) - 0.69314718f;
+    return logf(k) - 2.5310242f - lsinh + k * (f.mx * x + f.my * y + f.mz * z);
+}
+
+// Training head: per block site, loss -log q(target) and its gradient wrt
+// the four network outputs; dO is zero for the padded channels.
+__global__ void k_head_loss(const float* __restrict__ O, const float* __restrict__ target, int npatch,
+                            float scale, __bf16* __restrict__ dO, float* __restrict__ loss) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // site over npatch * 256
+    if (i >= npatch * 256) return;
+    const float* o = O + (size_t)i * 32;
+    const VMF f = vmf_of(o);
+    const float x = target[3 * i], y = target[3 * i + 1], z = target[3 * i + 2];
+    const float mux = f.mx * x + f.my * y + f.mz * z;
+    const float L = -vmf_logpdf(f, x, y, z);
+    // dL/dkappa = -(1/k - coth k + mu.x); dL/dmu = -k x; dL/dmu_raw = (I - mu mu^T) dL/dmu / r
+    const float k = f.kappa;
+    const float e2 = expf(-2.f * k);
+    const float coth = (1.f + e2) / (1.f - e2);
+    const float dk = -(1.f / k - coth + mux);
+    const float gx = -k * x, gy = -k * y, gz = -k * z;
+    const float proj = f.mx * gx + f.my * gy + f.mz * gz;
+    const float z3 = o[3];
+    const float sig = 1.f / (1.f + expf(-z3));
+    __bf16* d = dO + (size_t)i * 32;
+    d[0] = (__bf16)(scale * (gx - f.mx * proj) / f.r);
+    d[1] = (__bf16)(scale * (gy - f.my * proj) / f.r);
+    d[2] = (__bf16)(scale * (gz - f.mz * proj) / f.r);
+    d[3] = (__bf16)(scale * dk * sig);
+    for (int c = 4; c < 32; ++c) d[c] = (__bf16)0.f;
+    loss[i] = L;
+}
+
+// One Metropolis-Hastings block move per workgroup (256 work-items = the 16 x
+// 16 block sites).  x' ~ prod vMF(mu_i, kappa_i); dE is the change of E_s
+// over every bond touching the block; the block is written back if accepted.
+__global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, int64_t base, int64_t Sp, int nx, int ny,
+                                                  const int* __restrict__ bxy, int nblocks,
+                                                  const float* __restrict__ O, Stencil st, float beta, float h,
+                                                  uint32_t seed_lo, uint32_t seed_hi, uint32_t inst, uint64_t round,
+                                                  int* __restrict__ accepted, float* __restrict__ stats) {
+    __shared__ float nx_s[256], ny_s[256], nz_s[256];
+    __shared__ float red[3][4];
+    __shared__ int acc_s;
+    const int b = blockIdx.x;
+    if (b >= nblocks) return;
+    const int t = threadIdx.x;
+    const int u = t >> 4, v = t & 15;
+    const int bx = bxy[2 * b], by = bxy[2 * b + 1];
+    float* c0 = pool + base;
+    float* c1 = pool + base + 3 * Sp;
+    Lat2 L{c0, c1, Sp, nx, ny, ny / 2};
+    const int X = bx + u, Y = by + v;
+    float sx, sy, sz;
+    spin_at(L, X, Y, sx, sy, sz);
+    const VMF f = vmf_of(O + ((size_t)b * 256 + t) * 32);
+    uint32_t w[4];
+    philox4x32_10((uint32_t)(b * 256 + t), (uint32_t)(kPurposeBlock << 8), (uint32_t)round,
+                  (uint32_t)(round >> 32), seed_lo, seed_hi + inst, w);
+    // sample the vMF: cos angle w = 1 + log(u + (1 - u) e^{-2k}) / k
+    const float k = f.kappa;
+    const float u1 = u24_open(w[0]);
+    const float cw = fminf(1.f, fmaxf(-1.f, 1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k));
+    const float sw = sqrtf(fmaxf(0.f, 1.f - cw * cw));
+    const float ph = u24(w[1]);
+    const float cp = __builtin_amdgcn_cosf(ph), sp = __builtin_amdgcn_sinf(ph);
+    const float sgn = f.mz >= 0.f ? 1.f : -1.f;
+    const float a = -1.f / (sgn + f.mz);
+    const float bb = f.mx * f.my * a;
+    const float e1x = 1.f + sgn * f.mx * f.mx * a, e1y = sgn * bb, e1z = -sgn * f.mx;
+    const float e2x = bb, e2y = sgn + f.my * f.my * a, e2z = -f.my;
+    float tx = cw * f.mx + sw * (cp * e1x + sp * e2x);
+    float ty = cw * f.my + sw * (cp * e1y + sp * e2y);
+    float tz = cw * f.mz + sw * (cp * e1z + sp * e2z);
+    const float inv = rsqrtf(tx * tx + ty * ty + tz * tz);
+    tx *= inv; ty *= inv; tz *= inv;
+    nx_s[t] = tx; ny_s[t] = ty; nz_s[t] = tz;
+    const float lq_new = vmf_logpdf(f, tx, ty, tz);
+    const float lq_old = vmf_logpdf(f, sx, sy, sz);
+    __syncthreads();
+    // dE_s: field term + every bond (i, j) with i in the block; a bond with
+    // j outside counts for both orientations, one inside for its half
+    float dE = -h * (tz - sz);
+    const int dxs[4] = {0, 0, 1, -1}, dys[4] = {1, -1, 0, 0};
+    const int gs[4] = {5, 4, 3, 2};   // stencil slots of (0,+1), (0,-1), (+1,0), (-1,0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int uu = u + dxs[q], vv = v + dys[q];
+        float ox_, oy_, oz_, nx_, ny_, nz_;
+        spin_at(L, X + dxs[q], Y + dys[q], ox_, oy_, oz_);
+        const bool in_b = uu >= 0 && uu < 16 && vv >= 0 && vv < 16;
+        if (in_b) { nx_ = nx_s[uu * 16 + vv]; ny_ = ny_s[uu * 16 + vv]; nz_ = nz_s[uu * 16 + vv]; }
+        else { nx_ = ox_; ny_ = oy_; nz_ = oz_; }
+        const int g = gs[q];
+        const float J = st.J[g];
+        const float Dx = st.D[g][0], Dy = st.D[g][1], Dz = st.D[g][2];
+        // e(si, sj) = -(J si.sj + si.(sj x D))
+        auto bond = [&](float ax, float ay, float az, float cx, float cy, float cz) {
+            const float crx = cy * Dz - cz * Dy, cry = cz * Dx - cx * Dz, crz = cx * Dy - cy * Dx;
+            return -(J * (ax * cx + ay * cy + az * cz) + ax * crx + ay * cry + az * crz);
+        };
+        const float d = bond(tx, ty, tz, nx_, ny_, nz_) - bond(sx, sy, sz, ox_, oy_, oz_);
+        dE += in_b ? 0.5f * d : d;
+    }
+    // block sums of dE, log q(x'), log q(x)
+    float vals[3] = {dE, lq_new, lq_old};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        float s = vals[j];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((t & 63) == 0) red[j][t >> 6] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const float DE = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const float LQN = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        const float LQO = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+        const float loga = -beta * DE + LQO - LQN;
+        const float ua = u24_open(w[2]);
+        acc_s = (loga >= 0.f || logf(ua) < loga) ? 1 : 0;
+        accepted[b] = acc_s;
+        if (stats) { stats[3 * b] = DE; stats[3 * b + 1] = LQN; stats[3 * b + 2] = LQO; }
+    }
+    __syncthreads();
+    if (acc_s) {
+        int xx = X % nx, yy = Y % ny;
+        float* c = ((xx + yy) & 1) ? c1 : c0;
+        const int64_t e = (int64_t)xx * (ny / 2) + (yy >> 1);
+        c[e] = tx; c[Sp + e] = ty; c[2 * Sp + e] = tz;
+    }
+}
+
+// Row gather (training minibatches): dst row i = src row idx[i]; rows are a
+// multiple of 16 bytes.
+__global__ void k_copy_rows(const uint4* __restrict__ src, uint4* __restrict__ dst, const int* __restrict__ idx,
+                            int n, int64_t row16) {
+    const int r = blockIdx.x;
+    if (r >= n) return;
+    const uint4* s = src + (int64_t)idx[r] * row16;
+    uint4* d = dst + (int64_t)r * row16;
+    for (int64_t i = threadIdx.x; i < row16; i += blockDim.x) d[i] = s[i];
+}
+
+// fp32 patches (n x 24 x 24 x 4: spins and halo flag) -> bf16 NHWC input.
+__global__ void k_put_patches(const float* __restrict__ in, int n, __bf16* __restrict__ X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // over n * 576 pixels
+    if (i >= n * 576) return;
+    __bf16* o = X + (size_t)i * 16;
+    for (int c = 0; c < 16; ++c) o[c] = (__bf16)(c < 4 ? in[(size_t)i * 4 + c] : 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+hipError_t launch_copy_rows(hipStream_t s, const void* src, void* dst, const int* idx, int n, int64_t row_bytes) {
+    if (n <= 0) return hipSuccess;
+    k_copy_rows<<<n, 256, 0, s>>>((const uint4*)src, (uint4*)dst, idx, n, row_bytes / 16);
+    return hipGetLastError();
+}
+
+hipError_t launch_put_patches(hipStream_t s, const float* in, int n, __bf16* X) {
+    if (n <= 0) return hipSuccess;
+    k_put_patches<<<(n * 576 + 255) / 256, 256, 0, s>>>(in, n, X);
+    return hipGetLastError();
+}
+hipError_t launch_conv_fwd(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* W, const float* bias,
+                           int npatch, int relu, __bf16* Ybf, float* Yf) {
+    const int Hout = Hin - 2;
+    const int tiles = (npatch * Hout * Hout + 31) / 32;
+    k_conv_fwd<<<(tiles + 3) / 4, 256, 0, s>>>(X, Hin, Cin, W, bias, npatch, relu, Ybf, Yf);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv_dgrad(hipStream_t s, const __bf16* dY, int Hout, const __bf16* Wd, int Cin, int npatch,
+                             const __bf16* relu_in, __bf16* dX) {
+    const int Hin = Hout + 2;
+    const int tiles = ((npatch * Hin * Hin + 31) / 32) * ((Cin + 31) / 32);
+    k_conv_dgrad<<<(tiles + 3) / 4, 256, 0, s>>>(dY, Hout, Wd, Cin, npatch, relu_in, dX);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv_wgrad(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* dY, int npatch,
+                             int chunk, float* Pw, int* nchunks_out) {
+    const int Hout = Hin - 2;
+    const int M = npatch * Hout * Hout;
+    const int nchunks = (M + chunk - 1) / chunk;
+    const int tiles = nchunks * ((9 * Cin + 31) / 32);
+    k_conv_wgrad<<<(tiles + 3) / 4, 256, 0, s>>>(X, Hin, Cin, dY, npatch, chunk, Pw);
+    *nchunks_out = nchunks;
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_reduce(hipStream_t s, const float* Pw, int nchunks, int K, const __bf16* dY, int M, float* gW,
+                              float* gB) {
+    k_grad_reduce<<<(K * 32 + 255) / 256, 256, 0, s>>>(Pw, nchunks, K, dY, M, gW, gB);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(hipStream_t s, float* w, float* mw, float* vw, const float* g, int K, int Cin, int nout,
+                       float lr, float b1, float b2, float eps, float bc1, float bc2, float* b, float* mb, float* vb,
+                       const float* gb, __bf16* W16, __bf16* Wd16) {
+    k_adam<<<(K * 32 + 255) / 256, 256, 0, s>>>(w, mw, vw, g, K, Cin, nout, lr, b1, b2, eps, bc1, bc2, b, mb, vb,
+                                                gb, W16, Wd16);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_patches(hipStream_t s, const float* pool, int64_t base, int64_t Sp, int nx, int ny,
+                                 const int* bxy, int npatch, __bf16* X, float* target) {
+    if (npatch <= 0) return hipSuccess;
+    k_gather_patches<<<npatch, 256, 0, s>>>(pool, base, Sp, nx, ny, bxy, npatch, X, target);
+    return hipGetLastError();
+}
+
+hipError_t launch_head_loss(hipStream_t s, const float* O, const float* target, int npatch, float scale, __bf16* dO,
+                            float* loss) {
+    const int n = npatch * 256;
+    k_head_loss<<<(n + 255) / 256, 256, 0, s>>>(O, target, npatch, scale, dO, loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_mh(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
+                           int nblocks, const float* O, const Stencil& st, float beta, float h, uint32_t seed_lo,
+                           uint32_t seed_hi, uint32_t inst, uint64_t round, int* accepted, float* stats) {
+    if (nblocks <= 0) return hipSuccess;
+    k_block_mh<<<nblocks, 256, 0, s>>>(pool, base, Sp, nx, ny, bxy, nblocks, O, st, beta, h, seed_lo, seed_hi, inst,
+                                       round, accepted, stats);
+    return hipGetLastError();
+}
+
+}  // namespace nmc

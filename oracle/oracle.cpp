@@ -368,7 +368,8 @@ void orc_equil_replay(int n, double* s, const int* neigh, const int* nchoice,
 //   u       = word[e & 1] * 2^-32,  trial = 2*bit(e & 1 of word[2]) - 1
 // with e = ((i0 * a1 + i1) * (a2 / 2) + (i2 >> 1)) in the instance box of
 // internal extents (a0, a1, a2) (2-D: (1, nx, ny), 3-D: (nx, ny, nz); a2
-// rounded up to even) and internal coordinates (i0, i1, i2).  site_e holds e
+// kept on periodic boxes, rounded up to a multiple of 8 on open ones) and
+// internal coordinates (i0, i1, i2).  site_e holds e
 // and site_c the colour of each site (computed by the caller).
 // ---------------------------------------------------------------------------
 void orc_checkerboard_ising(int n, double* s, const int* neigh, const int* nchoice,

@@ -204,7 +204,9 @@ class Field:
             i0 = np.zeros(self.n, dtype=np.int64)
             i1, i2 = loc[:, 0], loc[:, 1]
             a1, a2 = n_box[0], n_box[1]
-        a2 = a2 + (a2 & 1)
+        # periodic boxes keep their (even) extent; open boxes are padded to a
+        # multiple of 8 along the fast axis
+        a2 = a2 if self.periodic else (a2 + 7) // 8 * 8
         half = a2 // 2
         e = ((i0.astype(np.int64) * a1 + i1) * half + (i2 >> 1)).astype(np.int64)
         c = ((i0 + i1 + i2) & 1).astype(np.int32)
