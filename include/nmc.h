@@ -218,6 +218,48 @@ int nmc_replay_ising(nmc_ctx* ctx, int inst, const int* sites,
                      const int8_t* trial, const uint32_t* k, int n,
                      uint8_t* accept);
 
+/* ---- learned block proposals (no reference counterpart) ----
+ * For 2-D periodic Heisenberg lattices whose edges are multiples of 32.  A
+ * 4-layer 3x3 convolutional network (32 channels, bf16 on the matrix cores)
+ * reads the 24 x 24 patch around a 16 x 16 block with the block itself masked
+ * out and gives each block site a von Mises-Fisher distribution; a block move
+ * draws the block from it and is accepted by exact Metropolis-Hastings on the
+ * sampler's energy, so the stationary distribution is unchanged.  Training
+ * maximises the likelihood of blocks taken from configurations the plain
+ * sweep produced. */
+typedef struct nmc_net nmc_net;
+int nmc_net_create(uint64_t seed, nmc_net** out);
+void nmc_net_destroy(nmc_net* net);
+/* Number of trainable parameters; parameter vectors are, per layer l = 0..3,
+ * W_l as [9 * Cin_l][32] (row = (dy * 3 + dx) * Cin_l + ci, Cin = 16, 32,
+ * 32, 32; input channels 0..2 = spin of a halo site, 3 = 1 on the halo;
+ * layer 3 uses output columns 0..3 = mu (unnormalised) and the kappa
+ * pre-activation) followed by b_l[32]. */
+int nmc_net_num_params(int* out);
+int nmc_net_get_params(nmc_net* net, float* params);
+int nmc_net_set_params(nmc_net* net, const float* params);
+/* Append `nblocks` training blocks (random positions) taken from the current
+ * spins of instance `inst` of ctx. */
+int nmc_net_collect(nmc_net* net, nmc_ctx* ctx, int inst, int nblocks, uint64_t seed);
+int nmc_net_num_samples(nmc_net* net, int64_t* out);
+/* `nsteps` Adam steps on random minibatches of `batch` collected blocks;
+ * mean negative log-likelihood per site of the last minibatch in *loss. */
+int nmc_net_train(nmc_net* net, int nsteps, int batch, float lr, uint64_t seed, float* loss);
+/* Test mode: mean loss and its gradient (parameter order above) for n given
+ * patches (n x 24 x 24 x 4 floats: spin x, y, z and halo flag) and block
+ * targets (n x 256 x 3). */
+int nmc_net_loss_grad(nmc_net* net, const float* patches, const float* targets, int n, float* loss,
+                      float* grad);
+/* One round of block moves on each listed instance: blocks at
+ * (32 i + off_x, 32 j + off_y) (i, j over the 32 x 32 cells, so every block
+ * is 16 sites from the next one and the moves are independent); accepted[k]
+ * counts the accepted blocks of instance inst[k].  Test mode (both
+ * optional, blocks ordered by instance, then i, then j): stats gets
+ * (dE_s, log q(x'), log q(x)) per block, proposal the 16 x 16 x 3 proposed
+ * spins (row = x offset, column = y offset). */
+int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y,
+                    int64_t* accepted, float* stats, float* proposal);
+
 /* ---- lattice builder (state::init_lattice + shape::weibull::check) ---- */
 
 /* For each of `ngrains` grains, the edge^dim site mask of a Weibull grain

@@ -26,7 +26,9 @@ EXPORTS = [
     "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_set_path", "nmc_set_lds_block", "nmc_get_accepts",
     "nmc_reset_accepts", "nmc_snapshot", "nmc_measure", "nmc_series_reset", "nmc_sample",
     "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
-    "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_build_weibull",
+    "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_net_create", "nmc_net_destroy",
+    "nmc_net_num_params", "nmc_net_get_params", "nmc_net_set_params", "nmc_net_collect",
+    "nmc_net_num_samples", "nmc_net_train", "nmc_net_loss_grad", "nmc_block_moves", "nmc_build_weibull",
 ]
 
 
@@ -92,6 +94,16 @@ def lib():
         "nmc_delta_e": (i, [vp, i, P(i), P(d), i, P(d)]),
         "nmc_replay_ising": (i, [vp, i, P(i), P(C.c_int8), P(C.c_uint32), i, P(C.c_uint8)]),
         "nmc_build_weibull": (i, [i, i, P(d), i, d, u64, P(C.c_uint8)]),
+        "nmc_net_create": (i, [u64, P(vp)]),
+        "nmc_net_destroy": (None, [vp]),
+        "nmc_net_num_params": (i, [P(i)]),
+        "nmc_net_get_params": (i, [vp, P(C.c_float)]),
+        "nmc_net_set_params": (i, [vp, P(C.c_float)]),
+        "nmc_net_collect": (i, [vp, vp, i, i, u64]),
+        "nmc_net_num_samples": (i, [vp, P(C.c_int64)]),
+        "nmc_net_train": (i, [vp, i, i, C.c_float, u64, P(C.c_float)]),
+        "nmc_net_loss_grad": (i, [vp, P(C.c_float), P(C.c_float), i, P(C.c_float), P(C.c_float)]),
+        "nmc_block_moves": (i, [vp, vp, P(i), i, i, i, P(C.c_int64), P(C.c_float), P(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnmc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["nmc_sweep.hip", "nmc_kernels.hip", "nmc_api.hip"]
+SOURCES = ["nmc_sweep.hip", "nmc_kernels.hip", "nmc_learn.hip", "nmc_api.hip"]
 HIPFLAGS = [
     "--offload-arch=gfx950",
     "-O3",

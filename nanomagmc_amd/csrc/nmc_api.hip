@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <numeric>
+#include <random>
 #include <memory>
 #include <string>
 #include <vector>
@@ -15,6 +17,7 @@
 #include "../../include/nmc.h"
 #include "nmc_device.h"
 #include "nmc_launch.h"
+#include "nmc_learn.h"
 
 using namespace nmc;
 
@@ -135,6 +138,7 @@ struct nmc_ctx {
     size_t max_lds = 65536;   // dynamic LDS one workgroup may use on this device
     // instantiated sample-loop graphs keyed by (instance list, nsamples, sweeps)
     std::map<std::tuple<std::vector<int>, int, int>, hipGraphExec_t> graphs;
+    uint64_t block_round = 0;   // learned block-move rounds issued (RNG counter)
 
     void drop_graphs() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
@@ -1037,6 +1041,437 @@ int nmc_build_weibull(int dim, int edge, const double* size, int ngrains, double
     (void)hipFree(ds); (void)hipFree(dm);
     (void)hipStreamDestroy(st);
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned block proposals.
+// ---------------------------------------------------------------------------
+struct nmc_net {
+    static constexpr int kLayers = kNetLayers;
+    int cin[kLayers] = {kCin0, 32, 32, 32};
+    int hin[kLayers] = {24, 22, 20, 18};
+    int nout[kLayers] = {32, 32, 32, 4};
+    float* w[kLayers] = {};      // fp32 master weights [K][32]
+    float* b[kLayers] = {};
+    float* mw[kLayers] = {};
+    float* vw[kLayers] = {};
+    float* mb[kLayers] = {};
+    float* vb[kLayers] = {};
+    float* gw[kLayers] = {};
+    float* gb[kLayers] = {};
+    __bf16* w16[kLayers] = {};   // [32][K] (forward)
+    __bf16* wd16[kLayers] = {};  // [Cin][9][32] (input gradients)
+    int adam_t = 0;
+    // activations for up to `cap` patches
+    int cap = 0;
+    __bf16 *x0 = nullptr, *y1 = nullptr, *y2 = nullptr, *y3 = nullptr;
+    float* o = nullptr;
+    __bf16 *d_o = nullptr, *dy3 = nullptr, *dy2 = nullptr, *dy1 = nullptr;
+    float* tgt = nullptr;
+    float* loss = nullptr;
+    float* pw = nullptr;
+    size_t pw_cap = 0;
+    int* bxy = nullptr;
+    int bxy_cap = 0;
+    int* acc = nullptr;
+    int* idx = nullptr;
+    // collected training data
+    __bf16* data_x = nullptr;
+    float* data_t = nullptr;
+    int64_t ndata = 0, data_cap = 0;
+    hipStream_t stream = nullptr;
+    uint64_t seed = 0;
+
+    int K(int l) const { return 9 * cin[l]; }
+
+    ~nmc_net() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (int l = 0; l < kLayers; ++l) {
+            (void)hipFree(w[l]); (void)hipFree(b[l]); (void)hipFree(mw[l]); (void)hipFree(vw[l]);
+            (void)hipFree(mb[l]); (void)hipFree(vb[l]); (void)hipFree(gw[l]); (void)hipFree(gb[l]);
+            (void)hipFree(w16[l]); (void)hipFree(wd16[l]);
+        }
+        (void)hipFree(x0); (void)hipFree(y1); (void)hipFree(y2); (void)hipFree(y3); (void)hipFree(o);
+        (void)hipFree(d_o); (void)hipFree(dy3); (void)hipFree(dy2); (void)hipFree(dy1); (void)hipFree(tgt);
+        (void)hipFree(loss); (void)hipFree(pw); (void)hipFree(bxy); (void)hipFree(acc); (void)hipFree(idx);
+        (void)hipFree(data_x); (void)hipFree(data_t);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+constexpr int64_t kPatchElems = 24 * 24 * 16;   // bf16 per input patch
+constexpr int64_t kTargetElems = 256 * 3;       // fp32 per block target
+
+int net_reserve(nmc_net* net, int n) {
+    if (n <= net->cap) return NMC_OK;
+    NMC_HIP(hipStreamSynchronize(net->stream));
+    __bf16** bufs[] = {&net->x0, &net->y1, &net->y2, &net->y3, &net->d_o, &net->dy3, &net->dy2, &net->dy1};
+    const int64_t per[] = {kPatchElems, 22 * 22 * 32, 20 * 20 * 32, 18 * 18 * 32, 16 * 16 * 32, 18 * 18 * 32,
+                           20 * 20 * 32, 22 * 22 * 32};
+    for (int i = 0; i < 8; ++i) {
+        (void)hipFree(*bufs[i]);
+        *bufs[i] = nullptr;
+        if (int rc = dalloc(bufs[i], (size_t)per[i] * n)) return rc;
+    }
+    (void)hipFree(net->o); (void)hipFree(net->tgt); (void)hipFree(net->loss); (void)hipFree(net->bxy);
+    (void)hipFree(net->acc); (void)hipFree(net->idx);
+    net->o = net->tgt = net->loss = nullptr;
+    net->bxy = net->acc = net->idx = nullptr;
+    if (int rc = dalloc(&net->o, (size_t)256 * 32 * n)) return rc;
+    if (int rc = dalloc(&net->tgt, (size_t)kTargetElems * n)) return rc;
+    if (int rc = dalloc(&net->loss, (size_t)256 * n)) return rc;
+    if (int rc = dalloc(&net->bxy, (size_t)2 * n)) return rc;
+    if (int rc = dalloc(&net->acc, (size_t)n)) return rc;
+    if (int rc = dalloc(&net->idx, (size_t)n)) return rc;
+    net->cap = n;
+    return NMC_OK;
+}
+
+// Forward pass of n patches already in x0; output pre-head in o.
+int net_forward(nmc_net* net, int n) {
+    __bf16* in[] = {net->x0, net->y1, net->y2, net->y3};
+    __bf16* out[] = {net->y1, net->y2, net->y3, nullptr};
+    for (int l = 0; l < 4; ++l)
+        NMC_HIP(launch_conv_fwd(net->stream, in[l], net->hin[l], net->cin[l], net->w16[l], net->b[l], n, l < 3 ? 1 : 0,
+                                out[l], l == 3 ? net->o : nullptr));
+    return NMC_OK;
+}
+
+// Loss (mean over n * 256 sites) and gradients into gw/gb for n patches with
+// targets in tgt.
+int net_backward(nmc_net* net, int n) {
+    NMC_HIP(launch_head_loss(net->stream, net->o, net->tgt, n, 1.0f / (256.0f * n), net->d_o, net->loss));
+    __bf16* xin[] = {net->x0, net->y1, net->y2, net->y3};
+    __bf16* dy[] = {net->dy1, net->dy2, net->dy3, net->d_o};
+    const int chunk = 4096;
+    for (int l = 3; l >= 0; --l) {
+        const int Hout = net->hin[l] - 2;
+        const int M = n * Hout * Hout;
+        const size_t need = (size_t)((M + chunk - 1) / chunk) * net->K(l) * 32;
+        if (need > net->pw_cap) {
+            NMC_HIP(hipStreamSynchronize(net->stream));
+            (void)hipFree(net->pw);
+            net->pw = nullptr;
+            if (int rc = dalloc(&net->pw, need)) return rc;
+            net->pw_cap = need;
+        }
+        int nchunks = 0;
+        NMC_HIP(launch_conv_wgrad(net->stream, xin[l], net->hin[l], net->cin[l], dy[l], n, chunk, net->pw, &nchunks));
+        NMC_HIP(launch_grad_reduce(net->stream, net->pw, nchunks, net->K(l), dy[l], M, net->gw[l], net->gb[l]));
+        if (l > 0)   // gradient wrt this layer's input, through the previous layer's ReLU
+            NMC_HIP(launch_conv_dgrad(net->stream, dy[l], Hout, net->wd16[l], net->cin[l], n, xin[l], dy[l - 1]));
+    }
+    return NMC_OK;
+}
+
+int net_adam(nmc_net* net, float lr) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    net->adam_t += 1;
+    const float bc1 = 1.f - std::pow(b1, (float)net->adam_t), bc2 = 1.f - std::pow(b2, (float)net->adam_t);
+    for (int l = 0; l < 4; ++l)
+        NMC_HIP(launch_adam(net->stream, net->w[l], net->mw[l], net->vw[l], net->gw[l], net->K(l), net->cin[l],
+                            net->nout[l], lr, b1, b2, eps, bc1, bc2, net->b[l], net->mb[l], net->vb[l], net->gb[l],
+                            net->w16[l], net->wd16[l]));
+    return NMC_OK;
+}
+
+// Refresh the bf16 weight copies from the fp32 masters (Adam with lr = 0
+// leaves the masters unchanged and rewrites both copies).
+int net_sync_bf16(nmc_net* net) {
+    for (int l = 0; l < 4; ++l) {
+        const size_t KN = (size_t)net->K(l) * 32;
+        NMC_HIP(hipMemsetAsync(net->gw[l], 0, KN * sizeof(float), net->stream));
+        NMC_HIP(hipMemsetAsync(net->gb[l], 0, 32 * sizeof(float), net->stream));
+        NMC_HIP(launch_adam(net->stream, net->w[l], net->mw[l], net->vw[l], net->gw[l], net->K(l), net->cin[l],
+                            net->nout[l], 0.f, 1.f, 1.f, 1.f, 1.f, 1.f, net->b[l], net->mb[l], net->vb[l], net->gb[l],
+                            net->w16[l], net->wd16[l]));
+    }
+    return NMC_OK;
+}
+
+int check_learn_ctx(nmc_ctx* ctx, int inst) {
+    if (ctx->dim3 || ctx->ising || !ctx->periodic || ctx->masked)
+        return fail(NMC_E_UNSUPPORTED, "learned block moves need a 2-D periodic Heisenberg context");
+    const DevInst& d = ctx->din[inst];
+    if (d.a1 % 32 != 0 || d.a2 % 32 != 0)
+        return fail(NMC_E_UNSUPPORTED, "learned block moves need lattice edges that are multiples of 32");
+    // E_s block differences assume the reverse bond of every bond (J equal, D opposite)
+    for (int g = 2; g < 6; g += 2) {
+        if (ctx->st.J[g] != ctx->st.J[g + 1]) return fail(NMC_E_UNSUPPORTED, "bond table is not symmetric in J");
+        for (int a = 0; a < 3; ++a)
+            if (ctx->st.D[g][a] != -ctx->st.D[g + 1][a])
+                return fail(NMC_E_UNSUPPORTED, "bond table is not antisymmetric in D");
+    }
+    return NMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nmc_net_num_params(int* out) {
+    if (!out) return fail(NMC_E_INVALID, "null argument");
+    *out = (9 * kCin0 + 3 * 9 * 32) * 32 + 4 * 32;
+    return NMC_OK;
+}
+
+int nmc_net_create(uint64_t seed, nmc_net** out) {
+    if (!out) return fail(NMC_E_INVALID, "null argument");
+    *out = nullptr;
+    auto net = std::make_unique<nmc_net>();
+    net->seed = seed;
+    NMC_HIP(hipStreamCreate(&net->stream));
+    std::mt19937_64 rng(seed ^ 0x6E65746Bull);
+    for (int l = 0; l < 4; ++l) {
+        const size_t KN = (size_t)net->K(l) * 32;
+        float** f32s[] = {&net->w[l], &net->mw[l], &net->vw[l], &net->gw[l]};
+        for (float** p : f32s) {
+            if (int rc = dalloc(p, KN)) return rc;
+            NMC_HIP(hipMemsetAsync(*p, 0, KN * sizeof(float), net->stream));
+        }
+        float** bs[] = {&net->b[l], &net->mb[l], &net->vb[l], &net->gb[l]};
+        for (float** p : bs) {
+            if (int rc = dalloc(p, 32)) return rc;
+            NMC_HIP(hipMemsetAsync(*p, 0, 32 * sizeof(float), net->stream));
+        }
+        if (int rc = dalloc(&net->w16[l], KN)) return rc;
+        if (int rc = dalloc(&net->wd16[l], KN)) return rc;
+        // He-uniform initialisation of the used columns (fan-in = 9 * used inputs)
+        const int fan_in = 9 * (l == 0 ? 4 : 32);
+        const float lim = std::sqrt(6.0f / fan_in) * (l == 3 ? 0.1f : 1.0f);
+        std::uniform_real_distribution<float> U(-lim, lim);
+        std::vector<float> hw(KN, 0.f);
+        for (int k = 0; k < net->K(l); ++k) {
+            const int ci = k % net->cin[l];
+            if (l == 0 && ci >= 4) continue;
+            for (int n = 0; n < net->nout[l]; ++n) hw[(size_t)k * 32 + n] = U(rng);
+        }
+        NMC_HIP(hipMemcpyAsync(net->w[l], hw.data(), KN * sizeof(float), hipMemcpyHostToDevice, net->stream));
+        NMC_HIP(hipStreamSynchronize(net->stream));
+    }
+    if (int rc = net_sync_bf16(net.get())) return rc;
+    NMC_HIP(hipStreamSynchronize(net->stream));
+    *out = net.release();
+    return NMC_OK;
+}
+
+void nmc_net_destroy(nmc_net* net) { delete net; }
+
+int nmc_net_get_params(nmc_net* net, float* params) {
+    if (!net || !params) return fail(NMC_E_INVALID, "null argument");
+    NMC_HIP(hipStreamSynchronize(net->stream));
+    size_t off = 0;
+    for (int l = 0; l < 4; ++l) {
+        const size_t KN = (size_t)net->K(l) * 32;
+        NMC_HIP(hipMemcpy(params + off, net->w[l], KN * sizeof(float), hipMemcpyDeviceToHost));
+        off += KN;
+        NMC_HIP(hipMemcpy(params + off, net->b[l], 32 * sizeof(float), hipMemcpyDeviceToHost));
+        off += 32;
+    }
+    return NMC_OK;
+}
+
+int nmc_net_set_params(nmc_net* net, const float* params) {
+    if (!net || !params) return fail(NMC_E_INVALID, "null argument");
+    NMC_HIP(hipStreamSynchronize(net->stream));
+    size_t off = 0;
+    for (int l = 0; l < 4; ++l) {
+        const size_t KN = (size_t)net->K(l) * 32;
+        std::vector<float> w(params + off, params + off + KN);
+        // unused input channels / output columns stay exactly zero
+        for (int k = 0; k < net->K(l); ++k)
+            for (int n = 0; n < 32; ++n)
+                if (n >= net->nout[l] || (l == 0 && k % net->cin[0] >= 4)) w[(size_t)k * 32 + n] = 0.f;
+        NMC_HIP(hipMemcpy(net->w[l], w.data(), KN * sizeof(float), hipMemcpyHostToDevice));
+        off += KN;
+        std::vector<float> b(params + off, params + off + 32);
+        for (int n = net->nout[l]; n < 32; ++n) b[n] = 0.f;
+        NMC_HIP(hipMemcpy(net->b[l], b.data(), 32 * sizeof(float), hipMemcpyHostToDevice));
+        off += 32;
+    }
+    if (int rc = net_sync_bf16(net)) return rc;
+    NMC_HIP(hipStreamSynchronize(net->stream));
+    return NMC_OK;
+}
+
+int nmc_net_collect(nmc_net* net, nmc_ctx* ctx, int inst, int nblocks, uint64_t seed) {
+    if (!net || !ctx || nblocks < 0) return fail(NMC_E_INVALID, "bad argument");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    if (int rc = check_learn_ctx(ctx, inst)) return rc;
+    if (nblocks == 0) return NMC_OK;
+    const DevInst& d = ctx->din[inst];
+    const int nx = d.a1, ny = d.a2;
+    if (net->ndata + nblocks > net->data_cap) {
+        const int64_t cap = std::max<int64_t>(net->ndata + nblocks, 2 * net->data_cap);
+        __bf16* nxp = nullptr;
+        float* ntp = nullptr;
+        if (int rc = dalloc(&nxp, (size_t)cap * kPatchElems)) return rc;
+        if (int rc = dalloc(&ntp, (size_t)cap * kTargetElems)) { (void)hipFree(nxp); return rc; }
+        NMC_HIP(hipStreamSynchronize(net->stream));
+        if (net->ndata) {
+            NMC_HIP(hipMemcpy(nxp, net->data_x, (size_t)net->ndata * kPatchElems * 2, hipMemcpyDeviceToDevice));
+            NMC_HIP(hipMemcpy(ntp, net->data_t, (size_t)net->ndata * kTargetElems * 4, hipMemcpyDeviceToDevice));
+        }
+        (void)hipFree(net->data_x); (void)hipFree(net->data_t);
+        net->data_x = nxp; net->data_t = ntp; net->data_cap = cap;
+    }
+    std::mt19937_64 rng(seed);
+    std::vector<int> bxy(2 * (size_t)nblocks);
+    for (int i = 0; i < nblocks; ++i) {
+        bxy[2 * i] = (int)(rng() % (uint64_t)nx);
+        bxy[2 * i + 1] = (int)(rng() % (uint64_t)ny);
+    }
+    int* dbxy = nullptr;
+    if (int rc = upload(ctx, &dbxy, bxy)) return rc;   // synchronises ctx: spins are current
+    hipError_t e = launch_gather_patches(ctx->stream, (const float*)ctx->pool, d.base, d.Sp, nx, ny, dbxy, nblocks,
+                                         net->data_x + (size_t)net->ndata * kPatchElems,
+                                         net->data_t + (size_t)net->ndata * kTargetElems);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dbxy);
+    if (e != hipSuccess) return fail(NMC_E_HIP, std::string("nmc_net_collect: ") + hipGetErrorString(e));
+    net->ndata += nblocks;
+    return NMC_OK;
+}
+
+int nmc_net_num_samples(nmc_net* net, int64_t* out) {
+    if (!net || !out) return fail(NMC_E_INVALID, "null argument");
+    *out = net->ndata;
+    return NMC_OK;
+}
+
+int nmc_net_train(nmc_net* net, int nsteps, int batch, float lr, uint64_t seed, float* loss) {
+    if (!net || nsteps < 0 || batch <= 0) return fail(NMC_E_INVALID, "bad argument");
+    if (net->ndata < 1) return fail(NMC_E_INVALID, "no training blocks collected");
+    if (int rc = net_reserve(net, batch)) return rc;
+    std::mt19937_64 rng(seed);
+    std::vector<int> idx(batch);
+    for (int s = 0; s < nsteps; ++s) {
+        for (int i = 0; i < batch; ++i) idx[i] = (int)(rng() % (uint64_t)net->ndata);
+        NMC_HIP(hipMemcpyAsync(net->idx, idx.data(), batch * sizeof(int), hipMemcpyHostToDevice, net->stream));
+        NMC_HIP(launch_copy_rows(net->stream, net->data_x, net->x0, net->idx, batch, kPatchElems * 2));
+        NMC_HIP(launch_copy_rows(net->stream, net->data_t, net->tgt, net->idx, batch, kTargetElems * 4));
+        if (int rc = net_forward(net, batch)) return rc;
+        if (int rc = net_backward(net, batch)) return rc;
+        if (int rc = net_adam(net, lr)) return rc;
+        NMC_HIP(hipStreamSynchronize(net->stream));   // idx is reused next step
+    }
+    if (loss) {
+        std::vector<float> l((size_t)256 * batch);
+        NMC_HIP(hipMemcpy(l.data(), net->loss, l.size() * sizeof(float), hipMemcpyDeviceToHost));
+        double s = 0;
+        for (float v : l) s += v;
+        *loss = (float)(s / l.size());
+    }
+    return NMC_OK;
+}
+
+int nmc_net_loss_grad(nmc_net* net, const float* patches, const float* targets, int n, float* loss, float* grad) {
+    if (!net || !patches || !targets || n <= 0) return fail(NMC_E_INVALID, "bad argument");
+    if (int rc = net_reserve(net, n)) return rc;
+    float* dp = nullptr;
+    if (int rc = dalloc(&dp, (size_t)n * 576 * 4)) return rc;
+    NMC_HIP(hipMemcpy(dp, patches, (size_t)n * 576 * 4 * sizeof(float), hipMemcpyHostToDevice));
+    NMC_HIP(hipMemcpy(net->tgt, targets, (size_t)n * kTargetElems * sizeof(float), hipMemcpyHostToDevice));
+    hipError_t e = launch_put_patches(net->stream, dp, n, net->x0);
+    if (e != hipSuccess) { (void)hipFree(dp); return fail(NMC_E_HIP, hipGetErrorString(e)); }
+    int rc = net_forward(net, n);
+    if (!rc) rc = net_backward(net, n);
+    if (!rc) {
+        e = hipStreamSynchronize(net->stream);
+        if (e != hipSuccess) rc = fail(NMC_E_HIP, hipGetErrorString(e));
+    }
+    (void)hipFree(dp);
+    if (rc) return rc;
+    if (loss) {
+        std::vector<float> l((size_t)256 * n);
+        NMC_HIP(hipMemcpy(l.data(), net->loss, l.size() * sizeof(float), hipMemcpyDeviceToHost));
+        double s = 0;
+        for (float v : l) s += v;
+        *loss = (float)(s / l.size());
+    }
+    if (grad) {
+        size_t off = 0;
+        for (int l = 0; l < 4; ++l) {
+            const size_t KN = (size_t)net->K(l) * 32;
+            NMC_HIP(hipMemcpy(grad + off, net->gw[l], KN * sizeof(float), hipMemcpyDeviceToHost));
+            off += KN;
+            NMC_HIP(hipMemcpy(grad + off, net->gb[l], 32 * sizeof(float), hipMemcpyDeviceToHost));
+            off += 32;
+        }
+    }
+    return NMC_OK;
+}
+
+int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y, int64_t* accepted,
+                    float* stats, float* proposal) {
+    if (!ctx || !net || !inst || n <= 0) return fail(NMC_E_INVALID, "bad argument");
+    if (off_x < 0 || off_x >= 32 || off_y < 0 || off_y >= 32) return fail(NMC_E_INVALID, "block offset outside [0, 32)");
+    int total = 0;
+    std::vector<int> first(n + 1, 0);
+    for (int j = 0; j < n; ++j) {
+        if (int rc = check_inst(ctx, inst[j])) return rc;
+        if (int rc = check_learn_ctx(ctx, inst[j])) return rc;
+        const DevInst& d = ctx->din[inst[j]];
+        first[j] = total;
+        total += (d.a1 / 32) * (d.a2 / 32);
+    }
+    first[n] = total;
+    if (int rc = net_reserve(net, total)) return rc;
+    std::vector<int> bxy(2 * (size_t)total);
+    for (int j = 0; j < n; ++j) {
+        const DevInst& d = ctx->din[inst[j]];
+        int b = first[j];
+        for (int cx = 0; cx < d.a1 / 32; ++cx)
+            for (int cy = 0; cy < d.a2 / 32; ++cy, ++b) {
+                bxy[2 * b] = cx * 32 + off_x;
+                bxy[2 * b + 1] = cy * 32 + off_y;
+            }
+    }
+    // the network reads the lattice on the context's stream; one stream for both
+    NMC_HIP(hipStreamSynchronize(net->stream));
+    NMC_HIP(hipMemcpyAsync(net->bxy, bxy.data(), bxy.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    for (int j = 0; j < n; ++j) {
+        const DevInst& d = ctx->din[inst[j]];
+        NMC_HIP(launch_gather_patches(ctx->stream, (const float*)ctx->pool, d.base, d.Sp, d.a1, d.a2,
+                                      net->bxy + 2 * first[j], first[j + 1] - first[j],
+                                      net->x0 + (size_t)first[j] * kPatchElems, nullptr));
+    }
+    hipStream_t keep = net->stream;
+    net->stream = ctx->stream;
+    int rc = net_forward(net, total);
+    net->stream = keep;
+    if (rc) return rc;
+    const uint64_t round = ++ctx->block_round;
+    const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
+    float *dstats = nullptr, *dprop = nullptr;
+    if (stats && (rc = dalloc(&dstats, (size_t)3 * total))) return rc;
+    if (proposal && (rc = dalloc(&dprop, (size_t)768 * total))) { (void)hipFree(dstats); return rc; }
+    hipError_t e = hipSuccess;
+    for (int j = 0; j < n && e == hipSuccess; ++j) {
+        const DevInst& d = ctx->din[inst[j]];
+        e = launch_block_mh(ctx->stream, (float*)ctx->pool, d.base, d.Sp, d.a1, d.a2, net->bxy + 2 * first[j],
+                            first[j + 1] - first[j], net->o + (size_t)first[j] * 256 * 32, ctx->st, d.beta, d.h, slo,
+                            shi, d.inst_id, round, net->acc + first[j], dstats ? dstats + 3 * first[j] : nullptr,
+                            dprop ? dprop + (size_t)768 * first[j] : nullptr);
+    }
+    std::vector<int> acc(total);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(acc.data(), net->acc, total * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && dstats)
+        e = hipMemcpyAsync(stats, dstats, (size_t)3 * total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && dprop)
+        e = hipMemcpyAsync(proposal, dprop, (size_t)768 * total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dstats); (void)hipFree(dprop);
+    if (e != hipSuccess) return fail(NMC_E_HIP, std::string("nmc_block_moves: ") + hipGetErrorString(e));
+    if (accepted)
+        for (int j = 0; j < n; ++j)
+            accepted[j] = std::accumulate(acc.begin() + first[j], acc.begin() + first[j + 1], (int64_t)0);
+    return NMC_OK;
 }
 
 }  // extern "C"

@@ -169,19 +169,29 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const __bf16* __restrict__ X
     }
 }
 
-// Sum the weight-gradient partials and the bias gradient (sum of dY over m).
-__global__ void k_grad_reduce(const float* __restrict__ Pw, int nchunks, int K, const __bf16* __restrict__ dY,
-                              int M, float* __restrict__ gW, float* __restrict__ gB) {
+// Sum the weight-gradient partials (fixed chunk order: deterministic).
+__global__ void k_grad_reduce(const float* __restrict__ Pw, int nchunks, int K, float* __restrict__ gW) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < K * 32) {
         float s = 0.f;
         for (int c = 0; c < nchunks; ++c) s += Pw[(size_t)c * K * 32 + i];
         gW[i] = s;   // [k][n]
     }
-    if (i < 32) {
-        float s = 0.f;
-        for (int m = 0; m < M; ++m) s += (float)dY[(size_t)m * 32 + i];
-        gB[i] = s;
+}
+
+// Bias gradient gB[n] = sum over m of dY[m][n]: one workgroup, 32 row
+// groups x 32 channels, then a fixed-order sum of the 32 group partials.
+__global__ __launch_bounds__(1024) void k_bias_grad(const __bf16* __restrict__ dY, int M, float* __restrict__ gB) {
+    __shared__ float part[32][33];
+    const int n = threadIdx.x & 31, g = threadIdx.x >> 5;
+    float s = 0.f;
+    for (int m = g; m < M; m += 32) s += (float)dY[(size_t)m * 32 + n];
+    part[g][n] = s;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float t = 0.f;
+        for (int j = 0; j < 32; ++j) t += part[j][threadIdx.x];
+        gB[threadIdx.x] = t;
     }
 }
 
@@ -273,6 +283,22 @@ __device__ __forceinline__ VMF vmf_of(const float* o) {
     return f;
 }
 
+// The proposal's law: the same kappa with the mean direction normalised
+// exactly (the training head's |m| + 1e-6 keeps gradients finite at m = 0;
+// a proposal needs the density it is drawn from, so it samples and scores
+// with one unit vector; m = 0 falls back to +z).
+__device__ __forceinline__ VMF vmf_unit(const float* o) {
+    VMF f = vmf_of(o);
+    const float r2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    if (r2 > 1e-30f) {
+        const float inv = 1.f / sqrtf(r2);
+        f.mx = o[0] * inv; f.my = o[1] * inv; f.mz = o[2] * inv;
+    } else {
+        f.mx = 0.f; f.my = 0.f; f.mz = 1.f;
+    }
+    return f;
+}
+
 // log density of the vMF distribution on S^2: log(kappa / (4 pi sinh kappa)) + kappa mu.x
 __device__ __forceinline__ float vmf_logpdf(const VMF& f, float x, float y, float z) {
     const float k = f.kappa;
@@ -316,7 +342,8 @@ __global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, int6
                                                   const int* __restrict__ bxy, int nblocks,
                                                   const float* __restrict__ O, Stencil st, float beta, float h,
                                                   uint32_t seed_lo, uint32_t seed_hi, uint32_t inst, uint64_t round,
-                                                  int* __restrict__ accepted, float* __restrict__ stats) {
+                                                  int* __restrict__ accepted, float* __restrict__ stats,
+                                                  float* __restrict__ proposal) {
     __shared__ float nx_s[256], ny_s[256], nz_s[256];
     __shared__ float red[3][4];
     __shared__ int acc_s;
@@ -331,7 +358,7 @@ __global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, int6
     const int X = bx + u, Y = by + v;
     float sx, sy, sz;
     spin_at(L, X, Y, sx, sy, sz);
-    const VMF f = vmf_of(O + ((size_t)b * 256 + t) * 32);
+    const VMF f = vmf_unit(O + ((size_t)b * 256 + t) * 32);
     uint32_t w[4];
     philox4x32_10((uint32_t)(b * 256 + t), (uint32_t)(kPurposeBlock << 8), (uint32_t)round,
                   (uint32_t)(round >> 32), seed_lo, seed_hi + inst, w);
@@ -353,6 +380,10 @@ __global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, int6
     const float inv = rsqrtf(tx * tx + ty * ty + tz * tz);
     tx *= inv; ty *= inv; tz *= inv;
     nx_s[t] = tx; ny_s[t] = ty; nz_s[t] = tz;
+    if (proposal) {
+        float* pr = proposal + ((size_t)b * 256 + t) * 3;
+        pr[0] = tx; pr[1] = ty; pr[2] = tz;
+    }
     const float lq_new = vmf_logpdf(f, tx, ty, tz);
     const float lq_old = vmf_logpdf(f, sx, sy, sz);
     __syncthreads();
@@ -470,7 +501,8 @@ hipError_t launch_conv_wgrad(hipStream_t s, const __bf16* X, int Hin, int Cin, c
 
 hipError_t launch_grad_reduce(hipStream_t s, const float* Pw, int nchunks, int K, const __bf16* dY, int M, float* gW,
                               float* gB) {
-    k_grad_reduce<<<(K * 32 + 255) / 256, 256, 0, s>>>(Pw, nchunks, K, dY, M, gW, gB);
+    k_grad_reduce<<<(K * 32 + 255) / 256, 256, 0, s>>>(Pw, nchunks, K, gW);
+    k_bias_grad<<<1, 1024, 0, s>>>(dY, M, gB);
     return hipGetLastError();
 }
 
@@ -498,10 +530,11 @@ hipError_t launch_head_loss(hipStream_t s, const float* O, const float* target, 
 
 hipError_t launch_block_mh(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
                            int nblocks, const float* O, const Stencil& st, float beta, float h, uint32_t seed_lo,
-                           uint32_t seed_hi, uint32_t inst, uint64_t round, int* accepted, float* stats) {
+                           uint32_t seed_hi, uint32_t inst, uint64_t round, int* accepted, float* stats,
+                           float* proposal) {
     if (nblocks <= 0) return hipSuccess;
     k_block_mh<<<nblocks, 256, 0, s>>>(pool, base, Sp, nx, ny, bxy, nblocks, O, st, beta, h, seed_lo, seed_hi, inst,
-                                       round, accepted, stats);
+                                       round, accepted, stats, proposal);
     return hipGetLastError();
 }
 

@@ -1,0 +1,158 @@
+"""FP64 NumPy restatement of the learned block-proposal network -- TEST
+INFRASTRUCTURE ONLY (the checker of the device's forward and backward
+kernels; never imported by the nanomagmc_amd package).
+
+Network: four valid 3x3 convolutions on a 24 x 24 patch, input channels
+(spin x, y, z, halo flag) padded to 16, then 32, 32, 32 channels with ReLU,
+last layer 4 outputs per block site: mu (unnormalised) and the kappa
+pre-activation.  Per site a von Mises-Fisher law on S^2:
+    kappa = softplus(z) + 0.05,  mu = m / (|m| + 1e-6),
+    log q(x) = log kappa - log(4 pi) - log sinh(kappa) + kappa mu.x.
+Loss = mean over sites of -log q(target).
+
+Parameter vector order (the device's): per layer W as [9 * Cin][32]
+(row = (dy * 3 + dx) * Cin + ci) then b[32].  With bf16=True the arrays the
+device keeps in bf16 (input patch, weights, hidden activations, output and
+hidden gradients) are rounded to bf16 (round to nearest even) at the same
+points, so the comparison isolates accumulation-order differences.
+"""
+import numpy as np
+
+CIN = [16, 32, 32, 32]
+NOUT = [32, 32, 32, 4]
+HIN = [24, 22, 20, 18]
+
+
+def n_params():
+    return sum(9 * c * 32 + 32 for c in CIN)
+
+
+def bf16(a):
+    a = np.asarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def unpack(p):
+    p = np.asarray(p, dtype=np.float64)
+    out, off = [], 0
+    for c in CIN:
+        K = 9 * c
+        W = p[off:off + K * 32].reshape(K, 32)
+        off += K * 32
+        b = p[off:off + 32]
+        off += 32
+        out.append((W, b))
+    return out
+
+
+def im2col(X):
+    """(n, H, H, C) -> (n, H-2, H-2, 9C) with k = (dy*3+dx)*C + c."""
+    n, H, _, C = X.shape
+    Ho = H - 2
+    cols = np.empty((n, Ho, Ho, 9 * C))
+    for dy in range(3):
+        for dx in range(3):
+            t = dy * 3 + dx
+            cols[..., t * C:(t + 1) * C] = X[:, dy:dy + Ho, dx:dx + Ho, :]
+    return cols
+
+
+def col2im(dcols, C):
+    """adjoint of im2col: (n, Ho, Ho, 9C) -> (n, Ho+2, Ho+2, C)."""
+    n, Ho, _, _ = dcols.shape
+    dX = np.zeros((n, Ho + 2, Ho + 2, C))
+    for dy in range(3):
+        for dx in range(3):
+            t = dy * 3 + dx
+            dX[:, dy:dy + Ho, dx:dx + Ho, :] += dcols[..., t * C:(t + 1) * C]
+    return dX
+
+
+def forward(params, patches, use_bf16=False):
+    """patches (n, 24, 24, 4) -> per-layer inputs and the output (n,16,16,32)."""
+    rnd = bf16 if use_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    n = patches.shape[0]
+    X = np.zeros((n, 24, 24, 16))
+    X[..., :4] = patches
+    X = rnd(X)
+    layers = unpack(params)
+    xs = []
+    for l, (W, b) in enumerate(layers):
+        xs.append(X)
+        Y = im2col(X) @ rnd(W) + b
+        if l < 3:
+            Y = rnd(np.maximum(Y, 0.0))
+        X = Y
+    return xs, X
+
+
+def head(O):
+    m = O[..., :3]
+    r = np.sqrt((m * m).sum(-1, keepdims=True)) + 1e-6
+    mu = m / r
+    z = O[..., 3]
+    kappa = np.where(z > 20, z, np.log1p(np.exp(np.minimum(z, 20)))) + 0.05
+    return mu, kappa, r
+
+
+def logq(mu, kappa, x):
+    lsinh = kappa + np.log1p(-np.exp(-2 * kappa)) - np.log(2.0)
+    return np.log(kappa) - np.log(4 * np.pi) - lsinh + kappa * (mu * x).sum(-1)
+
+
+def loss_grad(params, patches, targets, use_bf16=False):
+    """Mean loss and gradient (device parameter order)."""
+    rnd = bf16 if use_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    n = patches.shape[0]
+    xs, O = forward(params, patches, use_bf16)
+    x = targets.reshape(n, 16, 16, 3)
+    mu, kappa, r = head(O)
+    L = -logq(mu, kappa, x)
+    loss = L.mean()
+    scale = 1.0 / L.size
+    # dL/dkappa, dL/dmu_raw, dL/dz
+    e2 = np.exp(-2 * kappa)
+    coth = (1 + e2) / (1 - e2)
+    mux = (mu * x).sum(-1)
+    dk = -(1 / kappa - coth + mux)
+    g = -kappa[..., None] * x
+    proj = (mu * g).sum(-1, keepdims=True)
+    dm = (g - mu * proj) / r
+    z = O[..., 3]
+    sig = 1 / (1 + np.exp(-z))
+    dO = np.zeros_like(O)
+    dO[..., :3] = scale * dm
+    dO[..., 3] = scale * dk * sig
+    dY = rnd(dO)
+    layers = unpack(params)
+    grads = [None] * 4
+    for l in range(3, -1, -1):
+        W, b = layers[l]
+        cols = im2col(xs[l])
+        gW = cols.reshape(-1, cols.shape[-1]).T @ dY.reshape(-1, 32)
+        gb = dY.reshape(-1, 32).sum(0)
+        grads[l] = (gW, gb)
+        if l > 0:
+            dX = col2im(dY @ rnd(W).T, CIN[l])
+            dX = np.where(xs[l] > 0, dX, 0.0)   # through the ReLU of the layer below
+            dY = rnd(dX)
+    flat = np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads])
+    return loss, flat
+
+
+def init_params(rng):
+    p = []
+    for l, c in enumerate(CIN):
+        fan = 9 * (4 if l == 0 else 32)
+        lim = np.sqrt(6.0 / fan) * (0.1 if l == 3 else 1.0)
+        W = np.zeros((9 * c, 32))
+        for k in range(9 * c):
+            if l == 0 and k % c >= 4:
+                continue
+            W[k, :NOUT[l]] = rng.uniform(-lim, lim, NOUT[l])
+        b = np.zeros(32)
+        b[:NOUT[l]] = rng.uniform(-0.1, 0.1, NOUT[l])
+        p += [W.reshape(-1), b]
+    return np.concatenate(p)

@@ -1,0 +1,202 @@
+"""Learned block proposals on the device: gradients of the network against
+the FP64 oracle, the Metropolis-Hastings quantities against oracle energies,
+exact stationarity on a solvable case, training, and the effect on
+autocorrelation times.  Needs an MI355X."""
+import math
+
+import numpy as np
+import pytest
+
+from tests import exact
+from tests.fixtures import box_locs, neel2d, random_unit
+
+pytestmark = pytest.mark.gpu
+
+
+def lattice(L, D, J=1.0, ninst=1, seed=9):
+    from nanomagmc_amd import Bond, Lattices
+    d, Jv, Dv = neel2d(D)
+    bonds = [Bond(d[k], J * Jv[k], Dv[k]) for k in range(4)]
+    return Lattices(2, False, True, bonds, [{"n": (L, L), "edge": L}] * ninst, use_j=J != 0, use_d=D != 0,
+                    seed=seed)
+
+
+def last_bias_offset():
+    """Offset of the output layer's bias (mu_x, mu_y, mu_z, kappa pre-activation)."""
+    from oracle import netref
+    return sum(9 * c * 32 + 32 for c in netref.CIN[:3]) + 9 * netref.CIN[3] * 32
+
+
+def patches_of(spins, L, blocks):
+    """The device's network input for blocks at (bx, by): 24 x 24 x 4 with the
+    16 x 16 interior masked, and the interior targets."""
+    s = spins.reshape(L, L, 3)
+    P = np.zeros((len(blocks), 24, 24, 4))
+    T = np.zeros((len(blocks), 256, 3))
+    for i, (bx, by) in enumerate(blocks):
+        xs = (np.arange(bx - 4, bx + 20) % L)
+        ys = (np.arange(by - 4, by + 20) % L)
+        patch = s[np.ix_(xs, ys)]
+        P[i, ..., :3] = patch
+        P[i, ..., 3] = 1.0
+        T[i] = patch[4:20, 4:20].reshape(256, 3)
+        P[i, 4:20, 4:20, :] = 0.0
+    return P, T
+
+
+def test_network_gradients_match_oracle():
+    from nanomagmc_amd.learn import Net
+    from oracle import netref
+    rng = np.random.default_rng(3)
+    net = Net(seed=1)
+    p = netref.init_params(rng).astype(np.float32)
+    net.set_params(p)
+    assert np.array_equal(net.get_params(), p)
+    n = 6
+    P = np.zeros((n, 24, 24, 4))
+    s = random_unit(n * 576, rng).reshape(n, 24, 24, 3)
+    P[..., :3] = s
+    P[..., 3] = 1.0
+    P[:, 4:20, 4:20, :] = 0.0
+    T = random_unit(n * 256, rng).reshape(n, 256, 3)
+    loss, g = net.loss_grad(P, T)
+    ref_loss, ref_g = netref.loss_grad(p.astype(np.float64), P, T, use_bf16=True)
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
+    off = 0
+    for l, c in enumerate(netref.CIN):
+        K = 9 * c
+        for part, size in (("W", K * 32), ("b", 32)):
+            a, b = g[off:off + size], ref_g[off:off + size]
+            off += size
+            err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+            assert err < 3e-2, (l, part, err)
+
+
+@pytest.mark.parametrize("T,kappa0", [(2.0, None), (100.0, 0.05)])
+def test_block_move_quantities_match_oracle(orc, T, kappa0):
+    # kappa0 = None: a random network at T = 2 (almost every block rejected);
+    # kappa0 = 0.05: a near-uniform constant proposal at T = 100, where
+    # accepted and rejected blocks mix
+    from nanomagmc_amd.learn import Net, block_moves
+    from oracle import netref
+    L, D, h = 64, 0.3, 0.06
+    lat = lattice(L, D)
+    lat.set_params(1 / T, h)
+    rng = np.random.default_rng(5)
+    before = random_unit(L * L, rng).astype(np.float32).astype(np.float64)
+    lat.set_spins(0, before)
+    net = Net(seed=2)
+    p = netref.init_params(rng).astype(np.float32)
+    if kappa0 is not None:
+        p[:] = 0.0
+        p[last_bias_offset():last_bias_offset() + 4] = [0.3, -0.2, 0.9, math.log(math.expm1(kappa0 - 0.05 + 1e-3))]
+    net.set_params(p)
+    off = (7, 19)
+    acc, stats, prop = block_moves(lat, net, [0], *off, diagnostics=True)
+    after = lat.get_spins(0)
+    blocks = [(32 * i + off[0], 32 * j + off[1]) for i in range(2) for j in range(2)]
+    d, Jv, Dv = neel2d(D)
+    f = orc.Field(box_locs(2, L), 2, L, 1, d, Jv, Dv)
+    H = [0, 0, h]
+    E0 = f.energy(before, H, use_d=True, convention=1)
+    P, Tg = patches_of(before, L, blocks)
+    _, O = netref.forward(p.astype(np.float64), P, use_bf16=True)
+    _, kappa, _ = netref.head(O)
+    mu = O[..., :3] / np.linalg.norm(O[..., :3], axis=-1, keepdims=True)   # proposals use the exact unit mean
+    changed = np.zeros(L * L, dtype=bool)
+    naccepted = 0
+    for b, (bx, by) in enumerate(blocks):
+        xs = np.arange(bx, bx + 16) % L
+        ys = np.arange(by, by + 16) % L
+        idx = (xs[:, None] * L + ys[None, :]).reshape(-1)
+        trial = before.copy()
+        trial[idx] = prop[b].reshape(256, 3)
+        assert np.allclose(np.linalg.norm(trial[idx], axis=1), 1.0, atol=1e-6)
+        dE = f.energy(trial, H, use_d=True, convention=1) - E0
+        assert abs(stats[b, 0] - dE) <= 2e-3 * (1 + abs(dE)), (b, stats[b, 0], dE)
+        lqn = netref.logq(mu[b], kappa[b], prop[b].astype(np.float64)).sum()
+        lqo = netref.logq(mu[b], kappa[b], Tg[b].reshape(16, 16, 3)).sum()
+        assert abs(stats[b, 1] - lqn) <= 0.05 * (1 + abs(lqn)), (b, stats[b, 1], lqn)
+        assert abs(stats[b, 2] - lqo) <= 0.05 * (1 + abs(lqo)), (b, stats[b, 2], lqo)
+        # the decision: thread 0's Philox word 2, counter (b * 256, block purpose, round 1)
+        w = orc.philox([b * 256, 3 << 8, 1, 0], [9, 0])
+        u = np.float32(((int(w[2]) >> 8) + 0.5) / 16777216.0)
+        loga = np.float32(-np.float32(1 / T) * stats[b, 0] + stats[b, 2] - stats[b, 1])
+        accept = bool(loga >= 0 or np.log(u) < loga)
+        now = after[idx]
+        if accept:
+            assert np.array_equal(now, prop[b].reshape(256, 3).astype(np.float64))
+            naccepted += 1
+        else:
+            assert np.array_equal(now, before[idx])
+        changed[idx] = True
+    assert acc[0] == naccepted
+    assert np.array_equal(after[~changed], before[~changed])   # nothing outside the blocks moves
+
+
+def test_block_moves_alone_sample_the_exact_distribution():
+    # independent spins in a field (J = D = 0): <s_z> = L(beta h).  The
+    # network is set to a constant vMF(z, kappa0) with kappa0 != beta h, so
+    # only the Hastings correction makes the chain exact.
+    from nanomagmc_amd.learn import Net, block_moves
+    from oracle import netref
+    L, h, beta = 64, 1.0, 1.0
+    lat = lattice(L, 0.0, J=0.0)
+    lat.set_params(beta, h)
+    lat.randomize()
+    net = Net(seed=3)
+    p = np.zeros(netref.n_params(), dtype=np.float32)
+    kappa0 = 1.2
+    z = math.log(math.expm1(kappa0 - 0.05))
+    off = last_bias_offset()
+    p[off:off + 4] = [0.0, 0.0, 1.0, z]           # last-layer bias: mu = z-axis, kappa = kappa0
+    net.set_params(p)
+    rng = np.random.default_rng(11)
+    lat.series_reset(1500, with_charge=False)
+    nacc = 0
+    for k in range(1500):
+        for _ in range(2):
+            ox, oy = rng.integers(0, 32, 2)
+            nacc += int(block_moves(lat, net, [0], int(ox), int(oy))[0])
+        lat.sample(k)
+    obs, _, _ = lat.get_series()
+    sz = obs[0, 300:, 2] / (L * L)
+    m, err, _ = exact.mean_err(sz)
+    assert 0.05 < nacc / (1500 * 2 * 4) < 0.95
+    assert abs(m - exact.langevin(beta * h)) < 4 * err + 1e-4, (m, exact.langevin(beta * h), err)
+    assert abs(exact.langevin(kappa0) - exact.langevin(beta * h)) > 20 * err   # the test can see a bias
+
+
+def test_training_lowers_the_loss_and_keeps_the_physics():
+    # T = 0.6: short correlation length, so two 64 x 64 instances mix fast
+    # enough that their mean energies must agree within the error bars
+    from nanomagmc_amd.learn import Net, block_moves
+    L, D, h, T = 64, 0.3, 0.06, 0.6
+    lat = lattice(L, D, ninst=2, seed=21)
+    lat.set_params(1 / T, h)
+    lat.randomize()
+    lat.equil(500)
+    net = Net(seed=4)
+    for k in range(40):
+        lat.equil(10)
+        net.collect(lat, 0, 64, seed=k)
+    assert net.num_samples() == 40 * 64
+    first = net.train(1, batch=128, lr=0.0, seed=1)
+    last = net.train(300, batch=128, lr=2e-3, seed=2)
+    assert last < first - 0.1, (first, last)
+    assert last < math.log(4 * math.pi) - 0.05, last     # better than a uniform proposal
+    # instance 0: plain sweeps only; instance 1: sweeps + learned rounds
+    rng = np.random.default_rng(2)
+    n = 600
+    lat.series_reset(n, with_charge=False)
+    acc = 0
+    for k in range(n):
+        lat.equil(2)
+        ox, oy = rng.integers(0, 32, 2)
+        acc += int(block_moves(lat, net, [1], int(ox), int(oy))[0])
+        lat.sample(k)
+    obs, _, _ = lat.get_series()
+    e0, s0, _ = exact.mean_err(obs[0, 100:, 10])
+    e1, s1, _ = exact.mean_err(obs[1, 100:, 10])
+    assert abs(e0 - e1) < 4 * math.hypot(s0, s1), (e0, e1, s0, s1)
+    print("loss %.3f -> %.3f, learned block acceptance %.3f" % (first, last, acc / (n * 4)))
