@@ -13,6 +13,13 @@ Other workloads (--workload): `grains` (512 Weibull spheres x 64 (T, h)
 points = 32768 uneven instances, Heisenberg + Bloch DMI, open boundaries),
 `ising2d` (2-D Ising L = 64 periodic, 21 temperatures).
 
+The `ess` object (skip with --no-ess) is the metric's second half:
+tau_int-normalised effective samples/s of E and |Q| on the 2-D skyrmion
+Hamiltonian (L = 512 periodic, J = 1, Neel D = 0.3, h_z = 0.06,
+T in {0.1, 0.2, 0.3}), for plain checkerboard sweeps and for sweeps
+interleaved 4:1 with learned 16 x 16 block proposals (one network per
+temperature trained on 1e5 blocks of plain-sampler configurations).
+
 Prints one JSON line.
 """
 import argparse
@@ -129,6 +136,33 @@ def cpu_baseline(budget_s=12.0):
                       "lattice, T=1.4, h=0, uniform-sphere trials; %.1f s" % (threads, attempts, secs)}
 
 
+def ess_leg():
+    """Effective samples/s of E and |Q| (Q_wrap, periodic plaquette sum),
+    plain sweeps vs sweeps + learned block rounds; tau_int in samples of 4
+    sweeps each, Sokal window c = 6 (`window_closed` false: the series is
+    shorter than the window, tau is a lower bound and ESS an upper bound)."""
+    from nanomagmc_amd.learn import compare_ess
+    r = compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_sweeps=3000,
+                    collect_rounds=100, collect_blocks=1000, train_steps=2000, batch=256, nsamples=2000)
+    rows = []
+    for row in r["temps"]:
+        def pick(m, k):
+            st = row[m][k]
+            return {"tau_int": st["tau_int"], "ess_per_s": st["ess_per_s"], "window_closed": st["window_closed"],
+                    "mean": st["mean"], "err": st["err"]}
+        rows.append({"T": row["T"], "block_acceptance": row["acceptance"], "train_loss": row["loss"],
+                     "plain": {"E": pick("plain", "E"), "absQ": pick("plain", "absQ")},
+                     "learned": {"E": pick("learned", "E"), "absQ": pick("learned", "absQ")},
+                     "gain_absQ": row["gain_absQ"], "gain_E": row["gain_E"]})
+    return {"metric": "tau_int-normalised effective samples/s", "unit": "samples/s",
+            "config": "2-D skyrmion L=512 periodic, J=1, Neel D=0.3, h_z=0.06, %d replicas per method and T, "
+                      "%d samples x 4 sweeps, learned: 1 block round (256 16x16 blocks) per 4 sweeps"
+                      % (r["replicas"], r["nsamples"]),
+            "train": {"blocks": r["train_blocks"], "steps": r["train_steps"], "batch": r["batch"],
+                      "seconds": r["train_s"]},
+            "s_per_sample": r["s_per_sample"], "temps": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +172,7 @@ def main():
     ap.add_argument("--trial", default=None, choices=["cone", "sphere"])
     ap.add_argument("--tune", action="store_true", help="autotune the sweep path before timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ess", action="store_true", help="skip the effective-samples leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -200,7 +235,7 @@ def main():
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     out = {
-        "metric": "Metropolis spin-updates/s, 1 GPU",
+        "metric": "Metropolis spin-updates/s and \u03c4_int-normalised effective samples/s, 1 GPU",
         "value": value,
         "unit": "spin-updates/s",
         "n_gpus": 1,
@@ -219,6 +254,8 @@ def main():
                      "kernel": "sweep colour pass (%d updates, %d B/update algorithmic)" % (total_sites // 2, bpu),
                      "avg_launch_us": launch_s * 1e6},
     }
+    if not args.no_ess:
+        out["ess"] = ess_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

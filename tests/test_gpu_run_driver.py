@@ -74,7 +74,6 @@ def test_ising_layout(tmp_path, protocol):
     assert sh["/Av_Latt/T_0-H_0"] == (8, 8, 1) and sh["/Sing_Latt/T_2-H_1"] == (8, 8, 1)
     E = h5_read(path, "/Fulldat/energies").reshape(2, 3, 10)
     Hs = [0.0, 0.1]
-    hrow = 0 if protocol != 1 else 0
     # H index 0 is h = 0: E = -sum s_i s_j over 128 bonds, a multiple of 4
     assert np.all(np.abs(E[0]) <= 2 * 64) and np.all(E[0] % 4 == 0)
     assert np.all(np.abs(E[1]) <= 2 * 64 + Hs[1] * 64 + 1e-4)

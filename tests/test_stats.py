@@ -1,0 +1,48 @@
+"""tau_int / effective-sample estimators (nanomagmc_amd.stats) against closed
+forms: white noise has tau_int = 1/2, an AR(1) process x_t = phi x_{t-1} +
+e_t has rho(t) = phi^t and tau_int = (1 + phi) / (2 (1 - phi)).  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from nanomagmc_amd import stats
+
+
+def ar1(phi, n, rng):
+    e = rng.normal(size=n)
+    x = np.empty(n)
+    x[0] = e[0] / math.sqrt(1 - phi * phi)
+    for t in range(1, n):
+        x[t] = phi * x[t - 1] + e[t]
+    return x
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.5, 0.9])
+def test_tau_int_of_ar1(phi):
+    rng = np.random.default_rng(int(phi * 10) + 1)
+    x = ar1(phi, 200_000, rng)
+    tau, W = stats.tau_int(x)
+    want = (1 + phi) / (2 * (1 - phi))
+    assert abs(tau - want) < 0.05 * want + 0.02, (tau, want)
+    n_eff, tau2, closed = stats.ess(x)
+    assert closed and tau2 == tau and abs(n_eff - x.size / (2 * tau)) < 1e-9
+
+
+def test_autocorrelation_is_exact_on_a_known_series():
+    x = np.array([1.0, -1.0, 1.0, -1.0])
+    rho = stats.autocorr(x)
+    assert np.allclose(rho, [1.0, -0.75, 0.5, -0.25])
+
+
+def test_unclosed_window_is_flagged():
+    x = np.cumsum(np.random.default_rng(0).normal(size=2000))   # random walk: no finite tau
+    _, _, closed = stats.ess(x)
+    assert not closed
+
+
+def test_mean_err_matches_independent_case():
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=100_000)
+    m, e, tau = stats.mean_err(x)
+    assert abs(m) < 5 * e and abs(e - 1 / math.sqrt(x.size)) < 0.05 / math.sqrt(x.size)
