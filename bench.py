@@ -38,26 +38,27 @@ HBM_PEAK = 8.0e12
 NN3 = [(0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)]
 
 
-def heis3d(trial):
-    """64 instances of the 64^3 periodic Heisenberg lattice (32 T x 2 h)."""
+def heis3d(trial, replicas=1):
+    """64 instances of the 64^3 periodic Heisenberg lattice (32 T x 2 h),
+    times `replicas` independent copies of each chain."""
     from nanomagmc_amd import Bond, Lattices
     L = 64
     temps = list(np.linspace(2.0, 0.8, 32))
     fields = [0.0, 0.05]
     bonds = [Bond(d, 1.0) for d in NN3]
-    insts = [{"n": (L, L, L), "edge": L} for _ in range(len(temps) * len(fields))]
+    points = [(h, T) for h in fields for T in temps] * replicas
+    insts = [{"n": (L, L, L), "edge": L} for _ in points]
     lat = Lattices(3, False, True, bonds, insts, use_j=True, use_d=False, trial=trial, seed=0x4E4D4302)
-    betas = [1.0 / T for h in fields for T in temps]
-    hs = [h for h in fields for T in temps]
-    lat.set_params(betas, hs)
+    lat.set_params([1.0 / T for h, T in points], [h for h, T in points])
     lat.randomize()
     sites = [L ** 3] * lat.ninst
-    desc = ("3-D cubic Heisenberg L=64 periodic, J=1, D=0, h_z in {0,0.05} x 32 T 2.0->0.8 = 64 instances, "
-            "one checkerboard sweep per step, %s trials" % trial)
+    desc = ("3-D cubic Heisenberg L=64 periodic, J=1, D=0, h_z in {0,0.05} x 32 T 2.0->0.8 = 64 instances%s, "
+            "one checkerboard sweep per step, %s trials"
+            % ("" if replicas == 1 else " x %d replicas" % replicas, trial))
     return lat, sites, 36, desc
 
 
-def grains(trial):
+def grains(trial, replicas=1):
     """512 Weibull spheres (LATTSHAPE x, open): radius ~ log-normal (mean 10,
     sd 2.5, truncated to [4, 20]), Weibull factor 8, J=1, Bloch D=0.2,
     h_z in {0,0.02,0.05,0.1} x 16 T 2.0->0.1 = 32768 instances."""
@@ -91,7 +92,7 @@ def grains(trial):
     return lat, sites, 36, desc
 
 
-def ising2d(trial):
+def ising2d(trial, replicas=1):
     from nanomagmc_amd import Bond, Lattices
     L = 64
     temps = list(np.linspace(3.5, 1.5, 21))
@@ -171,6 +172,9 @@ def main():
     ap.add_argument("--workload", default="heis3d", choices=sorted(WORKLOADS))
     ap.add_argument("--trial", default=None, choices=["cone", "sphere"])
     ap.add_argument("--tune", action="store_true", help="autotune the sweep path before timing")
+    ap.add_argument("--path", default=None, choices=["auto", "stream"], help="force a sweep path")
+    ap.add_argument("--replicas", type=int, default=1,
+                    help="heis3d: independent copies of every chain (working sets beyond the 256 MB cache)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the effective-samples leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -192,7 +196,7 @@ def main():
     import torch
     trial = args.trial or ("cone" if args.workload == "heis3d" else "sphere")
     t_setup = time.perf_counter()
-    lat, sites, bpu, desc = WORKLOADS[args.workload](trial)
+    lat, sites, bpu, desc = WORKLOADS[args.workload](trial, replicas=args.replicas)
     setup_s = time.perf_counter() - t_setup
     total_sites = int(np.sum(sites))
     tuned = None
@@ -200,6 +204,8 @@ def main():
         from nanomagmc_amd.autotune import tune
         choice, timings = tune(lat, "%s/%s/%d" % (args.workload, trial, lat.ninst))
         tuned = {"choice": list(choice), "seconds_per_candidate": timings}
+    if args.path:
+        lat.set_path(args.path)
     # warm-up; cone half-angles adapt toward 50% acceptance, then freeze
     theta = np.full(lat.ninst, 1.0)
     if trial == "cone":
@@ -232,7 +238,7 @@ def main():
     achieved = (total_sites / 2) * bpu / launch_s
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.replicas == 1:   # measured for the default batch only
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     out = {
         "metric": "Metropolis spin-updates/s and \u03c4_int-normalised effective samples/s, 1 GPU",

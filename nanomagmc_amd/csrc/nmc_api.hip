@@ -75,16 +75,19 @@ struct Set {
     int n_mt = 0;
     int* d_tb = nullptr;
     int* d_te = nullptr;
-    int* d_lds = nullptr;    // instances swept LDS-resident (the rest are in d_sweep)
+    int* d_lds = nullptr;    // instances swept LDS-resident (the rest are in d_sweep or d_qt)
     int n_lds = 0;
+    QTile* d_qt = nullptr;   // shaped Heisenberg lattices: colour-0 tiles, then colour-1 tiles
+    int n_qt[2] = {0, 0};
+    uint32_t* d_qlist = nullptr;   // their non-empty quads (first colour element of each)
     size_t lds_bytes = 0;    // dynamic LDS of the largest of them
     int lds_block = 256;
 
     void release() {
         (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
         (void)hipFree(d_mt); (void)hipFree(d_tb); (void)hipFree(d_te); (void)hipFree(d_lds);
-        (void)hipFree(d_tacc);
-        d_tacc = nullptr;
+        (void)hipFree(d_tacc); (void)hipFree(d_qt); (void)hipFree(d_qlist);
+        d_tacc = nullptr; d_qt = nullptr; d_qlist = nullptr;
         d_list = nullptr; d_sweep = nullptr; d_init = nullptr; d_mt = nullptr;
         d_tb = d_te = d_lds = nullptr;
     }
@@ -110,6 +113,7 @@ struct nmc_ctx {
     void* pool = nullptr;
     size_t pool_elems = 0;
     uint32_t* mpool = nullptr;
+    std::vector<uint32_t> h_mask;   // host copy of the presence bits (shaped lattices)
     uint32_t* d_tab = nullptr;
     double* d_dtab = nullptr;
     std::vector<uint32_t> h_tab;
@@ -231,8 +235,9 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     std::vector<MTile> mt;
     std::vector<int> tb, te;
     const int stile = c->ising ? kIsingTile : kSweepTile;
-    std::vector<int> lds;
+    std::vector<int> lds, qinst;
     int64_t lds_work = 0;
+    const bool quad_lists = c->masked && !c->ising;
     for (int i : list) {
         const DevInst& d = c->din[i];
         // both colour arrays, then (masked lattices) both colours' presence bits
@@ -242,6 +247,8 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             lds.push_back(i);
             s->lds_bytes = std::max(s->lds_bytes, bytes);
             lds_work = std::max<int64_t>(lds_work, c->ising ? (d.S + 1) / 2 : (d.S + 3) / 4);
+        } else if (quad_lists) {
+            qinst.push_back(i);
         } else {
             for (int64_t e = 0; e < d.S; e += stile) sw.push_back(Tile{i, (int)e});
         }
@@ -252,9 +259,32 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         for (int r = 0; r < rows; r += per) mt.push_back(MTile{i, r, std::min(per, rows - r)});
         te.push_back((int)mt.size());
     }
+    // shaped Heisenberg lattices: per colour, the quads with a present site,
+    // 256 to a tile (a2 is a multiple of 8 there, so S is a multiple of 4)
+    std::vector<QTile> qt;
+    std::vector<uint32_t> ql;
+    for (int col = 0; col < 2; ++col) {
+        for (int i : qinst) {
+            const DevInst& d = c->din[i];
+            const uint32_t* mw = c->h_mask.data() + d.mask_base + (int64_t)col * d.mask_stride;
+            const size_t q0 = ql.size();
+            for (int64_t e = 0; e < d.S; e += 4)
+                if ((mw[e >> 5] >> (e & 31)) & 0xFu) ql.push_back((uint32_t)e);
+            if (ql.size() > (size_t)INT32_MAX) return fail(NMC_E_UNSUPPORTED, "too many quads in one sweep set");
+            for (size_t q = q0; q < ql.size(); q += 256)
+                qt.push_back(QTile{i, (int)q, (int)std::min<size_t>(256, ql.size() - q)});
+        }
+        s->n_qt[col] = (int)qt.size() - (col ? s->n_qt[0] : 0);
+    }
+    for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});   // accept counters follow the plain tiles
+    const int n_plain = (int)sw.size() - (int)qt.size();
     int rc;
     if ((rc = upload(c, &s->d_list, list))) return rc;
+    if ((rc = upload(c, &s->d_qt, qt))) return rc;
+    if ((rc = upload(c, &s->d_qlist, ql))) return rc;
+    sw.resize(n_plain);
     if ((rc = upload(c, &s->d_sweep, sw))) return rc;
+    for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});
     if ((rc = upload(c, &s->d_init, ini))) return rc;
     if ((rc = upload(c, &s->d_mt, mt))) return rc;
     if ((rc = upload(c, &s->d_tb, tb))) return rc;
@@ -263,7 +293,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     s->n_lds = (int)lds.size();
     s->lds_block = c->lds_block > 0 ? c->lds_block
                                     : (int)std::min<int64_t>(1024, std::max<int64_t>(64, (lds_work + 63) / 64 * 64));
-    s->n_sweep = (int)sw.size();
+    s->n_sweep = n_plain;
     s->h_sweep = sw;
     if ((rc = dalloc(&s->d_tacc, sw.size()))) return rc;
     NMC_HIP(hipMemsetAsync(s->d_tacc, 0, std::max<size_t>(1, sw.size()) * sizeof(unsigned long long), c->stream));
@@ -487,6 +517,7 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
             });
         }
         if ((rc = upload(c.get(), &c->mpool, mw))) return rc;
+        c->h_mask = std::move(mw);
     }
     if (c->ising) {
         if ((rc = dalloc(&c->d_tab, (size_t)ninst * c->tabsize))) return rc;
@@ -635,8 +666,17 @@ int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
                                  s->d_lds, s->n_lds, ctx->pool, ctx->mpool, ctx->d_tab, ctx->d_sweeps, ctx->st,
                                  chunk, (uint32_t)k, slo, shi, ctx->d_acc));
     }
-    for (int k = 0; k < nsweeps && s->n_sweep > 0; ++k)
+    const bool streaming = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    for (int k = 0; k < nsweeps && streaming; ++k)
         for (int col = 0; col < 2; ++col) {
+            if (s->n_qt[col] > 0) {
+                const int off = col ? s->n_qt[0] : 0;
+                NMC_HIP(launch_sweep_heis_q(ctx->stream, ctx->dim3, ctx->periodic, ctx->use_d, ctx->cone, true,
+                                            ctx->d_inst, s->d_qt + off, s->n_qt[col], s->d_qlist, (float*)ctx->pool,
+                                            ctx->mpool, ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi,
+                                            s->d_tacc + s->n_sweep + off));
+            }
+            if (s->n_sweep == 0) continue;
             if (ctx->ising)
                 NMC_HIP(launch_sweep_ising(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, true, ctx->d_inst,
                                            s->d_sweep, s->n_sweep, (int8_t*)ctx->pool, ctx->mpool, ctx->d_tab,

@@ -98,6 +98,15 @@ struct MTile {
     int inst, row0, nrows;
 };
 
+// Sweep work item over a list of non-empty quads (shaped lattices): entries
+// q0 .. q0 + qn - 1 (qn <= 256) of the quad list, all of one instance and
+// one colour; each entry is the first colour element of a quad.
+struct QTile {
+    int inst;
+    int q0;
+    int qn;
+};
+
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                               uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll

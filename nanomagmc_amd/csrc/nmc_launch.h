@@ -10,6 +10,13 @@ hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool 
                              const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo,
                              uint32_t seed_hi, unsigned long long* acc);
 
+// Shaped lattices: colour pass over the listed non-empty quads only.
+hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, bool use_d, bool cone, bool count,
+                               const DevInst* insts, const QTile* tiles, int ntiles, const uint32_t* qlist,
+                               float* pool, const uint32_t* mpool, const uint64_t* sweeps, const Stencil& st,
+                               int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                               unsigned long long* acc);
+
 hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool masked,
                               bool count, const DevInst* insts, const Tile* tiles, int ntiles,
                               int8_t* pool, const uint32_t* mpool, const uint32_t* tab,

@@ -317,6 +317,39 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
     }
 }
 
+// Colour pass of shaped lattices over their non-empty quads only: a wave
+// takes 64 consecutive entries of the quad list, so no lane idles on a quad
+// outside the grain.  The quad's element index fixes its random numbers, so
+// the lattice evolves exactly as under k_sweep_heis.
+template <bool DIM3, bool PERIODIC, bool USE_D, bool CONE, bool COUNT>
+__global__ __launch_bounds__(256) void k_sweep_heis_q(const DevInst* __restrict__ insts,
+                                                      const QTile* __restrict__ tiles, int ntiles,
+                                                      const uint32_t* __restrict__ qlist,
+                                                      float* __restrict__ pool,
+                                                      const uint32_t* __restrict__ mpool,
+                                                      const uint64_t* __restrict__ sweeps,
+                                                      Stencil st, int colour, uint32_t s_off,
+                                                      uint32_t seed_lo, uint32_t seed_hi,
+                                                      unsigned long long* __restrict__ acc) {
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const QTile tl = tiles[t];
+    const DevInst& in = insts[tl.inst];
+    const int64_t Sp = in.Sp;
+    float* own_x = pool + in.base + (int64_t)colour * 3 * Sp;
+    const float* oth_x = pool + in.base + (int64_t)(colour ^ 1) * 3 * Sp;
+    int nacc = 0;
+    if ((int)threadIdx.x < tl.qn) {
+        const uint32_t e0 = qlist[tl.q0 + (int)threadIdx.x];
+        nacc = heis_quad<DIM3, PERIODIC, true, USE_D, CONE>(in, st, mpool + in.mask_base, own_x, own_x + Sp,
+                                                            own_x + 2 * Sp, oth_x, oth_x + Sp, oth_x + 2 * Sp, e0,
+                                                            colour, sweeps[tl.inst] + s_off, seed_lo, seed_hi);
+    }
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        tile_count(nacc, acc, t);
+    }
+}
+
 template <bool DIM3, bool PERIODIC, bool MASKED, bool COUNT>
 __global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__ insts,
                                                      const Tile* __restrict__ tiles, int ntiles,
@@ -444,6 +477,34 @@ hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool 
     NMC_HEIS_B(true, true, true)
     NMC_HEIS_B(false, true, true)
 #undef NMC_HEIS_B
+    return hipGetLastError();
+}
+
+#define NMC_HEISQ_CASE(D3, PER, UD, CN, CT)                                                      \
+    if (dim3 == D3 && periodic == PER && use_d == UD && cone == CN && count == CT)                \
+        k_sweep_heis_q<D3, PER, UD, CN, CT><<<ntiles, 256, 0, stream>>>(                          \
+            insts, tiles, ntiles, qlist, pool, mpool, sweeps, st, colour, s_off, seed_lo, seed_hi, acc);
+
+hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, bool use_d, bool cone, bool count,
+                               const DevInst* insts, const QTile* tiles, int ntiles, const uint32_t* qlist,
+                               float* pool, const uint32_t* mpool, const uint64_t* sweeps, const Stencil& st,
+                               int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                               unsigned long long* acc) {
+    if (ntiles <= 0) return hipSuccess;
+#define NMC_HEISQ_B(D3, PER)                  \
+    NMC_HEISQ_CASE(D3, PER, false, false, false) \
+    NMC_HEISQ_CASE(D3, PER, false, false, true)  \
+    NMC_HEISQ_CASE(D3, PER, false, true, false)  \
+    NMC_HEISQ_CASE(D3, PER, false, true, true)   \
+    NMC_HEISQ_CASE(D3, PER, true, false, false)  \
+    NMC_HEISQ_CASE(D3, PER, true, false, true)   \
+    NMC_HEISQ_CASE(D3, PER, true, true, false)   \
+    NMC_HEISQ_CASE(D3, PER, true, true, true)
+    NMC_HEISQ_B(true, false)
+    NMC_HEISQ_B(false, false)
+    NMC_HEISQ_B(true, true)
+    NMC_HEISQ_B(false, true)
+#undef NMC_HEISQ_B
     return hipGetLastError();
 }
 
