@@ -118,7 +118,7 @@ def test_block_move_quantities_match_oracle(orc, T, kappa0):
         lqo = netref.logq(mu[b], kappa[b], Tg[b].reshape(16, 16, 3)).sum()
         assert abs(stats[b, 1] - lqn) <= 0.05 * (1 + abs(lqn)), (b, stats[b, 1], lqn)
         assert abs(stats[b, 2] - lqo) <= 0.05 * (1 + abs(lqo)), (b, stats[b, 2], lqo)
-        # the decision: thread 0's Philox word 2, counter (b * 256, block purpose, round 1)
+        # the decision: thread 0's Philox word 2 at counter (b * 256, block purpose, first block move)
         w = orc.philox([b * 256, 3 << 8, 1, 0], [9, 0])
         u = np.float32(((int(w[2]) >> 8) + 0.5) / 16777216.0)
         loga = np.float32(-np.float32(1 / T) * stats[b, 0] + stats[b, 2] - stats[b, 1])

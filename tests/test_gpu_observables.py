@@ -1,0 +1,128 @@
+"""Sampled observables of the device sampler against exact answers and the
+FP64 oracle, each with its stated error (3 sigma, errors from the spread of
+independent replicas, so tau_int is accounted for):
+
+* 2-D Ising 4x4 periodic: <E>, <|M|>, C and chi against exact enumeration
+  of all 2^16 states;
+* 2-D Heisenberg 6x6 periodic with Neel DMI and a field: <E_s>, <|M|>, C,
+  chi and <|Q|> (plaquette charge with periodic wrap) against the oracle's
+  random-site Metropolis chain (the reference's state::equil with
+  dE_func, Old/lib/state.cpp:171-203), run in FP64 on the host.
+
+Everything is seeded: device sweeps draw from Philox counters and the
+reductions have a fixed order, so the outcome is the same on every run.
+Needs an MI355X."""
+import math
+
+import numpy as np
+import pytest
+
+from tests import exact
+from tests.fixtures import box_locs, neel2d
+
+pytestmark = pytest.mark.gpu
+
+
+def replica_stats(per_replica):
+    """Mean over replicas and its standard error."""
+    x = np.asarray(per_replica, dtype=np.float64)
+    return x.mean(), x.std(ddof=1) / math.sqrt(x.size)
+
+
+def estimators(E, M, beta, N):
+    """Per-replica <E>, <|M|>, C = beta^2 var(E) / N, chi = beta (<M^2> - <|M|>^2) / N."""
+    aM = np.abs(M) if M.ndim == 1 else np.linalg.norm(M, axis=-1)
+    return {"E": E.mean(), "absM": aM.mean(), "C": beta * beta * E.var() / N,
+            "chi": beta * ((aM * aM).mean() - aM.mean() ** 2) / N}
+
+
+@pytest.mark.parametrize("T", [1.8, 2.4, 3.5])
+def test_ising_4x4_observables_match_exact_enumeration(T):
+    from nanomagmc_amd import Bond, Lattices
+    L, R, ns = 4, 24, 4000
+    bonds = [Bond((0, 1), 1.0), Bond((0, -1), 1.0), Bond((1, 0), 1.0), Bond((-1, 0), 1.0)]
+    lat = Lattices(2, True, True, bonds, [{"n": (L, L), "edge": L}] * R, seed=0x4E4D4311 + int(10 * T))
+    beta = 1.0 / T
+    lat.set_params([beta] * R, 0.0)
+    lat.randomize()
+    lat.equil(500)
+    lat.series_reset(ns, with_charge=False)
+    lat.run_samples(ns, 3)
+    obs, _, _ = lat.get_series()
+    E_all, M_all = exact.ising_periodic_energies(L, L)
+    ex = exact.enum_averages(E_all, M_all, beta)
+    want = {"E": ex["E"], "absM": ex["absM"], "C": ex["C"] / (L * L),
+            "chi": beta * (ex["M2"] - ex["absM"] ** 2) / (L * L)}
+    per = [estimators(obs[i, :, 9], obs[i, :, 0], beta, L * L) for i in range(R)]
+    for k in want:
+        m, err = replica_stats([p[k] for p in per])
+        print("T=%.2f %-5s device %.5f +- %.5f exact %.5f" % (T, k, m, err, want[k]))
+        assert abs(m - want[k]) < 3 * err, (T, k, m, err, want[k])
+
+
+def oracle_chain(orc, f, H, beta, nsamples, sweeps, rng, burn=200):
+    """The reference's random-site Metropolis on the host in FP64: uniform
+    sphere trials, accept iff dE <= 0 or -dE*beta > log(u).  Returns the
+    E_s, M and Q_wrap series."""
+    n = f.n
+    s = rng.normal(size=(n, 3))
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+
+    def attempts(k):
+        z = rng.uniform(-1, 1, k)
+        ph = rng.uniform(0, 2 * math.pi, k)
+        r = np.sqrt(1 - z * z)
+        t = np.stack([r * np.cos(ph), r * np.sin(ph), z], axis=1)
+        return rng.integers(0, n, k), t, rng.uniform(size=k)
+
+    for _ in range(burn):
+        f.equil_replay(s, H, beta, *attempts(sweeps * n), use_d=True)
+    E = np.zeros(nsamples)
+    M = np.zeros((nsamples, 3))
+    Q = np.zeros(nsamples)
+    for k in range(nsamples):
+        f.equil_replay(s, H, beta, *attempts(sweeps * n), use_d=True)
+        E[k] = f.energy(s, H, use_d=True, convention=1)
+        M[k] = f.mag(s)
+        Q[k] = f.tcharge(s, wrap=True)[0]
+    return E, M, Q
+
+
+def test_heisenberg_dmi_observables_match_oracle_chain(orc):
+    from nanomagmc_amd import Bond, Lattices
+    L, D, h, T = 6, 0.8, 0.3, 1.0
+    beta = 1.0 / T
+    d, Jv, Dv = neel2d(D)
+    R, ns = 16, 8000
+    lat = Lattices(2, False, True, [Bond(d[k], Jv[k], Dv[k]) for k in range(4)], [{"n": (L, L), "edge": L}] * R,
+                   use_d=True, seed=0x4E4D4312)
+    lat.set_params([beta] * R, h)
+    lat.randomize()
+    lat.equil(500)
+    lat.series_reset(ns, with_charge=True)
+    lat.run_samples(ns, 5)
+    obs, _, qw = lat.get_series()
+    N = L * L
+    dev = []
+    for i in range(R):
+        st = estimators(obs[i, :, 10], obs[i, :, 0:3], beta, N)
+        st["absQ"] = np.abs(qw[i, :, 0]).mean()
+        dev.append(st)
+    f = orc.Field(box_locs(2, L), 2, L, 1, d, Jv, Dv)
+    rng = np.random.default_rng(0x4E4D4313)
+    ref = []
+    for _ in range(8):
+        E, M, Q = oracle_chain(orc, f, [0.0, 0.0, h], beta, 2500, 5, rng)
+        st = estimators(E, M, beta, N)
+        st["absQ"] = np.abs(Q).mean()
+        ref.append(st)
+    # the plaquette charge on a torus is an integer for every configuration
+    # (to ~1e-7 here: the stored fp32 spins are unit vectors only to 1e-7)
+    assert np.allclose(qw[:, :, 0], np.round(qw[:, :, 0]), atol=1e-5)
+    for k in ("E", "absM", "C", "chi", "absQ"):
+        m, e = replica_stats([p[k] for p in dev])
+        mr, er = replica_stats([p[k] for p in ref])
+        sig = math.hypot(e, er)
+        print("%-5s device %.5f +- %.5f   oracle %.5f +- %.5f   (%.2f sigma)" % (k, m, e, mr, er, abs(m - mr) / sig))
+        assert abs(m - mr) < 3 * sig, (k, m, e, mr, er)
+    assert np.mean([p["absQ"] for p in ref]) > 0.05     # the charge does fluctuate at this T

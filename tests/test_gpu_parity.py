@@ -286,7 +286,9 @@ def test_graph_replays_match_eager_loop(orc, ising):
 
 # ------------------------------------------------------------ statistics ----
 def test_ising_2d_energy_matches_kaufman(orc):
-    # configs[0]-style lattice: 2-D square L=64 periodic, J=1, h=0
+    # 2-D square Ising L=64 periodic, J=1, h=0 (the reference's CPU-runnable
+    # case): <E> against Kaufman's exact finite-lattice energy, and against
+    # Onsager's infinite-lattice u(T) within the stated finite-size term
     L = 64
     d, J, D = nn2d()
     Ts = [1.5, 2.0, 2.5, 3.5]
@@ -304,6 +306,9 @@ def test_ising_2d_energy_matches_kaufman(orc):
         m, err, tau = exact.mean_err(obs[i, :, 9])
         want = exact.kaufman_energy(L, T)
         assert abs(m - want) < 4 * err + 1e-9, (T, m, want, err, tau)
+        u = exact.onsager_energy_per_site(T)
+        finite_size = abs(want / (L * L) - u)
+        assert abs(m / (L * L) - u) < 4 * err / (L * L) + finite_size, (T, m / (L * L), u, finite_size)
 
 
 @pytest.mark.parametrize("trial", ["sphere", "cone"])
