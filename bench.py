@@ -239,7 +239,9 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath) and args.replicas == 1:   # measured for the default batch only
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        tj = json.load(open(tpath))
+        if tj.get("path", "auto") == (args.path or "auto"):   # and on the sweep path it was measured on
+            traffic = tj.get("hbm_bytes_per_launch")
     out = {
         "metric": "Metropolis spin-updates/s and \u03c4_int-normalised effective samples/s, 1 GPU",
         "value": value,
