@@ -6,6 +6,8 @@
 // and solid_angle (Old/lib/thermodynamics.cpp:71-221), field_type::all_rand
 // (Old/lib/field_type.cpp:197-204), state::add_to_av (Old/lib/state.cpp:
 // 288-296) and shape::weibull::check (Old/lib/shape.cpp:40-58).
+#include <algorithm>
+
 #include "nmc_device.h"
 #include "nmc_launch.h"
 
@@ -17,10 +19,14 @@ __device__ __forceinline__ bool mask_bit(const uint32_t* __restrict__ mp, const 
 
 // ---------------------------------------------------------------------------
 // Observables.  One workgroup = 4 waves over a range of rows of one
-// instance; lane l of a wave owns the columns k = l, l + 64, ... of each row
-// (sites i2 = 2k and 2k + 1), so every per-slice accumulator in LDS has a
-// single writer and the reduction order is fixed: results are bitwise
-// reproducible from run to run.  Accumulators are fp64.
+// instance.  A wave takes G rows at a time (G = 64 / half for rows shorter
+// than a wave, capped by the LDS the launch provides), lane group g = lane /
+// half owning row g of the G and column k = lane % half (long rows: one row,
+// columns k = lane, lane + 64, ...); a column covers sites i2 = 2k and
+// 2k + 1.  Every per-slice accumulator in LDS (one bank per wave and lane
+// group) therefore has a single writer, and all reductions run in a fixed
+// order: results are bitwise reproducible from run to run.  Accumulators are
+// fp64.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double solid_angle_d(double s1x, double s1y, double s1z, double s2x,
                                                 double s2y, double s2z, double s3x, double s3y,
@@ -65,20 +71,26 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
                                                  const void* __restrict__ pool,
                                                  const uint32_t* __restrict__ mpool, Stencil st,
                                                  double* __restrict__ partials, int pstride,
-                                                 int a2max) {
+                                                 int a2max, int gmax) {
 #pragma clang fp contract(off)
-    extern __shared__ double lds[];   // [4 waves][2][a2max] (3-D Heisenberg) + [4][15]
+    extern __shared__ double lds[];   // [4 waves][gmax][2][a2max] (3-D Heisenberg) + [4][17]
     const MTile tl = tiles[blockIdx.x];
     const DevInst& in = insts[tl.inst];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr bool TC = !ISING;
     constexpr bool TCS = TC && DIM3;   // per-slice charges in LDS
-    double* qs = lds;                  // [4][2][a2max]
-    double* red = lds + (TCS ? 8 * a2max : 0);
+    const int half = in.half;
+    const int G = half >= 64 ? 1 : min(gmax, 64 / half);
+    const int lrow = half >= 64 ? 0 : lane / half;           // lane group = row within the G
+    const int kbeg = half >= 64 ? lane : lane - lrow * half;
+    const int kstep = half >= 64 ? 64 : half;
+    double* qs = lds;                  // [4][G][2][a2max]
+    double* red = lds + (TCS ? 8 * gmax * a2max : 0);
     if (TCS) {
-        for (int i = threadIdx.x; i < 8 * a2max; i += 256) qs[i] = 0.0;
+        for (int i = threadIdx.x; i < 8 * G * a2max; i += 256) qs[i] = 0.0;
         __syncthreads();
     }
+    double* qbank = qs + (size_t)((wv * G + lrow) * 2) * a2max;
     double acc[15];
 #pragma unroll
     for (int i = 0; i < 15; ++i) acc[i] = 0.0;
@@ -87,10 +99,12 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
     const int sx_slot = DIM3 ? 0 : 2;   // first stencil slot of the reference x axis
     const int sy_slot = DIM3 ? 2 : 4;   // ... of the reference y axis
 
-    for (int row = tl.row0 + wv; row < tl.row0 + tl.nrows; row += 4) {
+    for (int rbase = tl.row0 + wv * G; rbase < tl.row0 + tl.nrows; rbase += 4 * G) {
+        const int row = rbase + lrow;
+        if (lrow >= G || row >= tl.row0 + tl.nrows) continue;   // lanes beyond G * half idle
         const int i0 = DIM3 ? row / in.a1 : 0;
         const int i1 = DIM3 ? row - i0 * in.a1 : row;
-        for (int k = lane; k < in.half; k += 64) {
+        for (int k = kbeg; k < half; k += kstep) {
             for (int q2 = 0; q2 < 2; ++q2) {
                 const int i2 = 2 * k + q2;
                 if (i2 >= in.n2) continue;
@@ -136,27 +150,40 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
                 if (TC) {
                     // calc_TC triangles (i, -x, +y) and (i, +x, -y); q_ref without
                     // periodic wrap (the reference's adj), q_wrap with it.
+                    // Where both unwrapped links exist the wrapped triangle is the
+                    // same triangle, so its angle is computed once.
                     double t_ref = 0.0, t_wrap = 0.0;
                     for (int tri = 0; tri < 2; ++tri) {
                         const int ga = tri == 0 ? sx_slot : sx_slot + 1;
                         const int gb = tri == 0 ? sy_slot + 1 : sy_slot;
-                        for (int wrap = 0; wrap < (PERIODIC ? 2 : 1); ++wrap) {
-                            const int na = wrap ? nbr_elem<true>(in, p, ga) : nbr_elem<false>(in, p, ga);
-                            const int nb = wrap ? nbr_elem<true>(in, p, gb) : nbr_elem<false>(in, p, gb);
-                            if (na < 0 || nb < 0) continue;
-                            if (MASKED && (!mask_bit(mpool, in, c ^ 1, na) || !mask_bit(mpool, in, c ^ 1, nb)))
+                        const int na = nbr_elem<false>(in, p, ga);
+                        const int nb = nbr_elem<false>(in, p, gb);
+                        const bool inner = na >= 0 && nb >= 0;
+                        int wa = na, wb = nb;
+                        if (PERIODIC && !inner) {
+                            wa = nbr_elem<true>(in, p, ga);
+                            wb = nbr_elem<true>(in, p, gb);
+                        }
+                        for (int wrap = 0; wrap < (PERIODIC && !inner ? 2 : 1); ++wrap) {
+                            const int ua = wrap ? wa : na, ub = wrap ? wb : nb;
+                            if (ua < 0 || ub < 0) continue;
+                            if (MASKED && (!mask_bit(mpool, in, c ^ 1, ua) || !mask_bit(mpool, in, c ^ 1, ub)))
                                 continue;
                             double a0, a1, a2, b0, b1, b2;
-                            load_spin<false>(pool, in, c ^ 1, na, a0, a1, a2);
-                            load_spin<false>(pool, in, c ^ 1, nb, b0, b1, b2);
+                            load_spin<false>(pool, in, c ^ 1, ua, a0, a1, a2);
+                            load_spin<false>(pool, in, c ^ 1, ub, b0, b1, b2);
                             const double ang = solid_angle_d(s0, s1, s2, a0, a1, a2, b0, b1, b2);
-                            if (wrap) t_wrap += ang; else t_ref += ang;
+                            if (wrap) t_wrap += ang;
+                            else {
+                                t_ref += ang;
+                                if (inner) t_wrap += ang;
+                            }
                         }
                     }
                     if (!PERIODIC) t_wrap = t_ref;
                     if (TCS) {
-                        qs[(wv * 2 + 0) * a2max + i2] += t_ref;
-                        qs[(wv * 2 + 1) * a2max + i2] += t_wrap;
+                        qbank[i2] += t_ref;
+                        qbank[a2max + i2] += t_wrap;
                     } else {
                         qr += t_ref;
                         qw += t_wrap;
@@ -185,7 +212,7 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
         for (int i = threadIdx.x; i < 2 * in.a2; i += 256) {
             const int which = i / in.a2, z = i - which * in.a2;
             double v = 0.0;
-            for (int w = 0; w < 4; ++w) v += qs[(w * 2 + which) * a2max + z];
+            for (int b = 0; b < 4 * G; ++b) v += qs[(b * 2 + which) * a2max + z];
             out[15 + which * a2max + z] = v;
         }
     }
@@ -454,11 +481,14 @@ hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool mas
                           const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                           int a2max) {
     if (ntiles <= 0) return hipSuccess;
-    const size_t shm = (size_t)(((!ising && dim3) ? 8 * a2max : 0) + 4 * 17) * sizeof(double);
+    // per-slice charge banks: 4 waves x gmax lane groups x 2 x a2max doubles;
+    // gmax = 64 / (a2max / 2) covers the longest rows exactly
+    const int gmax = std::max(1, std::min(64, 128 / std::max(2, a2max)));
+    const size_t shm = (size_t)(((!ising && dim3) ? 8 * gmax * a2max : 0) + 4 * 17) * sizeof(double);
 #define NMC_MEAS(D3, PER, MSK, IS)                                                           \
     if (dim3 == D3 && periodic == PER && masked == MSK && ising == IS)                       \
         k_measure<D3, PER, MSK, IS><<<ntiles, 256, shm, stream>>>(insts, t, pool, mpool, st, \
-                                                                  partials, pstride, a2max);
+                                                                  partials, pstride, a2max, gmax);
 #define NMC_MEAS_B(D3, PER) \
     NMC_MEAS(D3, PER, false, false) NMC_MEAS(D3, PER, true, false) NMC_MEAS(D3, PER, false, true) NMC_MEAS(D3, PER, true, true)
     NMC_MEAS_B(true, true)
