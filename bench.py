@@ -117,6 +117,26 @@ def adapt_cone(lat, theta, sweeps, sites):
     return theta, acc
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Read + write bytes per second of device-to-device copies of a 1 GiB
+    buffer (larger than the 256 MB Infinity Cache, so HBM-resident): the
+    practical ceiling next to the 8 TB/s peak."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    secs = e0.elapsed_time(e1) / 1e3
+    del a, b
+    return 2.0 * nbytes * reps / secs
+
+
 def cpu_baseline(budget_s=12.0):
     """The reference algorithm (oracle port) on this host: random-site
     single-site Metropolis over AoS double spins and vector<vector<int>>
@@ -260,7 +280,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "sweep colour pass (%d updates, %d B/update algorithmic)" % (total_sites // 2, bpu),
-                     "avg_launch_us": launch_s * 1e6},
+                     "avg_launch_us": launch_s * 1e6,
+                     "copy_GBps": copy_bandwidth() / 1e9},
     }
     if not args.no_ess:
         out["ess"] = ess_leg()
