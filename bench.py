@@ -158,27 +158,33 @@ def cpu_baseline(budget_s=12.0):
 
 
 def ess_leg():
-    """Effective samples/s of E and |Q| (Q_wrap, periodic plaquette sum),
-    plain sweeps vs sweeps + learned block rounds; tau_int in samples of 4
-    sweeps each, Sokal window c = 6 (`window_closed` false: the series is
-    shorter than the window, tau is a lower bound and ESS an upper bound)."""
+    """Effective samples/s of E and |Q| (Q_wrap, periodic plaquette sum):
+    plain sweeps, sweeps + learned vMF block proposals, and sweeps + learned
+    texture swaps; tau_int in samples of 4 sweeps each, Sokal window c = 6
+    (`window_closed` false: the series is shorter than the window, tau is a
+    lower bound and ESS an upper bound).  q_changes_per_ksweep counts the
+    samples whose integer charge differs from the previous one: a frozen
+    charge has a small tau_int without exploring anything."""
     from nanomagmc_amd.learn import compare_ess
+    methods = ("vmf", "swap")
     r = compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_sweeps=3000,
-                    collect_rounds=100, collect_blocks=1000, train_steps=2000, batch=256, nsamples=2000)
+                    collect_rounds=100, collect_blocks=1000, train_steps=2000, batch=256, nsamples=2000,
+                    methods=methods)
     rows = []
     for row in r["temps"]:
         def pick(m, k):
             st = row[m][k]
             return {"tau_int": st["tau_int"], "ess_per_s": st["ess_per_s"], "window_closed": st["window_closed"],
                     "mean": st["mean"], "err": st["err"]}
-        rows.append({"T": row["T"], "block_acceptance": row["acceptance"], "train_loss": row["loss"],
-                     "plain": {"E": pick("plain", "E"), "absQ": pick("plain", "absQ")},
-                     "learned": {"E": pick("learned", "E"), "absQ": pick("learned", "absQ")},
+        rows.append({"T": row["T"], "train_loss": row["loss"], "block_acceptance": row["acceptance"],
+                     **{m: {"E": pick(m, "E"), "absQ": pick(m, "absQ"),
+                            "q_changes_per_ksweep": row[m]["q_changes_per_ksweep"]} for m in ("plain",) + methods},
                      "gain_absQ": row["gain_absQ"], "gain_E": row["gain_E"]})
     return {"metric": "tau_int-normalised effective samples/s", "unit": "samples/s",
             "config": "2-D skyrmion L=512 periodic, J=1, Neel D=0.3, h_z=0.06, %d replicas per method and T, "
-                      "%d samples x 4 sweeps, learned: 1 block round (256 16x16 blocks) per 4 sweeps"
-                      % (r["replicas"], r["nsamples"]),
+                      "%d samples x 4 sweeps, learned: 1 round of 256 16x16 blocks per 4 sweeps "
+                      "(swap: %d relaxation steps, core radius %.1f)"
+                      % (r["replicas"], r["nsamples"], r["relax_iters"], r["core_radius"]),
             "train": {"blocks": r["train_blocks"], "steps": r["train_steps"], "batch": r["batch"],
                       "seconds": r["train_s"]},
             "s_per_sample": r["s_per_sample"], "temps": rows}

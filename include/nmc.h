@@ -259,6 +259,20 @@ int nmc_net_loss_grad(nmc_net* net, const float* patches, const float* targets, 
  * spins (row = x offset, column = y offset). */
 int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y,
                     int64_t* accepted, float* stats, float* proposal);
+/* One round of learned texture swaps on the same blocks.  Per block, from
+ * its surroundings only: texture A = the network's mean directions relaxed
+ * (relax_iters pairs of checkerboard Gauss-Seidel half-steps with
+ * over-relaxation factor 1.85, the 1-site ring held fixed) to the energy
+ * minimum; texture B = the same with a skyrmion of radius
+ * core_radius in the block centre, relaxed likewise.  The move rotates every
+ * block spin by pi about the bisector of A_i and B_i (an involution that maps
+ * A to B and preserves the measure) and is accepted with
+ * min(1, exp(-beta dE_s)).  No counterpart in the reference.  Test mode as
+ * above: stats gets (dE_s, sum_i |A_i - B_i|^2, 0) per block, proposal the
+ * rotated block. */
+int nmc_block_swaps(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y,
+                    int relax_iters, float core_radius, int64_t* accepted, float* stats,
+                    float* proposal);
 
 /* ---- lattice builder (state::init_lattice + shape::weibull::check) ---- */
 

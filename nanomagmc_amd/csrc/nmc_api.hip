@@ -1446,8 +1446,14 @@ int nmc_net_loss_grad(nmc_net* net, const float* patches, const float* targets, 
     return NMC_OK;
 }
 
-int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y, int64_t* accepted,
-                    float* stats, float* proposal) {
+}  // extern "C"
+
+namespace {
+
+// One round of learned block moves (swap = false: vMF proposals; true:
+// texture swaps) on each listed instance.
+int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y, bool swap,
+                int relax_iters, float core_radius, int64_t* accepted, float* stats, float* proposal) {
     if (!ctx || !net || !inst || n <= 0) return fail(NMC_E_INVALID, "bad argument");
     if (off_x < 0 || off_x >= 32 || off_y < 0 || off_y >= 32) return fail(NMC_E_INVALID, "block offset outside [0, 32)");
     int total = 0;
@@ -1493,10 +1499,17 @@ int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_
     hipError_t e = hipSuccess;
     for (int j = 0; j < n && e == hipSuccess; ++j) {
         const DevInst& d = ctx->din[inst[j]];
-        e = launch_block_mh(ctx->stream, (float*)ctx->pool, d.base, d.Sp, d.a1, d.a2, net->bxy + 2 * first[j],
-                            first[j + 1] - first[j], net->o + (size_t)first[j] * 256 * 32, ctx->st, d.beta, d.h, slo,
-                            shi, d.inst_id, round, net->acc + first[j], dstats ? dstats + 3 * first[j] : nullptr,
-                            dprop ? dprop + (size_t)768 * first[j] : nullptr);
+        float* sj = dstats ? dstats + 3 * first[j] : nullptr;
+        float* pj = dprop ? dprop + (size_t)768 * first[j] : nullptr;
+        if (swap)
+            e = launch_block_swap(ctx->stream, (float*)ctx->pool, d.base, d.Sp, d.a1, d.a2, net->bxy + 2 * first[j],
+                                  first[j + 1] - first[j], net->o + (size_t)first[j] * 256 * 32, ctx->st, d.beta,
+                                  d.h, relax_iters, core_radius, slo, shi, d.inst_id, round, net->acc + first[j], sj,
+                                  pj);
+        else
+            e = launch_block_mh(ctx->stream, (float*)ctx->pool, d.base, d.Sp, d.a1, d.a2, net->bxy + 2 * first[j],
+                                first[j + 1] - first[j], net->o + (size_t)first[j] * 256 * 32, ctx->st, d.beta, d.h,
+                                slo, shi, d.inst_id, round, net->acc + first[j], sj, pj);
     }
     std::vector<int> acc(total);
     if (e == hipSuccess)
@@ -1507,11 +1520,28 @@ int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_
         e = hipMemcpyAsync(proposal, dprop, (size_t)768 * total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(dstats); (void)hipFree(dprop);
-    if (e != hipSuccess) return fail(NMC_E_HIP, std::string("nmc_block_moves: ") + hipGetErrorString(e));
+    if (e != hipSuccess)
+        return fail(NMC_E_HIP, std::string(swap ? "nmc_block_swaps: " : "nmc_block_moves: ") + hipGetErrorString(e));
     if (accepted)
         for (int j = 0; j < n; ++j)
             accepted[j] = std::accumulate(acc.begin() + first[j], acc.begin() + first[j + 1], (int64_t)0);
     return NMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y, int64_t* accepted,
+                    float* stats, float* proposal) {
+    return block_round(ctx, net, inst, n, off_x, off_y, false, 0, 0.f, accepted, stats, proposal);
+}
+
+int nmc_block_swaps(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y, int relax_iters,
+                    float core_radius, int64_t* accepted, float* stats, float* proposal) {
+    if (relax_iters < 0 || relax_iters > 100000) return fail(NMC_E_INVALID, "relax_iters outside [0, 100000]");
+    if (!(core_radius > 0.f && core_radius <= 8.f)) return fail(NMC_E_INVALID, "core_radius outside (0, 8]");
+    return block_round(ctx, net, inst, n, off_x, off_y, true, relax_iters, core_radius, accepted, stats, proposal);
 }
 
 }  // extern "C"

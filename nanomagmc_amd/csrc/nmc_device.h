@@ -32,6 +32,7 @@ constexpr int kPurposeSweep = 0;
 constexpr int kPurposeInit = 1;
 constexpr int kPurposeBuild = 2;
 constexpr int kPurposeBlock = 3;
+constexpr int kPurposeSwap = 4;
 
 // Unsigned division by a runtime constant d >= 1, exact for every 32-bit n
 // (Granlund & Montgomery, PLDI'94, round-up variant with the add fix-up).

@@ -31,6 +31,10 @@ hipError_t launch_block_mh(hipStream_t s, float* pool, int64_t base, int64_t Sp,
                            int nblocks, const float* O, const Stencil& st, float beta, float h, uint32_t seed_lo,
                            uint32_t seed_hi, uint32_t inst, uint64_t round, int* accepted, float* stats,
                            float* proposal);
+hipError_t launch_block_swap(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
+                             int nblocks, const float* O, const Stencil& st, float beta, float h, int relax_iters,
+                             float core_radius, uint32_t seed_lo, uint32_t seed_hi, uint32_t inst, uint64_t round,
+                             int* accepted, float* stats, float* proposal);
 hipError_t launch_copy_rows(hipStream_t s, const void* src, void* dst, const int* idx, int n, int64_t row_bytes);
 hipError_t launch_put_patches(hipStream_t s, const float* in, int n, __bf16* X);
 

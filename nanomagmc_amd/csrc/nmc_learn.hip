@@ -439,6 +439,217 @@ __global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, int6
     }
 }
 
+// ---------------------------------------------------------------------------
+// Learned texture swaps.  Per 16 x 16 block, from the frozen surroundings
+// only: texture A = the network's mean directions relaxed to the energy
+// minimum with the 1-site ring around the block held fixed; texture B = the
+// same start with a skyrmion (core against the ring's mean direction) in the
+// block centre, relaxed likewise.  The move is x'_i = T_i x_i, T_i the
+// rotation by pi about the bisector of A_i and B_i: T_i A_i = B_i, T_i T_i =
+// 1 and T_i preserves the measure on S^2, so the move is a deterministic
+// involution and exact Metropolis accepts it with min(1, exp(-beta dE_s)).
+// Thermal deviations from A are carried to the same deviations from B, so
+// the energy cost is essentially E(B) - E(A): the block can gain or lose a
+// skyrmion in one step.
+// ---------------------------------------------------------------------------
+constexpr int kRing = 18;   // 16 x 16 block + 1-site ring
+
+__device__ __forceinline__ int ring_idx(int u, int v) { return (u + 1) * kRing + (v + 1); }
+
+// Over-relaxation factor of the texture relaxation: with it the 16 x 16
+// block converges in ~80 half-step pairs instead of ~500.
+constexpr float kRelaxOmega = 1.85f;
+
+// One over-relaxed checkerboard Gauss-Seidel half-step on the sites of parity
+// `par` (global parity of X + Y): s_i <- normalise(s_i + omega (F_i - s_i)),
+// F_i the unit local field (H + sum_j J s_j + s_j x D) / |..| that minimises
+// the site's energy.
+__device__ __forceinline__ void relax_half(float* S, int u, int v, int par, int X, int Y, const Stencil& st,
+                                           float h) {
+    if (((X + Y) & 1) != par) return;
+    const int du[4] = {0, 0, 1, -1}, dv[4] = {1, -1, 0, 0};
+    const int gs[4] = {5, 4, 3, 2};
+    float vx = 0.f, vy = 0.f, vz = h;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = ring_idx(u + du[q], v + dv[q]);
+        const float nx = S[r], ny = S[kRing * kRing + r], nz = S[2 * kRing * kRing + r];
+        const int g = gs[q];
+        vx += st.J[g] * nx + (ny * st.D[g][2] - nz * st.D[g][1]);
+        vy += st.J[g] * ny + (nz * st.D[g][0] - nx * st.D[g][2]);
+        vz += st.J[g] * nz + (nx * st.D[g][1] - ny * st.D[g][0]);
+    }
+    const float n2 = vx * vx + vy * vy + vz * vz;
+    if (n2 > 1e-30f) {
+        const float inv = 1.f / sqrtf(n2);
+        const int r = ring_idx(u, v);
+        const float cx = S[r], cy = S[kRing * kRing + r], cz = S[2 * kRing * kRing + r];
+        float mx = cx + kRelaxOmega * (vx * inv - cx);
+        float my = cy + kRelaxOmega * (vy * inv - cy);
+        float mz = cz + kRelaxOmega * (vz * inv - cz);
+        const float m2 = mx * mx + my * my + mz * mz;
+        if (m2 > 1e-30f) {
+            const float minv = 1.f / sqrtf(m2);
+            S[r] = mx * minv; S[kRing * kRing + r] = my * minv; S[2 * kRing * kRing + r] = mz * minv;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_block_swap(float* __restrict__ pool, int64_t base, int64_t Sp, int nx,
+                                                    int ny, const int* __restrict__ bxy, int nblocks,
+                                                    const float* __restrict__ O, Stencil st, float beta, float h,
+                                                    int relax_iters, float core_radius, uint32_t seed_lo,
+                                                    uint32_t seed_hi, uint32_t inst, uint64_t round,
+                                                    int* __restrict__ accepted, float* __restrict__ stats,
+                                                    float* __restrict__ proposal) {
+    constexpr int RR = kRing * kRing;
+    __shared__ float A[3 * RR], B[3 * RR];
+    __shared__ float nx_s[256], ny_s[256], nz_s[256];
+    __shared__ float red[2][4];
+    __shared__ float mring[3];
+    __shared__ int acc_s;
+    const int b = blockIdx.x;
+    if (b >= nblocks) return;
+    const int t = threadIdx.x;
+    const int u = t >> 4, v = t & 15;
+    const int bx = bxy[2 * b], by = bxy[2 * b + 1];
+    float* c0 = pool + base;
+    float* c1 = pool + base + 3 * Sp;
+    Lat2 L{c0, c1, Sp, nx, ny, ny / 2};
+    const int X = bx + u, Y = by + v;
+    // the fixed ring (corners unused) into both textures
+    for (int i = t; i < RR; i += 256) {
+        const int uu = i / kRing - 1, vv = i % kRing - 1;
+        if (uu == -1 || uu == 16 || vv == -1 || vv == 16) {
+            float sx, sy, sz;
+            spin_at(L, bx + uu, by + vv, sx, sy, sz);
+            A[i] = sx; A[RR + i] = sy; A[2 * RR + i] = sz;
+            B[i] = sx; B[RR + i] = sy; B[2 * RR + i] = sz;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {   // mean direction of the ring's edge sites, fixed order
+        float mx = 0.f, my = 0.f, mz = 0.f;
+        for (int i = 0; i < RR; ++i) {
+            const int uu = i / kRing - 1, vv = i % kRing - 1;
+            const bool edge = (uu == -1 || uu == 16) != (vv == -1 || vv == 16);
+            if (edge) { mx += A[i]; my += A[RR + i]; mz += A[2 * RR + i]; }
+        }
+        const float n2 = mx * mx + my * my + mz * mz;
+        if (n2 > 1e-20f) { const float inv = 1.f / sqrtf(n2); mx *= inv; my *= inv; mz *= inv; }
+        else { mx = 0.f; my = 0.f; mz = 1.f; }
+        mring[0] = mx; mring[1] = my; mring[2] = mz;
+    }
+    __syncthreads();
+    {   // starting textures: network means; B gets a skyrmion in the centre
+        const VMF f = vmf_unit(O + ((size_t)b * 256 + t) * 32);
+        const int r = ring_idx(u, v);
+        A[r] = f.mx; A[RR + r] = f.my; A[2 * RR + r] = f.mz;
+        const float du = (float)u - 7.5f, dv = (float)v - 7.5f;
+        const float rad = sqrtf(du * du + dv * dv);
+        if (rad < core_radius) {
+            // Neel template in the frame (e1, e2, m) of the ring direction m;
+            // the in-plane part points against D_{+x} . y (the chirality that
+            // lowers the DMI bond energy -s_i . (s_j x D))
+            const float chir = st.D[3][1] > 0.f ? -1.f : 1.f;
+            const float th = 3.14159265f * (1.f - rad / core_radius);
+            const float cph = rad > 0.f ? chir * du / rad : chir, sph = rad > 0.f ? chir * dv / rad : 0.f;
+            const float ta = sinf(th) * cph, tb = sinf(th) * sph, tc = cosf(th);
+            const float mx = mring[0], my = mring[1], mz = mring[2];
+            const float sgn = mz >= 0.f ? 1.f : -1.f;
+            const float a = -1.f / (sgn + mz);
+            const float bb = mx * my * a;
+            const float e1x = 1.f + sgn * mx * mx * a, e1y = sgn * bb, e1z = -sgn * mx;
+            const float e2x = bb, e2y = sgn + my * my * a, e2z = -my;
+            B[r] = ta * e1x + tb * e2x + tc * mx;
+            B[RR + r] = ta * e1y + tb * e2y + tc * my;
+            B[2 * RR + r] = ta * e1z + tb * e2z + tc * mz;
+        } else {
+            B[r] = f.mx; B[RR + r] = f.my; B[2 * RR + r] = f.mz;
+        }
+    }
+    __syncthreads();
+    for (int it = 0; it < relax_iters; ++it) {
+        for (int par = 0; par < 2; ++par) {
+            relax_half(A, u, v, par, X, Y, st, h);
+            relax_half(B, u, v, par, X, Y, st, h);
+            __syncthreads();
+        }
+    }
+    // the involution: pi rotation about the bisector of A_i and B_i
+    const int r = ring_idx(u, v);
+    const float ax = A[r], ay = A[RR + r], az = A[2 * RR + r];
+    const float qx = B[r], qy = B[RR + r], qz = B[2 * RR + r];
+    float wx = ax + qx, wy = ay + qy, wz = az + qz;
+    float w2 = wx * wx + wy * wy + wz * wz;
+    if (w2 < 1e-12f) {   // antipodal: any axis perpendicular to A_i
+        const float ex = fabsf(ax) < 0.9f ? 1.f : 0.f, ey = 1.f - ex;
+        wx = ay * 0.f - az * ey; wy = az * ex - ax * 0.f; wz = ax * ey - ay * ex;
+        w2 = wx * wx + wy * wy + wz * wz;
+    }
+    const float winv = 1.f / sqrtf(w2);
+    wx *= winv; wy *= winv; wz *= winv;
+    float sx, sy, sz;
+    spin_at(L, X, Y, sx, sy, sz);
+    const float d = 2.f * (wx * sx + wy * sy + wz * sz);
+    const float tx = d * wx - sx, ty = d * wy - sy, tz = d * wz - sz;
+    nx_s[t] = tx; ny_s[t] = ty; nz_s[t] = tz;
+    if (proposal) {
+        float* pr = proposal + ((size_t)b * 256 + t) * 3;
+        pr[0] = tx; pr[1] = ty; pr[2] = tz;
+    }
+    __syncthreads();
+    // dE_s over every bond touching the block (inside: half from each end)
+    float dE = -h * (tz - sz);
+    const int dxs[4] = {0, 0, 1, -1}, dys[4] = {1, -1, 0, 0};
+    const int gs[4] = {5, 4, 3, 2};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int uu = u + dxs[q], vv = v + dys[q];
+        float ox_, oy_, oz_, mx_, my_, mz_;
+        spin_at(L, X + dxs[q], Y + dys[q], ox_, oy_, oz_);
+        const bool in_b = uu >= 0 && uu < 16 && vv >= 0 && vv < 16;
+        if (in_b) { mx_ = nx_s[uu * 16 + vv]; my_ = ny_s[uu * 16 + vv]; mz_ = nz_s[uu * 16 + vv]; }
+        else { mx_ = ox_; my_ = oy_; mz_ = oz_; }
+        const int g = gs[q];
+        const float J = st.J[g];
+        const float Dx = st.D[g][0], Dy = st.D[g][1], Dz = st.D[g][2];
+        auto bond = [&](float px, float py, float pz, float cx, float cy, float cz) {
+            const float crx = cy * Dz - cz * Dy, cry = cz * Dx - cx * Dz, crz = cx * Dy - cy * Dx;
+            return -(J * (px * cx + py * cy + pz * cz) + px * crx + py * cry + pz * crz);
+        };
+        const float dd = bond(tx, ty, tz, mx_, my_, mz_) - bond(sx, sy, sz, ox_, oy_, oz_);
+        dE += in_b ? 0.5f * dd : dd;
+    }
+    const float sep = (ax - qx) * (ax - qx) + (ay - qy) * (ay - qy) + (az - qz) * (az - qz);
+    float vals[2] = {dE, sep};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        float s = vals[j];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((t & 63) == 0) red[j][t >> 6] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const float DE = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const float SEP = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        uint32_t w[4];
+        philox4x32_10((uint32_t)(b * 256), (uint32_t)(kPurposeSwap << 8), (uint32_t)round, (uint32_t)(round >> 32),
+                      seed_lo, seed_hi + inst, w);
+        const float loga = -beta * DE;
+        acc_s = (loga >= 0.f || logf(u24_open(w[2])) < loga) ? 1 : 0;
+        accepted[b] = acc_s;
+        if (stats) { stats[3 * b] = DE; stats[3 * b + 1] = SEP; stats[3 * b + 2] = 0.f; }
+    }
+    __syncthreads();
+    if (acc_s) {
+        const int xx = X % nx, yy = Y % ny;
+        float* c = ((xx + yy) & 1) ? c1 : c0;
+        const int64_t e = (int64_t)xx * (ny / 2) + (yy >> 1);
+        c[e] = tx; c[Sp + e] = ty; c[2 * Sp + e] = tz;
+    }
+}
+
 // Row gather (training minibatches): dst row i = src row idx[i]; rows are a
 // multiple of 16 bytes.
 __global__ void k_copy_rows(const uint4* __restrict__ src, uint4* __restrict__ dst, const int* __restrict__ idx,
@@ -525,6 +736,16 @@ hipError_t launch_head_loss(hipStream_t s, const float* O, const float* target, 
                             float* loss) {
     const int n = npatch * 256;
     k_head_loss<<<(n + 255) / 256, 256, 0, s>>>(O, target, npatch, scale, dO, loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_swap(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
+                             int nblocks, const float* O, const Stencil& st, float beta, float h, int relax_iters,
+                             float core_radius, uint32_t seed_lo, uint32_t seed_hi, uint32_t inst, uint64_t round,
+                             int* accepted, float* stats, float* proposal) {
+    if (nblocks <= 0) return hipSuccess;
+    k_block_swap<<<nblocks, 256, 0, s>>>(pool, base, Sp, nx, ny, bxy, nblocks, O, st, beta, h, relax_iters,
+                                         core_radius, seed_lo, seed_hi, inst, round, accepted, stats, proposal);
     return hipGetLastError();
 }
 

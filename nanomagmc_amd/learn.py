@@ -85,8 +85,32 @@ def block_moves(lat, net, inst, off_x, off_y, diagnostics=False):
     return (acc, stats, prop) if diagnostics else acc
 
 
+def block_swaps(lat, net, inst, off_x, off_y, relax_iters=80, core_radius=5.0, diagnostics=False):
+    """One round of learned texture swaps (create or remove a skyrmion per
+    block by an exact measure-preserving involution); returns the
+    accepted-block counts (and per-block (dE, |A - B|^2, 0) / rotated blocks
+    with diagnostics)."""
+    a = np.ascontiguousarray(np.asarray(inst, dtype=np.int32).reshape(-1))
+    acc = np.zeros(a.size, dtype=np.int64)
+    stats = prop = None
+    if diagnostics:
+        nb = sum((lat_edge(lat, i) // 32) ** 2 for i in a)
+        stats = np.zeros((nb, 3), dtype=np.float32)
+        prop = np.zeros((nb, 16, 16, 3), dtype=np.float32)
+    check(lib().nmc_block_swaps(lat._h, net._h, a.ctypes.data_as(C.POINTER(C.c_int)), int(a.size), int(off_x),
+                                int(off_y), int(relax_iters), float(core_radius),
+                                acc.ctypes.data_as(C.POINTER(C.c_int64)),
+                                _fptr(stats) if stats is not None else None,
+                                _fptr(prop) if prop is not None else None))
+    return (acc, stats, prop) if diagnostics else acc
+
+
 def lat_edge(lat, i):
-    return int(round(lat.num_sites(int(i)) ** 0.5))
+    """Edge of a learned-move lattice (periodic 2-D, so the full edge x edge box)."""
+    n = lat.num_sites(int(i))
+    edge = int(round(n ** 0.5))
+    assert edge * edge == n, "learned block moves need square periodic lattices"
+    return edge
 
 
 def neel_bonds(D):
@@ -102,21 +126,23 @@ def neel_bonds(D):
 
 def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_sweeps=3000,
                 collect_rounds=100, collect_blocks=1000, train_steps=2000, batch=256, lr=1e-3,
-                nsamples=2000, sweeps_per_round=4, timing_samples=100, seed=0x4E4D4305, log=None):
+                nsamples=2000, sweeps_per_round=4, timing_samples=100, methods=("vmf", "swap"),
+                relax_iters=80, core_radius=5.0, seed=0x4E4D4305, log=None):
     """Plain checkerboard sweeps against sweeps interleaved with learned block
     rounds (sweeps_per_round : 1), on the 2-D skyrmion Hamiltonian (J = 1,
-    Neel DMI D, field h along z, periodic L x L).
+    Neel DMI D, field h along z, periodic L x L).  Learned methods: "vmf"
+    (block proposals drawn from the network's von Mises-Fisher laws) and
+    "swap" (texture swaps built on the network's mean directions).
 
-    Per temperature: 2 * replicas instances (half plain, half learned) in
-    one context; all are annealed from random spins (T = 1 -> T over
-    anneal_sweeps); one network per temperature is trained on
-    collect_rounds x collect_blocks blocks of the plain instances; then
-    nsamples samples of E_s and |Q| (periodic plaquette sum) are taken,
-    each after sweeps_per_round sweeps (plus one block round for the
-    learned instances).  The cost of one sample of each method is timed
-    on its own.  Returns a dict with tau_int of E and |Q| (in samples),
-    acceptance, seconds per sample and tau_int-normalised effective
-    samples per second for both methods."""
+    Per temperature: (1 + len(methods)) * replicas instances in one context;
+    all are annealed from random spins (T = 1 -> T over anneal_sweeps); one
+    network per temperature is trained on collect_rounds x collect_blocks
+    blocks of the plain instances; then nsamples samples of E_s and |Q|
+    (periodic plaquette sum) are taken, each after sweeps_per_round sweeps
+    (plus one block round for the learned instances).  The cost of one
+    sample of each method is timed on its own.  Returns a dict with tau_int
+    of E and |Q| (in samples), acceptance, seconds per sample and
+    tau_int-normalised effective samples per second for every method."""
     import time
 
     from .engine import Lattices
@@ -125,11 +151,12 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
     rng = np.random.default_rng(seed & 0xFFFFFFFF)
     nT = len(temps)
     R = int(replicas)
-    insts = [{"n": (L, L), "edge": L}] * (nT * 2 * R)
+    names = ["plain"] + list(methods)
+    G = len(names)
+    insts = [{"n": (L, L), "edge": L}] * (nT * G * R)
     lat = Lattices(2, False, True, neel_bonds(D), insts, use_j=True, use_d=True, seed=seed)
-    plain = [[t * 2 * R + r for r in range(R)] for t in range(nT)]
-    learned = [[t * 2 * R + R + r for r in range(R)] for t in range(nT)]
-    target = np.repeat(np.asarray(temps, dtype=np.float64), 2 * R)
+    group = {nm: [[t * G * R + gi * R + r for r in range(R)] for t in range(nT)] for gi, nm in enumerate(names)}
+    target = np.repeat(np.asarray(temps, dtype=np.float64), G * R)
     lat.randomize()
     t0 = time.perf_counter()
     nstep = 20
@@ -141,6 +168,7 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
     say("annealed %d sweeps in %.1f s" % (anneal_sweeps, time.perf_counter() - t0))
     nets, losses = [], []
     t0 = time.perf_counter()
+    plain = group["plain"]
     for t in range(nT):
         net = Net(seed=seed + 17 * t)
         for k in range(collect_rounds):
@@ -154,35 +182,37 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
     train_s = time.perf_counter() - t0
     say("trained %d networks in %.1f s: %s" % (nT, train_s, losses))
 
-    def block_round(which):
+    def block_round(nm):
         acc = np.zeros(nT, dtype=np.int64)
         for t in range(nT):
-            ox, oy = rng.integers(0, 32, 2)
-            acc[t] = int(np.sum(block_moves(lat, nets[t], which[t], int(ox), int(oy))))
+            ox, oy = (int(x) for x in rng.integers(0, 32, 2))
+            if nm == "vmf":
+                a = block_moves(lat, nets[t], group[nm][t], ox, oy)
+            else:
+                a = block_swaps(lat, nets[t], group[nm][t], ox, oy, relax_iters, core_radius)
+            acc[t] = int(np.sum(a))
         return acc
 
-    flat_plain = [i for g in plain for i in g]
-    flat_learned = [i for g in learned for i in g]
     # cost of one sample of each method, each on its own instances
-    lat.sync()
-    t0 = time.perf_counter()
-    for _ in range(timing_samples):
-        lat.equil(sweeps_per_round, inst=flat_plain)
-    lat.sync()
-    t_plain = (time.perf_counter() - t0) / timing_samples
-    t0 = time.perf_counter()
-    for _ in range(timing_samples):
-        lat.equil(sweeps_per_round, inst=flat_learned)
-        block_round(learned)
-    lat.sync()
-    t_learned = (time.perf_counter() - t0) / timing_samples
+    cost = {}
+    for nm in names:
+        ids = [i for g in group[nm] for i in g]
+        lat.sync()
+        t0 = time.perf_counter()
+        for _ in range(timing_samples):
+            lat.equil(sweeps_per_round, inst=ids)
+            if nm != "plain":
+                block_round(nm)
+        lat.sync()
+        cost[nm] = (time.perf_counter() - t0) / timing_samples
     # production run
     lat.series_reset(nsamples, with_charge=True)
-    nacc = np.zeros(nT, dtype=np.int64)
+    nacc = {nm: np.zeros(nT, dtype=np.int64) for nm in methods}
     t0 = time.perf_counter()
     for k in range(nsamples):
         lat.equil(sweeps_per_round)
-        nacc += block_round(learned)
+        for nm in methods:
+            nacc[nm] += block_round(nm)
         lat.sample(k)
     lat.sync()
     say("sampled %d x %d sweeps in %.1f s" % (nsamples, sweeps_per_round, time.perf_counter() - t0))
@@ -190,15 +220,16 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
     blocks_per_round = (L // 32) ** 2
     out = {"L": L, "h": h, "D": D, "replicas": R, "nsamples": nsamples, "sweeps_per_round": sweeps_per_round,
            "train_blocks": collect_rounds * collect_blocks, "train_steps": train_steps, "batch": batch,
-           "train_s": train_s, "s_per_sample": {"plain": t_plain, "learned": t_learned}, "temps": []}
+           "train_s": train_s, "relax_iters": relax_iters, "core_radius": core_radius,
+           "s_per_sample": cost, "temps": []}
     for t, T in enumerate(temps):
         row = {"T": T, "loss": losses[t],
-               "acceptance": float(nacc[t]) / (nsamples * blocks_per_round * R)}
-        for name, ids in (("plain", plain[t]), ("learned", learned[t])):
+               "acceptance": {nm: float(nacc[nm][t]) / (nsamples * blocks_per_round * R) for nm in methods}}
+        for nm in names:
             st = {}
             for key, series in (("E", obs[:, :, 10]), ("absQ", np.abs(qw[:, :, 0]))):
                 taus, means, errs, closed = [], [], [], True
-                for i in ids:
+                for i in group[nm][t]:
                     _, tau, ok = ess(series[i])
                     m, e, _ = mean_err(series[i])
                     taus.append(tau)
@@ -206,14 +237,17 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
                     errs.append(e)
                     closed = closed and ok
                 tau = float(np.mean(taus))
-                cost = t_plain if name == "plain" else t_learned
-                st[key] = {"mean": float(np.mean(means)), "err": float(math.sqrt(np.sum(np.square(errs))) / len(ids)),
+                st[key] = {"mean": float(np.mean(means)), "err": float(math.sqrt(np.sum(np.square(errs))) / R),
                            "tau_int": tau, "window_closed": closed,
                            # effective samples per second over this method's instances of this temperature,
                            # charged the method's full per-sample cost share
-                           "ess_per_s": len(ids) / (2.0 * tau * cost / nT)}
-            row[name] = st
-        row["gain_absQ"] = row["learned"]["absQ"]["ess_per_s"] / row["plain"]["absQ"]["ess_per_s"]
-        row["gain_E"] = row["learned"]["E"]["ess_per_s"] / row["plain"]["E"]["ess_per_s"]
+                           "ess_per_s": R / (2.0 * tau * cost[nm] / nT)}
+            # topological changes: samples whose (integer) charge differs from the previous sample
+            q = np.rint(qw[group[nm][t], :, 0])
+            st["q_changes_per_ksweep"] = float((q[:, 1:] != q[:, :-1]).sum()) / (R * (nsamples - 1) *
+                                                                                  sweeps_per_round) * 1000.0
+            row[nm] = st
+        row["gain_absQ"] = {nm: row[nm]["absQ"]["ess_per_s"] / row["plain"]["absQ"]["ess_per_s"] for nm in methods}
+        row["gain_E"] = {nm: row[nm]["E"]["ess_per_s"] / row["plain"]["E"]["ess_per_s"] for nm in methods}
         out["temps"].append(row)
     return out

@@ -156,3 +156,77 @@ def init_params(rng):
         b[:NOUT[l]] = rng.uniform(-0.1, 0.1, NOUT[l])
         p += [W.reshape(-1), b]
     return np.concatenate(p)
+
+
+def swap_proposal(spins, L, bx, by, mu_raw, J, D, h, iters, core_radius):
+    """FP64 restatement of the learned texture swap of one 16 x 16 block of a
+    periodic L x L lattice (spins (L*L, 3), site index x * L + y).
+
+    J[g], D[g] (3-vector) per stencil slot g: 2 = (-1, 0), 3 = (+1, 0),
+    4 = (0, -1), 5 = (0, +1) (x = first coordinate).  mu_raw (16, 16, 3): the
+    network's unnormalised mean directions.  Relaxation: `iters` pairs of
+    checkerboard half-steps s <- normalise(s + 1.85 (F - s)), F the unit local
+    field.  Returns (x', A, B): the rotated block and the two relaxed
+    textures, each (16, 16, 3)."""
+    s = np.asarray(spins, dtype=np.float64).reshape(L, L, 3)
+    xs = np.arange(bx - 1, bx + 17) % L
+    ys = np.arange(by - 1, by + 17) % L
+    ring = s[np.ix_(xs, ys)].copy()                      # (18, 18, 3); index (u + 1, v + 1)
+    edge = np.zeros((18, 18), dtype=bool)
+    edge[0, 1:17] = edge[17, 1:17] = edge[1:17, 0] = edge[1:17, 17] = True
+    m = ring[edge].sum(axis=0)
+    nm = np.linalg.norm(m)
+    m = m / nm if nm > 1e-10 else np.array([0.0, 0.0, 1.0])
+    mu = mu_raw / np.linalg.norm(mu_raw, axis=-1, keepdims=True)
+    A = ring.copy()
+    A[1:17, 1:17] = mu
+    B = A.copy()
+    u, v = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    du, dv = u - 7.5, v - 7.5
+    rad = np.sqrt(du * du + dv * dv)
+    th = np.pi * (1 - rad / core_radius)
+    chir = -1.0 if D[3][1] > 0 else 1.0
+    cph, sph = chir * du / rad, chir * dv / rad
+    ta, tb, tc = np.sin(th) * cph, np.sin(th) * sph, np.cos(th)
+    sgn = 1.0 if m[2] >= 0 else -1.0
+    a = -1.0 / (sgn + m[2])
+    bb = m[0] * m[1] * a
+    e1 = np.array([1 + sgn * m[0] * m[0] * a, sgn * bb, -sgn * m[0]])
+    e2 = np.array([bb, sgn + m[1] * m[1] * a, -m[1]])
+    tmpl = ta[..., None] * e1 + tb[..., None] * e2 + tc[..., None] * m
+    core = rad < core_radius
+    B[1:17, 1:17][core] = tmpl[core]
+    par = (bx + u + by + v) & 1
+    nbrs = [((0, 1), 5), ((0, -1), 4), ((1, 0), 3), ((-1, 0), 2)]
+
+    omega = 1.85
+
+    def half(S, p):
+        H = np.zeros((16, 16, 3))
+        H[..., 2] = h
+        for (a_, b_), g in nbrs:
+            n = S[1 + a_:17 + a_, 1 + b_:17 + b_]
+            H += J[g] * n + np.cross(n, D[g])
+        nH = np.linalg.norm(H, axis=-1, keepdims=True)
+        cur = S[1:17, 1:17]
+        m = cur + omega * (H / np.maximum(nH, 1e-300) - cur)
+        new = m / np.linalg.norm(m, axis=-1, keepdims=True)
+        sel = (par == p) & (nH[..., 0] > 1e-30)
+        S[1:17, 1:17][sel] = new[sel]
+
+    for _ in range(iters):
+        for p in (0, 1):
+            half(A, p)
+            half(B, p)
+    Ab, Bb = A[1:17, 1:17], B[1:17, 1:17]
+    w = Ab + Bb
+    nw = np.linalg.norm(w, axis=-1, keepdims=True)
+    anti = nw[..., 0] < 1e-6
+    if anti.any():
+        e = np.where(np.abs(Ab[..., :1]) < 0.9, [1.0, 0.0, 0.0], [0.0, 1.0, 0.0])
+        w[anti] = np.cross(Ab, e)[anti]
+        nw = np.linalg.norm(w, axis=-1, keepdims=True)
+    w = w / nw
+    x = s[np.ix_(np.arange(bx, bx + 16) % L, np.arange(by, by + 16) % L)]
+    xp = 2 * (w * x).sum(-1, keepdims=True) * w - x
+    return xp, Ab, Bb

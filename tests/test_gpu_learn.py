@@ -209,8 +209,8 @@ def test_training_lowers_the_loss_and_keeps_the_physics():
 
 
 def test_learned_rounds_against_plain_sweeps_tau_and_ess():
-    # the comparison protocol at a reduced size (L = 128, T = 0.3): both
-    # methods run, tau_int of E and |Q| and the ESS/s ratio are reported
+    # the comparison protocol at a reduced size (L = 128, T = 0.3): every
+    # method runs, tau_int of E and |Q| and the ESS/s ratios are reported
     from nanomagmc_amd.learn import compare_ess
     r = compare_ess(L=128, temps=(0.3,), replicas=2, anneal_sweeps=2000, collect_rounds=20, collect_blocks=500,
                     train_steps=300, batch=128, nsamples=800, timing_samples=50, log=print)
@@ -218,10 +218,129 @@ def test_learned_rounds_against_plain_sweeps_tau_and_ess():
     print(row)
     assert r["train_blocks"] == 10_000
     assert row["loss"][1] < row["loss"][0]
-    assert 0.0 <= row["acceptance"] <= 1.0
-    for m in ("plain", "learned"):
+    for m in ("vmf", "swap"):
+        assert 0.0 <= row["acceptance"][m] <= 1.0
+        assert r["s_per_sample"][m] > r["s_per_sample"]["plain"] > 0
+        assert math.isfinite(row["gain_absQ"][m]) and math.isfinite(row["gain_E"][m])
+    for m in ("plain", "vmf", "swap"):
         for key in ("E", "absQ"):
             st = row[m][key]
             assert math.isfinite(st["mean"]) and st["tau_int"] >= 0.5 and st["ess_per_s"] > 0
-    assert r["s_per_sample"]["learned"] > r["s_per_sample"]["plain"] > 0
-    assert math.isfinite(row["gain_absQ"]) and math.isfinite(row["gain_E"])
+
+
+def stencil_slots(D):
+    """J and D per stencil slot (2 = -x, 3 = +x, 4 = -y, 5 = +y) of the Neel table."""
+    d, Jv, Dv = neel2d(D)
+    J6, D6 = np.zeros(6), np.zeros((6, 3))
+    slot = {(0, 1): 5, (0, -1): 4, (1, 0): 3, (-1, 0): 2}
+    for k in range(4):
+        g = slot[(int(d[k][0]), int(d[k][1]))]
+        J6[g], D6[g] = Jv[k], Dv[k]
+    return J6, D6
+
+
+def constant_net(mu, kappa=1.0):
+    """A network whose output is the same (mu, kappa) at every block site."""
+    from nanomagmc_amd.learn import Net
+    from oracle import netref
+    p = np.zeros(netref.n_params(), dtype=np.float32)
+    off = last_bias_offset()
+    p[off:off + 4] = list(mu) + [math.log(math.expm1(kappa - 0.05))]
+    net = Net(seed=5)
+    net.set_params(p)
+    return net
+
+
+def test_block_swaps_match_oracle(orc):
+    from nanomagmc_amd.learn import block_swaps
+    from oracle import netref
+    L, D, h, T = 64, 0.3, 0.06, 0.3
+    lat = lattice(L, D)
+    lat.set_params(1 / T, h)
+    rng = np.random.default_rng(8)
+    s = np.zeros((L * L, 3))
+    s[:, 2] = 1.0
+    s += 0.25 * rng.normal(size=s.shape)
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+    # one block starts near a relaxed skyrmion, so both directions of the swap occur
+    before = s.astype(np.float32).astype(np.float64)
+    lat.set_spins(0, before)
+    mu = (0.1, -0.05, 1.0)
+    net = constant_net(mu)
+    off = (3, 21)
+    iters, R0 = 80, 5.0
+    acc, stats, prop = block_swaps(lat, net, [0], *off, relax_iters=iters, core_radius=R0, diagnostics=True)
+    after = lat.get_spins(0)
+    J6, D6 = stencil_slots(D)
+    d, Jv, Dv = neel2d(D)
+    f = orc.Field(box_locs(2, L), 2, L, 1, d, Jv, Dv)
+    H = [0, 0, h]
+    E0 = f.energy(before, H, use_d=True, convention=1)
+    blocks = [(32 * i + off[0], 32 * j + off[1]) for i in range(2) for j in range(2)]
+    mu_raw = np.broadcast_to(np.asarray(mu, dtype=np.float64), (16, 16, 3))
+    naccepted = 0
+    for b, (bx, by) in enumerate(blocks):
+        xp, A, B = netref.swap_proposal(before, L, bx, by, mu_raw, J6, D6, h, iters, R0)
+        assert np.abs(prop[b] - xp).max() < 2e-3, (b, np.abs(prop[b] - xp).max())
+        sep = ((A - B) ** 2).sum()
+        assert abs(stats[b, 1] - sep) <= 1e-2 * sep + 1e-3, (b, stats[b, 1], sep)
+        assert sep > 10.0          # B holds a skyrmion that A does not
+        idx = ((np.arange(bx, bx + 16)[:, None] % L) * L + (np.arange(by, by + 16)[None, :] % L)).reshape(-1)
+        trial = before.copy()
+        trial[idx] = prop[b].reshape(256, 3)
+        dE = f.energy(trial, H, use_d=True, convention=1) - E0
+        assert abs(stats[b, 0] - dE) <= 2e-3 * (1 + abs(dE)), (b, stats[b, 0], dE)
+        w = orc.philox([b * 256, 4 << 8, 1, 0], [9, 0])
+        u = np.float32(((int(w[2]) >> 8) + 0.5) / 16777216.0)
+        loga = np.float32(-np.float32(1 / T) * stats[b, 0])
+        accept = bool(loga >= 0 or np.log(u) < loga)
+        if accept:
+            assert np.array_equal(after[idx], prop[b].reshape(256, 3).astype(np.float64))
+            naccepted += 1
+        else:
+            assert np.array_equal(after[idx], before[idx])
+    assert acc[0] == naccepted
+
+
+def test_block_swap_is_an_involution():
+    # at negligible beta every swap is accepted; two rounds on the same
+    # blocks give the lattice back
+    from nanomagmc_amd.learn import block_swaps
+    L, D, h = 64, 0.3, 0.06
+    lat = lattice(L, D)
+    lat.set_params(1e-7, h)
+    lat.randomize()
+    before = lat.get_spins(0)
+    net = constant_net((0.0, 0.0, 1.0))
+    a1 = block_swaps(lat, net, [0], 11, 5)
+    mid = lat.get_spins(0)
+    a2 = block_swaps(lat, net, [0], 11, 5)
+    after = lat.get_spins(0)
+    assert a1[0] == 4 and a2[0] == 4
+    assert np.abs(mid - before).max() > 0.5
+    assert np.abs(after - before).max() < 1e-5
+
+
+def test_block_swaps_keep_the_physics():
+    # T = 0.6: instance 0 plain sweeps, instance 1 sweeps + texture swaps
+    from nanomagmc_amd.learn import block_swaps
+    L, D, h, T = 64, 0.3, 0.06, 0.6
+    lat = lattice(L, D, ninst=2, seed=23)
+    lat.set_params(1 / T, h)
+    lat.randomize()
+    lat.equil(500)
+    net = constant_net((0.0, 0.0, 1.0))
+    rng = np.random.default_rng(4)
+    n = 600
+    lat.series_reset(n, with_charge=False)
+    acc = 0
+    for k in range(n):
+        lat.equil(2)
+        ox, oy = rng.integers(0, 32, 2)
+        acc += int(block_swaps(lat, net, [1], int(ox), int(oy))[0])
+        lat.sample(k)
+    obs, _, _ = lat.get_series()
+    e0, s0, _ = exact.mean_err(obs[0, 100:, 10])
+    e1, s1, _ = exact.mean_err(obs[1, 100:, 10])
+    print("swap acceptance %.4f, <E> plain %.2f +- %.2f, with swaps %.2f +- %.2f" % (acc / (n * 4), e0, s0, e1, s1))
+    assert abs(e0 - e1) < 4 * math.hypot(s0, s1), (e0, e1, s0, s1)
