@@ -1187,11 +1187,12 @@ int net_backward(nmc_net* net, int n) {
     NMC_HIP(launch_head_loss(net->stream, net->o, net->tgt, n, 1.0f / (256.0f * n), net->d_o, net->loss));
     __bf16* xin[] = {net->x0, net->y1, net->y2, net->y3};
     __bf16* dy[] = {net->dy1, net->dy2, net->dy3, net->d_o};
-    const int chunk = 4096;
+    const int chunk = 1024;   // output pixels per weight-gradient partial
     for (int l = 3; l >= 0; --l) {
         const int Hout = net->hin[l] - 2;
         const int M = n * Hout * Hout;
-        const size_t need = (size_t)((M + chunk - 1) / chunk) * net->K(l) * 32;
+        const size_t nch = (size_t)((M + chunk - 1) / chunk);
+        const size_t need = nch * net->K(l) * 32 + nch * 32;   // weight partials, then bias partials
         if (need > net->pw_cap) {
             NMC_HIP(hipStreamSynchronize(net->stream));
             (void)hipFree(net->pw);
@@ -1201,7 +1202,8 @@ int net_backward(nmc_net* net, int n) {
         }
         int nchunks = 0;
         NMC_HIP(launch_conv_wgrad(net->stream, xin[l], net->hin[l], net->cin[l], dy[l], n, chunk, net->pw, &nchunks));
-        NMC_HIP(launch_grad_reduce(net->stream, net->pw, nchunks, net->K(l), dy[l], M, net->gw[l], net->gb[l]));
+        NMC_HIP(launch_grad_reduce(net->stream, net->pw, nchunks, net->K(l), dy[l], M, chunk,
+                                   net->pw + nch * net->K(l) * 32, net->gw[l], net->gb[l]));
         if (l > 0)   // gradient wrt this layer's input, through the previous layer's ReLU
             NMC_HIP(launch_conv_dgrad(net->stream, dy[l], Hout, net->wd16[l], net->cin[l], n, xin[l], dy[l - 1]));
     }

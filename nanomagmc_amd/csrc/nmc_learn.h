@@ -18,8 +18,8 @@ hipError_t launch_conv_dgrad(hipStream_t s, const __bf16* dY, int Hout, const __
                              const __bf16* relu_in, __bf16* dX);
 hipError_t launch_conv_wgrad(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* dY, int npatch,
                              int chunk, float* Pw, int* nchunks_out);
-hipError_t launch_grad_reduce(hipStream_t s, const float* Pw, int nchunks, int K, const __bf16* dY, int M, float* gW,
-                              float* gB);
+hipError_t launch_grad_reduce(hipStream_t s, const float* Pw, int nchunks, int K, const __bf16* dY, int M,
+                              int chunk, float* Pb, float* gW, float* gB);
 hipError_t launch_adam(hipStream_t s, float* w, float* mw, float* vw, const float* g, int K, int Cin, int nout,
                        float lr, float b1, float b2, float eps, float bc1, float bc2, float* b, float* mb, float* vb,
                        const float* gb, __bf16* W16, __bf16* Wd16);

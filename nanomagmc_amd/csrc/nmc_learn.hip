@@ -179,20 +179,33 @@ __global__ void k_grad_reduce(const float* __restrict__ Pw, int nchunks, int K, 
     }
 }
 
-// Bias gradient gB[n] = sum over m of dY[m][n]: one workgroup, 32 row
-// groups x 32 channels, then a fixed-order sum of the 32 group partials.
-__global__ __launch_bounds__(1024) void k_bias_grad(const __bf16* __restrict__ dY, int M, float* __restrict__ gB) {
-    __shared__ float part[32][33];
+// Bias gradient gB[n] = sum over m of dY[m][n] in two deterministic stages:
+// workgroup c sums rows [c * chunk, (c + 1) * chunk) (8 row groups x 32
+// channels, then a fixed-order sum of the groups) into Pb[c][n]; then the
+// chunk partials are summed in chunk order.
+__global__ __launch_bounds__(256) void k_bias_partial(const __bf16* __restrict__ dY, int M, int chunk,
+                                                      float* __restrict__ Pb) {
+    __shared__ float part[8][33];
+    const int c = blockIdx.x;
     const int n = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int mbeg = c * chunk, mend = min(M, mbeg + chunk);
     float s = 0.f;
-    for (int m = g; m < M; m += 32) s += (float)dY[(size_t)m * 32 + n];
+    for (int m = mbeg + g; m < mend; m += 8) s += (float)dY[(size_t)m * 32 + n];
     part[g][n] = s;
     __syncthreads();
     if (threadIdx.x < 32) {
         float t = 0.f;
-        for (int j = 0; j < 32; ++j) t += part[j][threadIdx.x];
-        gB[threadIdx.x] = t;
+        for (int j = 0; j < 8; ++j) t += part[j][threadIdx.x];
+        Pb[(size_t)c * 32 + threadIdx.x] = t;
     }
+}
+
+__global__ void k_bias_grad(const float* __restrict__ Pb, int nchunks, float* __restrict__ gB) {
+    const int n = threadIdx.x;
+    if (n >= 32) return;
+    float t = 0.f;
+    for (int c = 0; c < nchunks; ++c) t += Pb[(size_t)c * 32 + n];
+    gB[n] = t;
 }
 
 // Adam on the fp32 master weights ([k][n] layout, columns >= nout frozen at
@@ -495,13 +508,15 @@ __device__ __forceinline__ void relax_half(float* S, int u, int v, int par, int 
     }
 }
 
-__global__ __launch_bounds__(256) void k_block_swap(float* __restrict__ pool, int64_t base, int64_t Sp, int nx,
+__global__ __launch_bounds__(512) void k_block_swap(float* __restrict__ pool, int64_t base, int64_t Sp, int nx,
                                                     int ny, const int* __restrict__ bxy, int nblocks,
                                                     const float* __restrict__ O, Stencil st, float beta, float h,
                                                     int relax_iters, float core_radius, uint32_t seed_lo,
                                                     uint32_t seed_hi, uint32_t inst, uint64_t round,
                                                     int* __restrict__ accepted, float* __restrict__ stats,
                                                     float* __restrict__ proposal) {
+    // 512 work-items: items 0..255 relax texture A, 256..511 texture B, one
+    // block site each; items 0..255 then apply and judge the move.
     constexpr int RR = kRing * kRing;
     __shared__ float A[3 * RR], B[3 * RR];
     __shared__ float nx_s[256], ny_s[256], nz_s[256];
@@ -511,14 +526,15 @@ __global__ __launch_bounds__(256) void k_block_swap(float* __restrict__ pool, in
     const int b = blockIdx.x;
     if (b >= nblocks) return;
     const int t = threadIdx.x;
-    const int u = t >> 4, v = t & 15;
+    const int site = t & 255, which = t >> 8;
+    const int u = site >> 4, v = site & 15;
     const int bx = bxy[2 * b], by = bxy[2 * b + 1];
     float* c0 = pool + base;
     float* c1 = pool + base + 3 * Sp;
     Lat2 L{c0, c1, Sp, nx, ny, ny / 2};
     const int X = bx + u, Y = by + v;
     // the fixed ring (corners unused) into both textures
-    for (int i = t; i < RR; i += 256) {
+    for (int i = t; i < RR; i += 512) {
         const int uu = i / kRing - 1, vv = i % kRing - 1;
         if (uu == -1 || uu == 16 || vv == -1 || vv == 16) {
             float sx, sy, sz;
@@ -542,12 +558,13 @@ __global__ __launch_bounds__(256) void k_block_swap(float* __restrict__ pool, in
     }
     __syncthreads();
     {   // starting textures: network means; B gets a skyrmion in the centre
-        const VMF f = vmf_unit(O + ((size_t)b * 256 + t) * 32);
+        const VMF f = vmf_unit(O + ((size_t)b * 256 + site) * 32);
         const int r = ring_idx(u, v);
-        A[r] = f.mx; A[RR + r] = f.my; A[2 * RR + r] = f.mz;
+        float* S = which ? B : A;
+        float px = f.mx, py = f.my, pz = f.mz;
         const float du = (float)u - 7.5f, dv = (float)v - 7.5f;
         const float rad = sqrtf(du * du + dv * dv);
-        if (rad < core_radius) {
+        if (which && rad < core_radius) {
             // Neel template in the frame (e1, e2, m) of the ring direction m;
             // the in-plane part points against D_{+x} . y (the chirality that
             // lowers the DMI bond energy -s_i . (s_j x D))
@@ -561,73 +578,75 @@ __global__ __launch_bounds__(256) void k_block_swap(float* __restrict__ pool, in
             const float bb = mx * my * a;
             const float e1x = 1.f + sgn * mx * mx * a, e1y = sgn * bb, e1z = -sgn * mx;
             const float e2x = bb, e2y = sgn + my * my * a, e2z = -my;
-            B[r] = ta * e1x + tb * e2x + tc * mx;
-            B[RR + r] = ta * e1y + tb * e2y + tc * my;
-            B[2 * RR + r] = ta * e1z + tb * e2z + tc * mz;
-        } else {
-            B[r] = f.mx; B[RR + r] = f.my; B[2 * RR + r] = f.mz;
+            px = ta * e1x + tb * e2x + tc * mx;
+            py = ta * e1y + tb * e2y + tc * my;
+            pz = ta * e1z + tb * e2z + tc * mz;
         }
+        S[r] = px; S[RR + r] = py; S[2 * RR + r] = pz;
     }
     __syncthreads();
     for (int it = 0; it < relax_iters; ++it) {
         for (int par = 0; par < 2; ++par) {
-            relax_half(A, u, v, par, X, Y, st, h);
-            relax_half(B, u, v, par, X, Y, st, h);
+            relax_half(which ? B : A, u, v, par, X, Y, st, h);
             __syncthreads();
         }
     }
-    // the involution: pi rotation about the bisector of A_i and B_i
-    const int r = ring_idx(u, v);
-    const float ax = A[r], ay = A[RR + r], az = A[2 * RR + r];
-    const float qx = B[r], qy = B[RR + r], qz = B[2 * RR + r];
-    float wx = ax + qx, wy = ay + qy, wz = az + qz;
-    float w2 = wx * wx + wy * wy + wz * wz;
-    if (w2 < 1e-12f) {   // antipodal: any axis perpendicular to A_i
-        const float ex = fabsf(ax) < 0.9f ? 1.f : 0.f, ey = 1.f - ex;
-        wx = ay * 0.f - az * ey; wy = az * ex - ax * 0.f; wz = ax * ey - ay * ex;
-        w2 = wx * wx + wy * wy + wz * wz;
-    }
-    const float winv = 1.f / sqrtf(w2);
-    wx *= winv; wy *= winv; wz *= winv;
-    float sx, sy, sz;
-    spin_at(L, X, Y, sx, sy, sz);
-    const float d = 2.f * (wx * sx + wy * sy + wz * sz);
-    const float tx = d * wx - sx, ty = d * wy - sy, tz = d * wz - sz;
-    nx_s[t] = tx; ny_s[t] = ty; nz_s[t] = tz;
-    if (proposal) {
-        float* pr = proposal + ((size_t)b * 256 + t) * 3;
-        pr[0] = tx; pr[1] = ty; pr[2] = tz;
+    float dE = 0.f, sep = 0.f, tx = 0.f, ty = 0.f, tz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    if (t < 256) {
+        // the involution: pi rotation about the bisector of A_i and B_i
+        const int r = ring_idx(u, v);
+        const float ax = A[r], ay = A[RR + r], az = A[2 * RR + r];
+        const float qx = B[r], qy = B[RR + r], qz = B[2 * RR + r];
+        float wx = ax + qx, wy = ay + qy, wz = az + qz;
+        float w2 = wx * wx + wy * wy + wz * wz;
+        if (w2 < 1e-12f) {   // antipodal: any axis perpendicular to A_i
+            const float ex = fabsf(ax) < 0.9f ? 1.f : 0.f, ey = 1.f - ex;
+            wx = -az * ey; wy = az * ex; wz = ax * ey - ay * ex;
+            w2 = wx * wx + wy * wy + wz * wz;
+        }
+        const float winv = 1.f / sqrtf(w2);
+        wx *= winv; wy *= winv; wz *= winv;
+        spin_at(L, X, Y, sx, sy, sz);
+        const float d = 2.f * (wx * sx + wy * sy + wz * sz);
+        tx = d * wx - sx; ty = d * wy - sy; tz = d * wz - sz;
+        nx_s[t] = tx; ny_s[t] = ty; nz_s[t] = tz;
+        if (proposal) {
+            float* pr = proposal + ((size_t)b * 256 + t) * 3;
+            pr[0] = tx; pr[1] = ty; pr[2] = tz;
+        }
+        sep = (ax - qx) * (ax - qx) + (ay - qy) * (ay - qy) + (az - qz) * (az - qz);
     }
     __syncthreads();
-    // dE_s over every bond touching the block (inside: half from each end)
-    float dE = -h * (tz - sz);
-    const int dxs[4] = {0, 0, 1, -1}, dys[4] = {1, -1, 0, 0};
-    const int gs[4] = {5, 4, 3, 2};
+    if (t < 256) {
+        // dE_s over every bond touching the block (inside: half from each end)
+        dE = -h * (tz - sz);
+        const int dxs[4] = {0, 0, 1, -1}, dys[4] = {1, -1, 0, 0};
+        const int gs[4] = {5, 4, 3, 2};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int uu = u + dxs[q], vv = v + dys[q];
-        float ox_, oy_, oz_, mx_, my_, mz_;
-        spin_at(L, X + dxs[q], Y + dys[q], ox_, oy_, oz_);
-        const bool in_b = uu >= 0 && uu < 16 && vv >= 0 && vv < 16;
-        if (in_b) { mx_ = nx_s[uu * 16 + vv]; my_ = ny_s[uu * 16 + vv]; mz_ = nz_s[uu * 16 + vv]; }
-        else { mx_ = ox_; my_ = oy_; mz_ = oz_; }
-        const int g = gs[q];
-        const float J = st.J[g];
-        const float Dx = st.D[g][0], Dy = st.D[g][1], Dz = st.D[g][2];
-        auto bond = [&](float px, float py, float pz, float cx, float cy, float cz) {
-            const float crx = cy * Dz - cz * Dy, cry = cz * Dx - cx * Dz, crz = cx * Dy - cy * Dx;
-            return -(J * (px * cx + py * cy + pz * cz) + px * crx + py * cry + pz * crz);
-        };
-        const float dd = bond(tx, ty, tz, mx_, my_, mz_) - bond(sx, sy, sz, ox_, oy_, oz_);
-        dE += in_b ? 0.5f * dd : dd;
-    }
-    const float sep = (ax - qx) * (ax - qx) + (ay - qy) * (ay - qy) + (az - qz) * (az - qz);
-    float vals[2] = {dE, sep};
+        for (int q = 0; q < 4; ++q) {
+            const int uu = u + dxs[q], vv = v + dys[q];
+            float ox_, oy_, oz_, mx_, my_, mz_;
+            spin_at(L, X + dxs[q], Y + dys[q], ox_, oy_, oz_);
+            const bool in_b = uu >= 0 && uu < 16 && vv >= 0 && vv < 16;
+            if (in_b) { mx_ = nx_s[uu * 16 + vv]; my_ = ny_s[uu * 16 + vv]; mz_ = nz_s[uu * 16 + vv]; }
+            else { mx_ = ox_; my_ = oy_; mz_ = oz_; }
+            const int g = gs[q];
+            const float J = st.J[g];
+            const float Dx = st.D[g][0], Dy = st.D[g][1], Dz = st.D[g][2];
+            auto bond = [&](float px, float py, float pz, float cx, float cy, float cz) {
+                const float crx = cy * Dz - cz * Dy, cry = cz * Dx - cx * Dz, crz = cx * Dy - cy * Dx;
+                return -(J * (px * cx + py * cy + pz * cz) + px * crx + py * cry + pz * crz);
+            };
+            const float dd = bond(tx, ty, tz, mx_, my_, mz_) - bond(sx, sy, sz, ox_, oy_, oz_);
+            dE += in_b ? 0.5f * dd : dd;
+        }
+        float vals[2] = {dE, sep};
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        float s = vals[j];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if ((t & 63) == 0) red[j][t >> 6] = s;
+        for (int j = 0; j < 2; ++j) {
+            float s2 = vals[j];
+            for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
+            if ((t & 63) == 0) red[j][t >> 6] = s2;
+        }
     }
     __syncthreads();
     if (t == 0) {
@@ -642,7 +661,7 @@ __global__ __launch_bounds__(256) void k_block_swap(float* __restrict__ pool, in
         if (stats) { stats[3 * b] = DE; stats[3 * b + 1] = SEP; stats[3 * b + 2] = 0.f; }
     }
     __syncthreads();
-    if (acc_s) {
+    if (t < 256 && acc_s) {
         const int xx = X % nx, yy = Y % ny;
         float* c = ((xx + yy) & 1) ? c1 : c0;
         const int64_t e = (int64_t)xx * (ny / 2) + (yy >> 1);
@@ -710,10 +729,11 @@ hipError_t launch_conv_wgrad(hipStream_t s, const __bf16* X, int Hin, int Cin, c
     return hipGetLastError();
 }
 
-hipError_t launch_grad_reduce(hipStream_t s, const float* Pw, int nchunks, int K, const __bf16* dY, int M, float* gW,
-                              float* gB) {
+hipError_t launch_grad_reduce(hipStream_t s, const float* Pw, int nchunks, int K, const __bf16* dY, int M,
+                              int chunk, float* Pb, float* gW, float* gB) {
     k_grad_reduce<<<(K * 32 + 255) / 256, 256, 0, s>>>(Pw, nchunks, K, gW);
-    k_bias_grad<<<1, 1024, 0, s>>>(dY, M, gB);
+    k_bias_partial<<<nchunks, 256, 0, s>>>(dY, M, chunk, Pb);
+    k_bias_grad<<<1, 64, 0, s>>>(Pb, nchunks, gB);
     return hipGetLastError();
 }
 
@@ -744,7 +764,7 @@ hipError_t launch_block_swap(hipStream_t s, float* pool, int64_t base, int64_t S
                              float core_radius, uint32_t seed_lo, uint32_t seed_hi, uint32_t inst, uint64_t round,
                              int* accepted, float* stats, float* proposal) {
     if (nblocks <= 0) return hipSuccess;
-    k_block_swap<<<nblocks, 256, 0, s>>>(pool, base, Sp, nx, ny, bxy, nblocks, O, st, beta, h, relax_iters,
+    k_block_swap<<<nblocks, 512, 0, s>>>(pool, base, Sp, nx, ny, bxy, nblocks, O, st, beta, h, relax_iters,
                                          core_radius, seed_lo, seed_hi, inst, round, accepted, stats, proposal);
     return hipGetLastError();
 }
