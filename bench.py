@@ -13,12 +13,14 @@ Other workloads (--workload): `grains` (512 Weibull spheres x 64 (T, h)
 points = 32768 uneven instances, Heisenberg + Bloch DMI, open boundaries),
 `ising2d` (2-D Ising L = 64 periodic, 21 temperatures).
 
-The `ess` object (skip with --no-ess) is the metric's second half:
-tau_int-normalised effective samples/s of E and |Q| on the 2-D skyrmion
+The metric's second half, tau_int-normalised effective samples/s (skip
+with --no-ess): `ess_plain` for the plain sampler on the benchmark batch
+itself (E and |M|, 400 samples of 5 sweeps), and `ess` on the 2-D skyrmion
 Hamiltonian (L = 512 periodic, J = 1, Neel D = 0.3, h_z = 0.06,
-T in {0.1, 0.2, 0.3}), for plain checkerboard sweeps and for sweeps
-interleaved 4:1 with learned 16 x 16 block proposals (one network per
-temperature trained on 1e5 blocks of plain-sampler configurations).
+T in {0.1, 0.2, 0.3}) for plain checkerboard sweeps and for sweeps
+interleaved 4:1 with learned 16 x 16 block rounds -- vMF block proposals and
+texture swaps (one network per temperature trained on 1e5 blocks of
+plain-sampler configurations).
 
 Prints one JSON line.
 """
@@ -157,6 +159,33 @@ def cpu_baseline(budget_s=12.0):
                       "lattice, T=1.4, h=0, uniform-sphere trials; %.1f s" % (threads, attempts, secs)}
 
 
+def plain_ess(lat, nsamples=400, samp_sweeps=5, extra_equil=1000):
+    """tau_int-normalised effective samples/s of the plain sampler on the
+    benchmark batch itself: after `extra_equil` more sweeps, `nsamples`
+    samples of samp_sweeps sweeps + measurement (one replayed HIP graph);
+    ESS/s = sum over chains of n / (2 tau_int) / wall time of the sampling."""
+    from nanomagmc_amd.stats import ess
+    lat.equil(extra_equil)
+    lat.sync()
+    lat.series_reset(nsamples, with_charge=False)
+    t0 = time.perf_counter()
+    lat.run_samples(nsamples, samp_sweeps)
+    lat.sync()
+    wall = time.perf_counter() - t0
+    obs, _, _ = lat.get_series()
+    out = {"samples": nsamples, "sweeps_per_sample": samp_sweeps, "chains": lat.ninst, "wall_s": wall}
+    for key, series in (("E", obs[:, :, 10]), ("absM", np.linalg.norm(obs[:, :, 0:3], axis=-1))):
+        n_eff, taus, closed = 0.0, [], 0
+        for i in range(lat.ninst):
+            ne, tau, ok = ess(series[i])
+            n_eff += ne
+            taus.append(tau)
+            closed += int(ok)
+        out[key] = {"ess_per_s": n_eff / wall, "tau_int_median": float(np.median(taus)),
+                    "tau_int_max": float(np.max(taus)), "windows_closed": closed}
+    return out
+
+
 def ess_leg():
     """Effective samples/s of E and |Q| (Q_wrap, periodic plaquette sum):
     plain sweeps, sweeps + learned vMF block proposals, and sweeps + learned
@@ -290,6 +319,8 @@ def main():
                      "copy_GBps": copy_bandwidth() / 1e9},
     }
     if not args.no_ess:
+        if args.workload == "heis3d" and args.replicas == 1:
+            out["ess_plain"] = plain_ess(lat)
         out["ess"] = ess_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
