@@ -1172,12 +1172,12 @@ int net_reserve(nmc_net* net, int n) {
 }
 
 // Forward pass of n patches already in x0; output pre-head in o.
-int net_forward(nmc_net* net, int n) {
-    __bf16* in[] = {net->x0, net->y1, net->y2, net->y3};
-    __bf16* out[] = {net->y1, net->y2, net->y3, nullptr};
-    for (int l = 0; l < 4; ++l)
-        NMC_HIP(launch_conv_fwd(net->stream, in[l], net->hin[l], net->cin[l], net->w16[l], net->b[l], n, l < 3 ? 1 : 0,
-                                out[l], l == 3 ? net->o : nullptr));
+// (keep: also store the hidden activations y1..y3 the backward pass reads.)
+int net_forward(nmc_net* net, int n, bool keep) {
+    const __bf16* W[4] = {net->w16[0], net->w16[1], net->w16[2], net->w16[3]};
+    const float* b[4] = {net->b[0], net->b[1], net->b[2], net->b[3]};
+    NMC_HIP(launch_net_fwd(net->stream, net->x0, n, W, b, keep ? net->y1 : nullptr, keep ? net->y2 : nullptr,
+                           keep ? net->y3 : nullptr, net->o));
     return NMC_OK;
 }
 
@@ -1396,7 +1396,7 @@ int nmc_net_train(nmc_net* net, int nsteps, int batch, float lr, uint64_t seed, 
         NMC_HIP(hipMemcpyAsync(net->idx, idx.data(), batch * sizeof(int), hipMemcpyHostToDevice, net->stream));
         NMC_HIP(launch_copy_rows(net->stream, net->data_x, net->x0, net->idx, batch, kPatchElems * 2));
         NMC_HIP(launch_copy_rows(net->stream, net->data_t, net->tgt, net->idx, batch, kTargetElems * 4));
-        if (int rc = net_forward(net, batch)) return rc;
+        if (int rc = net_forward(net, batch, true)) return rc;
         if (int rc = net_backward(net, batch)) return rc;
         if (int rc = net_adam(net, lr)) return rc;
         NMC_HIP(hipStreamSynchronize(net->stream));   // idx is reused next step
@@ -1420,7 +1420,7 @@ int nmc_net_loss_grad(nmc_net* net, const float* patches, const float* targets, 
     NMC_HIP(hipMemcpy(net->tgt, targets, (size_t)n * kTargetElems * sizeof(float), hipMemcpyHostToDevice));
     hipError_t e = launch_put_patches(net->stream, dp, n, net->x0);
     if (e != hipSuccess) { (void)hipFree(dp); return fail(NMC_E_HIP, hipGetErrorString(e)); }
-    int rc = net_forward(net, n);
+    int rc = net_forward(net, n, true);
     if (!rc) rc = net_backward(net, n);
     if (!rc) {
         e = hipStreamSynchronize(net->stream);
@@ -1490,7 +1490,7 @@ int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, i
     }
     hipStream_t keep = net->stream;
     net->stream = ctx->stream;
-    int rc = net_forward(net, total);
+    int rc = net_forward(net, total, false);
     net->stream = keep;
     if (rc) return rc;
     const uint64_t round = ++ctx->block_round;

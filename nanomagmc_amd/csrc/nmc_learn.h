@@ -12,8 +12,10 @@ constexpr int kHalo = 4;
 constexpr int kCin0 = 16;
 constexpr int kNetLayers = 4;
 
-hipError_t launch_conv_fwd(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* W, const float* bias,
-                           int npatch, int relu, __bf16* Ybf, float* Yf);
+// All four layers for npatch patches in one launch (activations in LDS);
+// y1..y3 (optional) receive the intermediate activations for the backward pass.
+hipError_t launch_net_fwd(hipStream_t s, const __bf16* X0, int npatch, const __bf16* const* W, const float* const* b,
+                          __bf16* y1, __bf16* y2, __bf16* y3, float* O);
 hipError_t launch_conv_dgrad(hipStream_t s, const __bf16* dY, int Hout, const __bf16* Wd, int Cin, int npatch,
                              const __bf16* relu_in, __bf16* dX);
 hipError_t launch_conv_wgrad(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* dY, int npatch,

@@ -36,50 +36,91 @@ __device__ __forceinline__ bf16x8 zero8() {
 
 // ---------------------------------------------------------------------------
 // Convolution GEMMs.
-//   fwd  : Y[m, n]  = sum_k X(m, k) W[n][k]          m = (p, oy, ox), k = (dy, dx, ci)
+//   fwd  : Y[m, n]  = sum_k X(m, k) W[n][k]          m = (p, oy, ox), k = (dy, dx, ci)  (k_net_fwd)
 //   dgrad: dX[m, ci] = sum_k dY(m, k) Wd[ci][k]       m = (p, iy, ix), k = (dy, dx, co)
 //   wgrad: dW[k, n] = sum_m X(m, k) dY[m, n]          reduction over m, split in chunks
-// Tiles: blockDim = 256 (4 waves); each wave owns one 32 x 32 tile.
+// Tiles: blockDim = 256 (4 waves); each wave owns 32 x 32 output tiles.
 // MFMA 32x32x16 operand layout: lane l holds A[row = l % 32][k0 + 8 (l / 32) .. +7],
 // B[k0 + 8 (l / 32) .. +7][col = l % 32]; result element v of lane l is
 // C[8 (v / 4) + 4 (l / 32) + (v % 4)][l % 32].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_conv_fwd(const __bf16* __restrict__ X, int Hin, int Cin,
-                                                  const __bf16* __restrict__ W, const float* __restrict__ bias,
-                                                  int npatch, int relu, __bf16* __restrict__ Ybf,
-                                                  float* __restrict__ Yf) {
-    const int Hout = Hin - 2;
-    const int M = npatch * Hout * Hout;
-    const int K = 9 * Cin;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + wave;
-    const int m0 = tile * 32;
-    if (m0 >= M) return;
-    const int row = lane & 31, kh = (lane >> 5) * 8;
-    const int m = m0 + row;
-    const bool mv = m < M;
-    int p = 0, oy = 0, ox = 0;
-    if (mv) { p = m / (Hout * Hout); const int r = m - p * Hout * Hout; oy = r / Hout; ox = r - oy * Hout; }
-    const int n = lane & 31;   // output channel of this lane's B column
-    f32x16 acc = {};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const int k = k0 + kh;
-        const int tap = k / Cin, ci = k - tap * Cin;
-        const int dy = tap / 3, dx = tap - dy * 3;
-        bf16x8 a = zero8();
-        if (mv) a = *(const bf16x8*)(X + (((size_t)p * Hin + oy + dy) * Hin + ox + dx) * Cin + ci);
-        const bf16x8 b = *(const bf16x8*)(W + (size_t)n * K + k);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
+// Fused forward pass: one workgroup per patch runs all four layers with the
+// activations resident in LDS (two regions used in turn: input -> R0,
+// layer 1 -> R1, layer 2 -> R0, layer 3 -> R1), so HBM sees only the input
+// patch, the weights (L2-resident) and the network output.  Each layer is an
+// implicit GEMM over 32 x 32 output tiles (wave w takes tiles w, w + 4, ...)
+// with fp32 accumulation; hidden activations are rounded to bf16 after bias
+// and ReLU, the last layer's output stays fp32.  The hidden activations are
+// also written to global memory when the backward pass needs them.
+struct NetW {
+    const __bf16* w[4];
+    const float* b[4];
+};
+
+template <int HIN, int CIN, bool LAST>
+__device__ __forceinline__ void fused_layer(const __bf16* X, const __bf16* __restrict__ W,
+                                            const float* __restrict__ bias, __bf16* Y, __bf16* __restrict__ Yg,
+                                            float* __restrict__ Yf, int wave, int lane) {
+    constexpr int HOUT = HIN - 2, M = HOUT * HOUT, K = 9 * CIN, TILES = (M + 31) / 32;
+    const int n = lane & 31, kh = (lane >> 5) * 8;
+    for (int tile = wave; tile < TILES; tile += 4) {
+        const int m0 = tile * 32;
+        const int m = m0 + (lane & 31);
+        const bool mv = m < M;
+        int oy = 0, ox = 0;
+        if (mv) { oy = m / HOUT; ox = m - oy * HOUT; }
+        f32x16 acc = {};
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            const int k = k0 + kh;
+            const int tap = k / CIN, ci = k - tap * CIN;
+            const int dy = tap / 3, dx = tap - dy * 3;
+            bf16x8 a = zero8();
+            if (mv) a = *(const bf16x8*)(X + ((oy + dy) * HIN + ox + dx) * CIN + ci);
+            const bf16x8 bv = *(const bf16x8*)(W + (size_t)n * K + k);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
+        }
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int mr = m0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (mr >= M) continue;
-        float y = acc[v] + bias[n];
-        if (relu) y = fmaxf(y, 0.f);
-        if (Ybf) Ybf[(size_t)mr * 32 + n] = (__bf16)y;
-        if (Yf) Yf[(size_t)mr * 32 + n] = y;
+        for (int v = 0; v < 16; ++v) {
+            const int mr = m0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            if (mr >= M) continue;
+            float y = acc[v] + bias[n];
+            if (LAST) {
+                Yf[(size_t)mr * 32 + n] = y;
+            } else {
+                y = fmaxf(y, 0.f);
+                Y[mr * 32 + n] = (__bf16)y;
+                if (Yg) Yg[(size_t)mr * 32 + n] = (__bf16)y;
+            }
+        }
     }
+}
+
+constexpr int kR0 = 20 * 20 * 32;   // bf16 elements: input patch (24*24*16) and layer-2 output
+constexpr int kR1 = 22 * 22 * 32;   // layer-1 output and layer-3 output (18*18*32)
+
+__global__ __launch_bounds__(256) void k_net_fwd(const __bf16* __restrict__ X0, int npatch, NetW w,
+                                                 __bf16* __restrict__ y1g, __bf16* __restrict__ y2g,
+                                                 __bf16* __restrict__ y3g, float* __restrict__ O) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __bf16* R0 = (__bf16*)lds_raw;
+    __bf16* R1 = R0 + kR0;
+    const int p = blockIdx.x;
+    if (p >= npatch) return;
+    const uint4* src = (const uint4*)(X0 + (size_t)p * (24 * 24 * 16));
+    uint4* dst = (uint4*)R0;
+    for (int i = threadIdx.x; i < 24 * 24 * 16 / 8; i += 256) dst[i] = src[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    fused_layer<24, 16, false>(R0, w.w[0], w.b[0], R1, y1g ? y1g + (size_t)p * (22 * 22 * 32) : nullptr, nullptr,
+                               wave, lane);
+    __syncthreads();
+    fused_layer<22, 32, false>(R1, w.w[1], w.b[1], R0, y2g ? y2g + (size_t)p * (20 * 20 * 32) : nullptr, nullptr,
+                               wave, lane);
+    __syncthreads();
+    fused_layer<20, 32, false>(R0, w.w[2], w.b[2], R1, y3g ? y3g + (size_t)p * (18 * 18 * 32) : nullptr, nullptr,
+                               wave, lane);
+    __syncthreads();
+    fused_layer<18, 32, true>(R1, w.w[3], w.b[3], nullptr, nullptr, O + (size_t)p * 256 * 32, wave, lane);
 }
 
 // dX (pre-activation of the layer below when relu_in != null) of a valid 3x3
@@ -702,11 +743,12 @@ hipError_t launch_put_patches(hipStream_t s, const float* in, int n, __bf16* X) 
     k_put_patches<<<(n * 576 + 255) / 256, 256, 0, s>>>(in, n, X);
     return hipGetLastError();
 }
-hipError_t launch_conv_fwd(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* W, const float* bias,
-                           int npatch, int relu, __bf16* Ybf, float* Yf) {
-    const int Hout = Hin - 2;
-    const int tiles = (npatch * Hout * Hout + 31) / 32;
-    k_conv_fwd<<<(tiles + 3) / 4, 256, 0, s>>>(X, Hin, Cin, W, bias, npatch, relu, Ybf, Yf);
+hipError_t launch_net_fwd(hipStream_t s, const __bf16* X0, int npatch, const __bf16* const* W, const float* const* b,
+                          __bf16* y1, __bf16* y2, __bf16* y3, float* O) {
+    if (npatch <= 0) return hipSuccess;
+    NetW w;
+    for (int l = 0; l < 4; ++l) { w.w[l] = W[l]; w.b[l] = b[l]; }
+    k_net_fwd<<<npatch, 256, (size_t)(kR0 + kR1) * sizeof(__bf16), s>>>(X0, npatch, w, y1, y2, y3, O);
     return hipGetLastError();
 }
 
