@@ -314,7 +314,10 @@ def main():
                    "setup_s": setup_s, "tuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "sweep colour pass (%d updates, %d B/update algorithmic)" % (total_sites // 2, bpu),
+                     "kernel": ("sweep colour pass (%d updates, %d B/update algorithmic)" % (total_sites // 2, bpu)
+                                if args.workload != "grains" or args.path == "stream" else
+                                "grain sweeps: compacted LDS-resident grains + streamed large grains; achieved = "
+                                "%d B/update algorithmic x updates / time (half a sweep per unit)" % bpu),
                      "avg_launch_us": launch_s * 1e6,
                      "copy_GBps": copy_bandwidth() / 1e9},
     }

@@ -80,6 +80,12 @@ struct Set {
     QTile* d_qt = nullptr;   // shaped Heisenberg lattices: colour-0 tiles, then colour-1 tiles
     int n_qt[2] = {0, 0};
     uint32_t* d_qlist = nullptr;   // their non-empty quads (first colour element of each)
+    CInst* d_cl = nullptr;         // shaped Heisenberg lattices swept compacted in LDS
+    int n_cl = 0;
+    uint32_t* d_cs_e = nullptr;    // their present sites (element index) ...
+    uint16_t* d_cs_nb = nullptr;   // ... and six neighbour slots per site
+    size_t c_lds_bytes = 0;
+    int c_block = 256;
     size_t lds_bytes = 0;    // dynamic LDS of the largest of them
     int lds_block = 256;
 
@@ -87,7 +93,9 @@ struct Set {
         (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
         (void)hipFree(d_mt); (void)hipFree(d_tb); (void)hipFree(d_te); (void)hipFree(d_lds);
         (void)hipFree(d_tacc); (void)hipFree(d_qt); (void)hipFree(d_qlist);
+        (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb);
         d_tacc = nullptr; d_qt = nullptr; d_qlist = nullptr;
+        d_cl = nullptr; d_cs_e = nullptr; d_cs_nb = nullptr;
         d_list = nullptr; d_sweep = nullptr; d_init = nullptr; d_mt = nullptr;
         d_tb = d_te = d_lds = nullptr;
     }
@@ -235,15 +243,26 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     std::vector<MTile> mt;
     std::vector<int> tb, te;
     const int stile = c->ising ? kIsingTile : kSweepTile;
-    std::vector<int> lds, qinst;
+    std::vector<int> lds, qinst, cinst;
     int64_t lds_work = 0;
     const bool quad_lists = c->masked && !c->ising;
+    const bool compact = quad_lists && !c->periodic && c->path == NMC_PATH_AUTO;
     for (int i : list) {
         const DevInst& d = c->din[i];
+        bool fits_compact = false;
+        if (compact) {
+            // present sites (12 B each) + the zero slot, compacted in LDS if they fit
+            int64_t np = 0;
+            const uint32_t* mw = c->h_mask.data() + d.mask_base;
+            for (int64_t w = 0; w < 2 * d.mask_stride; ++w) np += __builtin_popcount(mw[w]);
+            fits_compact = (size_t)(np + 1) * 12 <= c->max_lds && np + 1 <= 65535;
+        }
         // both colour arrays, then (masked lattices) both colours' presence bits
         const size_t bytes = (size_t)(c->ising ? 2 * d.Sp : 6 * d.Sp * (int64_t)sizeof(float)) +
                              (c->masked ? (size_t)(2 * d.mask_stride * 4 + 15) / 16 * 16 : 0);
-        if (c->path == NMC_PATH_AUTO && bytes <= c->max_lds) {
+        if (fits_compact) {
+            cinst.push_back(i);
+        } else if (c->path == NMC_PATH_AUTO && bytes <= c->max_lds) {
             lds.push_back(i);
             s->lds_bytes = std::max(s->lds_bytes, bytes);
             lds_work = std::max<int64_t>(lds_work, c->ising ? (d.S + 1) / 2 : (d.S + 3) / 4);
@@ -278,8 +297,66 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     }
     for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});   // accept counters follow the plain tiles
     const int n_plain = (int)sw.size() - (int)qt.size();
+    // compacted shaped lattices: present sites per colour, neighbour slots
+    std::vector<CInst> cl;
+    std::vector<uint32_t> cs_e;
+    std::vector<uint16_t> cs_nb;
+    int64_t cmax_colour = 0;
+    for (int i : cinst) {
+        const DevInst& d = c->din[i];
+        const uint32_t* mw = c->h_mask.data() + d.mask_base;
+        std::vector<int> slot[2];
+        CInst ci{i, 0, 0, (int64_t)cs_e.size()};
+        int next = 0;
+        for (int col = 0; col < 2; ++col) {
+            slot[col].assign((size_t)d.S, -1);
+            for (int64_t e = 0; e < d.S; ++e)
+                if ((mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 1u) {
+                    slot[col][e] = next++;
+                    cs_e.push_back((uint32_t)e);
+                }
+            (col ? ci.n1 : ci.n0) = next - (col ? ci.n0 : 0);
+        }
+        const int zero = ci.n0 + ci.n1;
+        cmax_colour = std::max<int64_t>(cmax_colour, std::max(ci.n0, ci.n1));
+        for (int col = 0; col < 2; ++col)
+            for (int64_t j = col ? ci.n0 : 0; j < (col ? zero : ci.n0); ++j) {
+                const int64_t e = cs_e[ci.site0 + j];
+                // position of element e (open box), then each stencil neighbour
+                const int64_t row = e / d.half, k = e - row * d.half;
+                const int i0 = c->dim3 ? (int)(row / d.a1) : 0;
+                const int i1 = c->dim3 ? (int)(row - (int64_t)i0 * d.a1) : (int)row;
+                const int q = (col + i0 + i1) & 1;
+                const int i2 = 2 * (int)k + q;
+                for (int g = 0; g < 6; ++g) {
+                    int n0 = i0, n1 = i1;
+                    int64_t kk = k;
+                    bool ok = true;
+                    if (g < 2) { if (!c->dim3) ok = false; n0 += (g & 1) ? 1 : -1; ok = ok && n0 >= 0 && n0 < d.a0; }
+                    else if (g < 4) { n1 += (g & 1) ? 1 : -1; ok = n1 >= 0 && n1 < d.a1; }
+                    else if (g & 1) { ok = i2 + 1 < d.n2; kk = k + q; }
+                    else { ok = i2 > 0; kk = k - 1 + q; }
+                    int sv = zero;
+                    if (ok) {
+                        const int64_t ne = ((int64_t)n0 * d.a1 + n1) * d.half + kk;
+                        const int t = slot[col ^ 1][ne];
+                        if (t >= 0) sv = t;
+                    }
+                    cs_nb.push_back((uint16_t)sv);
+                }
+            }
+        cl.push_back(ci);
+    }
     int rc;
     if ((rc = upload(c, &s->d_list, list))) return rc;
+    if ((rc = upload(c, &s->d_cl, cl))) return rc;
+    if ((rc = upload(c, &s->d_cs_e, cs_e))) return rc;
+    if ((rc = upload(c, &s->d_cs_nb, cs_nb))) return rc;
+    s->n_cl = (int)cl.size();
+    for (const CInst& ci : cl)
+        s->c_lds_bytes = std::max(s->c_lds_bytes, (size_t)(ci.n0 + ci.n1 + 1) * 3 * sizeof(float));
+    s->c_block = c->lds_block > 0 ? c->lds_block
+                                  : (int)std::min<int64_t>(1024, std::max<int64_t>(64, (cmax_colour + 63) / 64 * 64));
     if ((rc = upload(c, &s->d_qt, qt))) return rc;
     if ((rc = upload(c, &s->d_qlist, ql))) return rc;
     sw.resize(n_plain);
@@ -659,6 +736,13 @@ int release_sets(nmc_ctx* ctx) {
 // no synchronisation, so it can be recorded into a graph).
 int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
+    for (int k = 0; k < nsweeps && s->n_cl > 0; k += kLdsSweepChunk) {
+        const int chunk = std::min(kLdsSweepChunk, nsweeps - k);
+        NMC_HIP(launch_sweep_lds_c(ctx->stream, ctx->dim3, ctx->use_d, ctx->cone, std::min(s->n_cl, 65535),
+                                   s->c_block, s->c_lds_bytes, ctx->d_inst, s->d_cl, s->n_cl, (float*)ctx->pool,
+                                   s->d_cs_e, s->d_cs_nb, ctx->d_sweeps, ctx->st, chunk, (uint32_t)k, slo, shi,
+                                   ctx->d_acc));
+    }
     for (int k = 0; k < nsweeps && s->n_lds > 0; k += kLdsSweepChunk) {
         const int chunk = std::min(kLdsSweepChunk, nsweeps - k);
         NMC_HIP(launch_sweep_lds(ctx->stream, ctx->ising, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d,

@@ -108,6 +108,16 @@ struct QTile {
     int qn;
 };
 
+// Shaped instance in compacted form for LDS-resident sweeps: its present
+// sites, colour 0 first (n0 of them) then colour 1 (n1), stored from site0 on
+// in the per-set site tables (element index; six neighbour slots, each a
+// compact index into the other colour or n0 + n1 = the zero slot).
+struct CInst {
+    int inst;
+    int n0, n1;
+    int64_t site0;
+};
+
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                               uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll

@@ -446,6 +446,99 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Compacted LDS-resident sweeps of shaped Heisenberg lattices (open
+// boundaries): one workgroup per instance loads only the present sites into
+// LDS (12 B each, plus one zero slot for absent neighbours), runs `nsweeps`
+// sweeps there with precomputed neighbour slots, and stores them back.  A
+// site draws the Philox words its quad gives it in k_sweep_heis (element
+// e0 + m takes words 3m .. 3m + 2 of the quad's three blocks) and adds its
+// bonds in the same slot order, so both kernels produce the same lattice.
+// ---------------------------------------------------------------------------
+template <bool DIM3, bool USE_D, bool CONE>
+__global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict__ insts,
+                                                      const CInst* __restrict__ cl, int n,
+                                                      float* __restrict__ pool,
+                                                      const uint32_t* __restrict__ cs_e,
+                                                      const uint16_t* __restrict__ cs_nb,
+                                                      const uint64_t* __restrict__ sweeps, Stencil st, int nsweeps,
+                                                      uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                                                      unsigned long long* __restrict__ acc) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float* lx = (float*)lds_raw;
+    for (int li = blockIdx.x; li < n; li += gridDim.x) {
+        const CInst ci = cl[li];
+        const DevInst& in = insts[ci.inst];
+        const int N = ci.n0 + ci.n1;
+        float* ly = lx + (N + 1);
+        float* lz = ly + (N + 1);
+        const uint32_t* E = cs_e + ci.site0;
+        const uint16_t* NB = cs_nb + 6 * ci.site0;
+        const int64_t Sp = in.Sp;
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            const float* p = pool + in.base + (j < ci.n0 ? 0 : 3 * Sp);
+            const uint32_t e = E[j];
+            lx[j] = ld32(p, e); ly[j] = ld32(p + Sp, e); lz[j] = ld32(p + 2 * Sp, e);
+        }
+        if (threadIdx.x == 0) { lx[N] = 0.f; ly[N] = 0.f; lz[N] = 0.f; }
+        __syncthreads();
+        const uint64_t sweep0 = sweeps[ci.inst] + s_off;
+        const uint32_t k1 = seed_hi + in.inst_id;
+        int nacc = 0;
+        for (int s = 0; s < nsweeps; ++s) {
+            const uint64_t sw = sweep0 + (uint64_t)s;
+            const uint32_t slo = (uint32_t)sw, shi = (uint32_t)(sw >> 32);
+            for (int c = 0; c < 2; ++c) {
+                const int jb = c ? ci.n0 : 0, je = c ? N : ci.n0;
+                for (int j = jb + (int)threadIdx.x; j < je; j += blockDim.x) {
+                    const uint32_t e = E[j];
+                    const uint16_t* nb = NB + 6 * j;
+                    float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+                    for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+                        const int q = nb[g];
+                        add_bond<USE_D>(st, g, lx[q], ly[q], lz[q], vx, vy, vz);
+                    }
+                    const uint32_t q3 = (e >> 2) * 3u, m = e & 3u;
+                    uint32_t a[4], b[4];
+                    const uint32_t blk0 = (3u * m) >> 2;
+                    philox4x32_10(q3 + blk0, (uint32_t)c, slo, shi, seed_lo, k1, a);
+                    uint32_t wa, wb, wc;
+                    if (m == 0) { wa = a[0]; wb = a[1]; wc = a[2]; }
+                    else if (m == 3) { wa = a[1]; wb = a[2]; wc = a[3]; }
+                    else {
+                        philox4x32_10(q3 + blk0 + 1u, (uint32_t)c, slo, shi, seed_lo, k1, b);
+                        if (m == 1) { wa = a[3]; wb = b[0]; wc = b[1]; }
+                        else { wa = a[2]; wb = a[3]; wc = b[0]; }
+                    }
+                    float tx, ty, tz;
+                    if (heis_decide<CONE>(in, lx[j], ly[j], lz[j], vx, vy, vz, wa, wb, wc, tx, ty, tz)) {
+                        lx[j] = tx; ly[j] = ty; lz[j] = tz;
+                        ++nacc;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            float* p = pool + in.base + (j < ci.n0 ? 0 : 3 * Sp);
+            const uint32_t e = E[j];
+            st32(p, e, lx[j]); st32(p + Sp, e, ly[j]); st32(p + 2 * Sp, e, lz[j]);
+        }
+        // accepted attempts: this workgroup is the only writer of acc[inst]
+        __shared__ int part[16];
+        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = nacc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+            if (t) acc[ci.inst] += (unsigned long long)t;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
 #define NMC_HEIS_CASE(D3, PER, MSK, UD, CN, CT)                                                     \
@@ -545,6 +638,35 @@ static hipError_t lds_one(hipStream_t stream, int grid, int block, size_t shm, c
     k<<<grid, block, shm, stream>>>(insts, list, n, pool, mpool, tab, sweeps, st, nsweeps, s_off, seed_lo, seed_hi,
                                    acc);
     return hipGetLastError();
+}
+
+template <bool D3, bool UD, bool CN>
+static hipError_t lds_c_one(hipStream_t stream, int grid, int block, size_t shm, const DevInst* insts,
+                            const CInst* cl, int n, float* pool, const uint32_t* cs_e, const uint16_t* cs_nb,
+                            const uint64_t* sweeps, const Stencil& st, int nsweeps, uint32_t s_off,
+                            uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc) {
+    auto k = k_sweep_lds_c<D3, UD, CN>;
+    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return e;
+    k<<<grid, block, shm, stream>>>(insts, cl, n, pool, cs_e, cs_nb, sweeps, st, nsweeps, s_off, seed_lo, seed_hi,
+                                   acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, bool use_d, bool cone, int grid, int block, size_t shm,
+                              const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
+                              const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st, int nsweeps,
+                              uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc) {
+    if (n <= 0 || nsweeps <= 0) return hipSuccess;
+#define NMC_LDSC(D3, UD, CN)                          \
+    if (dim3 == D3 && use_d == UD && cone == CN)      \
+        return lds_c_one<D3, UD, CN>(stream, grid, block, shm, insts, cl, n, pool, cs_e, cs_nb, sweeps, st, \
+                                     nsweeps, s_off, seed_lo, seed_hi, acc);
+    NMC_LDSC(true, false, false) NMC_LDSC(true, false, true) NMC_LDSC(true, true, false) NMC_LDSC(true, true, true)
+    NMC_LDSC(false, false, false) NMC_LDSC(false, false, true) NMC_LDSC(false, true, false)
+    NMC_LDSC(false, true, true)
+#undef NMC_LDSC
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool periodic, bool masked, bool use_d,

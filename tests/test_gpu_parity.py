@@ -230,7 +230,8 @@ def test_checkerboard_trajectory_identical(orc, bloch3d, dim, edge, periodic, ma
 
 @pytest.mark.parametrize("ising,dim,periodic,masked,trial", [
     (True, 2, 1, False, "sphere"), (True, 3, 0, True, "sphere"),
-    (False, 2, 1, False, "sphere"), (False, 3, 0, True, "cone"), (False, 3, 1, False, "cone")])
+    (False, 2, 1, False, "sphere"), (False, 3, 0, True, "cone"), (False, 3, 1, False, "cone"),
+    (False, 2, 0, True, "sphere")])
 def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, periodic, masked, trial):
     # the LDS-resident launch (whole sweeps in one workgroup) and the
     # streaming launches (one per colour pass) draw the same numbers
@@ -256,6 +257,33 @@ def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, period
     for i in range(3):
         assert np.array_equal(lats[0].get_spins(i), lats[1].get_spins(i)), i
     assert np.array_equal(lats[0].accepts(), lats[1].accepts())
+
+
+def test_compacted_and_streamed_grains_agree(bloch3d, orc):
+    # one set holding a grain small enough for the compacted LDS sweep and one
+    # too large for it (streamed over its non-empty quads): the lattice and the
+    # accept counts must equal an all-streaming run bit for bit
+    from nanomagmc_amd import Bond, Lattices
+    rng = np.random.default_rng(5)
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
+    bonds = [Bond(bd[k], J[k], D[k]) for k in range(len(J))]
+    insts = []
+    for n in (10, 32):
+        m = (rng.random(n ** 3) < 0.75).astype(np.uint8)
+        insts.append({"n": (n, n, n), "edge": n, "mask": m})
+    runs = []
+    for path in ("auto", "stream"):
+        lat = Lattices(3, False, False, bonds, insts, use_d=True, trial="cone", seed=91)
+        lat.set_path(path)
+        lat.set_params([0.7, 1.3], [0.05, 0.0])
+        lat.set_cone([0.8, 1.1])
+        lat.randomize()
+        lat.equil(5)
+        lat.equil(260, inst=[0, 1])
+        runs.append(lat)
+    for i in range(2):
+        assert np.array_equal(runs[0].get_spins(i), runs[1].get_spins(i)), i
+    assert np.array_equal(runs[0].accepts(), runs[1].accepts())
 
 
 @pytest.mark.parametrize("ising", [True, False])
