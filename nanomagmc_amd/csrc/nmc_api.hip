@@ -12,6 +12,7 @@
 #include <random>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nmc.h"
@@ -297,55 +298,75 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     }
     for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});   // accept counters follow the plain tiles
     const int n_plain = (int)sw.size() - (int)qt.size();
-    // compacted shaped lattices: present sites per colour, neighbour slots
-    std::vector<CInst> cl;
-    std::vector<uint32_t> cs_e;
-    std::vector<uint16_t> cs_nb;
-    int64_t cmax_colour = 0;
-    for (int i : cinst) {
-        const DevInst& d = c->din[i];
+    // compacted shaped lattices: present sites per colour, neighbour slots;
+    // instances are independent, so their tables are filled in parallel
+    std::vector<CInst> cl(cinst.size());
+    int64_t total_sites = 0, cmax_colour = 0;
+    for (size_t q = 0; q < cinst.size(); ++q) {
+        const DevInst& d = c->din[cinst[q]];
         const uint32_t* mw = c->h_mask.data() + d.mask_base;
-        std::vector<int> slot[2];
-        CInst ci{i, 0, 0, (int64_t)cs_e.size()};
-        int next = 0;
-        for (int col = 0; col < 2; ++col) {
-            slot[col].assign((size_t)d.S, -1);
-            for (int64_t e = 0; e < d.S; ++e)
-                if ((mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 1u) {
-                    slot[col][e] = next++;
-                    cs_e.push_back((uint32_t)e);
-                }
-            (col ? ci.n1 : ci.n0) = next - (col ? ci.n0 : 0);
-        }
-        const int zero = ci.n0 + ci.n1;
-        cmax_colour = std::max<int64_t>(cmax_colour, std::max(ci.n0, ci.n1));
+        int cnt[2] = {0, 0};
         for (int col = 0; col < 2; ++col)
-            for (int64_t j = col ? ci.n0 : 0; j < (col ? zero : ci.n0); ++j) {
-                const int64_t e = cs_e[ci.site0 + j];
+            for (int64_t w = 0; w < d.mask_stride; ++w) cnt[col] += __builtin_popcount(mw[col * d.mask_stride + w]);
+        cl[q] = CInst{cinst[q], cnt[0], cnt[1], total_sites};
+        total_sites += cnt[0] + cnt[1];
+        cmax_colour = std::max<int64_t>(cmax_colour, std::max(cnt[0], cnt[1]));
+    }
+    std::vector<uint32_t> cs_e((size_t)total_sites);
+    std::vector<uint16_t> cs_nb((size_t)total_sites * 6);
+    auto fill = [&](size_t qb, size_t qe) {
+        std::vector<int> slot[2];
+        for (size_t q = qb; q < qe; ++q) {
+            const CInst& ci = cl[q];
+            const DevInst& d = c->din[ci.inst];
+            const uint32_t* mw = c->h_mask.data() + d.mask_base;
+            uint32_t* E = cs_e.data() + ci.site0;
+            uint16_t* NB = cs_nb.data() + 6 * ci.site0;
+            int next = 0;
+            for (int col = 0; col < 2; ++col) {
+                slot[col].assign((size_t)d.S, -1);
+                for (int64_t e = 0; e < d.S; ++e)
+                    if ((mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 1u) {
+                        slot[col][e] = next;
+                        E[next++] = (uint32_t)e;
+                    }
+            }
+            const int zero = ci.n0 + ci.n1;
+            for (int j = 0; j < zero; ++j) {
+                const int col = j < ci.n0 ? 0 : 1;
+                const int64_t e = E[j];
                 // position of element e (open box), then each stencil neighbour
                 const int64_t row = e / d.half, k = e - row * d.half;
                 const int i0 = c->dim3 ? (int)(row / d.a1) : 0;
                 const int i1 = c->dim3 ? (int)(row - (int64_t)i0 * d.a1) : (int)row;
-                const int q = (col + i0 + i1) & 1;
-                const int i2 = 2 * (int)k + q;
+                const int par = (col + i0 + i1) & 1;
+                const int i2 = 2 * (int)k + par;
                 for (int g = 0; g < 6; ++g) {
                     int n0 = i0, n1 = i1;
                     int64_t kk = k;
-                    bool ok = true;
-                    if (g < 2) { if (!c->dim3) ok = false; n0 += (g & 1) ? 1 : -1; ok = ok && n0 >= 0 && n0 < d.a0; }
+                    bool ok;
+                    if (g < 2) { n0 += (g & 1) ? 1 : -1; ok = c->dim3 && n0 >= 0 && n0 < d.a0; }
                     else if (g < 4) { n1 += (g & 1) ? 1 : -1; ok = n1 >= 0 && n1 < d.a1; }
-                    else if (g & 1) { ok = i2 + 1 < d.n2; kk = k + q; }
-                    else { ok = i2 > 0; kk = k - 1 + q; }
+                    else if (g & 1) { ok = i2 + 1 < d.n2; kk = k + par; }
+                    else { ok = i2 > 0; kk = k - 1 + par; }
                     int sv = zero;
                     if (ok) {
-                        const int64_t ne = ((int64_t)n0 * d.a1 + n1) * d.half + kk;
-                        const int t = slot[col ^ 1][ne];
+                        const int t = slot[col ^ 1][((int64_t)n0 * d.a1 + n1) * d.half + kk];
                         if (t >= 0) sv = t;
                     }
-                    cs_nb.push_back((uint16_t)sv);
+                    NB[6 * j + g] = (uint16_t)sv;
                 }
             }
-        cl.push_back(ci);
+        }
+    };
+    {
+        const size_t nq = cl.size();
+        const size_t nt = std::max<size_t>(1, std::min<size_t>(16, std::min<size_t>(
+                                                                       std::thread::hardware_concurrency(), nq / 64)));
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(fill, nq * t / nt, nq * (t + 1) / nt);
+        fill(0, nq / nt);
+        for (auto& th : pool) th.join();
     }
     int rc;
     if ((rc = upload(c, &s->d_list, list))) return rc;
