@@ -142,6 +142,10 @@ struct nmc_ctx {
     std::map<int, double*> avg;
     uint64_t init_epoch = 0;
     hipStream_t stream = nullptr;
+    // Second stream (default priority) for compacted-grain sweeps that run
+    // beside the streamed ones; forked from and joined to `stream` by events.
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::map<std::vector<int>, std::unique_ptr<Set>> sets;
     // Ising threshold rows keyed by (beta, h)
     std::map<std::pair<double, double>, std::pair<std::vector<uint32_t>, std::vector<double>>> tab_cache;
@@ -167,6 +171,9 @@ struct nmc_ctx {
         (void)hipFree(d_dtab); (void)hipFree(d_sweeps); (void)hipFree(d_acc);
         (void)hipFree(d_partials); (void)hipFree(d_obs); (void)hipFree(d_series);
         (void)hipFree(d_series_qr); (void)hipFree(d_series_qw);
+        if (aux) { (void)hipStreamSynchronize(aux); (void)hipStreamDestroy(aux); }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -602,6 +609,9 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
         c->max_lds = (size_t)std::max(0, v);
     }
     NMC_HIP(hipStreamCreate(&c->stream));
+    NMC_HIP(hipStreamCreate(&c->aux));
+    NMC_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    NMC_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     int rc;
     if ((rc = dalloc(&c->d_inst, c->din.size()))) return rc;
     if (c->ising) { int8_t* p; if ((rc = dalloc(&p, (size_t)pool))) return rc; c->pool = p; }
@@ -757,13 +767,27 @@ int release_sets(nmc_ctx* ctx) {
 // no synchronisation, so it can be recorded into a graph).
 int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
+    // Compacted grains (LDS/VALU-bound) and streamed grains (HBM-bound) are
+    // disjoint instances: outside graph capture they run side by side on two
+    // streams, joined before anything that follows.
+    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    NMC_HIP(hipStreamIsCapturing(ctx->stream, &cap));
+    const bool side = s->n_cl > 0 && streamed && nsweeps > 0 && cap == hipStreamCaptureStatusNone;
+    hipStream_t cstream = ctx->stream;
+    if (side) {
+        NMC_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+        NMC_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
+        cstream = ctx->aux;
+    }
     for (int k = 0; k < nsweeps && s->n_cl > 0; k += kLdsSweepChunk) {
         const int chunk = std::min(kLdsSweepChunk, nsweeps - k);
-        NMC_HIP(launch_sweep_lds_c(ctx->stream, ctx->dim3, ctx->use_d, ctx->cone, std::min(s->n_cl, 65535),
+        NMC_HIP(launch_sweep_lds_c(cstream, ctx->dim3, ctx->use_d, ctx->cone, std::min(s->n_cl, 65535),
                                    s->c_block, s->c_lds_bytes, ctx->d_inst, s->d_cl, s->n_cl, (float*)ctx->pool,
                                    s->d_cs_e, s->d_cs_nb, ctx->d_sweeps, ctx->st, chunk, (uint32_t)k, slo, shi,
                                    ctx->d_acc));
     }
+    if (side) NMC_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
     for (int k = 0; k < nsweeps && s->n_lds > 0; k += kLdsSweepChunk) {
         const int chunk = std::min(kLdsSweepChunk, nsweeps - k);
         NMC_HIP(launch_sweep_lds(ctx->stream, ctx->ising, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d,
@@ -791,6 +815,7 @@ int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
                                           true, ctx->d_inst, s->d_sweep, s->n_sweep, (float*)ctx->pool, ctx->mpool,
                                           ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc));
         }
+    if (side) NMC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     if (nsweeps > 0)
         NMC_HIP(launch_bump_sweeps(ctx->stream, ctx->d_sweeps, s->d_list, (int)s->list.size(), (uint64_t)nsweeps));
     return NMC_OK;
