@@ -192,6 +192,9 @@ int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp
  * it in site order (3 doubles per site), nmc_zero_avg is field_type::all_zero
  * on the summed field (main.cpp:176-179). */
 int nmc_accumulate_avg(nmc_ctx* ctx, int inst);
+/* add_to_av for every listed instance in one launch (the same sums as n
+ * calls of nmc_accumulate_avg). */
+int nmc_accumulate_avgs(nmc_ctx* ctx, const int* inst, int n);
 int nmc_get_avg(nmc_ctx* ctx, int inst, double* out);
 int nmc_zero_avg(nmc_ctx* ctx, int inst);
 

@@ -25,7 +25,7 @@ EXPORTS = [
     "nmc_last_error", "nmc_create", "nmc_destroy", "nmc_get_stream", "nmc_sync", "nmc_num_sites", "nmc_num_slices",
     "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_set_path", "nmc_set_lds_block", "nmc_get_accepts",
     "nmc_reset_accepts", "nmc_snapshot", "nmc_measure", "nmc_series_reset", "nmc_sample",
-    "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
+    "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_accumulate_avgs", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
     "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_net_create", "nmc_net_destroy",
     "nmc_net_num_params", "nmc_net_get_params", "nmc_net_set_params", "nmc_net_collect",
     "nmc_net_num_samples", "nmc_net_train", "nmc_net_loss_grad", "nmc_block_moves", "nmc_block_swaps", "nmc_build_weibull",
@@ -87,6 +87,7 @@ def lib():
         "nmc_get_series": (i, [vp, P(i), i, P(d), P(d), P(d)]),
         "nmc_run_samples": (i, [vp, P(i), i, i, i]),
         "nmc_accumulate_avg": (i, [vp, i]),
+        "nmc_accumulate_avgs": (i, [vp, P(i), i]),
         "nmc_get_avg": (i, [vp, i, P(d)]),
         "nmc_zero_avg": (i, [vp, i]),
         "nmc_get_spins": (i, [vp, i, P(d)]),

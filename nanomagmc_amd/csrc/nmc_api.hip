@@ -140,6 +140,9 @@ struct nmc_ctx {
     double* d_series_qr = nullptr;
     double* d_series_qw = nullptr;
     std::map<int, double*> avg;
+    std::vector<double*> h_avgptr;   // per instance: running sum (device) or null ...
+    double** d_avgptr = nullptr;     // ... and its device copy, read by the batched accumulate
+    std::map<std::vector<int>, std::pair<Tile*, int>> avg_tiles;   // tile lists per instance list
     uint64_t init_epoch = 0;
     hipStream_t stream = nullptr;
     // Second stream (default priority) for compacted-grain sweeps that run
@@ -167,6 +170,8 @@ struct nmc_ctx {
         drop_graphs();
         for (auto& kv : sets) kv.second->release();
         for (auto& kv : avg) (void)hipFree(kv.second);
+        for (auto& kv : avg_tiles) (void)hipFree(kv.second.first);
+        (void)hipFree(d_avgptr);
         (void)hipFree(d_inst); (void)hipFree(pool); (void)hipFree(mpool); (void)hipFree(d_tab);
         (void)hipFree(d_dtab); (void)hipFree(d_sweeps); (void)hipFree(d_acc);
         (void)hipFree(d_partials); (void)hipFree(d_obs); (void)hipFree(d_series);
@@ -1025,16 +1030,64 @@ int nmc_get_series(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_
     return NMC_OK;
 }
 
-int nmc_accumulate_avg(nmc_ctx* ctx, int inst) {
-    if (!ctx) return fail(NMC_E_INVALID, "null context");
-    if (int rc = check_inst(ctx, inst)) return rc;
+}  // extern "C"
+
+namespace {
+
+// The running sum of instance `inst`, allocated (zeroed) on first use; the
+// device pointer table used by the batched accumulate is kept in step.
+int ensure_avg(nmc_ctx* ctx, int inst, double** out) {
     const DevInst& d = ctx->din[inst];
     double*& a = ctx->avg[inst];
     if (!a) {
         if (int rc = dalloc(&a, (size_t)6 * d.Sp)) return rc;
         NMC_HIP(hipMemsetAsync(a, 0, (size_t)6 * d.Sp * sizeof(double), ctx->stream));
+        if (!ctx->d_avgptr) {
+            if (int rc = dalloc(&ctx->d_avgptr, ctx->din.size())) return rc;
+            NMC_HIP(hipMemsetAsync(ctx->d_avgptr, 0, ctx->din.size() * sizeof(double*), ctx->stream));
+            ctx->h_avgptr.assign(ctx->din.size(), nullptr);
+        }
+        ctx->h_avgptr[inst] = a;   // stable host storage for the asynchronous copy
+        NMC_HIP(hipMemcpyAsync(ctx->d_avgptr + inst, &ctx->h_avgptr[inst], sizeof(double*), hipMemcpyHostToDevice,
+                               ctx->stream));
     }
-    NMC_HIP(launch_accum_avg(ctx->stream, ctx->ising, ctx->d_inst, inst, d.Sp, ctx->pool, a));
+    *out = a;
+    return NMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nmc_accumulate_avg(nmc_ctx* ctx, int inst) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (int rc = check_inst(ctx, inst)) return rc;
+    double* a = nullptr;
+    if (int rc = ensure_avg(ctx, inst, &a)) return rc;
+    NMC_HIP(launch_accum_avg(ctx->stream, ctx->ising, ctx->d_inst, inst, ctx->din[inst].Sp, ctx->pool, a));
+    return NMC_OK;
+}
+
+int nmc_accumulate_avgs(nmc_ctx* ctx, const int* inst, int n) {
+    if (!ctx || (!inst && n > 0) || n < 0) return fail(NMC_E_INVALID, "bad argument");
+    if (n == 0) return NMC_OK;
+    std::vector<int> list(inst, inst + n);
+    for (int i : list) {
+        if (int rc = check_inst(ctx, i)) return rc;
+        double* a = nullptr;
+        if (int rc = ensure_avg(ctx, i, &a)) return rc;
+    }
+    auto it = ctx->avg_tiles.find(list);
+    if (it == ctx->avg_tiles.end()) {
+        std::vector<Tile> tiles;
+        for (int i : list)
+            for (int64_t e = 0; e < ctx->din[i].Sp; e += 256) tiles.push_back(Tile{i, (int)e});
+        Tile* dt = nullptr;
+        if (int rc = upload(ctx, &dt, tiles)) return rc;
+        it = ctx->avg_tiles.emplace(list, std::make_pair(dt, (int)tiles.size())).first;
+    }
+    NMC_HIP(launch_accum_avg_tiles(ctx->stream, ctx->ising, ctx->d_inst, it->second.first, it->second.second,
+                                   ctx->pool, ctx->d_avgptr));
     return NMC_OK;
 }
 

@@ -349,6 +349,26 @@ __global__ void k_accum_avg(const DevInst* __restrict__ insts, int inst, const v
     avg[(c * 3 + 2) * in.Sp + e] += z;
 }
 
+// add_to_av over a tile list spanning several instances: tile (inst, e0)
+// covers colour elements e0 .. e0 + 255 of both colours; avgptr[inst] is the
+// instance's running sum.
+template <bool ISING>
+__global__ void k_accum_avg_tiles(const DevInst* __restrict__ insts, const Tile* __restrict__ tiles,
+                                  const void* __restrict__ pool, double* const* __restrict__ avgptr) {
+    const Tile tl = tiles[blockIdx.x];
+    const DevInst& in = insts[tl.inst];
+    const int64_t e = (int64_t)tl.e0 + threadIdx.x;
+    if (e >= in.Sp) return;
+    double* avg = avgptr[tl.inst];
+    for (int c = 0; c < 2; ++c) {
+        double x, y, z;
+        load_spin<ISING>(pool, in, c, e, x, y, z);
+        avg[(c * 3 + 0) * in.Sp + e] += x;
+        avg[(c * 3 + 1) * in.Sp + e] += y;
+        avg[(c * 3 + 2) * in.Sp + e] += z;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Test mode.
 // ---------------------------------------------------------------------------
@@ -537,6 +557,14 @@ hipError_t launch_accum_avg(hipStream_t stream, bool ising, const DevInst* insts
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (ising) k_accum_avg<true><<<nb, 256, 0, stream>>>(insts, inst, pool, avg);
     else k_accum_avg<false><<<nb, 256, 0, stream>>>(insts, inst, pool, avg);
+    return hipGetLastError();
+}
+
+hipError_t launch_accum_avg_tiles(hipStream_t stream, bool ising, const DevInst* insts, const Tile* tiles,
+                                  int ntiles, const void* pool, double* const* avgptr) {
+    if (ntiles <= 0) return hipSuccess;
+    if (ising) k_accum_avg_tiles<true><<<ntiles, 256, 0, stream>>>(insts, tiles, pool, avgptr);
+    else k_accum_avg_tiles<false><<<ntiles, 256, 0, stream>>>(insts, tiles, pool, avgptr);
     return hipGetLastError();
 }
 

@@ -53,6 +53,8 @@ hipError_t launch_init_spins(hipStream_t stream, bool ising, const DevInst* inst
 
 hipError_t launch_bump_sweeps(hipStream_t stream, uint64_t* sweeps, const int* list, int n, uint64_t by);
 
+hipError_t launch_accum_avg_tiles(hipStream_t stream, bool ising, const DevInst* insts, const Tile* tiles,
+                                  int ntiles, const void* pool, double* const* avgptr);
 hipError_t launch_accum_avg(hipStream_t stream, bool ising, const DevInst* insts, int inst,
                             int64_t Sp, const void* pool, double* avg);
 

@@ -220,6 +220,11 @@ class Lattices:
     def accumulate_avg(self, i):
         check(lib().nmc_accumulate_avg(self._h, int(i)))
 
+    def accumulate_avgs(self, inst=None):
+        """add_to_av for every listed instance (all when None) in one launch."""
+        a, n = _ilist(range(self.ninst) if inst is None else inst)
+        check(lib().nmc_accumulate_avgs(self._h, _iptr(a), n))
+
     def zero_avg(self, i):
         check(lib().nmc_zero_avg(self._h, int(i)))
 

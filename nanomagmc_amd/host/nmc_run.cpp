@@ -607,12 +607,15 @@ int main(int argc, char** argv) {
         push_params(active);
         equilibrate(active, sim.Eq_steps);
         for (int i = 0; i < v1n; ++i) nmc_check(nmc_zero_avg(ctx, curr(0, i)), "nmc_zero_avg");
+        std::vector<int> avg_ids;   // add_to_av for lattice 0 (main.cpp:236-239)
+        if (!sim.distrib)
+            for (int id : active)
+                if (id < curr(1, 0)) avg_ids.push_back(id);
         for (int s = 0; s < sim.N_samp; ++s) {
             nmc_check(nmc_sweep(ctx, active.data(), (int)active.size(), sim.Samp_steps), "nmc_sweep");
             nmc_check(nmc_sample(ctx, active.data(), (int)active.size(), s), "nmc_sample");
-            if (!sim.distrib)   // add_to_av for lattice 0 (main.cpp:236-239)
-                for (int id : active)
-                    if (id < curr(1, 0)) nmc_check(nmc_accumulate_avg(ctx, id), "nmc_accumulate_avg");
+            if (!avg_ids.empty())
+                nmc_check(nmc_accumulate_avgs(ctx, avg_ids.data(), (int)avg_ids.size()), "nmc_accumulate_avgs");
         }
         nmc_check(nmc_get_series(ctx, active.data(), (int)active.size(), obs.data(), tc ? qr.data() : nullptr,
                                  tc ? qw.data() : nullptr), "nmc_get_series");

@@ -424,6 +424,24 @@ def test_snapshot_series_and_average(orc):
     assert np.array_equal(obs[:, 3, :], o2) and np.array_equal(qr[:, 3, :], q2)
 
 
+def test_batched_average_equals_single_calls(orc):
+    d, J, D = nn2d()
+    sums = []
+    for batched in (True, False):
+        lat, f = make_pair(orc, 2, 8, 1, d, J, D, False, ninst=3, seed=17)
+        lat.randomize()
+        for _ in range(4):
+            lat.equil(1)
+            if batched:
+                lat.accumulate_avgs([2, 0])
+            else:
+                lat.accumulate_avg(0)
+                lat.accumulate_avg(2)
+        sums.append([lat.get_avg(i) for i in (0, 2)])
+    for a, b in zip(*sums):
+        assert np.array_equal(a, b)
+
+
 def test_invalid_arguments_raise(orc):
     from nanomagmc_amd import Bond, Lattices
     from nanomagmc_amd._lib import NmcError
