@@ -68,6 +68,7 @@ struct Set {
     int* d_list = nullptr;
     Tile* d_sweep = nullptr;
     int n_sweep = 0;
+    int sweep_block = 256;   // work-items per streaming tile (64 for small sets)
     std::vector<Tile> h_sweep;                // host copy (tile -> instance)
     unsigned long long* d_tacc = nullptr;     // accepted attempts per streaming tile
     Tile* d_init = nullptr;
@@ -255,7 +256,14 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     std::vector<Tile> sw, ini;
     std::vector<MTile> mt;
     std::vector<int> tb, te;
-    const int stile = c->ising ? kIsingTile : kSweepTile;
+    // streaming tiles: full size, or a quarter when the set would give fewer
+    // than 1024 full tiles (too few workgroups to fill 256 CUs)
+    int64_t stream_elems = 0;
+    for (int i : list) stream_elems += c->din[i].S;
+    const int full = c->ising ? kIsingTile : kSweepTile;
+    const bool quarter = stream_elems < (int64_t)1024 * full;
+    const int stile = quarter ? full / 4 : full;
+    s->sweep_block = quarter ? 64 : 256;
     std::vector<int> lds, qinst, cinst;
     int64_t lds_work = 0;
     const bool quad_lists = c->masked && !c->ising;
@@ -770,59 +778,69 @@ int release_sets(nmc_ctx* ctx) {
 
 // Enqueue `nsweeps` sweeps of a prepared set (launches only: no allocation,
 // no synchronisation, so it can be recorded into a graph).
-int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
+// The launches of `nsweeps` sweeps of one set.  Compacted grains
+// (LDS/VALU-bound) and streamed grains (HBM-bound) are disjoint instances:
+// outside graph capture the compacted ones run on the auxiliary stream beside
+// the others, and the main stream waits for them before anything that
+// follows -- also when a launch on the main stream fails.
+hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream, bool side) {
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
-    // Compacted grains (LDS/VALU-bound) and streamed grains (HBM-bound) are
-    // disjoint instances: outside graph capture they run side by side on two
-    // streams, joined before anything that follows.
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < nsweeps && s->n_cl > 0 && e == hipSuccess; k += kLdsSweepChunk)
+        e = launch_sweep_lds_c(cstream, ctx->dim3, ctx->use_d, ctx->cone, std::min(s->n_cl, 65535), s->c_block,
+                               s->c_lds_bytes, ctx->d_inst, s->d_cl, s->n_cl, (float*)ctx->pool, s->d_cs_e,
+                               s->d_cs_nb, ctx->d_sweeps, ctx->st, std::min(kLdsSweepChunk, nsweeps - k),
+                               (uint32_t)k, slo, shi, ctx->d_acc);
+    if (side) {
+        const hipError_t j = hipEventRecord(ctx->ev_join, ctx->aux);
+        if (e == hipSuccess) e = j;
+    }
+    for (int k = 0; k < nsweeps && s->n_lds > 0 && e == hipSuccess; k += kLdsSweepChunk)
+        e = launch_sweep_lds(ctx->stream, ctx->ising, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d, ctx->cone,
+                             std::min(s->n_lds, 65535), s->lds_block, s->lds_bytes, ctx->d_inst, s->d_lds, s->n_lds,
+                             ctx->pool, ctx->mpool, ctx->d_tab, ctx->d_sweeps, ctx->st,
+                             std::min(kLdsSweepChunk, nsweeps - k), (uint32_t)k, slo, shi, ctx->d_acc);
+    const bool streaming = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    for (int k = 0; k < nsweeps && streaming && e == hipSuccess; ++k)
+        for (int col = 0; col < 2 && e == hipSuccess; ++col) {
+            if (s->n_qt[col] > 0) {
+                const int off = col ? s->n_qt[0] : 0;
+                e = launch_sweep_heis_q(ctx->stream, ctx->dim3, ctx->periodic, ctx->use_d, ctx->cone, true,
+                                        ctx->d_inst, s->d_qt + off, s->n_qt[col], s->d_qlist, (float*)ctx->pool,
+                                        ctx->mpool, ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi,
+                                        s->d_tacc + s->n_sweep + off);
+            }
+            if (s->n_sweep == 0 || e != hipSuccess) continue;
+            if (ctx->ising)
+                e = launch_sweep_ising(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, true, s->sweep_block,
+                                       ctx->d_inst,
+                                       s->d_sweep, s->n_sweep, (int8_t*)ctx->pool, ctx->mpool, ctx->d_tab,
+                                       ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc);
+            else
+                e = launch_sweep_heis(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d, ctx->cone, true,
+                                      s->sweep_block, ctx->d_inst, s->d_sweep, s->n_sweep, (float*)ctx->pool, ctx->mpool,
+                                      ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc);
+        }
+    return e;
+}
+
+int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
     const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     NMC_HIP(hipStreamIsCapturing(ctx->stream, &cap));
     const bool side = s->n_cl > 0 && streamed && nsweeps > 0 && cap == hipStreamCaptureStatusNone;
-    hipStream_t cstream = ctx->stream;
     if (side) {
         NMC_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
         NMC_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
-        cstream = ctx->aux;
     }
-    for (int k = 0; k < nsweeps && s->n_cl > 0; k += kLdsSweepChunk) {
-        const int chunk = std::min(kLdsSweepChunk, nsweeps - k);
-        NMC_HIP(launch_sweep_lds_c(cstream, ctx->dim3, ctx->use_d, ctx->cone, std::min(s->n_cl, 65535),
-                                   s->c_block, s->c_lds_bytes, ctx->d_inst, s->d_cl, s->n_cl, (float*)ctx->pool,
-                                   s->d_cs_e, s->d_cs_nb, ctx->d_sweeps, ctx->st, chunk, (uint32_t)k, slo, shi,
-                                   ctx->d_acc));
+    hipError_t e = sweep_kernels(ctx, s, nsweeps, side ? ctx->aux : ctx->stream, side);
+    if (side) {
+        const hipError_t j = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+        if (e == hipSuccess) e = j;
     }
-    if (side) NMC_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
-    for (int k = 0; k < nsweeps && s->n_lds > 0; k += kLdsSweepChunk) {
-        const int chunk = std::min(kLdsSweepChunk, nsweeps - k);
-        NMC_HIP(launch_sweep_lds(ctx->stream, ctx->ising, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d,
-                                 ctx->cone, std::min(s->n_lds, 65535), s->lds_block, s->lds_bytes, ctx->d_inst,
-                                 s->d_lds, s->n_lds, ctx->pool, ctx->mpool, ctx->d_tab, ctx->d_sweeps, ctx->st,
-                                 chunk, (uint32_t)k, slo, shi, ctx->d_acc));
-    }
-    const bool streaming = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
-    for (int k = 0; k < nsweeps && streaming; ++k)
-        for (int col = 0; col < 2; ++col) {
-            if (s->n_qt[col] > 0) {
-                const int off = col ? s->n_qt[0] : 0;
-                NMC_HIP(launch_sweep_heis_q(ctx->stream, ctx->dim3, ctx->periodic, ctx->use_d, ctx->cone, true,
-                                            ctx->d_inst, s->d_qt + off, s->n_qt[col], s->d_qlist, (float*)ctx->pool,
-                                            ctx->mpool, ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi,
-                                            s->d_tacc + s->n_sweep + off));
-            }
-            if (s->n_sweep == 0) continue;
-            if (ctx->ising)
-                NMC_HIP(launch_sweep_ising(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, true, ctx->d_inst,
-                                           s->d_sweep, s->n_sweep, (int8_t*)ctx->pool, ctx->mpool, ctx->d_tab,
-                                           ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc));
-            else
-                NMC_HIP(launch_sweep_heis(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d, ctx->cone,
-                                          true, ctx->d_inst, s->d_sweep, s->n_sweep, (float*)ctx->pool, ctx->mpool,
-                                          ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc));
-        }
-    if (side) NMC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    if (nsweeps > 0)
-        NMC_HIP(launch_bump_sweeps(ctx->stream, ctx->d_sweeps, s->d_list, (int)s->list.size(), (uint64_t)nsweeps));
+    if (e == hipSuccess && nsweeps > 0)
+        e = launch_bump_sweeps(ctx->stream, ctx->d_sweeps, s->d_list, (int)s->list.size(), (uint64_t)nsweeps);
+    if (e != hipSuccess) return fail(NMC_E_HIP, std::string("nmc_sweep: ") + hipGetErrorString(e));
     return NMC_OK;
 }
 }  // namespace

@@ -4,8 +4,10 @@
 
 namespace nmc {
 
+// block: 256 work-items per tile (1024 Heisenberg / 512 Ising elements), or
+// 64 (quarter tiles) for sets too small to fill the chip with full tiles.
 hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, bool use_d,
-                             bool cone, bool count, const DevInst* insts, const Tile* tiles,
+                             bool cone, bool count, int block, const DevInst* insts, const Tile* tiles,
                              int ntiles, float* pool, const uint32_t* mpool, const uint64_t* sweeps,
                              const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo,
                              uint32_t seed_hi, unsigned long long* acc);
@@ -18,7 +20,7 @@ hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, boo
                                unsigned long long* acc);
 
 hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool masked,
-                              bool count, const DevInst* insts, const Tile* tiles, int ntiles,
+                              bool count, int block, const DevInst* insts, const Tile* tiles, int ntiles,
                               int8_t* pool, const uint32_t* mpool, const uint32_t* tab,
                               const uint64_t* sweeps, const Stencil& st, int colour,
                               uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,

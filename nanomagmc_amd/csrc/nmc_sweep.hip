@@ -38,7 +38,8 @@ __device__ __forceinline__ void tile_count(int wave_count, unsigned long long* _
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = wave_count;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int s = part[0] + part[1] + part[2] + part[3];
+        int s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += part[w];   // 1 or 4 waves
         if (s) tile_acc[t] += (unsigned long long)s;
     }
 }
@@ -543,15 +544,16 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
 // ---------------------------------------------------------------------------
 #define NMC_HEIS_CASE(D3, PER, MSK, UD, CN, CT)                                                     \
     if (dim3 == D3 && periodic == PER && masked == MSK && use_d == UD && cone == CN && count == CT) \
-        k_sweep_heis<D3, PER, MSK, UD, CN, CT><<<ntiles, 256, 0, stream>>>(                         \
+        k_sweep_heis<D3, PER, MSK, UD, CN, CT><<<ntiles, block, 0, stream>>>(                       \
             insts, tiles, ntiles, pool, mpool, sweeps, st, colour, s_off, seed_lo, seed_hi, acc);
 
 hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, bool use_d,
-                             bool cone, bool count, const DevInst* insts, const Tile* tiles,
+                             bool cone, bool count, int block, const DevInst* insts, const Tile* tiles,
                              int ntiles, float* pool, const uint32_t* mpool, const uint64_t* sweeps,
                              const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo,
                              uint32_t seed_hi, unsigned long long* acc) {
     if (ntiles <= 0) return hipSuccess;
+    if (block != 64 && block != 256) return hipErrorInvalidValue;
 #define NMC_HEIS_B(D3, PER, MSK)                     \
     NMC_HEIS_CASE(D3, PER, MSK, false, false, false) \
     NMC_HEIS_CASE(D3, PER, MSK, false, false, true)  \
@@ -603,17 +605,18 @@ hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, boo
 
 #define NMC_ISING_CASE(D3, PER, MSK, CT)                                                 \
     if (dim3 == D3 && periodic == PER && masked == MSK && count == CT)                   \
-        k_sweep_ising<D3, PER, MSK, CT><<<ntiles, 256, 0, stream>>>(                     \
+        k_sweep_ising<D3, PER, MSK, CT><<<ntiles, block, 0, stream>>>(                   \
             insts, tiles, ntiles, pool, mpool, tab, sweeps, st, colour, s_off, seed_lo, \
             seed_hi, acc);
 
 hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool masked,
-                              bool count, const DevInst* insts, const Tile* tiles, int ntiles,
+                              bool count, int block, const DevInst* insts, const Tile* tiles, int ntiles,
                               int8_t* pool, const uint32_t* mpool, const uint32_t* tab,
                               const uint64_t* sweeps, const Stencil& st, int colour,
                               uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                               unsigned long long* acc) {
     if (ntiles <= 0) return hipSuccess;
+    if (block != 64 && block != 256) return hipErrorInvalidValue;
 #define NMC_ISING_B(D3, PER, MSK) NMC_ISING_CASE(D3, PER, MSK, false) NMC_ISING_CASE(D3, PER, MSK, true)
     NMC_ISING_B(true, true, false)
     NMC_ISING_B(true, true, true)
