@@ -104,6 +104,7 @@ struct Set {
 };
 
 constexpr int kLdsSweepChunk = 256;   // sweeps per LDS-resident launch
+constexpr int kGraphSweeps = 64;      // sweeps per captured graph of a small set
 
 }  // namespace
 
@@ -850,6 +851,29 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
     if (nsweeps < 0) return fail(NMC_E_INVALID, "negative sweep count");
     Set* s;
     if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    // Small sets (quarter tiles): a long call is launch-bound, so whole
+    // blocks of kGraphSweeps sweeps replay one captured graph.  The graph's
+    // sweep offsets are relative and its last node advances the device sweep
+    // counters, so replays continue the same random streams.
+    if (s->sweep_block == 64 && nsweeps >= 2 * kGraphSweeps) {
+        auto key = std::make_tuple(s->list, -1, kGraphSweeps);
+        auto it = ctx->graphs.find(key);
+        if (it == ctx->graphs.end()) {
+            NMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = sweep_launches(ctx, s, kGraphSweeps);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc != NMC_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return fail(NMC_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            hipGraphExec_t x = nullptr;
+            e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(NMC_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            it = ctx->graphs.emplace(key, x).first;
+        }
+        for (int r = 0; r < nsweeps / kGraphSweeps; ++r) NMC_HIP(hipGraphLaunch(it->second, ctx->stream));
+        return sweep_launches(ctx, s, nsweeps % kGraphSweeps);
+    }
     return sweep_launches(ctx, s, nsweeps);
 }
 

@@ -424,6 +424,30 @@ def test_snapshot_series_and_average(orc):
     assert np.array_equal(obs[:, 3, :], o2) and np.array_equal(qr[:, 3, :], q2)
 
 
+@pytest.mark.parametrize("ising", [True, False])
+@pytest.mark.parametrize("path", ["auto", "stream"])
+def test_long_sweep_calls_replay_graphs_exactly(orc, ising, path):
+    # a small set swept 200 times in one call (captured graphs of 64 sweeps
+    # plus the remainder) equals 200 single-sweep calls bit for bit
+    d, J, D = nn2d()
+    spins, accs = [], []
+    for one_call in (True, False):
+        lat, f = make_pair(orc, 2, 32, 1, d, J, D, ising, ninst=2, seed=29)
+        lat.set_path(path)
+        lat.set_params([0.5, 1.2], 0.05)
+        lat.randomize()
+        if one_call:
+            lat.equil(200)
+        else:
+            for _ in range(200):
+                lat.equil(1)
+        spins.append([lat.get_spins(i) for i in range(2)])
+        accs.append(lat.accepts())
+    for a, b in zip(*spins):
+        assert np.array_equal(a, b)
+    assert np.array_equal(accs[0], accs[1])
+
+
 def test_batched_average_equals_single_calls(orc):
     d, J, D = nn2d()
     sums = []
