@@ -85,7 +85,8 @@ struct Set {
     CInst* d_cl = nullptr;         // shaped Heisenberg lattices swept compacted in LDS
     int n_cl = 0;
     uint32_t* d_cs_e = nullptr;    // their present sites (element index) ...
-    uint16_t* d_cs_nb = nullptr;   // ... and six neighbour slots per site
+    uint16_t* d_cs_nb = nullptr;   // ... and six neighbour float indices per site
+    uint2* d_cs_q = nullptr;       // their quads with a present site
     size_t c_lds_bytes = 0;
     int c_block = 256;
     size_t lds_bytes = 0;    // dynamic LDS of the largest of them
@@ -95,9 +96,9 @@ struct Set {
         (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
         (void)hipFree(d_mt); (void)hipFree(d_tb); (void)hipFree(d_te); (void)hipFree(d_lds);
         (void)hipFree(d_tacc); (void)hipFree(d_qt); (void)hipFree(d_qlist);
-        (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb);
+        (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb); (void)hipFree(d_cs_q);
         d_tacc = nullptr; d_qt = nullptr; d_qlist = nullptr;
-        d_cl = nullptr; d_cs_e = nullptr; d_cs_nb = nullptr;
+        d_cl = nullptr; d_cs_e = nullptr; d_cs_nb = nullptr; d_cs_q = nullptr;
         d_list = nullptr; d_sweep = nullptr; d_init = nullptr; d_mt = nullptr;
         d_tb = d_te = d_lds = nullptr;
     }
@@ -105,6 +106,7 @@ struct Set {
 
 constexpr int kLdsSweepChunk = 256;   // sweeps per LDS-resident launch
 constexpr int kGraphSweeps = 64;      // sweeps per captured graph of a small set
+constexpr size_t kStreamLdsReserve = 1024;   // LDS a compacted lattice leaves free on its CU
 
 }  // namespace
 
@@ -273,11 +275,13 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         const DevInst& d = c->din[i];
         bool fits_compact = false;
         if (compact) {
-            // present sites (12 B each) + the zero slot, compacted in LDS if they fit
+            // present sites (12 B each) + the zero slot, compacted in LDS if
+            // they fit with kStreamLdsReserve to spare, so that a streaming
+            // workgroup can share the CU with a compacted one
             int64_t np = 0;
             const uint32_t* mw = c->h_mask.data() + d.mask_base;
             for (int64_t w = 0; w < 2 * d.mask_stride; ++w) np += __builtin_popcount(mw[w]);
-            fits_compact = (size_t)(np + 1) * 12 <= c->max_lds && np + 1 <= 65535;
+            fits_compact = (size_t)(np + 1) * 12 + kStreamLdsReserve <= c->max_lds && 3 * (np + 1) <= 65535;
         }
         // both colour arrays, then (masked lattices) both colours' presence bits
         const size_t bytes = (size_t)(c->ising ? 2 * d.Sp : 6 * d.Sp * (int64_t)sizeof(float)) +
@@ -319,22 +323,28 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     }
     for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});   // accept counters follow the plain tiles
     const int n_plain = (int)sw.size() - (int)qt.size();
-    // compacted shaped lattices: present sites per colour, neighbour slots;
+    // compacted shaped lattices: present sites per colour, neighbour indices;
     // instances are independent, so their tables are filled in parallel
     std::vector<CInst> cl(cinst.size());
-    int64_t total_sites = 0, cmax_colour = 0;
+    int64_t total_sites = 0, total_quads = 0, cmax_quads = 0;
     for (size_t q = 0; q < cinst.size(); ++q) {
         const DevInst& d = c->din[cinst[q]];
         const uint32_t* mw = c->h_mask.data() + d.mask_base;
-        int cnt[2] = {0, 0};
+        int cnt[2] = {0, 0}, nq[2] = {0, 0};
         for (int col = 0; col < 2; ++col)
-            for (int64_t w = 0; w < d.mask_stride; ++w) cnt[col] += __builtin_popcount(mw[col * d.mask_stride + w]);
-        cl[q] = CInst{cinst[q], cnt[0], cnt[1], total_sites};
+            for (int64_t w = 0; w < d.mask_stride; ++w) {
+                const uint32_t b = mw[col * d.mask_stride + w];
+                cnt[col] += __builtin_popcount(b);
+                for (int k = 0; k < 32; k += 4) nq[col] += ((b >> k) & 0xFu) != 0;
+            }
+        cl[q] = CInst{cinst[q], cnt[0], cnt[1], total_sites, nq[0], nq[1], total_quads};
         total_sites += cnt[0] + cnt[1];
-        cmax_colour = std::max<int64_t>(cmax_colour, std::max(cnt[0], cnt[1]));
+        total_quads += nq[0] + nq[1];
+        cmax_quads = std::max<int64_t>(cmax_quads, std::max(nq[0], nq[1]));
     }
     std::vector<uint32_t> cs_e((size_t)total_sites);
     std::vector<uint16_t> cs_nb((size_t)total_sites * 6);
+    std::vector<uint2> cs_q((size_t)total_quads);
     auto fill = [&](size_t qb, size_t qe) {
         std::vector<int> slot[2];
         for (size_t q = qb; q < qe; ++q) {
@@ -343,6 +353,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             const uint32_t* mw = c->h_mask.data() + d.mask_base;
             uint32_t* E = cs_e.data() + ci.site0;
             uint16_t* NB = cs_nb.data() + 6 * ci.site0;
+            uint2* QD = cs_q.data() + ci.quad0;
             int next = 0;
             for (int col = 0; col < 2; ++col) {
                 slot[col].assign((size_t)d.S, -1);
@@ -351,6 +362,23 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                         slot[col][e] = next;
                         E[next++] = (uint32_t)e;
                     }
+                // this colour's quads, counting-sorted by presence mask
+                // (S is a multiple of 4, so a quad never straddles a mask word)
+                int start[16] = {0};
+                auto nib = [&](int64_t e) { return (mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 0xFu; };
+                for (int64_t e = 0; e < d.S; e += 4) ++start[nib(e)];
+                for (int m = 1, run = 0; m < 16; ++m) {
+                    const int t = start[m];
+                    start[m] = run;
+                    run += t;
+                }
+                for (int64_t e = 0; e < d.S; e += 4) {
+                    const uint32_t m = nib(e);
+                    if (!m) continue;
+                    const int first = slot[col][e + __builtin_ctz(m)];
+                    QD[start[m]++] = make_uint2((uint32_t)(e >> 2), 3u * (uint32_t)first | (m << 16));
+                }
+                QD += col ? ci.nq1 : ci.nq0;
             }
             const int zero = ci.n0 + ci.n1;
             for (int j = 0; j < zero; ++j) {
@@ -375,7 +403,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                         const int t = slot[col ^ 1][((int64_t)n0 * d.a1 + n1) * d.half + kk];
                         if (t >= 0) sv = t;
                     }
-                    NB[6 * j + g] = (uint16_t)sv;
+                    NB[6 * j + g] = (uint16_t)(3 * sv);
                 }
             }
         }
@@ -394,11 +422,12 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     if ((rc = upload(c, &s->d_cl, cl))) return rc;
     if ((rc = upload(c, &s->d_cs_e, cs_e))) return rc;
     if ((rc = upload(c, &s->d_cs_nb, cs_nb))) return rc;
+    if ((rc = upload(c, &s->d_cs_q, cs_q))) return rc;
     s->n_cl = (int)cl.size();
     for (const CInst& ci : cl)
-        s->c_lds_bytes = std::max(s->c_lds_bytes, (size_t)(ci.n0 + ci.n1 + 1) * 3 * sizeof(float));
+        s->c_lds_bytes = std::max(s->c_lds_bytes, std::max<size_t>(64, (size_t)(ci.n0 + ci.n1 + 1) * 3 * sizeof(float)));
     s->c_block = c->lds_block > 0 ? c->lds_block
-                                  : (int)std::min<int64_t>(1024, std::max<int64_t>(64, (cmax_colour + 63) / 64 * 64));
+                                  : (int)std::min<int64_t>(1024, std::max<int64_t>(64, (cmax_quads + 63) / 64 * 64));
     if ((rc = upload(c, &s->d_qt, qt))) return rc;
     if ((rc = upload(c, &s->d_qlist, ql))) return rc;
     sw.resize(n_plain);
@@ -790,7 +819,7 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
     for (int k = 0; k < nsweeps && s->n_cl > 0 && e == hipSuccess; k += kLdsSweepChunk)
         e = launch_sweep_lds_c(cstream, ctx->dim3, ctx->use_d, ctx->cone, std::min(s->n_cl, 65535), s->c_block,
                                s->c_lds_bytes, ctx->d_inst, s->d_cl, s->n_cl, (float*)ctx->pool, s->d_cs_e,
-                               s->d_cs_nb, ctx->d_sweeps, ctx->st, std::min(kLdsSweepChunk, nsweeps - k),
+                               s->d_cs_q, s->d_cs_nb, ctx->d_sweeps, ctx->st, std::min(kLdsSweepChunk, nsweeps - k),
                                (uint32_t)k, slo, shi, ctx->d_acc);
     if (side) {
         const hipError_t j = hipEventRecord(ctx->ev_join, ctx->aux);

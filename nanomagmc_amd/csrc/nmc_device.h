@@ -110,12 +110,18 @@ struct QTile {
 
 // Shaped instance in compacted form for LDS-resident sweeps: its present
 // sites, colour 0 first (n0 of them) then colour 1 (n1), stored from site0 on
-// in the per-set site tables (element index; six neighbour slots, each a
-// compact index into the other colour or n0 + n1 = the zero slot).
+// in the per-set site tables (element index; six neighbours, each the float
+// index 3 x slot of a site of the other colour or of n0 + n1 = the zero
+// slot), and its quads with a present site, colour 0 first, from quad0 on in
+// the per-set quad table: {quad index e >> 2, 3 x first slot | 4-bit
+// presence mask << 16}, grouped by presence mask so that a wavefront mostly
+// sees one pattern.
 struct CInst {
     int inst;
-    int n0, n1;
-    int64_t site0;
+    int n0, n1;       // present sites of colour 0 and 1 (slots 0..n0-1, n0..n0+n1-1)
+    int64_t site0;    // first slot in the set's site tables
+    int nq0, nq1;     // quads with a present site, per colour
+    int64_t quad0;    // first quad in the set's quad table
 };
 
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,

@@ -35,8 +35,9 @@ hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool peri
 // Compacted LDS-resident sweeps of shaped Heisenberg lattices (see CInst).
 hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, bool use_d, bool cone, int grid, int block, size_t shm,
                               const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
-                              const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st, int nsweeps,
-                              uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc);
+                              const uint2* cs_q, const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st,
+                              int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                              unsigned long long* acc);
 
 hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising,
                           const DevInst* insts, const MTile* tiles, int ntiles, const void* pool,

@@ -55,18 +55,25 @@ __device__ __forceinline__ bool mask_bit_s(const uint32_t* mw, const DevInst& in
 // give the 12 words the quad needs, element m taking words 3m, 3m+1
 // (trial) and 3m+2 (acceptance uniform).
 
+// Every sweep kernel must round a site's update identically (the LDS-resident
+// and streaming paths are compared bit for bit), so the two helpers below fix
+// each fused multiply-add by hand and keep the compiler from contracting the
+// rest: an implicit a * b + c * d may otherwise be fused either way depending
+// on the surrounding code.
+
 // Local field of stencil neighbour n added to v: J n (+ n x D, the
 // orientation dE_func uses, thermodynamics.cpp:45-46).
 template <bool USE_D>
 __device__ __forceinline__ void add_bond(const Stencil& st, int g, float nx, float ny, float nz, float& vx, float& vy,
                                          float& vz) {
-    vx += st.J[g] * nx;
-    vy += st.J[g] * ny;
-    vz += st.J[g] * nz;
+#pragma clang fp contract(off)
+    vx = __builtin_fmaf(st.J[g], nx, vx);
+    vy = __builtin_fmaf(st.J[g], ny, vy);
+    vz = __builtin_fmaf(st.J[g], nz, vz);
     if (USE_D) {
-        vx += ny * st.D[g][2] - nz * st.D[g][1];
-        vy += nz * st.D[g][0] - nx * st.D[g][2];
-        vz += nx * st.D[g][1] - ny * st.D[g][0];
+        vx = __builtin_fmaf(-nz, st.D[g][1], __builtin_fmaf(ny, st.D[g][2], vx));
+        vy = __builtin_fmaf(-nx, st.D[g][2], __builtin_fmaf(nz, st.D[g][0], vy));
+        vz = __builtin_fmaf(-ny, st.D[g][0], __builtin_fmaf(nx, st.D[g][1], vz));
     }
 }
 
@@ -77,37 +84,38 @@ template <bool CONE>
 __device__ __forceinline__ bool heis_decide(const DevInst& in, float sx, float sy, float sz, float vx, float vy,
                                             float vz, uint32_t wa, uint32_t wb, uint32_t wc, float& tx, float& ty,
                                             float& tz) {
+#pragma clang fp contract(off)
     const float ua = u24_open(wa);
     const float uphi = u24(wb);
     const float cphi = __builtin_amdgcn_cosf(uphi);   // cos(2 pi u): v_cos_f32 takes revolutions
     const float sphi = __builtin_amdgcn_sinf(uphi);
     if (!CONE) {
         // uniform direction on S^2: z uniform in (-1, 1), azimuth uniform
-        tz = 2.f * ua - 1.f;
-        const float r = __builtin_sqrtf(fmaxf(0.f, 1.f - tz * tz));
+        tz = __builtin_fmaf(2.f, ua, -1.f);
+        const float r = __builtin_sqrtf(fmaxf(0.f, __builtin_fmaf(-tz, tz, 1.f)));
         tx = r * cphi;
         ty = r * sphi;
     } else {
         // uniform on the cap of half-angle theta around s
         // (orthonormal frame: Duff et al., JCGT 6(1), 2017)
-        const float ca = 1.f - ua * in.cone_w;
-        const float sa = __builtin_sqrtf(fmaxf(0.f, 1.f - ca * ca));
+        const float ca = __builtin_fmaf(-ua, in.cone_w, 1.f);
+        const float sa = __builtin_sqrtf(fmaxf(0.f, __builtin_fmaf(-ca, ca, 1.f)));
         const float sgn = sz >= 0.f ? 1.f : -1.f;
         const float a = -__builtin_amdgcn_rcpf(sgn + sz);   // |sgn + sz| >= 1
         const float b = sx * sy * a;
-        const float e1x = 1.f + sgn * sx * sx * a, e1y = sgn * b, e1z = -sgn * sx;
-        const float e2x = b, e2y = sgn + sy * sy * a, e2z = -sy;
-        const float px = cphi * e1x + sphi * e2x;
-        const float py = cphi * e1y + sphi * e2y;
-        const float pz = cphi * e1z + sphi * e2z;
-        tx = ca * sx + sa * px;
-        ty = ca * sy + sa * py;
-        tz = ca * sz + sa * pz;
-        const float inv = __builtin_amdgcn_rsqf(tx * tx + ty * ty + tz * tz);
+        const float e1x = __builtin_fmaf(sgn * sx * sx, a, 1.f), e1y = sgn * b, e1z = -sgn * sx;
+        const float e2x = b, e2y = __builtin_fmaf(sy * sy, a, sgn), e2z = -sy;
+        const float px = __builtin_fmaf(cphi, e1x, sphi * e2x);
+        const float py = __builtin_fmaf(cphi, e1y, sphi * e2y);
+        const float pz = __builtin_fmaf(cphi, e1z, sphi * e2z);
+        tx = __builtin_fmaf(ca, sx, sa * px);
+        ty = __builtin_fmaf(ca, sy, sa * py);
+        tz = __builtin_fmaf(ca, sz, sa * pz);
+        const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz)));
         tx *= inv; ty *= inv; tz *= inv;
     }
     // dE_func: (H + Vsum) . (s_old - s_trial), H = (0, 0, h)
-    const float dE = vx * (sx - tx) + vy * (sy - ty) + (in.h + vz) * (sz - tz);
+    const float dE = __builtin_fmaf(vx, sx - tx, __builtin_fmaf(vy, sy - ty, (in.h + vz) * (sz - tz)));
     return (dE <= 0.f) || (u24(wc) < __expf(-in.beta * dE));
 }
 
@@ -133,18 +141,20 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
                                          float* own_x, float* own_y, float* own_z, const float* oth_x,
                                          const float* oth_y, const float* oth_z, uint32_t e0, int colour,
                                          uint64_t sweep, uint32_t seed_lo, uint32_t seed_hi) {
-    if (MASKED) {
-        // e0 % 4 == 0, so the quad's four presence bits share one mask word;
-        // a quad with no present site (outside a grain) costs nothing
-        if (((mw[colour * (uint32_t)in.mask_stride + (e0 >> 5)] >> (e0 & 31)) & 0xFu) == 0) return 0;
-    }
+    // e0 % 4 == 0, so the quad's four presence bits share one mask word
+    const uint32_t qmask =
+        MASKED ? (mw[colour * (uint32_t)in.mask_stride + (e0 >> 5)] >> (e0 & 31)) & 0xFu : 0xFu;
+    // a quad with no present site (outside a grain) costs nothing
+    if (MASKED && qmask == 0) return 0;
     uint32_t w[12];
     {
+        // element j uses words 3j .. 3j + 2: block 0 serves j = 0, 1, block 1
+        // j = 1, 2, block 2 j = 2, 3; a block no present element uses is skipped
         const uint32_t q3 = (e0 >> 2) * 3u;
         const uint32_t slo = (uint32_t)sweep, shi = (uint32_t)(sweep >> 32), k1 = seed_hi + in.inst_id;
-        philox4x32_10(q3, (uint32_t)colour, slo, shi, seed_lo, k1, w);
-        philox4x32_10(q3 + 1, (uint32_t)colour, slo, shi, seed_lo, k1, w + 4);
-        philox4x32_10(q3 + 2, (uint32_t)colour, slo, shi, seed_lo, k1, w + 8);
+        if (qmask & 3u) philox4x32_10(q3, (uint32_t)colour, slo, shi, seed_lo, k1, w);
+        if (qmask & 6u) philox4x32_10(q3 + 1, (uint32_t)colour, slo, shi, seed_lo, k1, w + 4);
+        if (qmask & 12u) philox4x32_10(q3 + 2, (uint32_t)colour, slo, shi, seed_lo, k1, w + 8);
     }
     const uint32_t S = (uint32_t)in.S;
     int nacc = 0;
@@ -449,39 +459,94 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
 // ---------------------------------------------------------------------------
 // Compacted LDS-resident sweeps of shaped Heisenberg lattices (open
 // boundaries): one workgroup per instance loads only the present sites into
-// LDS (12 B each, plus one zero slot for absent neighbours), runs `nsweeps`
-// sweeps there with precomputed neighbour slots, and stores them back.  A
-// site draws the Philox words its quad gives it in k_sweep_heis (element
-// e0 + m takes words 3m .. 3m + 2 of the quad's three blocks) and adds its
-// bonds in the same slot order, so both kernels produce the same lattice.
+// LDS (x, y, z of slot j at floats 3j .. 3j + 2, plus a zero slot for absent
+// neighbours), runs `nsweeps` sweeps there with precomputed neighbour
+// indices, and stores them back.  A work-item takes one quad with a present
+// site: it draws the quad's Philox blocks once (only those its present
+// elements use) and updates the present elements, element e0 + m with words
+// 3m .. 3m + 2 of the three blocks and its bonds in the same order as
+// k_sweep_heis, so both kernels produce the same lattice.  Quads are grouped
+// by presence pattern, so a wavefront mostly takes the same branches.
 // ---------------------------------------------------------------------------
+
+// Updates of one quad; returns the number accepted.  qr = {quad index q,
+// float index of its first present element (3 x slot) | presence mask << 16
+// (bit m: element e0 + m is present)}; the present elements have
+// consecutive slots.  NB holds, per slot, the float indices of its six
+// neighbours (16 bits each, three aligned words).  Element m uses words
+// 3m .. 3m + 2 of the quad's blocks 3q, 3q + 1, 3q + 2: block 0 serves
+// m = 0, 1, block 1 m = 1, 2 and block 2 m = 2, 3, and only the blocks a
+// present element uses are drawn.
+template <bool DIM3, bool USE_D, bool CONE>
+__device__ __forceinline__ int cquad_update(uint2 qr, const uint16_t* __restrict__ NB, float* ls,
+                                            const DevInst& in, const Stencil& st, uint32_t c, uint32_t slo,
+                                            uint32_t shi, uint32_t k0, uint32_t k1) {
+    const uint32_t msk = qr.y >> 16, q3 = qr.x * 3u, b0 = qr.y & 0xFFFFu;
+    uint32_t w[12];
+    if (msk & 3u) philox4x32_10(q3, c, slo, shi, k0, k1, w);
+    if (msk & 6u) philox4x32_10(q3 + 1u, c, slo, shi, k0, k1, w + 4);
+    if (msk & 12u) philox4x32_10(q3 + 2u, c, slo, shi, k0, k1, w + 8);
+    // every read before any write: the elements are one colour, so no update
+    // here changes a spin another element of the quad reads
+    float sx[4], sy[4], sz[4], vx[4], vy[4], vz[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        if (!((msk >> m) & 1u)) continue;
+        const uint32_t b = b0 + 3u * __builtin_popcount(msk & ((1u << m) - 1u));
+        // the slot's six neighbour indices: 12 B at byte 4 b
+        const uint32_t* nbw = (const uint32_t*)(NB + 2 * b);
+        const uint32_t nw[3] = {nbw[0], nbw[1], nbw[2]};
+        sx[m] = ls[b]; sy[m] = ls[b + 1]; sz[m] = ls[b + 2];
+        vx[m] = 0.f; vy[m] = 0.f; vz[m] = 0.f;
+#pragma unroll
+        for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+            const uint32_t o = (nw[g >> 1] >> (16 * (g & 1))) & 0xFFFFu;
+            add_bond<USE_D>(st, g, ls[o], ls[o + 1], ls[o + 2], vx[m], vy[m], vz[m]);
+        }
+    }
+    int nacc = 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        if (!((msk >> m) & 1u)) continue;
+        float tx, ty, tz;
+        if (heis_decide<CONE>(in, sx[m], sy[m], sz[m], vx[m], vy[m], vz[m], w[3 * m], w[3 * m + 1], w[3 * m + 2],
+                              tx, ty, tz)) {
+            const uint32_t b = b0 + 3u * __builtin_popcount(msk & ((1u << m) - 1u));
+            ls[b] = tx; ls[b + 1] = ty; ls[b + 2] = tz;
+            ++nacc;
+        }
+    }
+    return nacc;
+}
+
 template <bool DIM3, bool USE_D, bool CONE>
 __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict__ insts,
                                                       const CInst* __restrict__ cl, int n,
                                                       float* __restrict__ pool,
                                                       const uint32_t* __restrict__ cs_e,
+                                                      const uint2* __restrict__ cs_q,
                                                       const uint16_t* __restrict__ cs_nb,
                                                       const uint64_t* __restrict__ sweeps, Stencil st, int nsweeps,
                                                       uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                                                       unsigned long long* __restrict__ acc) {
+    // the only LDS of this kernel, so float index i is byte address 4 i
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    float* lx = (float*)lds_raw;
+    float* ls = (float*)lds_raw;
     for (int li = blockIdx.x; li < n; li += gridDim.x) {
         const CInst ci = cl[li];
         const DevInst& in = insts[ci.inst];
         const int N = ci.n0 + ci.n1;
-        float* ly = lx + (N + 1);
-        float* lz = ly + (N + 1);
         const uint32_t* E = cs_e + ci.site0;
         const uint16_t* NB = cs_nb + 6 * ci.site0;
         const int64_t Sp = in.Sp;
         for (int j = threadIdx.x; j < N; j += blockDim.x) {
             const float* p = pool + in.base + (j < ci.n0 ? 0 : 3 * Sp);
             const uint32_t e = E[j];
-            lx[j] = ld32(p, e); ly[j] = ld32(p + Sp, e); lz[j] = ld32(p + 2 * Sp, e);
+            ls[3 * j] = ld32(p, e); ls[3 * j + 1] = ld32(p + Sp, e); ls[3 * j + 2] = ld32(p + 2 * Sp, e);
         }
-        if (threadIdx.x == 0) { lx[N] = 0.f; ly[N] = 0.f; lz[N] = 0.f; }
+        if (threadIdx.x < 3) ls[3 * N + threadIdx.x] = 0.f;
         __syncthreads();
+        const uint2* Q = cs_q + ci.quad0;
         const uint64_t sweep0 = sweeps[ci.inst] + s_off;
         const uint32_t k1 = seed_hi + in.inst_id;
         int nacc = 0;
@@ -489,45 +554,23 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
             const uint64_t sw = sweep0 + (uint64_t)s;
             const uint32_t slo = (uint32_t)sw, shi = (uint32_t)(sw >> 32);
             for (int c = 0; c < 2; ++c) {
-                const int jb = c ? ci.n0 : 0, je = c ? N : ci.n0;
-                for (int j = jb + (int)threadIdx.x; j < je; j += blockDim.x) {
-                    const uint32_t e = E[j];
-                    const uint16_t* nb = NB + 6 * j;
-                    float vx = 0.f, vy = 0.f, vz = 0.f;
-#pragma unroll
-                    for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-                        const int q = nb[g];
-                        add_bond<USE_D>(st, g, lx[q], ly[q], lz[q], vx, vy, vz);
-                    }
-                    const uint32_t q3 = (e >> 2) * 3u, m = e & 3u;
-                    uint32_t a[4], b[4];
-                    const uint32_t blk0 = (3u * m) >> 2;
-                    philox4x32_10(q3 + blk0, (uint32_t)c, slo, shi, seed_lo, k1, a);
-                    uint32_t wa, wb, wc;
-                    if (m == 0) { wa = a[0]; wb = a[1]; wc = a[2]; }
-                    else if (m == 3) { wa = a[1]; wb = a[2]; wc = a[3]; }
-                    else {
-                        philox4x32_10(q3 + blk0 + 1u, (uint32_t)c, slo, shi, seed_lo, k1, b);
-                        if (m == 1) { wa = a[3]; wb = b[0]; wc = b[1]; }
-                        else { wa = a[2]; wb = a[3]; wc = b[0]; }
-                    }
-                    float tx, ty, tz;
-                    if (heis_decide<CONE>(in, lx[j], ly[j], lz[j], vx, vy, vz, wa, wb, wc, tx, ty, tz)) {
-                        lx[j] = tx; ly[j] = ty; lz[j] = tz;
-                        ++nacc;
-                    }
-                }
+                const int tb = c ? ci.nq0 : 0, te = c ? ci.nq0 + ci.nq1 : ci.nq0;
+                for (int t = tb + (int)threadIdx.x; t < te; t += blockDim.x)
+                    nacc += cquad_update<DIM3, USE_D, CONE>(Q[t], NB, ls, in, st, (uint32_t)c, slo, shi, seed_lo,
+                                                            k1);
                 __syncthreads();
             }
         }
         for (int j = threadIdx.x; j < N; j += blockDim.x) {
             float* p = pool + in.base + (j < ci.n0 ? 0 : 3 * Sp);
             const uint32_t e = E[j];
-            st32(p, e, lx[j]); st32(p + Sp, e, ly[j]); st32(p + 2 * Sp, e, lz[j]);
+            st32(p, e, ls[3 * j]); st32(p + Sp, e, ls[3 * j + 1]); st32(p + 2 * Sp, e, ls[3 * j + 2]);
         }
-        // accepted attempts: this workgroup is the only writer of acc[inst]
-        __shared__ int part[16];
+        // accepted attempts: this workgroup is the only writer of acc[inst];
+        // the per-wave partial sums reuse the start of the spin area
         for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        __syncthreads();
+        int* part = (int*)lds_raw;
         if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = nacc;
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -645,26 +688,27 @@ static hipError_t lds_one(hipStream_t stream, int grid, int block, size_t shm, c
 
 template <bool D3, bool UD, bool CN>
 static hipError_t lds_c_one(hipStream_t stream, int grid, int block, size_t shm, const DevInst* insts,
-                            const CInst* cl, int n, float* pool, const uint32_t* cs_e, const uint16_t* cs_nb,
-                            const uint64_t* sweeps, const Stencil& st, int nsweeps, uint32_t s_off,
-                            uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc) {
+                            const CInst* cl, int n, float* pool, const uint32_t* cs_e, const uint2* cs_q,
+                            const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st, int nsweeps,
+                            uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc) {
     auto k = k_sweep_lds_c<D3, UD, CN>;
     hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
-    k<<<grid, block, shm, stream>>>(insts, cl, n, pool, cs_e, cs_nb, sweeps, st, nsweeps, s_off, seed_lo, seed_hi,
-                                   acc);
+    k<<<grid, block, shm, stream>>>(insts, cl, n, pool, cs_e, cs_q, cs_nb, sweeps, st, nsweeps, s_off, seed_lo,
+                                   seed_hi, acc);
     return hipGetLastError();
 }
 
 hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, bool use_d, bool cone, int grid, int block, size_t shm,
                               const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
-                              const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st, int nsweeps,
-                              uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc) {
+                              const uint2* cs_q, const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st,
+                              int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                              unsigned long long* acc) {
     if (n <= 0 || nsweeps <= 0) return hipSuccess;
 #define NMC_LDSC(D3, UD, CN)                          \
     if (dim3 == D3 && use_d == UD && cone == CN)      \
-        return lds_c_one<D3, UD, CN>(stream, grid, block, shm, insts, cl, n, pool, cs_e, cs_nb, sweeps, st, \
-                                     nsweeps, s_off, seed_lo, seed_hi, acc);
+        return lds_c_one<D3, UD, CN>(stream, grid, block, shm, insts, cl, n, pool, cs_e, cs_q, cs_nb, sweeps, \
+                                     st, nsweeps, s_off, seed_lo, seed_hi, acc);
     NMC_LDSC(true, false, false) NMC_LDSC(true, false, true) NMC_LDSC(true, true, false) NMC_LDSC(true, true, true)
     NMC_LDSC(false, false, false) NMC_LDSC(false, false, true) NMC_LDSC(false, true, false)
     NMC_LDSC(false, true, true)
