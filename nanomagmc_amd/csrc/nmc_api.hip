@@ -114,6 +114,7 @@ struct nmc_ctx {
     nmc_config cfg{};
     bool dim3 = false, ising = false, periodic = false, masked = false, use_d = false, cone = false;
     int zslots = 4;          // stencil slots used (4 in 2-D, 6 in 3-D)
+    int dm = 0;              // DMI form of the sweep kernels (dmi_axis in nmc_sweep.hip)
     int tabsize = 0;         // Ising acceptance-table entries per instance
     int nsl = 1;             // topological-charge slices
     int a2max = 2;
@@ -577,6 +578,20 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
             c->st.Dd[g][a] = D;
         }
     }
+    // DMI form: none, one spin axis per bond slot (the lattice's Bloch or
+    // Neel pattern: 3-D slot g along axis g >> 1, 2-D slot g along 2 - (g >> 1)),
+    // or general
+    if (c->use_d) {
+        bool any = false, axial = true;
+        for (int g = c->dim3 ? 0 : 2; g < 6; ++g) {
+            const int k = c->dim3 ? (g >> 1) : 2 - (g >> 1);
+            for (int a = 0; a < 3; ++a) {
+                any = any || c->st.D[g][a] != 0.f;
+                if (a != k && c->st.D[g][a] != 0.f) axial = false;
+            }
+        }
+        c->dm = !any ? 0 : (axial ? 2 : 1);
+    }
     for (int g = 0, p = 1; g < 6; ++g) {
         if (!c->dim3 && g < 2) { c->st.pow3[g] = 0; continue; }
         c->st.pow3[g] = p;
@@ -817,7 +832,7 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
     hipError_t e = hipSuccess;
     for (int k = 0; k < nsweeps && s->n_cl > 0 && e == hipSuccess; k += kLdsSweepChunk)
-        e = launch_sweep_lds_c(cstream, ctx->dim3, ctx->use_d, ctx->cone, std::min(s->n_cl, 65535), s->c_block,
+        e = launch_sweep_lds_c(cstream, ctx->dim3, ctx->dm, ctx->cone, std::min(s->n_cl, 65535), s->c_block,
                                s->c_lds_bytes, ctx->d_inst, s->d_cl, s->n_cl, (float*)ctx->pool, s->d_cs_e,
                                s->d_cs_q, s->d_cs_nb, ctx->d_sweeps, ctx->st, std::min(kLdsSweepChunk, nsweeps - k),
                                (uint32_t)k, slo, shi, ctx->d_acc);
@@ -826,7 +841,7 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
         if (e == hipSuccess) e = j;
     }
     for (int k = 0; k < nsweeps && s->n_lds > 0 && e == hipSuccess; k += kLdsSweepChunk)
-        e = launch_sweep_lds(ctx->stream, ctx->ising, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d, ctx->cone,
+        e = launch_sweep_lds(ctx->stream, ctx->ising, ctx->dim3, ctx->periodic, ctx->masked, ctx->dm, ctx->cone,
                              std::min(s->n_lds, 65535), s->lds_block, s->lds_bytes, ctx->d_inst, s->d_lds, s->n_lds,
                              ctx->pool, ctx->mpool, ctx->d_tab, ctx->d_sweeps, ctx->st,
                              std::min(kLdsSweepChunk, nsweeps - k), (uint32_t)k, slo, shi, ctx->d_acc);
@@ -835,7 +850,7 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
         for (int col = 0; col < 2 && e == hipSuccess; ++col) {
             if (s->n_qt[col] > 0) {
                 const int off = col ? s->n_qt[0] : 0;
-                e = launch_sweep_heis_q(ctx->stream, ctx->dim3, ctx->periodic, ctx->use_d, ctx->cone, true,
+                e = launch_sweep_heis_q(ctx->stream, ctx->dim3, ctx->periodic, ctx->dm, ctx->cone, true,
                                         ctx->d_inst, s->d_qt + off, s->n_qt[col], s->d_qlist, (float*)ctx->pool,
                                         ctx->mpool, ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi,
                                         s->d_tacc + s->n_sweep + off);
@@ -847,7 +862,7 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
                                        s->d_sweep, s->n_sweep, (int8_t*)ctx->pool, ctx->mpool, ctx->d_tab,
                                        ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc);
             else
-                e = launch_sweep_heis(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, ctx->use_d, ctx->cone, true,
+                e = launch_sweep_heis(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, ctx->dm, ctx->cone, true,
                                       s->sweep_block, ctx->d_inst, s->d_sweep, s->n_sweep, (float*)ctx->pool, ctx->mpool,
                                       ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc);
         }

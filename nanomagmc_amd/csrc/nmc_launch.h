@@ -6,14 +6,14 @@ namespace nmc {
 
 // block: 256 work-items per tile (1024 Heisenberg / 512 Ising elements), or
 // 64 (quarter tiles) for sets too small to fill the chip with full tiles.
-hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, bool use_d,
+hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, int dm,
                              bool cone, bool count, int block, const DevInst* insts, const Tile* tiles,
                              int ntiles, float* pool, const uint32_t* mpool, const uint64_t* sweeps,
                              const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo,
                              uint32_t seed_hi, unsigned long long* acc);
 
 // Shaped lattices: colour pass over the listed non-empty quads only.
-hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, bool use_d, bool cone, bool count,
+hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, int dm, bool cone, bool count,
                                const DevInst* insts, const QTile* tiles, int ntiles, const uint32_t* qlist,
                                float* pool, const uint32_t* mpool, const uint64_t* sweeps, const Stencil& st,
                                int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
@@ -26,14 +26,14 @@ hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool
                               uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                               unsigned long long* acc);
 
-hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool periodic, bool masked, bool use_d,
+hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool periodic, bool masked, int dm,
                             bool cone, int grid, int block, size_t shm, const DevInst* insts, const int* list,
                             int n, void* pool, const uint32_t* mpool, const uint32_t* tab, const uint64_t* sweeps,
                             const Stencil& st, int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                             unsigned long long* acc);
 
 // Compacted LDS-resident sweeps of shaped Heisenberg lattices (see CInst).
-hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, bool use_d, bool cone, int grid, int block, size_t shm,
+hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,
                               const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
                               const uint2* cs_q, const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st,
                               int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,

@@ -61,19 +61,40 @@ __device__ __forceinline__ bool mask_bit_s(const uint32_t* mw, const DevInst& in
 // rest: an implicit a * b + c * d may otherwise be fused either way depending
 // on the surrounding code.
 
+// DMI forms of the sweep kernels (template argument DM):
+//  0  no DMI;
+//  1  any D per bond: n x D costs six multiply-adds;
+//  2  D of bond slot g along one spin axis, the one fixed by the lattice:
+//     3-D, D parallel to the bond (Bloch; slot g lies on axis g >> 1);
+//     2-D, D in the plane and perpendicular to the bond (Neel; slots 2, 3
+//     lie on x and carry D along y, slots 4, 5 on y with D along x).
+//     n x D then has two non-zero terms.  The host picks 2 only when every
+//     other component of every D is exactly zero.
+template <bool DIM3>
+__host__ __device__ constexpr int dmi_axis(int g) {
+    return DIM3 ? (g >> 1) : 2 - (g >> 1);
+}
+
 // Local field of stencil neighbour n added to v: J n (+ n x D, the
 // orientation dE_func uses, thermodynamics.cpp:45-46).
-template <bool USE_D>
+template <int DM, bool DIM3>
 __device__ __forceinline__ void add_bond(const Stencil& st, int g, float nx, float ny, float nz, float& vx, float& vy,
                                          float& vz) {
 #pragma clang fp contract(off)
     vx = __builtin_fmaf(st.J[g], nx, vx);
     vy = __builtin_fmaf(st.J[g], ny, vy);
     vz = __builtin_fmaf(st.J[g], nz, vz);
-    if (USE_D) {
+    if (DM == 1) {
         vx = __builtin_fmaf(-nz, st.D[g][1], __builtin_fmaf(ny, st.D[g][2], vx));
         vy = __builtin_fmaf(-nx, st.D[g][2], __builtin_fmaf(nz, st.D[g][0], vy));
         vz = __builtin_fmaf(-ny, st.D[g][0], __builtin_fmaf(nx, st.D[g][1], vz));
+    } else if (DM == 2) {
+        // the terms of the general form whose D component is non-zero
+        const int k = dmi_axis<DIM3>(g);
+        const float d = st.D[g][k];
+        if (k == 0) { vy = __builtin_fmaf(nz, d, vy); vz = __builtin_fmaf(-ny, d, vz); }
+        else if (k == 1) { vx = __builtin_fmaf(-nz, d, vx); vz = __builtin_fmaf(nx, d, vz); }
+        else { vx = __builtin_fmaf(ny, d, vx); vy = __builtin_fmaf(-nx, d, vy); }
     }
 }
 
@@ -136,7 +157,7 @@ __device__ __forceinline__ float f4(const float4& v, int j) {
 // colour (plus one element past the quad's end for the +/-axis-2 bond), and
 // 16-byte stores of the quad.  Otherwise one site at a time.  Returns the
 // number of accepted attempts.
-template <bool DIM3, bool PERIODIC, bool MASKED, bool USE_D, bool CONE>
+template <bool DIM3, bool PERIODIC, bool MASKED, int DM, bool CONE>
 __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, const uint32_t* mw,
                                          float* own_x, float* own_y, float* own_z, const float* oth_x,
                                          const float* oth_y, const float* oth_z, uint32_t e0, int colour,
@@ -171,7 +192,7 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
                 nx = ld4(oth_x, (uint32_t)ne); ny = ld4(oth_y, (uint32_t)ne); nz = ld4(oth_z, (uint32_t)ne);
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) add_bond<USE_D>(st, g, f4(nx, j), f4(ny, j), f4(nz, j), vx[j], vy[j], vz[j]);
+            for (int j = 0; j < 4; ++j) add_bond<DM, DIM3>(st, g, f4(nx, j), f4(ny, j), f4(nz, j), vx[j], vy[j], vz[j]);
         }
         // axis 2 within the row: element k-1+q (minus) and k+q (plus)
         {
@@ -200,8 +221,8 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
                     mx = f4(ax, j); my = f4(ay, j); mz = f4(az, j);
                     qx = j < 3 ? f4(ax, j + 1) : ex; qy = j < 3 ? f4(ay, j + 1) : ey; qz = j < 3 ? f4(az, j + 1) : ez;
                 }
-                add_bond<USE_D>(st, 4, mx, my, mz, vx[j], vy[j], vz[j]);
-                add_bond<USE_D>(st, 5, qx, qy, qz, vx[j], vy[j], vz[j]);
+                add_bond<DM, DIM3>(st, 4, mx, my, mz, vx[j], vy[j], vz[j]);
+                add_bond<DM, DIM3>(st, 5, qx, qy, qz, vx[j], vy[j], vz[j]);
             }
         }
         float ox[4], oy[4], oz[4];
@@ -240,7 +261,7 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
             const int ne = nbr_elem<PERIODIC>(in, p, g);
             if (PERIODIC || ne >= 0) {
                 const uint32_t u = (uint32_t)ne;
-                add_bond<USE_D>(st, g, ld32(oth_x, u), ld32(oth_y, u), ld32(oth_z, u), vx, vy, vz);
+                add_bond<DM, DIM3>(st, g, ld32(oth_x, u), ld32(oth_y, u), ld32(oth_z, u), vx, vy, vz);
             }
         }
         float tx, ty, tz;
@@ -299,7 +320,7 @@ __device__ __forceinline__ int ising_update2(const DevInst& in, const Stencil& s
 // ---------------------------------------------------------------------------
 // Streaming colour passes.
 // ---------------------------------------------------------------------------
-template <bool DIM3, bool PERIODIC, bool MASKED, bool USE_D, bool CONE, bool COUNT>
+template <bool DIM3, bool PERIODIC, bool MASKED, int DM, bool CONE, bool COUNT>
 __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ insts,
                                                     const Tile* __restrict__ tiles, int ntiles,
                                                     float* __restrict__ pool,
@@ -319,7 +340,7 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
     const float* oth_x = pool + in.base + (int64_t)(colour ^ 1) * 3 * Sp;
     int nacc = 0;
     if (e0 < (uint32_t)in.S)
-        nacc = heis_quad<DIM3, PERIODIC, MASKED, USE_D, CONE>(in, st, mpool + in.mask_base, own_x, own_x + Sp, own_x + 2 * Sp,
+        nacc = heis_quad<DIM3, PERIODIC, MASKED, DM, CONE>(in, st, mpool + in.mask_base, own_x, own_x + Sp, own_x + 2 * Sp,
                                                               oth_x, oth_x + Sp, oth_x + 2 * Sp, e0, colour,
                                                               sweeps[tl.inst] + s_off, seed_lo, seed_hi);
     if (COUNT) {
@@ -332,7 +353,7 @@ __global__ __launch_bounds__(256) void k_sweep_heis(const DevInst* __restrict__ 
 // takes 64 consecutive entries of the quad list, so no lane idles on a quad
 // outside the grain.  The quad's element index fixes its random numbers, so
 // the lattice evolves exactly as under k_sweep_heis.
-template <bool DIM3, bool PERIODIC, bool USE_D, bool CONE, bool COUNT>
+template <bool DIM3, bool PERIODIC, int DM, bool CONE, bool COUNT>
 __global__ __launch_bounds__(256) void k_sweep_heis_q(const DevInst* __restrict__ insts,
                                                       const QTile* __restrict__ tiles, int ntiles,
                                                       const uint32_t* __restrict__ qlist,
@@ -351,7 +372,7 @@ __global__ __launch_bounds__(256) void k_sweep_heis_q(const DevInst* __restrict_
     int nacc = 0;
     if ((int)threadIdx.x < tl.qn) {
         const uint32_t e0 = qlist[tl.q0 + (int)threadIdx.x];
-        nacc = heis_quad<DIM3, PERIODIC, true, USE_D, CONE>(in, st, mpool + in.mask_base, own_x, own_x + Sp,
+        nacc = heis_quad<DIM3, PERIODIC, true, DM, CONE>(in, st, mpool + in.mask_base, own_x, own_x + Sp,
                                                             own_x + 2 * Sp, oth_x, oth_x + Sp, oth_x + 2 * Sp, e0,
                                                             colour, sweeps[tl.inst] + s_off, seed_lo, seed_hi);
     }
@@ -393,7 +414,7 @@ __global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__
 // `nsweeps` full sweeps on the lattice held in LDS.  Every workgroup leaves
 // the instance loop at the same trip (li >= n), so barriers are uniform.
 // ---------------------------------------------------------------------------
-template <bool ISING, bool DIM3, bool PERIODIC, bool MASKED, bool USE_D, bool CONE>
+template <bool ISING, bool DIM3, bool PERIODIC, bool MASKED, int DM, bool CONE>
 __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ insts, const int* __restrict__ list,
                                                     int n, void* __restrict__ pool,
                                                     const uint32_t* __restrict__ mpool,
@@ -430,7 +451,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
                     float* own = (float*)lds_raw + c * 3 * Sp;
                     const float* oth = (const float*)lds_raw + (c ^ 1) * 3 * Sp;
                     for (uint32_t e0 = 4u * threadIdx.x; e0 < (uint32_t)in.S; e0 += 4u * blockDim.x)
-                        nacc += heis_quad<DIM3, PERIODIC, MASKED, USE_D, CONE>(in, st, lmask, own, own + Sp,
+                        nacc += heis_quad<DIM3, PERIODIC, MASKED, DM, CONE>(in, st, lmask, own, own + Sp,
                                                                                own + 2 * Sp, oth, oth + Sp,
                                                                                oth + 2 * Sp, e0, c, sweep0 + s,
                                                                                seed_lo, seed_hi);
@@ -477,7 +498,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
 // 3m .. 3m + 2 of the quad's blocks 3q, 3q + 1, 3q + 2: block 0 serves
 // m = 0, 1, block 1 m = 1, 2 and block 2 m = 2, 3, and only the blocks a
 // present element uses are drawn.
-template <bool DIM3, bool USE_D, bool CONE>
+template <bool DIM3, int DM, bool CONE>
 __device__ __forceinline__ int cquad_update(uint2 qr, const uint16_t* __restrict__ NB, float* ls,
                                             const DevInst& in, const Stencil& st, uint32_t c, uint32_t slo,
                                             uint32_t shi, uint32_t k0, uint32_t k1) {
@@ -501,7 +522,7 @@ __device__ __forceinline__ int cquad_update(uint2 qr, const uint16_t* __restrict
 #pragma unroll
         for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
             const uint32_t o = (nw[g >> 1] >> (16 * (g & 1))) & 0xFFFFu;
-            add_bond<USE_D>(st, g, ls[o], ls[o + 1], ls[o + 2], vx[m], vy[m], vz[m]);
+            add_bond<DM, DIM3>(st, g, ls[o], ls[o + 1], ls[o + 2], vx[m], vy[m], vz[m]);
         }
     }
     int nacc = 0;
@@ -519,7 +540,7 @@ __device__ __forceinline__ int cquad_update(uint2 qr, const uint16_t* __restrict
     return nacc;
 }
 
-template <bool DIM3, bool USE_D, bool CONE>
+template <bool DIM3, int DM, bool CONE>
 __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict__ insts,
                                                       const CInst* __restrict__ cl, int n,
                                                       float* __restrict__ pool,
@@ -556,7 +577,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
             for (int c = 0; c < 2; ++c) {
                 const int tb = c ? ci.nq0 : 0, te = c ? ci.nq0 + ci.nq1 : ci.nq0;
                 for (int t = tb + (int)threadIdx.x; t < te; t += blockDim.x)
-                    nacc += cquad_update<DIM3, USE_D, CONE>(Q[t], NB, ls, in, st, (uint32_t)c, slo, shi, seed_lo,
+                    nacc += cquad_update<DIM3, DM, CONE>(Q[t], NB, ls, in, st, (uint32_t)c, slo, shi, seed_lo,
                                                             k1);
                 __syncthreads();
             }
@@ -586,26 +607,24 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
 // Launchers.
 // ---------------------------------------------------------------------------
 #define NMC_HEIS_CASE(D3, PER, MSK, UD, CN, CT)                                                     \
-    if (dim3 == D3 && periodic == PER && masked == MSK && use_d == UD && cone == CN && count == CT) \
+    if (dim3 == D3 && periodic == PER && masked == MSK && dm == UD && cone == CN && count == CT)    \
         k_sweep_heis<D3, PER, MSK, UD, CN, CT><<<ntiles, block, 0, stream>>>(                       \
             insts, tiles, ntiles, pool, mpool, sweeps, st, colour, s_off, seed_lo, seed_hi, acc);
 
-hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, bool use_d,
+hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool masked, int dm,
                              bool cone, bool count, int block, const DevInst* insts, const Tile* tiles,
                              int ntiles, float* pool, const uint32_t* mpool, const uint64_t* sweeps,
                              const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo,
                              uint32_t seed_hi, unsigned long long* acc) {
     if (ntiles <= 0) return hipSuccess;
     if (block != 64 && block != 256) return hipErrorInvalidValue;
-#define NMC_HEIS_B(D3, PER, MSK)                     \
-    NMC_HEIS_CASE(D3, PER, MSK, false, false, false) \
-    NMC_HEIS_CASE(D3, PER, MSK, false, false, true)  \
-    NMC_HEIS_CASE(D3, PER, MSK, false, true, false)  \
-    NMC_HEIS_CASE(D3, PER, MSK, false, true, true)   \
-    NMC_HEIS_CASE(D3, PER, MSK, true, false, false)  \
-    NMC_HEIS_CASE(D3, PER, MSK, true, false, true)   \
-    NMC_HEIS_CASE(D3, PER, MSK, true, true, false)   \
-    NMC_HEIS_CASE(D3, PER, MSK, true, true, true)
+#define NMC_HEIS_D(D3, PER, MSK, UD)            \
+    NMC_HEIS_CASE(D3, PER, MSK, UD, false, false) \
+    NMC_HEIS_CASE(D3, PER, MSK, UD, false, true)  \
+    NMC_HEIS_CASE(D3, PER, MSK, UD, true, false)  \
+    NMC_HEIS_CASE(D3, PER, MSK, UD, true, true)
+#define NMC_HEIS_B(D3, PER, MSK) \
+    NMC_HEIS_D(D3, PER, MSK, 0) NMC_HEIS_D(D3, PER, MSK, 1) NMC_HEIS_D(D3, PER, MSK, 2)
     NMC_HEIS_B(true, true, false)
     NMC_HEIS_B(true, false, false)
     NMC_HEIS_B(true, false, true)
@@ -615,34 +634,33 @@ hipError_t launch_sweep_heis(hipStream_t stream, bool dim3, bool periodic, bool 
     NMC_HEIS_B(true, true, true)
     NMC_HEIS_B(false, true, true)
 #undef NMC_HEIS_B
+#undef NMC_HEIS_D
     return hipGetLastError();
 }
 
 #define NMC_HEISQ_CASE(D3, PER, UD, CN, CT)                                                      \
-    if (dim3 == D3 && periodic == PER && use_d == UD && cone == CN && count == CT)                \
+    if (dim3 == D3 && periodic == PER && dm == UD && cone == CN && count == CT)                   \
         k_sweep_heis_q<D3, PER, UD, CN, CT><<<ntiles, 256, 0, stream>>>(                          \
             insts, tiles, ntiles, qlist, pool, mpool, sweeps, st, colour, s_off, seed_lo, seed_hi, acc);
 
-hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, bool use_d, bool cone, bool count,
+hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, int dm, bool cone, bool count,
                                const DevInst* insts, const QTile* tiles, int ntiles, const uint32_t* qlist,
                                float* pool, const uint32_t* mpool, const uint64_t* sweeps, const Stencil& st,
                                int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                                unsigned long long* acc) {
     if (ntiles <= 0) return hipSuccess;
-#define NMC_HEISQ_B(D3, PER)                  \
-    NMC_HEISQ_CASE(D3, PER, false, false, false) \
-    NMC_HEISQ_CASE(D3, PER, false, false, true)  \
-    NMC_HEISQ_CASE(D3, PER, false, true, false)  \
-    NMC_HEISQ_CASE(D3, PER, false, true, true)   \
-    NMC_HEISQ_CASE(D3, PER, true, false, false)  \
-    NMC_HEISQ_CASE(D3, PER, true, false, true)   \
-    NMC_HEISQ_CASE(D3, PER, true, true, false)   \
-    NMC_HEISQ_CASE(D3, PER, true, true, true)
+#define NMC_HEISQ_D(D3, PER, UD)            \
+    NMC_HEISQ_CASE(D3, PER, UD, false, false) \
+    NMC_HEISQ_CASE(D3, PER, UD, false, true)  \
+    NMC_HEISQ_CASE(D3, PER, UD, true, false)  \
+    NMC_HEISQ_CASE(D3, PER, UD, true, true)
+#define NMC_HEISQ_B(D3, PER) NMC_HEISQ_D(D3, PER, 0) NMC_HEISQ_D(D3, PER, 1) NMC_HEISQ_D(D3, PER, 2)
     NMC_HEISQ_B(true, false)
     NMC_HEISQ_B(false, false)
     NMC_HEISQ_B(true, true)
     NMC_HEISQ_B(false, true)
 #undef NMC_HEISQ_B
+#undef NMC_HEISQ_D
     return hipGetLastError();
 }
 
@@ -673,7 +691,7 @@ hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool
     return hipGetLastError();
 }
 
-template <bool IS, bool D3, bool PER, bool MSK, bool UD, bool CN>
+template <bool IS, bool D3, bool PER, bool MSK, int UD, bool CN>
 static hipError_t lds_one(hipStream_t stream, int grid, int block, size_t shm, const DevInst* insts, const int* list,
                           int n, void* pool, const uint32_t* mpool, const uint32_t* tab, const uint64_t* sweeps,
                           const Stencil& st, int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
@@ -686,7 +704,7 @@ static hipError_t lds_one(hipStream_t stream, int grid, int block, size_t shm, c
     return hipGetLastError();
 }
 
-template <bool D3, bool UD, bool CN>
+template <bool D3, int UD, bool CN>
 static hipError_t lds_c_one(hipStream_t stream, int grid, int block, size_t shm, const DevInst* insts,
                             const CInst* cl, int n, float* pool, const uint32_t* cs_e, const uint2* cs_q,
                             const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st, int nsweeps,
@@ -699,37 +717,39 @@ static hipError_t lds_c_one(hipStream_t stream, int grid, int block, size_t shm,
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, bool use_d, bool cone, int grid, int block, size_t shm,
+hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,
                               const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
                               const uint2* cs_q, const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st,
                               int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                               unsigned long long* acc) {
     if (n <= 0 || nsweeps <= 0) return hipSuccess;
 #define NMC_LDSC(D3, UD, CN)                          \
-    if (dim3 == D3 && use_d == UD && cone == CN)      \
+    if (dim3 == D3 && dm == UD && cone == CN)         \
         return lds_c_one<D3, UD, CN>(stream, grid, block, shm, insts, cl, n, pool, cs_e, cs_q, cs_nb, sweeps, \
                                      st, nsweeps, s_off, seed_lo, seed_hi, acc);
-    NMC_LDSC(true, false, false) NMC_LDSC(true, false, true) NMC_LDSC(true, true, false) NMC_LDSC(true, true, true)
-    NMC_LDSC(false, false, false) NMC_LDSC(false, false, true) NMC_LDSC(false, true, false)
-    NMC_LDSC(false, true, true)
+#define NMC_LDSC_D(D3, UD) NMC_LDSC(D3, UD, false) NMC_LDSC(D3, UD, true)
+    NMC_LDSC_D(true, 0) NMC_LDSC_D(true, 1) NMC_LDSC_D(true, 2)
+    NMC_LDSC_D(false, 0) NMC_LDSC_D(false, 1) NMC_LDSC_D(false, 2)
+#undef NMC_LDSC_D
 #undef NMC_LDSC
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool periodic, bool masked, bool use_d,
+hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool periodic, bool masked, int dm,
                             bool cone, int grid, int block, size_t shm, const DevInst* insts, const int* list,
                             int n, void* pool, const uint32_t* mpool, const uint32_t* tab, const uint64_t* sweeps,
                             const Stencil& st, int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                             unsigned long long* acc) {
     if (n <= 0 || nsweeps <= 0) return hipSuccess;
 #define NMC_LDS(IS, D3, PER, MSK, UD, CN)                                                                   \
-    if (ising == IS && dim3 == D3 && periodic == PER && masked == MSK && use_d == UD && cone == CN)        \
+    if (ising == IS && dim3 == D3 && periodic == PER && masked == MSK && dm == UD && cone == CN)           \
         return lds_one<IS, D3, PER, MSK, UD, CN>(stream, grid, block, shm, insts, list, n, pool, mpool, tab, \
                                                  sweeps, st, nsweeps, s_off, seed_lo, seed_hi, acc);
-#define NMC_LDS_H(D3, PER, MSK)                                                        \
-    NMC_LDS(false, D3, PER, MSK, false, false) NMC_LDS(false, D3, PER, MSK, false, true) \
-    NMC_LDS(false, D3, PER, MSK, true, false) NMC_LDS(false, D3, PER, MSK, true, true)   \
-    NMC_LDS(true, D3, PER, MSK, false, false)
+#define NMC_LDS_H(D3, PER, MSK)                                                \
+    NMC_LDS(false, D3, PER, MSK, 0, false) NMC_LDS(false, D3, PER, MSK, 0, true) \
+    NMC_LDS(false, D3, PER, MSK, 1, false) NMC_LDS(false, D3, PER, MSK, 1, true) \
+    NMC_LDS(false, D3, PER, MSK, 2, false) NMC_LDS(false, D3, PER, MSK, 2, true) \
+    NMC_LDS(true, D3, PER, MSK, 0, false)
     NMC_LDS_H(true, true, false)
     NMC_LDS_H(true, true, true)
     NMC_LDS_H(true, false, false)

@@ -100,6 +100,29 @@ def langevin(x):
     return 1 / math.tanh(x) - 1 / x
 
 
+def dmi_dimer_energy(beta, J, D):
+    """<E> of two classical unit spins with E = -J s1.s2 - D.(s1 x s2),
+    |D| = D.  Integrating s1 over the sphere against the field
+    b = J s2 + s2 x D, |b|^2 = J^2 + D^2 (1 - u^2) with u = cos(s2, D), leaves
+    Z ~ int_{-1}^{1} sinh(beta K) / (beta K) du, K = |b|; <E> = -d ln Z / d beta."""
+    from scipy import integrate
+
+    def K(u):
+        return math.sqrt(J * J + D * D * (1 - u * u))
+
+    def f(u):
+        x = beta * K(u)
+        return math.sinh(x) / x
+
+    def df(u):
+        x = beta * K(u)
+        return (math.cosh(x) - math.sinh(x) / x) / beta
+
+    Z, _ = integrate.quad(f, -1, 1, epsabs=0, epsrel=1e-12)
+    dZ, _ = integrate.quad(df, -1, 1, epsabs=0, epsrel=1e-12)
+    return -dZ / Z
+
+
 def tau_int(x, c=6.0):
     """Sokal's automatic windowing: tau = 1/2 + sum_{t=1}^{W} rho(t), with the
     smallest W >= c * tau(W)."""

@@ -228,16 +228,20 @@ def test_checkerboard_trajectory_identical(orc, bloch3d, dim, edge, periodic, ma
         assert np.array_equal(lat.get_spins(i), s), "instance %d" % i
 
 
-@pytest.mark.parametrize("ising,dim,periodic,masked,trial", [
-    (True, 2, 1, False, "sphere"), (True, 3, 0, True, "sphere"),
-    (False, 2, 1, False, "sphere"), (False, 3, 0, True, "cone"), (False, 3, 1, False, "cone"),
-    (False, 2, 0, True, "sphere")])
-def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, periodic, masked, trial):
+@pytest.mark.parametrize("ising,dim,periodic,masked,trial,general", [
+    (True, 2, 1, False, "sphere", False), (True, 3, 0, True, "sphere", False),
+    (False, 2, 1, False, "sphere", False), (False, 3, 0, True, "cone", False), (False, 3, 1, False, "cone", False),
+    (False, 2, 0, True, "sphere", False), (False, 3, 0, True, "sphere", True), (False, 2, 1, False, "cone", True)])
+def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, periodic, masked, trial, general):
     # the LDS-resident launch (whole sweeps in one workgroup) and the
-    # streaming launches (one per colour pass) draw the same numbers
+    # streaming launches (one per colour pass) draw the same numbers; with
+    # `general` the DMI vectors are not on one spin axis per bond (the
+    # kernels' six-term form instead of the two-term one)
     rng = np.random.default_rng(31 + dim)
     edge = 10 if dim == 3 else 24
     bd, J, D = (orc.parse_bonds(bloch3d, 3, 1.0, 0.2) if dim == 3 else neel2d(0.2))
+    if general:
+        D = np.asarray(D) + 0.1 * np.sign(np.asarray(D).sum(axis=1, keepdims=True))
     locs = None
     if masked:
         locs = box_locs(dim, edge)
@@ -365,6 +369,46 @@ def test_heisenberg_single_spin_and_dimer_langevin(orc, trial):
     for i in range(3, 6):
         m, err, _ = exact.mean_err(obs[i, :, 9])
         assert abs(m - (-exact.langevin(betas[i]))) < 4 * err
+
+
+@pytest.mark.parametrize("dim,form", [(2, "neel"), (2, "general"), (3, "bloch"), (3, "general")])
+def test_dmi_dimer_energy(dim, form):
+    # open dimer with exchange and DMI: <E_s> = -d ln Z / d beta of
+    # E = -J s1.s2 - D.(s1 x s2).  "neel"/"bloch" tables put each bond's D on
+    # one spin axis (the sweep kernels' two-term form), "general" tables do
+    # not (the six-term form)
+    from nanomagmc_amd import Bond, Lattices
+    dirs = [(1, 0), (-1, 0), (0, 1), (0, -1)] if dim == 2 else \
+        [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    d = 0.8
+    if form == "neel":
+        Dp = {0: (0.0, d, 0.0), 1: (-d, 0.0, 0.0)}           # z x r for +x, +y
+    elif form == "bloch":
+        Dp = {a: tuple(d * float(a == k) for k in range(3)) for a in range(3)}   # along the bond
+    else:
+        v = np.array([0.3, -0.4, 0.5])
+        v *= d / np.linalg.norm(v)
+        Dp = {a: tuple(np.roll(v, a)) for a in range(dim)}
+    bonds = []
+    for r in dirs:
+        a = int(np.flatnonzero(r)[0])
+        sgn = float(r[a])
+        bonds.append(Bond(r, 1.0, tuple(sgn * x for x in Dp[a])))
+    n = (2, 1) if dim == 2 else (2, 1, 1)
+    betas = [0.5, 1.0, 2.0]
+    lat = Lattices(dim, False, False, bonds, [{"n": n, "edge": 2}] * 3, use_d=True, seed=23)
+    lat.set_params(betas, 0.0)
+    lat.randomize()
+    lat.equil(200)
+    nsamp = 20000
+    lat.series_reset(nsamp, with_charge=False)
+    for k in range(nsamp):
+        lat.equil(2)
+        lat.sample(k)
+    obs, _, _ = lat.get_series()
+    for i, beta in enumerate(betas):
+        m, err, _ = exact.mean_err(obs[i, :, 10])
+        assert abs(m - exact.dmi_dimer_energy(beta, 1.0, d)) < 4 * err, (i, m, err)
 
 
 def test_randomize_draws_uniform_directions(orc):
