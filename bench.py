@@ -60,10 +60,12 @@ def heis3d(trial, replicas=1):
     return lat, sites, 36, desc
 
 
-def grains(trial, replicas=1):
+def grains(trial, replicas=1, site_range=None):
     """512 Weibull spheres (LATTSHAPE x, open): radius ~ log-normal (mean 10,
     sd 2.5, truncated to [4, 20]), Weibull factor 8, J=1, Bloch D=0.2,
-    h_z in {0,0.02,0.05,0.1} x 16 T 2.0->0.1 = 32768 instances."""
+    h_z in {0,0.02,0.05,0.1} x 16 T 2.0->0.1 = 32768 instances.  With
+    site_range = (lo, hi) only the grains with lo <= present sites <= hi are
+    kept (to time the LDS-resident and streamed parts of the batch apart)."""
     from nanomagmc_amd import Bond, Lattices, build_weibull, crop_to_box
     ngr, edge, wb = 512, 45, 8.0
     am, asd = 10.0, 2.5
@@ -78,6 +80,8 @@ def grains(trial, replicas=1):
     for g in range(ngr):
         n, lo, sub = crop_to_box(masks[g], 3, edge)
         nk = int(sub.sum())
+        if site_range is not None and not site_range[0] <= nk <= site_range[1]:
+            continue
         for h in fields:
             for T in temps:
                 insts.append({"n": tuple(n), "origin": tuple(lo), "edge": edge, "mask": sub})
@@ -91,6 +95,9 @@ def grains(trial, replicas=1):
     desc = ("512 Weibull grains (radius log-normal 10+-2.5 in [4,20], factor 8, open) x 4 h x 16 T = "
             "32768 instances, Heisenberg J=1 + Bloch D=0.2, one sweep of every instance per step, %s trials"
             % trial)
+    if site_range is not None:
+        desc += "; only the %d instances of grains with %d..%d present sites" % (len(insts), site_range[0],
+                                                                                 site_range[1])
     return lat, sites, 36, desc
 
 
@@ -230,6 +237,8 @@ def main():
     ap.add_argument("--path", default=None, choices=["auto", "stream"], help="force a sweep path")
     ap.add_argument("--replicas", type=int, default=1,
                     help="heis3d: independent copies of every chain (working sets beyond the 256 MB cache)")
+    ap.add_argument("--grain-sites", default=None, metavar="LO:HI",
+                    help="grains: keep only grains with LO..HI present sites")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the effective-samples leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -251,7 +260,13 @@ def main():
     import torch
     trial = args.trial or ("cone" if args.workload == "heis3d" else "sphere")
     t_setup = time.perf_counter()
-    lat, sites, bpu, desc = WORKLOADS[args.workload](trial, replicas=args.replicas)
+    extra = {}
+    if args.grain_sites:
+        if args.workload != "grains":
+            sys.exit("--grain-sites applies to --workload grains")
+        lo, hi = (int(x) for x in args.grain_sites.split(":"))
+        extra["site_range"] = (lo, hi)
+    lat, sites, bpu, desc = WORKLOADS[args.workload](trial, replicas=args.replicas, **extra)
     setup_s = time.perf_counter() - t_setup
     total_sites = int(np.sum(sites))
     tuned = None
@@ -293,7 +308,7 @@ def main():
     achieved = (total_sites / 2) * bpu / launch_s
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
-    if os.path.exists(tpath) and args.replicas == 1:   # measured for the default batch only
+    if os.path.exists(tpath) and args.replicas == 1 and not extra:   # measured for the default batch only
         tj = json.load(open(tpath))
         if tj.get("path", "auto") == (args.path or "auto"):   # and on the sweep path it was measured on
             traffic = tj.get("hbm_bytes_per_launch")

@@ -4,16 +4,19 @@
 // dE_func (Old/lib/thermodynamics.cpp:51-69) with the trial spin of
 // field_type::gen_rand (Old/lib/field_type.cpp:185-195).
 //
-// A sweep is two colour passes.  Two launch forms share one update routine:
-//  * streaming (k_sweep_heis / k_sweep_ising): one launch per colour pass over
+// A sweep is two colour passes.  The launch forms share the update helpers:
+//  * streaming (k_sweep_heis / k_sweep_ising, and k_sweep_heis_q over the
+//    non-empty quads of shaped lattices): one launch per colour pass over
 //    tiles of every instance; each work-item owns four consecutive sites of
 //    the colour (Ising: two), reads their neighbours from the other colour
 //    array in HBM/L2 with coalesced 16-byte loads, and writes them back;
 //  * LDS-resident (k_sweep_lds): one persistent workgroup per instance keeps
 //    both colour arrays in LDS for `nsweeps` sweeps, with a workgroup barrier
-//    between colour passes; HBM is touched only to load and store the lattice.
-// Both draw the same Philox4x32-10 stream per (instance, sweep, colour,
-// element), so both produce the same trajectory bit for bit.
+//    between colour passes; HBM is touched only to load and store the lattice;
+//  * compacted LDS-resident (k_sweep_lds_c): the same for shaped lattices
+//    whose present sites alone fit in LDS, with precomputed neighbour indices.
+// All draw the same Philox4x32-10 stream per (instance, sweep, colour,
+// element), so all produce the same trajectory bit for bit.
 #include "nmc_device.h"
 #include "nmc_launch.h"
 
