@@ -107,6 +107,7 @@ struct Set {
 constexpr int kLdsSweepChunk = 256;   // sweeps per LDS-resident launch
 constexpr int kGraphSweeps = 64;      // sweeps per captured graph of a small set
 constexpr size_t kStreamLdsReserve = 1024;   // LDS a compacted lattice leaves free on its CU
+constexpr int kMeasureTile = 1024;    // colour elements per measurement tile (one workgroup)
 
 }  // namespace
 
@@ -300,7 +301,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         }
         for (int64_t e = 0; e < d.Sp; e += 256) ini.push_back(Tile{i, (int)e});
         const int rows = d.a0 * d.a1;
-        int per = std::max(4, (int)std::min<int64_t>(rows, (4096 + d.half - 1) / std::max(1, d.half)));
+        int per = std::max(4, (int)std::min<int64_t>(rows, (kMeasureTile + d.half - 1) / std::max(1, d.half)));
         tb.push_back((int)mt.size());
         for (int r = 0; r < rows; r += per) mt.push_back(MTile{i, r, std::min(per, rows - r)});
         te.push_back((int)mt.size());

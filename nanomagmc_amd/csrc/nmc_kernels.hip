@@ -32,7 +32,7 @@ __device__ __forceinline__ double solid_angle_d(double s1x, double s1y, double s
                                                 double s2y, double s2z, double s3x, double s3y,
                                                 double s3z) {
 #pragma clang fp contract(off)
-    // solid_angle (thermodynamics.cpp:200-221), same expression order
+    // solid_angle (thermodynamics.cpp:200-221): the same products and sums
     const double s1s2 = s1x * s2x + s1y * s2y + s1z * s2z;
     const double s1s3 = s1x * s3x + s1y * s3y + s1z * s3z;
     const double s2s3 = s2x * s3x + s2y * s3y + s2z * s3z;
@@ -42,7 +42,19 @@ __device__ __forceinline__ double solid_angle_d(double s1x, double s1y, double s
     const double cy = s1z * s3x - s1x * s3z;
     const double cz = s1x * s3y - s1y * s3x;
     const double crosssum = cx * s2x + cy * s2y + cz * s2z;
-    return -atan2(crosssum / rho, dotsum / rho);
+    // atan2(crosssum / rho, dotsum / rho) without the two divisions: scaling
+    // both arguments by |rho| leaves the angle unchanged, only the sign of rho
+    // matters.  rho = 0 (an antiparallel pair) keeps the reference's inf/NaN
+    // quotients.
+    double y = crosssum, x = dotsum;
+    if (rho == 0.0) {
+        y = crosssum / rho;
+        x = dotsum / rho;
+    } else if (rho < 0.0) {
+        y = -y;
+        x = -x;
+    }
+    return -atan2(y, x);
 }
 
 template <bool ISING>
