@@ -263,29 +263,36 @@ def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, period
     assert np.array_equal(lats[0].accepts(), lats[1].accepts())
 
 
-def test_compacted_and_streamed_grains_agree(bloch3d, orc):
-    # one set holding a grain small enough for the compacted LDS sweep and one
+@pytest.mark.parametrize("dim,trial,table", [(3, "cone", "bloch"), (3, "sphere", "general"), (2, "sphere", "neel"),
+                                              (2, "cone", "general")])
+def test_compacted_and_streamed_grains_agree(bloch3d, orc, dim, trial, table):
+    # one set holding grains small enough for the compacted LDS sweep and one
     # too large for it (streamed over its non-empty quads): the lattice and the
-    # accept counts must equal an all-streaming run bit for bit
+    # accept counts must equal an all-streaming run bit for bit, for both DMI
+    # forms of the kernels
     from nanomagmc_amd import Bond, Lattices
-    rng = np.random.default_rng(5)
-    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
-    bonds = [Bond(bd[k], J[k], D[k]) for k in range(len(J))]
+    rng = np.random.default_rng(5 + dim)
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2) if dim == 3 else neel2d(0.2)
+    D = np.asarray(D)
+    if table == "general":
+        D = D + 0.1 * np.sign(D.sum(axis=1, keepdims=True))
+    bonds = [Bond(bd[k][:dim], J[k], D[k]) for k in range(len(J))]
     insts = []
-    for n in (10, 32):
-        m = (rng.random(n ** 3) < 0.75).astype(np.uint8)
-        insts.append({"n": (n, n, n), "edge": n, "mask": m})
+    for n in ((10, 7, 32) if dim == 3 else (40, 23, 200)):
+        m = (rng.random(n ** dim) < 0.75).astype(np.uint8)
+        insts.append({"n": (n,) * dim, "edge": n, "mask": m})
     runs = []
     for path in ("auto", "stream"):
-        lat = Lattices(3, False, False, bonds, insts, use_d=True, trial="cone", seed=91)
+        lat = Lattices(dim, False, False, bonds, insts, use_d=True, trial=trial, seed=91)
         lat.set_path(path)
-        lat.set_params([0.7, 1.3], [0.05, 0.0])
-        lat.set_cone([0.8, 1.1])
+        lat.set_params([0.7, 1.3, 2.0], [0.05, 0.0, -0.1])
+        if trial == "cone":
+            lat.set_cone([0.8, 1.1, 0.5])
         lat.randomize()
         lat.equil(5)
-        lat.equil(260, inst=[0, 1])
+        lat.equil(260, inst=[0, 2])
         runs.append(lat)
-    for i in range(2):
+    for i in range(3):
         assert np.array_equal(runs[0].get_spins(i), runs[1].get_spins(i)), i
     assert np.array_equal(runs[0].accepts(), runs[1].accepts())
 
