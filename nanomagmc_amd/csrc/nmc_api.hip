@@ -74,6 +74,7 @@ struct Set {
     Tile* d_init = nullptr;
     int n_init = 0;
     MTile* d_mt = nullptr;
+    uint32_t* d_msites = nullptr;   // shaped lattices: present sites to measure (e | colour << 31)
     int n_mt = 0;
     int* d_tb = nullptr;
     int* d_te = nullptr;
@@ -95,6 +96,8 @@ struct Set {
     void release() {
         (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
         (void)hipFree(d_mt); (void)hipFree(d_tb); (void)hipFree(d_te); (void)hipFree(d_lds);
+        (void)hipFree(d_msites);
+        d_msites = nullptr;
         (void)hipFree(d_tacc); (void)hipFree(d_qt); (void)hipFree(d_qlist);
         (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb); (void)hipFree(d_cs_q);
         d_tacc = nullptr; d_qt = nullptr; d_qlist = nullptr;
@@ -270,6 +273,8 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     const int stile = quarter ? full / 4 : full;
     s->sweep_block = quarter ? 64 : 256;
     std::vector<int> lds, qinst, cinst;
+    std::vector<int64_t> mbase;   // shaped lattices: first measured site of each instance in the list
+    int64_t msite_total = 0;
     int64_t lds_work = 0;
     const bool quad_lists = c->masked && !c->ising;
     const bool compact = quad_lists && !c->periodic && c->path == NMC_PATH_AUTO;
@@ -301,10 +306,47 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         }
         for (int64_t e = 0; e < d.Sp; e += 256) ini.push_back(Tile{i, (int)e});
         const int rows = d.a0 * d.a1;
-        int per = std::max(4, (int)std::min<int64_t>(rows, (kMeasureTile + d.half - 1) / std::max(1, d.half)));
         tb.push_back((int)mt.size());
-        for (int r = 0; r < rows; r += per) mt.push_back(MTile{i, r, std::min(per, rows - r)});
+        if (c->masked) {
+            // measured from the list of present sites: tiles of one colour
+            const uint32_t* mw = c->h_mask.data() + d.mask_base;
+            mbase.push_back(msite_total);
+            for (int col = 0; col < 2; ++col) {
+                int64_t np = 0;
+                for (int64_t w = 0; w < d.mask_stride; ++w) np += __builtin_popcount(mw[col * d.mask_stride + w]);
+                for (int64_t q = 0; q < np; q += kMeasureSites)
+                    mt.push_back(MTile{i, (int)(msite_total + q), (int)std::min<int64_t>(kMeasureSites, np - q)});
+                msite_total += np;
+                if (msite_total > (int64_t)INT32_MAX) return fail(NMC_E_UNSUPPORTED, "too many sites in one set");
+            }
+        } else {
+            int per = std::max(4, (int)std::min<int64_t>(rows, (kMeasureTile + d.half - 1) / std::max(1, d.half)));
+            for (int r = 0; r < rows; r += per) mt.push_back(MTile{i, r, std::min(per, rows - r)});
+        }
         te.push_back((int)mt.size());
+    }
+    // shaped lattices: the present sites, colour 0 then colour 1, in element
+    // order
+    std::vector<uint32_t> msites((size_t)msite_total);
+    if (c->masked) {
+        auto mfill = [&](size_t lb, size_t le) {
+            for (size_t li = lb; li < le; ++li) {
+                const DevInst& d = c->din[list[li]];
+                const uint32_t* mw = c->h_mask.data() + d.mask_base;
+                uint32_t* out = msites.data() + mbase[li];
+                for (int col = 0; col < 2; ++col)
+                    for (int64_t e = 0; e < d.S; ++e)
+                        if ((mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 1u)
+                            *out++ = (uint32_t)e | ((uint32_t)col << 31);
+            }
+        };
+        const size_t nl = list.size();
+        const size_t nt = std::max<size_t>(1, std::min<size_t>(16, std::min<size_t>(
+                                                                       std::thread::hardware_concurrency(), nl / 64)));
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(mfill, nl * t / nt, nl * (t + 1) / nt);
+        mfill(0, nl / nt);
+        for (auto& th : pool) th.join();
     }
     // shaped Heisenberg lattices: per colour, the quads with a present site,
     // 256 to a tile (a2 is a multiple of 8 there, so S is a multiple of 4)
@@ -437,6 +479,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});
     if ((rc = upload(c, &s->d_init, ini))) return rc;
     if ((rc = upload(c, &s->d_mt, mt))) return rc;
+    if ((rc = upload(c, &s->d_msites, msites))) return rc;
     if ((rc = upload(c, &s->d_tb, tb))) return rc;
     if ((rc = upload(c, &s->d_te, te))) return rc;
     if ((rc = upload(c, &s->d_lds, lds))) return rc;
@@ -523,12 +566,22 @@ int pstride_of(const nmc_ctx* c) {
     return c->dim3 ? 15 + 2 * c->a2max : 17;
 }
 
-int run_measure(nmc_ctx* c, Set* s) {
+// The measurement launches of a set (no allocation: the partials buffer must
+// already hold n_mt tiles).
+int measure_launches(nmc_ctx* c, Set* s) {
     const int ps = pstride_of(c);
-    if (int rc = ensure_partials(c, (size_t)s->n_mt * ps)) return rc;
-    NMC_HIP(launch_measure(c->stream, c->dim3, c->periodic, c->masked, c->ising, c->d_inst, s->d_mt,
-                           s->n_mt, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
+    if (c->masked)
+        NMC_HIP(launch_measure_sites(c->stream, c->dim3, c->periodic, c->ising, c->d_inst, s->d_mt, s->n_mt,
+                                     s->d_msites, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
+    else
+        NMC_HIP(launch_measure(c->stream, c->dim3, c->periodic, c->masked, c->ising, c->d_inst, s->d_mt,
+                               s->n_mt, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
     return NMC_OK;
+}
+
+int run_measure(nmc_ctx* c, Set* s) {
+    if (int rc = ensure_partials(c, (size_t)s->n_mt * pstride_of(c))) return rc;
+    return measure_launches(c, s);
 }
 
 }  // namespace
@@ -1045,8 +1098,7 @@ namespace {
 // Enqueue one measurement of a prepared set into series slot `slot`
 // (launches only; the partials buffer must already be large enough).
 int sample_launches(nmc_ctx* ctx, Set* s, int slot) {
-    NMC_HIP(launch_measure(ctx->stream, ctx->dim3, ctx->periodic, ctx->masked, ctx->ising, ctx->d_inst, s->d_mt,
-                           s->n_mt, ctx->pool, ctx->mpool, ctx->st, ctx->d_partials, pstride_of(ctx), ctx->a2max));
+    if (int rc = measure_launches(ctx, s)) return rc;
     NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, (int)s->list.size(), s->d_tb, s->d_te,
                                     ctx->d_partials, pstride_of(ctx), ctx->a2max, ctx->dim3, ctx->series_q, ctx->nsl,
                                     ctx->d_series, ctx->d_series_qr, ctx->d_series_qw, true, ctx->series_n, slot,

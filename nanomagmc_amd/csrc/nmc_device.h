@@ -99,6 +99,11 @@ struct MTile {
     int inst, row0, nrows;
 };
 
+// Shaped lattices are measured from a list of their present sites instead:
+// the same MTile then names kMeasureSites or fewer consecutive list entries
+// (row0 = first entry, nrows = count) of one colour of one instance.
+constexpr int kMeasureSites = 1024;
+
 // Sweep work item over a list of non-empty quads (shaped lattices): entries
 // q0 .. q0 + qn - 1 (qn <= 256) of the quad list, all of one instance and
 // one colour; each entry is the first colour element of a quad.

@@ -18,7 +18,8 @@ __device__ __forceinline__ bool mask_bit(const uint32_t* __restrict__ mp, const 
 }
 
 // ---------------------------------------------------------------------------
-// Observables.  One workgroup = 4 waves over a range of rows of one
+// Observables of full boxes (k_measure; shaped lattices use
+// k_measure_sites below).  One workgroup = 4 waves over a range of rows of one
 // instance.  A wave takes G rows at a time (G = 64 / half for rows shorter
 // than a wave, capped by the LDS the launch provides), lane group g = lane /
 // half owning row g of the G and column k = lane % half (long rows: one row,
@@ -77,6 +78,89 @@ __device__ __forceinline__ double wave_sum(double v) {
 // C remainder semantics (truncating), as the reference's % on int.
 __device__ __forceinline__ int crem(int a, int b) { return a % b; }
 
+// One site's contributions to the observables: M (acc 0-2), subM(0) (3-5),
+// sub4M (6-8) and the two local-field sums s . (X +/- Y) (9-14), and for
+// Heisenberg spins the solid angles of its calc_TC triangles, without wrap
+// (t_ref) and with it (t_wrap).
+template <bool DIM3, bool PERIODIC, bool MASKED, bool ISING>
+__device__ __forceinline__ void measure_site(const DevInst& in, const void* __restrict__ pool,
+                                             const uint32_t* __restrict__ mpool, const Stencil& st, int c,
+                                             int64_t e, const SitePos& p, double* acc, double& t_ref,
+                                             double& t_wrap) {
+#pragma clang fp contract(off)
+    const int i0 = p.i0, i1 = p.i1, i2 = p.i2;
+    const int sx_slot = DIM3 ? 0 : 2;   // first stencil slot of the reference x axis
+    const int sy_slot = DIM3 ? 2 : 4;   // ... of the reference y axis
+    double s0, s1, s2;
+    load_spin<ISING>(pool, in, c, e, s0, s1, s2);
+    acc[0] += s0; acc[1] += s1; acc[2] += s2;
+    // reference coordinates of the site
+    const int rx = DIM3 ? i0 + in.o0 : i1 + in.o1;
+    const int ry = DIM3 ? i1 + in.o1 : i2 + in.o2;
+    const int rz = DIM3 ? i2 + in.o2 : 0;
+    if (((i0 + in.o0 + i1 + in.o1 + i2 + in.o2) % 2) == 0) {   // calc_subM(0)
+        acc[3] += s0; acc[4] += s1; acc[5] += s2;
+    }
+    {   // calc_sub4M
+        const int possum = rx + ry;
+        const int posdiff = -rx + ry + (in.edge - in.edge % 4);
+        const int kz = rz;
+        if ((crem(possum, 4) == 0 && crem(posdiff, 4) == 0 && crem(kz, 4) == 0) ||
+            (crem(possum, 4) == 2 && crem(posdiff, 4) == 2 && crem(kz, 4) == 2)) {
+            acc[6] += s0; acc[7] += s1; acc[8] += s2;
+        }
+    }
+    // local fields: X = sum J s_j, Y = sum D x s_j
+    double X0 = 0, X1 = 0, X2 = 0, Y0 = 0, Y1 = 0, Y2 = 0;
+    for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+        const int ne = nbr_elem<PERIODIC>(in, p, g);
+        if (ne < 0) continue;
+        double n0, n1, n2;
+        load_spin<ISING>(pool, in, c ^ 1, ne, n0, n1, n2);
+        X0 += st.Jd[g] * n0; X1 += st.Jd[g] * n1; X2 += st.Jd[g] * n2;
+        const double* D = st.Dd[g];
+        Y0 += D[1] * n2 - D[2] * n1;
+        Y1 += D[2] * n0 - D[0] * n2;
+        Y2 += D[0] * n1 - D[1] * n0;
+    }
+    acc[9] += s0 * (X0 + Y0); acc[10] += s1 * (X1 + Y1); acc[11] += s2 * (X2 + Y2);
+    acc[12] += s0 * (X0 - Y0); acc[13] += s1 * (X1 - Y1); acc[14] += s2 * (X2 - Y2);
+    if (!ISING) {
+        // calc_TC triangles (i, -x, +y) and (i, +x, -y); q_ref without
+        // periodic wrap (the reference's adj), q_wrap with it.
+        // Where both unwrapped links exist the wrapped triangle is the
+        // same triangle, so its angle is computed once.
+        for (int tri = 0; tri < 2; ++tri) {
+            const int ga = tri == 0 ? sx_slot : sx_slot + 1;
+            const int gb = tri == 0 ? sy_slot + 1 : sy_slot;
+            const int na = nbr_elem<false>(in, p, ga);
+            const int nb = nbr_elem<false>(in, p, gb);
+            const bool inner = na >= 0 && nb >= 0;
+            int wa = na, wb = nb;
+            if (PERIODIC && !inner) {
+                wa = nbr_elem<true>(in, p, ga);
+                wb = nbr_elem<true>(in, p, gb);
+            }
+            for (int wrap = 0; wrap < (PERIODIC && !inner ? 2 : 1); ++wrap) {
+                const int ua = wrap ? wa : na, ub = wrap ? wb : nb;
+                if (ua < 0 || ub < 0) continue;
+                if (MASKED && (!mask_bit(mpool, in, c ^ 1, ua) || !mask_bit(mpool, in, c ^ 1, ub)))
+                    continue;
+                double a0, a1, a2, b0, b1, b2;
+                load_spin<false>(pool, in, c ^ 1, ua, a0, a1, a2);
+                load_spin<false>(pool, in, c ^ 1, ub, b0, b1, b2);
+                const double ang = solid_angle_d(s0, s1, s2, a0, a1, a2, b0, b1, b2);
+                if (wrap) t_wrap += ang;
+                else {
+                    t_ref += ang;
+                    if (inner) t_wrap += ang;
+                }
+            }
+        }
+        if (!PERIODIC) t_wrap = t_ref;
+    }
+}
+
 template <bool DIM3, bool PERIODIC, bool MASKED, bool ISING>
 __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ insts,
                                                  const MTile* __restrict__ tiles,
@@ -108,9 +192,6 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
     for (int i = 0; i < 15; ++i) acc[i] = 0.0;
     double qr = 0.0, qw = 0.0;   // 2-D charges
 
-    const int sx_slot = DIM3 ? 0 : 2;   // first stencil slot of the reference x axis
-    const int sy_slot = DIM3 ? 2 : 4;   // ... of the reference y axis
-
     for (int rbase = tl.row0 + wv * G; rbase < tl.row0 + tl.nrows; rbase += 4 * G) {
         const int row = rbase + lrow;
         if (lrow >= G || row >= tl.row0 + tl.nrows) continue;   // lanes beyond G * half idle
@@ -125,74 +206,9 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
                 if (MASKED && !mask_bit(mpool, in, c, e)) continue;
                 SitePos p;
                 p.i0 = i0; p.i1 = i1; p.i2 = i2; p.k = k; p.q = q2; p.row = row;
-                double s0, s1, s2;
-                load_spin<ISING>(pool, in, c, e, s0, s1, s2);
-                acc[0] += s0; acc[1] += s1; acc[2] += s2;
-                // reference coordinates of the site
-                const int rx = DIM3 ? i0 + in.o0 : i1 + in.o1;
-                const int ry = DIM3 ? i1 + in.o1 : i2 + in.o2;
-                const int rz = DIM3 ? i2 + in.o2 : 0;
-                if (((i0 + in.o0 + i1 + in.o1 + i2 + in.o2) % 2) == 0) {   // calc_subM(0)
-                    acc[3] += s0; acc[4] += s1; acc[5] += s2;
-                }
-                {   // calc_sub4M
-                    const int possum = rx + ry;
-                    const int posdiff = -rx + ry + (in.edge - in.edge % 4);
-                    const int kz = rz;
-                    if ((crem(possum, 4) == 0 && crem(posdiff, 4) == 0 && crem(kz, 4) == 0) ||
-                        (crem(possum, 4) == 2 && crem(posdiff, 4) == 2 && crem(kz, 4) == 2)) {
-                        acc[6] += s0; acc[7] += s1; acc[8] += s2;
-                    }
-                }
-                // local fields: X = sum J s_j, Y = sum D x s_j
-                double X0 = 0, X1 = 0, X2 = 0, Y0 = 0, Y1 = 0, Y2 = 0;
-                for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-                    const int ne = nbr_elem<PERIODIC>(in, p, g);
-                    if (ne < 0) continue;
-                    double n0, n1, n2;
-                    load_spin<ISING>(pool, in, c ^ 1, ne, n0, n1, n2);
-                    X0 += st.Jd[g] * n0; X1 += st.Jd[g] * n1; X2 += st.Jd[g] * n2;
-                    const double* D = st.Dd[g];
-                    Y0 += D[1] * n2 - D[2] * n1;
-                    Y1 += D[2] * n0 - D[0] * n2;
-                    Y2 += D[0] * n1 - D[1] * n0;
-                }
-                acc[9] += s0 * (X0 + Y0); acc[10] += s1 * (X1 + Y1); acc[11] += s2 * (X2 + Y2);
-                acc[12] += s0 * (X0 - Y0); acc[13] += s1 * (X1 - Y1); acc[14] += s2 * (X2 - Y2);
+                double t_ref = 0.0, t_wrap = 0.0;
+                measure_site<DIM3, PERIODIC, MASKED, ISING>(in, pool, mpool, st, c, e, p, acc, t_ref, t_wrap);
                 if (TC) {
-                    // calc_TC triangles (i, -x, +y) and (i, +x, -y); q_ref without
-                    // periodic wrap (the reference's adj), q_wrap with it.
-                    // Where both unwrapped links exist the wrapped triangle is the
-                    // same triangle, so its angle is computed once.
-                    double t_ref = 0.0, t_wrap = 0.0;
-                    for (int tri = 0; tri < 2; ++tri) {
-                        const int ga = tri == 0 ? sx_slot : sx_slot + 1;
-                        const int gb = tri == 0 ? sy_slot + 1 : sy_slot;
-                        const int na = nbr_elem<false>(in, p, ga);
-                        const int nb = nbr_elem<false>(in, p, gb);
-                        const bool inner = na >= 0 && nb >= 0;
-                        int wa = na, wb = nb;
-                        if (PERIODIC && !inner) {
-                            wa = nbr_elem<true>(in, p, ga);
-                            wb = nbr_elem<true>(in, p, gb);
-                        }
-                        for (int wrap = 0; wrap < (PERIODIC && !inner ? 2 : 1); ++wrap) {
-                            const int ua = wrap ? wa : na, ub = wrap ? wb : nb;
-                            if (ua < 0 || ub < 0) continue;
-                            if (MASKED && (!mask_bit(mpool, in, c ^ 1, ua) || !mask_bit(mpool, in, c ^ 1, ub)))
-                                continue;
-                            double a0, a1, a2, b0, b1, b2;
-                            load_spin<false>(pool, in, c ^ 1, ua, a0, a1, a2);
-                            load_spin<false>(pool, in, c ^ 1, ub, b0, b1, b2);
-                            const double ang = solid_angle_d(s0, s1, s2, a0, a1, a2, b0, b1, b2);
-                            if (wrap) t_wrap += ang;
-                            else {
-                                t_ref += ang;
-                                if (inner) t_wrap += ang;
-                            }
-                        }
-                    }
-                    if (!PERIODIC) t_wrap = t_ref;
                     if (TCS) {
                         qbank[i2] += t_ref;
                         qbank[a2max + i2] += t_wrap;
@@ -226,6 +242,96 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
             double v = 0.0;
             for (int b = 0; b < 4 * G; ++b) v += qs[(b * 2 + which) * a2max + z];
             out[15 + which * a2max + z] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Observables of shaped lattices from their lists of present sites (entries
+// e | colour << 31, in element order, so that a wavefront's loads stay
+// coalesced), so that no lane idles on the absent sites that fill most of a
+// grain's bounding box.  One workgroup takes up to kMeasureSites listed sites
+// of one colour of one instance (MTile {inst, first entry, count}) and writes
+// the same partials as k_measure.  3-D Heisenberg tiles keep each site's
+// solid angles and slice in LDS; each slice is then summed over the tile in
+// list order, a quarter of the tile per work-item.  Every sum runs in a fixed
+// order.
+// ---------------------------------------------------------------------------
+template <bool DIM3, bool PERIODIC, bool ISING>
+__global__ __launch_bounds__(256) void k_measure_sites(const DevInst* __restrict__ insts,
+                                                       const MTile* __restrict__ tiles,
+                                                       const uint32_t* __restrict__ sites,
+                                                       const void* __restrict__ pool,
+                                                       const uint32_t* __restrict__ mpool, Stencil st,
+                                                       double* __restrict__ partials, int pstride, int a2max) {
+#pragma clang fp contract(off)
+    constexpr bool TC = !ISING;
+    constexpr bool TCS = TC && DIM3;
+    constexpr int NV = TCS ? kMeasureSites : 1;
+    __shared__ double red[4 * 17];
+    __shared__ double vr[NV], vw[NV];
+    __shared__ int vz[NV];
+    extern __shared__ double qp[];   // [2][a2max][4] quarter sums per slice (3-D Heisenberg)
+    const MTile tl = tiles[blockIdx.x];   // row0 = first list entry, nrows = count
+    const DevInst& in = insts[tl.inst];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc[15];
+#pragma unroll
+    for (int i = 0; i < 15; ++i) acc[i] = 0.0;
+    double qr = 0.0, qw = 0.0;   // 2-D charges
+    for (int j = threadIdx.x; j < tl.nrows; j += 256) {
+        const uint32_t ent = sites[(int64_t)tl.row0 + j];
+        const int c = (int)(ent >> 31);
+        const uint32_t e = ent & 0x7FFFFFFFu;
+        const SitePos p = site_of(in, e, c, DIM3);
+        double t_ref = 0.0, t_wrap = 0.0;
+        measure_site<DIM3, PERIODIC, true, ISING>(in, pool, mpool, st, c, e, p, acc, t_ref, t_wrap);
+        if (TCS) {
+            vr[j] = t_ref;
+            vw[j] = t_wrap;
+            vz[j] = p.i2;
+        } else if (TC) {
+            qr += t_ref;
+            qw += t_wrap;
+        }
+    }
+    const int nsc = 15 + ((TC && !DIM3) ? 2 : 0);
+    if (TC && !DIM3) { qr = wave_sum(qr); qw = wave_sum(qw); }
+#pragma unroll
+    for (int i = 0; i < 15; ++i) acc[i] = wave_sum(acc[i]);
+    if (lane == 0) {
+        for (int i = 0; i < 15; ++i) red[wv * 17 + i] = acc[i];
+        red[wv * 17 + 15] = qr;
+        red[wv * 17 + 16] = qw;
+    }
+    __syncthreads();
+    double* out = partials + (int64_t)blockIdx.x * pstride;
+    if (threadIdx.x < nsc) {
+        const int i = threadIdx.x;
+        out[i] = ((red[i] + red[17 + i]) + red[34 + i]) + red[51 + i];
+    }
+    if (TCS) {
+        // slice z: four work-items scan a quarter of the tile each, then one
+        // adds the quarters in order
+        for (int i = threadIdx.x; i < 4 * a2max; i += 256) {
+            const int z = i >> 2, part = i & 3;
+            const int t0 = part * tl.nrows / 4, t1 = (part + 1) * tl.nrows / 4;
+            double sr = 0.0, sw = 0.0;
+            if (z < in.a2)
+                for (int t = t0; t < t1; ++t)
+                    if (vz[t] == z) {
+                        sr += vr[t];
+                        sw += vw[t];
+                    }
+            qp[i] = sr;
+            qp[4 * a2max + i] = sw;
+        }
+        __syncthreads();
+        for (int z = threadIdx.x; z < a2max; z += 256) {
+            const double* r = qp + 4 * z;
+            const double* w = qp + 4 * a2max + 4 * z;
+            out[15 + z] = ((r[0] + r[1]) + r[2]) + r[3];
+            out[15 + a2max + z] = ((w[0] + w[1]) + w[2]) + w[3];
         }
     }
 }
@@ -521,8 +627,9 @@ hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool mas
     if (dim3 == D3 && periodic == PER && masked == MSK && ising == IS)                       \
         k_measure<D3, PER, MSK, IS><<<ntiles, 256, shm, stream>>>(insts, t, pool, mpool, st, \
                                                                   partials, pstride, a2max, gmax);
-#define NMC_MEAS_B(D3, PER) \
-    NMC_MEAS(D3, PER, false, false) NMC_MEAS(D3, PER, true, false) NMC_MEAS(D3, PER, false, true) NMC_MEAS(D3, PER, true, true)
+    // shaped (masked) lattices are measured from site lists (k_measure_sites)
+    if (masked) return hipErrorInvalidValue;
+#define NMC_MEAS_B(D3, PER) NMC_MEAS(D3, PER, false, false) NMC_MEAS(D3, PER, false, true)
     NMC_MEAS_B(true, true)
     NMC_MEAS_B(true, false)
     NMC_MEAS_B(false, true)
@@ -531,6 +638,24 @@ hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool mas
 #undef NMC_MEAS
     return hipGetLastError();
 }
+
+hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, const DevInst* insts,
+                                const MTile* t, int ntiles, const uint32_t* sites, const void* pool,
+                                const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
+                                int a2max) {
+    if (ntiles <= 0) return hipSuccess;
+    const size_t shm = (!ising && dim3) ? 8 * (size_t)a2max * sizeof(double) : 0;
+#define NMC_MEAS_S(D3, PER, IS)                                                                           \
+    if (dim3 == D3 && periodic == PER && ising == IS)                                                     \
+        k_measure_sites<D3, PER, IS><<<ntiles, 256, shm, stream>>>(insts, t, sites, pool, mpool, st, partials, \
+                                                                   pstride, a2max);
+    NMC_MEAS_S(true, true, false) NMC_MEAS_S(true, true, true) NMC_MEAS_S(true, false, false)
+    NMC_MEAS_S(true, false, true) NMC_MEAS_S(false, true, false) NMC_MEAS_S(false, true, true)
+    NMC_MEAS_S(false, false, false) NMC_MEAS_S(false, false, true)
+#undef NMC_MEAS_S
+    return hipGetLastError();
+}
+
 
 hipError_t launch_measure_finalize(hipStream_t stream, const DevInst* insts, const int* list, int n,
                                    const int* t_begin, const int* t_end, const double* partials,

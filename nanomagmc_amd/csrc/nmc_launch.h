@@ -44,6 +44,13 @@ hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool mas
                           const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                           int a2max);
 
+// Shaped lattices: the same partials from lists of present sites
+// (e | colour << 31), tiles as described at kMeasureSites.
+hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, const DevInst* insts,
+                                const MTile* t, int ntiles, const uint32_t* sites, const void* pool,
+                                const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
+                                int a2max);
+
 hipError_t launch_measure_finalize(hipStream_t stream, const DevInst* insts, const int* list, int n,
                                    const int* t_begin, const int* t_end, const double* partials,
                                    int pstride, int a2max, bool dim3, bool tc, int nsl, double* obs,
