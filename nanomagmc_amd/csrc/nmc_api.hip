@@ -949,11 +949,14 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
     if (nsweeps < 0) return fail(NMC_E_INVALID, "negative sweep count");
     Set* s;
     if (int rc = get_set(ctx, inst, n, &s)) return rc;
-    // Small sets (quarter tiles): a long call is launch-bound, so whole
-    // blocks of kGraphSweeps sweeps replay one captured graph.  The graph's
-    // sweep offsets are relative and its last node advances the device sweep
-    // counters, so replays continue the same random streams.
-    if (s->sweep_block == 64 && nsweeps >= 2 * kGraphSweeps) {
+    // Small sets (quarter tiles) with streamed instances: a long call is
+    // launch-bound (two launches per sweep), so whole blocks of kGraphSweeps
+    // sweeps replay one captured graph.  The graph's sweep offsets are
+    // relative and its last node advances the device sweep counters, so
+    // replays continue the same random streams.  Sets swept only in LDS
+    // launch once per 256 sweeps anyway and skip the capture (~3 ms).
+    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    if (s->sweep_block == 64 && streamed && nsweeps >= 2 * kGraphSweeps) {
         auto key = std::make_tuple(s->list, -1, kGraphSweeps);
         auto it = ctx->graphs.find(key);
         if (it == ctx->graphs.end()) {
