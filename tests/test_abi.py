@@ -60,3 +60,32 @@ def test_device_index_math_on_host(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout
     assert "fdiv_bad=0 xcd_bad=0" in r.stdout
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/nmc.h compiles as C99 with warnings as errors, a C program
+    taking the address of every declared entry point links against
+    libnmc.so, and it runs (calling only nmc_last_error, which needs no
+    device)."""
+    import subprocess
+    from nanomagmc_amd import _lib
+    syms = header_symbols()
+    src = tmp_path / "use_nmc.c"
+    src.write_text(
+        '#include <stdio.h>\n#include "nmc.h"\n'
+        "typedef void (*fn)(void);\n"
+        "static fn table[] = {\n" + ",\n".join("    (fn)%s" % s for s in syms) + "\n};\n"
+        "int main(void) {\n"
+        "    const char* e = nmc_last_error();\n"
+        "    printf(\"%d %s\\n\", (int)(sizeof(table) / sizeof(table[0])), e ? \"ok\" : \"null\");\n"
+        "    return 0;\n}\n")
+    exe = str(tmp_path / "use_nmc")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-Wno-cast-function-type",
+                        "-I", os.path.join(ROOT, "include"), str(src), "-o", exe, "-L", libdir, "-lnmc",
+                        "-Wl,-rpath," + libdir, "-Wl,-rpath,/opt/rocm/lib"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split() == [str(len(syms)), "ok"]
