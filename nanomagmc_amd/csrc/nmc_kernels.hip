@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void k_measure_finalize(
 // ---------------------------------------------------------------------------
 template <bool ISING>
 __global__ __launch_bounds__(256) void k_init_spins(const DevInst* __restrict__ insts,
-                                                    const Tile* __restrict__ tiles, int ntiles,
+                                                    const Tile* __restrict__ tiles,
                                                     void* __restrict__ pool,
                                                     const uint32_t* __restrict__ mpool, int colour,
                                                     int random, uint64_t epoch, uint32_t seed_lo,
@@ -674,10 +674,10 @@ hipError_t launch_init_spins(hipStream_t stream, bool ising, const DevInst* inst
                              uint64_t epoch, uint32_t seed_lo, uint32_t seed_hi, bool dim3) {
     if (ntiles <= 0) return hipSuccess;
     if (ising)
-        k_init_spins<true><<<ntiles, 256, 0, stream>>>(insts, tiles, ntiles, pool, mpool, colour,
+        k_init_spins<true><<<ntiles, 256, 0, stream>>>(insts, tiles, pool, mpool, colour,
                                                        random ? 1 : 0, epoch, seed_lo, seed_hi, dim3 ? 1 : 0);
     else
-        k_init_spins<false><<<ntiles, 256, 0, stream>>>(insts, tiles, ntiles, pool, mpool, colour,
+        k_init_spins<false><<<ntiles, 256, 0, stream>>>(insts, tiles, pool, mpool, colour,
                                                         random ? 1 : 0, epoch, seed_lo, seed_hi, dim3 ? 1 : 0);
     return hipGetLastError();
 }
