@@ -256,6 +256,12 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         if (n < 0) return fail(NMC_E_INVALID, "negative instance count");
         list.assign(inst, inst + n);
         for (int i : list) if (int rc = check_inst(c, i)) return rc;
+        // an instance listed twice would be swept by two workgroups at once
+        std::vector<int> sorted(list);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(NMC_E_INVALID, "instance " + std::to_string(*std::adjacent_find(sorted.begin(), sorted.end())) +
+                                           " listed twice");
     }
     auto it = c->sets.find(list);
     if (it != c->sets.end()) { *out = it->second.get(); return NMC_OK; }

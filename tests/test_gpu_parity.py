@@ -524,6 +524,9 @@ def test_invalid_arguments_raise(orc):
     lat = Lattices(2, True, True, bonds, [{"n": (4, 4), "edge": 4}])
     with pytest.raises(NmcError):
         lat.set_params(-1.0, 0.0)          # T <= 0 (reference exit 104)
+    lat2 = Lattices(2, True, True, bonds, [{"n": (4, 4), "edge": 4}] * 2)
+    with pytest.raises(NmcError):
+        lat2.equil(1, inst=[1, 0, 1])      # one instance twice in a sweep list
     with pytest.raises(NmcError):
         Lattices(2, True, True, bonds, [{"n": (5, 5), "edge": 5}])   # odd periodic edge
     with pytest.raises(NmcError):
