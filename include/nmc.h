@@ -136,7 +136,10 @@ int nmc_randomize(nmc_ctx* ctx, const int* inst, int n);
 
 /* state::equil(nsweeps * N) (state.cpp:171-203) as `nsweeps` checkerboard
  * sweeps: every present site gets one single-site Metropolis attempt per
- * sweep.  inst == NULL means every instance. */
+ * sweep.  inst == NULL means every instance; a list may name an instance
+ * once (NMC_E_INVALID otherwise; the same holds for the lists of
+ * nmc_randomize, nmc_measure, nmc_sample, nmc_run_samples and
+ * nmc_accumulate_avgs). */
 int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps);
 /* Kernel path for nmc_sweep.  AUTO keeps every instance whose two colour
  * arrays fit in one workgroup's LDS resident there for whole sweeps (one
@@ -193,7 +196,7 @@ int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp
  * on the summed field (main.cpp:176-179). */
 int nmc_accumulate_avg(nmc_ctx* ctx, int inst);
 /* add_to_av for every listed instance in one launch (the same sums as n
- * calls of nmc_accumulate_avg). */
+ * calls of nmc_accumulate_avg); an instance may be listed once. */
 int nmc_accumulate_avgs(nmc_ctx* ctx, const int* inst, int n);
 int nmc_get_avg(nmc_ctx* ctx, int inst, double* out);
 int nmc_zero_avg(nmc_ctx* ctx, int inst);

@@ -1225,6 +1225,11 @@ int nmc_accumulate_avgs(nmc_ctx* ctx, const int* inst, int n) {
         double* a = nullptr;
         if (int rc = ensure_avg(ctx, i, &a)) return rc;
     }
+    // one launch adds every listed instance: each must appear once
+    std::vector<int> sorted(list);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return fail(NMC_E_INVALID, "nmc_accumulate_avgs: an instance is listed twice");
     auto it = ctx->avg_tiles.find(list);
     if (it == ctx->avg_tiles.end()) {
         std::vector<Tile> tiles;

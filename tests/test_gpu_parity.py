@@ -528,6 +528,8 @@ def test_invalid_arguments_raise(orc):
     with pytest.raises(NmcError):
         lat2.equil(1, inst=[1, 0, 1])      # one instance twice in a sweep list
     with pytest.raises(NmcError):
+        lat2.accumulate_avgs([0, 0])
+    with pytest.raises(NmcError):
         Lattices(2, True, True, bonds, [{"n": (5, 5), "edge": 5}])   # odd periodic edge
     with pytest.raises(NmcError):
         Lattices(2, True, False, [Bond((2, 0), 1.0)], [{"n": (4, 4), "edge": 4}])
