@@ -138,8 +138,8 @@ int nmc_randomize(nmc_ctx* ctx, const int* inst, int n);
  * sweeps: every present site gets one single-site Metropolis attempt per
  * sweep.  inst == NULL means every instance; a list may name an instance
  * once (NMC_E_INVALID otherwise; the same holds for the lists of
- * nmc_randomize, nmc_measure, nmc_sample, nmc_run_samples and
- * nmc_accumulate_avgs). */
+ * nmc_randomize, nmc_measure, nmc_sample, nmc_run_samples,
+ * nmc_accumulate_avgs, nmc_block_moves and nmc_block_swaps). */
 int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps);
 /* Kernel path for nmc_sweep.  AUTO keeps every instance whose two colour
  * arrays fit in one workgroup's LDS resident there for whole sweeps (one

@@ -1794,6 +1794,12 @@ int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, i
                 int relax_iters, float core_radius, int64_t* accepted, float* stats, float* proposal) {
     if (!ctx || !net || !inst || n <= 0) return fail(NMC_E_INVALID, "bad argument");
     if (off_x < 0 || off_x >= 32 || off_y < 0 || off_y >= 32) return fail(NMC_E_INVALID, "block offset outside [0, 32)");
+    {
+        std::vector<int> sorted(inst, inst + n);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(NMC_E_INVALID, "an instance is listed twice");
+    }
     int total = 0;
     std::vector<int> first(n + 1, 0);
     for (int j = 0; j < n; ++j) {
