@@ -198,7 +198,7 @@ def test_replay_boundary_uniforms(orc):
 
 # ------------------------------------ whole checkerboard sweeps (Ising) ------
 @pytest.mark.parametrize("dim,edge,periodic,masked", [(2, 16, 1, False), (2, 11, 0, True), (3, 8, 1, False),
-                                                      (3, 7, 0, True)])
+                                                      (3, 7, 0, True), (2, 12, 1, True), (3, 6, 1, True)])
 def test_checkerboard_trajectory_identical(orc, bloch3d, dim, edge, periodic, masked):
     rng = np.random.default_rng(edge + 17 * dim)
     bd, J, D = (orc.parse_bonds(bloch3d, 3, 1.0, 0.0) if dim == 3 else nn2d())
@@ -231,7 +231,8 @@ def test_checkerboard_trajectory_identical(orc, bloch3d, dim, edge, periodic, ma
 @pytest.mark.parametrize("ising,dim,periodic,masked,trial,general", [
     (True, 2, 1, False, "sphere", False), (True, 3, 0, True, "sphere", False),
     (False, 2, 1, False, "sphere", False), (False, 3, 0, True, "cone", False), (False, 3, 1, False, "cone", False),
-    (False, 2, 0, True, "sphere", False), (False, 3, 0, True, "sphere", True), (False, 2, 1, False, "cone", True)])
+    (False, 2, 0, True, "sphere", False), (False, 3, 0, True, "sphere", True), (False, 2, 1, False, "cone", True),
+    (True, 2, 1, True, "sphere", False), (False, 3, 1, True, "cone", False), (False, 2, 1, True, "sphere", True)])
 def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, periodic, masked, trial, general):
     # the LDS-resident launch (whole sweeps in one workgroup) and the
     # streaming launches (one per colour pass) draw the same numbers; with
