@@ -58,6 +58,9 @@ CASES = [
     (2, 13, 0, False, True, True),
     (3, 8, 1, False, True, False),
     (3, 9, 0, False, True, True),
+    (2, 16, 1, True, False, True),
+    (2, 14, 1, False, True, True),
+    (3, 8, 1, False, True, True),
 ]
 
 
