@@ -259,8 +259,10 @@ int nmc_net_loss_grad(nmc_net* net, const float* patches, const float* targets, 
 /* One round of block moves on each listed instance: blocks at
  * (32 i + off_x, 32 j + off_y) (i, j over the 32 x 32 cells, so every block
  * is 16 sites from the next one and the moves are independent); accepted[k]
- * counts the accepted blocks of instance inst[k].  Test mode (both
- * optional, blocks ordered by instance, then i, then j): stats gets
+ * counts the accepted blocks of instance inst[k].  With accepted, stats and
+ * proposal all NULL the round is only enqueued (no synchronisation); the
+ * accepted blocks also add to per-instance totals (nmc_get_block_accepts).
+ * Test mode (both optional, blocks ordered by instance, then i, then j): stats gets
  * (dE_s, log q(x'), log q(x)) per block, proposal the 16 x 16 x 3 proposed
  * spins (row = x offset, column = y offset). */
 int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y,
@@ -279,6 +281,10 @@ int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_
 int nmc_block_swaps(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y,
                     int relax_iters, float core_radius, int64_t* accepted, float* stats,
                     float* proposal);
+/* Accepted blocks of the listed instances over all block rounds since the
+ * context was created or nmc_reset_accepts was called.  No counterpart in the
+ * reference. */
+int nmc_get_block_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out);
 
 /* ---- lattice builder (state::init_lattice + shape::weibull::check) ---- */
 

@@ -28,7 +28,7 @@ EXPORTS = [
     "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_accumulate_avgs", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
     "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_net_create", "nmc_net_destroy",
     "nmc_net_num_params", "nmc_net_get_params", "nmc_net_set_params", "nmc_net_collect",
-    "nmc_net_num_samples", "nmc_net_train", "nmc_net_loss_grad", "nmc_block_moves", "nmc_block_swaps", "nmc_build_weibull",
+    "nmc_net_num_samples", "nmc_net_train", "nmc_net_loss_grad", "nmc_block_moves", "nmc_block_swaps", "nmc_get_block_accepts", "nmc_build_weibull",
 ]
 
 
@@ -106,6 +106,7 @@ def lib():
         "nmc_net_loss_grad": (i, [vp, P(C.c_float), P(C.c_float), i, P(C.c_float), P(C.c_float)]),
         "nmc_block_moves": (i, [vp, vp, P(i), i, i, i, P(C.c_int64), P(C.c_float), P(C.c_float)]),
         "nmc_block_swaps": (i, [vp, vp, P(i), i, i, i, i, C.c_float, P(C.c_int64), P(C.c_float), P(C.c_float)]),
+        "nmc_get_block_accepts": (i, [vp, P(i), i, P(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -138,6 +138,7 @@ struct nmc_ctx {
     std::vector<double> h_dtab;
     uint64_t* d_sweeps = nullptr;
     unsigned long long* d_acc = nullptr;
+    unsigned long long* d_bacc = nullptr;   // accepted learned block moves per instance
     std::vector<uint64_t> acc_folded;   // counts of released work lists
     double* d_partials = nullptr;
     size_t partials_cap = 0;
@@ -182,7 +183,7 @@ struct nmc_ctx {
         for (auto& kv : avg_tiles) (void)hipFree(kv.second.first);
         (void)hipFree(d_avgptr);
         (void)hipFree(d_inst); (void)hipFree(pool); (void)hipFree(mpool); (void)hipFree(d_tab);
-        (void)hipFree(d_dtab); (void)hipFree(d_sweeps); (void)hipFree(d_acc);
+        (void)hipFree(d_dtab); (void)hipFree(d_sweeps); (void)hipFree(d_acc); (void)hipFree(d_bacc);
         (void)hipFree(d_partials); (void)hipFree(d_obs); (void)hipFree(d_series);
         (void)hipFree(d_series_qr); (void)hipFree(d_series_qw);
         if (aux) { (void)hipStreamSynchronize(aux); (void)hipStreamDestroy(aux); }
@@ -753,8 +754,10 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
     }
     if ((rc = dalloc(&c->d_sweeps, c->din.size()))) return rc;
     if ((rc = dalloc(&c->d_acc, c->din.size()))) return rc;
+    if ((rc = dalloc(&c->d_bacc, c->din.size()))) return rc;
     NMC_HIP(hipMemsetAsync(c->d_sweeps, 0, c->din.size() * sizeof(uint64_t), c->stream));
     NMC_HIP(hipMemsetAsync(c->d_acc, 0, c->din.size() * sizeof(unsigned long long), c->stream));
+    NMC_HIP(hipMemsetAsync(c->d_bacc, 0, c->din.size() * sizeof(unsigned long long), c->stream));
     if ((rc = upload_insts(c.get()))) return rc;
     // default up spins (set_default_spins) with beta = 1, h = 0
     std::vector<double> b1(ninst, 1.0), h0(ninst, 0.0);
@@ -1029,6 +1032,7 @@ int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out) {
 int nmc_reset_accepts(nmc_ctx* ctx) {
     if (!ctx) return fail(NMC_E_INVALID, "null context");
     NMC_HIP(hipMemsetAsync(ctx->d_acc, 0, ctx->din.size() * sizeof(unsigned long long), ctx->stream));
+    NMC_HIP(hipMemsetAsync(ctx->d_bacc, 0, ctx->din.size() * sizeof(unsigned long long), ctx->stream));
     std::fill(ctx->acc_folded.begin(), ctx->acc_folded.end(), 0);
     for (auto& kv : ctx->sets)
         if (!kv.second->h_sweep.empty())
@@ -1458,6 +1462,7 @@ struct nmc_net {
     float* data_t = nullptr;
     int64_t ndata = 0, data_cap = 0;
     hipStream_t stream = nullptr;
+    hipEvent_t ready = nullptr;   // recorded on `stream` before a block round reads the network
     uint64_t seed = 0;
 
     int K(int l) const { return 9 * cin[l]; }
@@ -1473,6 +1478,7 @@ struct nmc_net {
         (void)hipFree(d_o); (void)hipFree(dy3); (void)hipFree(dy2); (void)hipFree(dy1); (void)hipFree(tgt);
         (void)hipFree(loss); (void)hipFree(pw); (void)hipFree(bxy); (void)hipFree(acc); (void)hipFree(idx);
         (void)hipFree(data_x); (void)hipFree(data_t);
+        if (ready) (void)hipEventDestroy(ready);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -1603,6 +1609,7 @@ int nmc_net_create(uint64_t seed, nmc_net** out) {
     auto net = std::make_unique<nmc_net>();
     net->seed = seed;
     NMC_HIP(hipStreamCreate(&net->stream));
+    NMC_HIP(hipEventCreateWithFlags(&net->ready, hipEventDisableTiming));
     std::mt19937_64 rng(seed ^ 0x6E65746Bull);
     for (int l = 0; l < 4; ++l) {
         const size_t KN = (size_t)net->K(l) * 32;
@@ -1811,21 +1818,12 @@ int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, i
     }
     first[n] = total;
     if (int rc = net_reserve(net, total)) return rc;
-    std::vector<int> bxy(2 * (size_t)total);
+    // the network's training work (its own stream) before the round reads it
+    NMC_HIP(hipEventRecord(net->ready, net->stream));
+    NMC_HIP(hipStreamWaitEvent(ctx->stream, net->ready, 0));
     for (int j = 0; j < n; ++j) {
         const DevInst& d = ctx->din[inst[j]];
-        int b = first[j];
-        for (int cx = 0; cx < d.a1 / 32; ++cx)
-            for (int cy = 0; cy < d.a2 / 32; ++cy, ++b) {
-                bxy[2 * b] = cx * 32 + off_x;
-                bxy[2 * b + 1] = cy * 32 + off_y;
-            }
-    }
-    // the network reads the lattice on the context's stream; one stream for both
-    NMC_HIP(hipStreamSynchronize(net->stream));
-    NMC_HIP(hipMemcpyAsync(net->bxy, bxy.data(), bxy.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    for (int j = 0; j < n; ++j) {
-        const DevInst& d = ctx->din[inst[j]];
+        NMC_HIP(launch_block_xy(ctx->stream, net->bxy + 2 * first[j], d.a1 / 32, d.a2 / 32, off_x, off_y));
         NMC_HIP(launch_gather_patches(ctx->stream, (const float*)ctx->pool, d.base, d.Sp, d.a1, d.a2,
                                       net->bxy + 2 * first[j], first[j + 1] - first[j],
                                       net->x0 + (size_t)first[j] * kPatchElems, nullptr));
@@ -1854,7 +1852,16 @@ int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, i
             e = launch_block_mh(ctx->stream, (float*)ctx->pool, d.base, d.Sp, d.a1, d.a2, net->bxy + 2 * first[j],
                                 first[j + 1] - first[j], net->o + (size_t)first[j] * 256 * 32, ctx->st, d.beta, d.h,
                                 slo, shi, d.inst_id, round, net->acc + first[j], sj, pj);
+        if (e == hipSuccess)
+            e = launch_block_acc_add(ctx->stream, net->acc + first[j], first[j + 1] - first[j],
+                                     ctx->d_bacc + inst[j]);
     }
+    // later work on the network's stream (training reuses its buffers) waits
+    // for this round
+    if (e == hipSuccess) e = hipEventRecord(net->ready, ctx->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(net->stream, net->ready, 0);
+    // nothing to return: the round stays queued on the context's stream
+    if (e == hipSuccess && !accepted && !dstats && !dprop) return NMC_OK;
     std::vector<int> acc(total);
     if (e == hipSuccess)
         e = hipMemcpyAsync(acc.data(), net->acc, total * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
@@ -1886,6 +1893,19 @@ int nmc_block_swaps(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_
     if (relax_iters < 0 || relax_iters > 100000) return fail(NMC_E_INVALID, "relax_iters outside [0, 100000]");
     if (!(core_radius > 0.f && core_radius <= 8.f)) return fail(NMC_E_INVALID, "core_radius outside (0, 8]");
     return block_round(ctx, net, inst, n, off_x, off_y, true, relax_iters, core_radius, accepted, stats, proposal);
+}
+
+int nmc_get_block_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out) {
+    if (!ctx || !out || n < 0 || (n > 0 && !inst)) return fail(NMC_E_INVALID, "bad argument");
+    std::vector<uint64_t> all(ctx->din.size());
+    NMC_HIP(hipMemcpyAsync(all.data(), ctx->d_bacc, all.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    for (int j = 0; j < n; ++j) {
+        if (int rc = check_inst(ctx, inst[j])) return rc;
+        out[j] = all[inst[j]];
+    }
+    return NMC_OK;
 }
 
 }  // extern "C"

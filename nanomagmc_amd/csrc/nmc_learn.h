@@ -27,6 +27,10 @@ hipError_t launch_adam(hipStream_t s, float* w, float* mw, float* vw, const floa
                        const float* gb, __bf16* W16, __bf16* Wd16);
 hipError_t launch_gather_patches(hipStream_t s, const float* pool, int64_t base, int64_t Sp, int nx, int ny,
                                  const int* bxy, int npatch, __bf16* X, float* target);
+// Block corners of a round (nbx x nby blocks of 32 x 32 sites, offset
+// (off_x, off_y)) and the running per-instance count of accepted blocks.
+hipError_t launch_block_xy(hipStream_t s, int* bxy, int nbx, int nby, int off_x, int off_y);
+hipError_t launch_block_acc_add(hipStream_t s, const int* acc, int nblocks, unsigned long long* total);
 hipError_t launch_head_loss(hipStream_t s, const float* O, const float* target, int npatch, float scale, __bf16* dO,
                             float* loss);
 hipError_t launch_block_mh(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,

@@ -787,6 +787,36 @@ hipError_t launch_adam(hipStream_t s, float* w, float* mw, float* vw, const floa
     return hipGetLastError();
 }
 
+// Corners of one round's blocks on one instance: block b = cx * nby + cy
+// sits at (32 cx + off_x, 32 cy + off_y), the order the host enumerates.
+__global__ void k_block_xy(int* __restrict__ bxy, int nbx, int nby, int off_x, int off_y) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbx * nby) return;
+    const int cx = b / nby, cy = b - cx * nby;
+    bxy[2 * b] = 32 * cx + off_x;
+    bxy[2 * b + 1] = 32 * cy + off_y;
+}
+
+// A round's accepted blocks of one instance added to its running count (one
+// work-item, in block order: the only writer of *total in the launch).
+__global__ void k_block_acc_add(const int* __restrict__ acc, int nblocks, unsigned long long* __restrict__ total) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    unsigned long long t = 0;
+    for (int b = 0; b < nblocks; ++b) t += (unsigned long long)acc[b];
+    *total += t;
+}
+
+hipError_t launch_block_xy(hipStream_t s, int* bxy, int nbx, int nby, int off_x, int off_y) {
+    if (nbx * nby <= 0) return hipSuccess;
+    k_block_xy<<<(nbx * nby + 255) / 256, 256, 0, s>>>(bxy, nbx, nby, off_x, off_y);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_acc_add(hipStream_t s, const int* acc, int nblocks, unsigned long long* total) {
+    k_block_acc_add<<<1, 64, 0, s>>>(acc, nblocks, total);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather_patches(hipStream_t s, const float* pool, int64_t base, int64_t Sp, int nx, int ny,
                                  const int* bxy, int npatch, __bf16* X, float* target) {
     if (npatch <= 0) return hipSuccess;

@@ -68,30 +68,33 @@ class Net:
         return loss.value, g
 
 
-def block_moves(lat, net, inst, off_x, off_y, diagnostics=False):
+def block_moves(lat, net, inst, off_x, off_y, diagnostics=False, counts=True):
     """One round of learned block moves on each listed instance; returns the
-    accepted-block counts (and per-block stats / proposals with diagnostics)."""
+    accepted-block counts (and per-block stats / proposals with diagnostics).
+    With counts=False (and no diagnostics) the round is only enqueued and
+    None is returned; block_accepts() reads the running totals."""
     a = np.ascontiguousarray(np.asarray(inst, dtype=np.int32).reshape(-1))
-    acc = np.zeros(a.size, dtype=np.int64)
+    acc = np.zeros(a.size, dtype=np.int64) if (counts or diagnostics) else None
     stats = prop = None
     if diagnostics:
         nb = sum((lat_edge(lat, i) // 32) ** 2 for i in a)
         stats = np.zeros((nb, 3), dtype=np.float32)
         prop = np.zeros((nb, 16, 16, 3), dtype=np.float32)
     check(lib().nmc_block_moves(lat._h, net._h, a.ctypes.data_as(C.POINTER(C.c_int)), int(a.size), int(off_x),
-                                int(off_y), acc.ctypes.data_as(C.POINTER(C.c_int64)),
+                                int(off_y), acc.ctypes.data_as(C.POINTER(C.c_int64)) if acc is not None else None,
                                 _fptr(stats) if stats is not None else None,
                                 _fptr(prop) if prop is not None else None))
     return (acc, stats, prop) if diagnostics else acc
 
 
-def block_swaps(lat, net, inst, off_x, off_y, relax_iters=80, core_radius=5.0, diagnostics=False):
+def block_swaps(lat, net, inst, off_x, off_y, relax_iters=80, core_radius=5.0, diagnostics=False, counts=True):
     """One round of learned texture swaps (create or remove a skyrmion per
     block by an exact measure-preserving involution); returns the
     accepted-block counts (and per-block (dE, |A - B|^2, 0) / rotated blocks
-    with diagnostics)."""
+    with diagnostics), or None with counts=False (the round is only
+    enqueued; block_accepts() reads the running totals)."""
     a = np.ascontiguousarray(np.asarray(inst, dtype=np.int32).reshape(-1))
-    acc = np.zeros(a.size, dtype=np.int64)
+    acc = np.zeros(a.size, dtype=np.int64) if (counts or diagnostics) else None
     stats = prop = None
     if diagnostics:
         nb = sum((lat_edge(lat, i) // 32) ** 2 for i in a)
@@ -99,10 +102,20 @@ def block_swaps(lat, net, inst, off_x, off_y, relax_iters=80, core_radius=5.0, d
         prop = np.zeros((nb, 16, 16, 3), dtype=np.float32)
     check(lib().nmc_block_swaps(lat._h, net._h, a.ctypes.data_as(C.POINTER(C.c_int)), int(a.size), int(off_x),
                                 int(off_y), int(relax_iters), float(core_radius),
-                                acc.ctypes.data_as(C.POINTER(C.c_int64)),
+                                acc.ctypes.data_as(C.POINTER(C.c_int64)) if acc is not None else None,
                                 _fptr(stats) if stats is not None else None,
                                 _fptr(prop) if prop is not None else None))
     return (acc, stats, prop) if diagnostics else acc
+
+
+def block_accepts(lat, inst):
+    """Accepted learned blocks per listed instance since the context was made
+    or its accept counters were reset."""
+    a = np.ascontiguousarray(np.asarray(inst, dtype=np.int32).reshape(-1))
+    out = np.zeros(a.size, dtype=np.uint64)
+    check(lib().nmc_get_block_accepts(lat._h, a.ctypes.data_as(C.POINTER(C.c_int)), int(a.size),
+                                      out.ctypes.data_as(C.POINTER(C.c_uint64))))
+    return out
 
 
 def lat_edge(lat, i):
@@ -183,15 +196,13 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
     say("trained %d networks in %.1f s: %s" % (nT, train_s, losses))
 
     def block_round(nm):
-        acc = np.zeros(nT, dtype=np.int64)
+        # enqueued only: acceptance is read from the running totals at the end
         for t in range(nT):
             ox, oy = (int(x) for x in rng.integers(0, 32, 2))
             if nm == "vmf":
-                a = block_moves(lat, nets[t], group[nm][t], ox, oy)
+                block_moves(lat, nets[t], group[nm][t], ox, oy, counts=False)
             else:
-                a = block_swaps(lat, nets[t], group[nm][t], ox, oy, relax_iters, core_radius)
-            acc[t] = int(np.sum(a))
-        return acc
+                block_swaps(lat, nets[t], group[nm][t], ox, oy, relax_iters, core_radius, counts=False)
 
     # cost of one sample of each method, each on its own instances
     cost = {}
@@ -207,14 +218,15 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
         cost[nm] = (time.perf_counter() - t0) / timing_samples
     # production run
     lat.series_reset(nsamples, with_charge=True)
-    nacc = {nm: np.zeros(nT, dtype=np.int64) for nm in methods}
+    lat.reset_accepts()
     t0 = time.perf_counter()
     for k in range(nsamples):
         lat.equil(sweeps_per_round)
         for nm in methods:
-            nacc[nm] += block_round(nm)
+            block_round(nm)
         lat.sample(k)
     lat.sync()
+    nacc = {nm: np.array([int(np.sum(block_accepts(lat, group[nm][t]))) for t in range(nT)]) for nm in methods}
     say("sampled %d x %d sweeps in %.1f s" % (nsamples, sweeps_per_round, time.perf_counter() - t0))
     obs, _, qw = lat.get_series()
     blocks_per_round = (L // 32) ** 2

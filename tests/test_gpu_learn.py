@@ -344,3 +344,38 @@ def test_block_swaps_keep_the_physics():
     e1, s1, _ = exact.mean_err(obs[1, 100:, 10])
     print("swap acceptance %.4f, <E> plain %.2f +- %.2f, with swaps %.2f +- %.2f" % (acc / (n * 4), e0, s0, e1, s1))
     assert abs(e0 - e1) < 4 * math.hypot(s0, s1), (e0, e1, s0, s1)
+
+
+@pytest.mark.parametrize("kind", ["vmf", "swap"])
+def test_enqueued_rounds_equal_counted_rounds(kind):
+    # rounds enqueued without outputs (no synchronisation) give the same
+    # lattice as rounds that return their counts, and the running totals
+    # equal the sum of the returned counts
+    from nanomagmc_amd.learn import block_accepts, block_moves, block_swaps
+    L, D, h, T = 64, 0.3, 0.06, 0.3
+    net = constant_net((0.0, 0.0, 1.0))
+    spins, totals, counted = [], [], []
+    for counts in (True, False):
+        lat = lattice(L, D, ninst=2, seed=31)
+        lat.set_params([1 / T, 2 / T], h)
+        lat.randomize()
+        lat.equil(20)
+        lat.reset_accepts()
+        rng = np.random.default_rng(8)
+        got = np.zeros(2, dtype=np.int64)
+        for _ in range(6):
+            ox, oy = (int(x) for x in rng.integers(0, 32, 2))
+            lat.equil(1)
+            if kind == "vmf":
+                a = block_moves(lat, net, [1, 0], ox, oy, counts=counts)
+            else:
+                a = block_swaps(lat, net, [1, 0], ox, oy, counts=counts)
+            if counts:
+                got += np.asarray(a)[::-1]   # listed as [1, 0]
+        spins.append([lat.get_spins(i) for i in range(2)])
+        totals.append(block_accepts(lat, [0, 1]).astype(np.int64))
+        counted.append(got)
+    for a, b in zip(*spins):
+        assert np.array_equal(a, b)
+    assert np.array_equal(totals[0], totals[1])
+    assert np.array_equal(totals[0], counted[0])
