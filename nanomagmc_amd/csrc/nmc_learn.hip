@@ -557,7 +557,7 @@ __global__ __launch_bounds__(512) void k_block_swap(float* __restrict__ pool, in
                                                     int* __restrict__ accepted, float* __restrict__ stats,
                                                     float* __restrict__ proposal) {
     // 512 work-items: items 0..255 relax texture A, 256..511 texture B, one
-    // block site each; items 0..255 then apply and judge the move.
+    // block site each; items 0..255 then apply the move and decide it.
     constexpr int RR = kRing * kRing;
     __shared__ float A[3 * RR], B[3 * RR];
     __shared__ float nx_s[256], ny_s[256], nz_s[256];
