@@ -75,6 +75,8 @@ struct Set {
     int n_init = 0;
     MTile* d_mt = nullptr;
     uint32_t* d_msites = nullptr;   // shaped lattices: present sites to measure (e | colour << 31)
+    uint16_t* d_mrank = nullptr;    // 3-D Heisenberg ones: each entry's rank in its tile sorted by slice ...
+    uint16_t* d_mzoff = nullptr;    // ... and a2max + 1 slice offsets into that order per tile
     int n_mt = 0;
     int* d_tb = nullptr;
     int* d_te = nullptr;
@@ -96,8 +98,8 @@ struct Set {
     void release() {
         (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
         (void)hipFree(d_mt); (void)hipFree(d_tb); (void)hipFree(d_te); (void)hipFree(d_lds);
-        (void)hipFree(d_msites);
-        d_msites = nullptr;
+        (void)hipFree(d_msites); (void)hipFree(d_mrank); (void)hipFree(d_mzoff);
+        d_msites = nullptr; d_mrank = nullptr; d_mzoff = nullptr;
         (void)hipFree(d_tacc); (void)hipFree(d_qt); (void)hipFree(d_qlist);
         (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb); (void)hipFree(d_cs_q);
         d_tacc = nullptr; d_qt = nullptr; d_qlist = nullptr;
@@ -118,7 +120,8 @@ struct nmc_ctx {
     nmc_config cfg{};
     bool dim3 = false, ising = false, periodic = false, masked = false, use_d = false, cone = false;
     int zslots = 4;          // stencil slots used (4 in 2-D, 6 in 3-D)
-    int dm = 0;              // DMI form of the sweep kernels (dmi_axis in nmc_sweep.hip)
+    int dm = 0;              // DMI form of the sweep kernels (dmi_axis in nmc_device.h), from the fp32 D
+    int dm_meas = 0;         // the same form from the fp64 D of the measurement
     int tabsize = 0;         // Ising acceptance-table entries per instance
     int nsl = 1;             // topological-charge slices
     int a2max = 2;
@@ -139,6 +142,7 @@ struct nmc_ctx {
     uint64_t* d_sweeps = nullptr;
     unsigned long long* d_acc = nullptr;
     unsigned long long* d_bacc = nullptr;   // accepted learned block moves per instance
+    bool bacc_dirty = false;                // d_bacc counted a block round since it was last cleared
     std::vector<uint64_t> acc_folded;   // counts of released work lists
     double* d_partials = nullptr;
     size_t partials_cap = 0;
@@ -315,17 +319,20 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         const int rows = d.a0 * d.a1;
         tb.push_back((int)mt.size());
         if (c->masked) {
-            // measured from the list of present sites: tiles of one colour
+            // measured from the list of present sites (both colours): the
+            // fewest tiles of at most kMeasureSites entries, of equal size
+            // give or take one, so that no workgroup idles on a short tail
             const uint32_t* mw = c->h_mask.data() + d.mask_base;
             mbase.push_back(msite_total);
-            for (int col = 0; col < 2; ++col) {
-                int64_t np = 0;
-                for (int64_t w = 0; w < d.mask_stride; ++w) np += __builtin_popcount(mw[col * d.mask_stride + w]);
-                for (int64_t q = 0; q < np; q += kMeasureSites)
-                    mt.push_back(MTile{i, (int)(msite_total + q), (int)std::min<int64_t>(kMeasureSites, np - q)});
-                msite_total += np;
-                if (msite_total > (int64_t)INT32_MAX) return fail(NMC_E_UNSUPPORTED, "too many sites in one set");
+            int64_t np = 0;
+            for (int64_t w = 0; w < 2 * d.mask_stride; ++w) np += __builtin_popcount(mw[w]);
+            const int64_t nt = (np + kMeasureSites - 1) / kMeasureSites;
+            for (int64_t t = 0; t < nt; ++t) {
+                const int64_t q0 = np * t / nt, q1 = np * (t + 1) / nt;
+                mt.push_back(MTile{i, (int)(msite_total + q0), (int)(q1 - q0)});
             }
+            msite_total += np;
+            if (msite_total > (int64_t)INT32_MAX) return fail(NMC_E_UNSUPPORTED, "too many sites in one set");
         } else {
             int per = std::max(4, (int)std::min<int64_t>(rows, (kMeasureTile + d.half - 1) / std::max(1, d.half)));
             for (int r = 0; r < rows; r += per) mt.push_back(MTile{i, r, std::min(per, rows - r)});
@@ -333,10 +340,16 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         te.push_back((int)mt.size());
     }
     // shaped lattices: the present sites, colour 0 then colour 1, in element
-    // order
+    // order; for 3-D Heisenberg also each tile's entries grouped by slice z
+    // (a stable counting sort, so each slice keeps list order)
     std::vector<uint32_t> msites((size_t)msite_total);
+    const bool mslices = c->masked && c->dim3 && !c->ising;
+    const int za = c->a2max + 1;
+    std::vector<uint16_t> mrank(mslices ? (size_t)msite_total : 0), mzoff(mslices ? mt.size() * za : 0);
     if (c->masked) {
         auto mfill = [&](size_t lb, size_t le) {
+            std::vector<int> zs(kMeasureSites);
+            std::vector<int> pos(za);
             for (size_t li = lb; li < le; ++li) {
                 const DevInst& d = c->din[list[li]];
                 const uint32_t* mw = c->h_mask.data() + d.mask_base;
@@ -345,6 +358,25 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                     for (int64_t e = 0; e < d.S; ++e)
                         if ((mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 1u)
                             *out++ = (uint32_t)e | ((uint32_t)col << 31);
+                if (!mslices) continue;
+                for (int t = tb[li]; t < te[li]; ++t) {
+                    const MTile& m = mt[t];
+                    uint16_t* zo = mzoff.data() + (size_t)t * za;
+                    std::fill(pos.begin(), pos.end(), 0);
+                    for (int u = 0; u < m.nrows; ++u) {
+                        const uint32_t ent = msites[(size_t)m.row0 + u];
+                        const int col = (int)(ent >> 31);
+                        const int64_t e = ent & 0x7FFFFFFFu;
+                        const int64_t row = e / d.half;
+                        const int k = (int)(e - row * d.half);
+                        const int i0 = (int)(row / d.a1), i1 = (int)(row - (int64_t)i0 * d.a1);
+                        zs[u] = 2 * k + ((col + i0 + i1) & 1);   // site_of's i2
+                        ++pos[zs[u] + 1];
+                    }
+                    for (int z = 0; z < za - 1; ++z) pos[z + 1] += pos[z];
+                    for (int z = 0; z < za; ++z) zo[z] = (uint16_t)pos[z];
+                    for (int u = 0; u < m.nrows; ++u) mrank[(size_t)m.row0 + u] = (uint16_t)pos[zs[u]]++;
+                }
             }
         };
         const size_t nl = list.size();
@@ -487,6 +519,8 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     if ((rc = upload(c, &s->d_init, ini))) return rc;
     if ((rc = upload(c, &s->d_mt, mt))) return rc;
     if ((rc = upload(c, &s->d_msites, msites))) return rc;
+    if ((rc = upload(c, &s->d_mrank, mrank))) return rc;
+    if ((rc = upload(c, &s->d_mzoff, mzoff))) return rc;
     if ((rc = upload(c, &s->d_tb, tb))) return rc;
     if ((rc = upload(c, &s->d_te, te))) return rc;
     if ((rc = upload(c, &s->d_lds, lds))) return rc;
@@ -574,21 +608,22 @@ int pstride_of(const nmc_ctx* c) {
 }
 
 // The measurement launches of a set (no allocation: the partials buffer must
-// already hold n_mt tiles).
-int measure_launches(nmc_ctx* c, Set* s) {
+// already hold n_mt tiles).  tc: compute the topological charges too.
+int measure_launches(nmc_ctx* c, Set* s, bool tc) {
     const int ps = pstride_of(c);
     if (c->masked)
-        NMC_HIP(launch_measure_sites(c->stream, c->dim3, c->periodic, c->ising, c->d_inst, s->d_mt, s->n_mt,
-                                     s->d_msites, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
+        NMC_HIP(launch_measure_sites(c->stream, c->dim3, c->periodic, c->ising, c->dm_meas, tc, c->d_inst, s->d_mt,
+                                     s->n_mt, s->d_msites, s->d_mrank, s->d_mzoff, c->pool, c->mpool, c->st,
+                                     c->d_partials, ps, c->a2max));
     else
-        NMC_HIP(launch_measure(c->stream, c->dim3, c->periodic, c->masked, c->ising, c->d_inst, s->d_mt,
-                               s->n_mt, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
+        NMC_HIP(launch_measure(c->stream, c->dim3, c->periodic, c->masked, c->ising, c->dm_meas, tc, c->d_inst,
+                               s->d_mt, s->n_mt, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
     return NMC_OK;
 }
 
-int run_measure(nmc_ctx* c, Set* s) {
+int run_measure(nmc_ctx* c, Set* s, bool tc) {
     if (int rc = ensure_partials(c, (size_t)s->n_mt * pstride_of(c))) return rc;
-    return measure_launches(c, s);
+    return measure_launches(c, s, tc);
 }
 
 }  // namespace
@@ -652,6 +687,15 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
             }
         }
         c->dm = !any ? 0 : (axial ? 2 : 1);
+        bool anyd = false, axiald = true;
+        for (int g = c->dim3 ? 0 : 2; g < 6; ++g) {
+            const int k = c->dim3 ? (g >> 1) : 2 - (g >> 1);
+            for (int a = 0; a < 3; ++a) {
+                anyd = anyd || c->st.Dd[g][a] != 0.0;
+                if (a != k && c->st.Dd[g][a] != 0.0) axiald = false;
+            }
+        }
+        c->dm_meas = !anyd ? 0 : (axiald ? 2 : 1);
     }
     for (int g = 0, p = 1; g < 6; ++g) {
         if (!c->dim3 && g < 2) { c->st.pow3[g] = 0; continue; }
@@ -1032,7 +1076,9 @@ int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out) {
 int nmc_reset_accepts(nmc_ctx* ctx) {
     if (!ctx) return fail(NMC_E_INVALID, "null context");
     NMC_HIP(hipMemsetAsync(ctx->d_acc, 0, ctx->din.size() * sizeof(unsigned long long), ctx->stream));
-    NMC_HIP(hipMemsetAsync(ctx->d_bacc, 0, ctx->din.size() * sizeof(unsigned long long), ctx->stream));
+    if (ctx->bacc_dirty)
+        NMC_HIP(hipMemsetAsync(ctx->d_bacc, 0, ctx->din.size() * sizeof(unsigned long long), ctx->stream));
+    ctx->bacc_dirty = false;
     std::fill(ctx->acc_folded.begin(), ctx->acc_folded.end(), 0);
     for (auto& kv : ctx->sets)
         if (!kv.second->h_sweep.empty())
@@ -1072,9 +1118,10 @@ int nmc_measure(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_ref
     }
     double* dq_r = ctx->d_obs + (size_t)cnt * 11;
     double* dq_w = dq_r + (size_t)cnt * ctx->nsl;
-    if (int rc = run_measure(ctx, s)) return rc;
+    const bool tc = !ctx->ising && (q_ref || q_wrap);
+    if (int rc = run_measure(ctx, s, tc)) return rc;
     NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, cnt, s->d_tb, s->d_te, ctx->d_partials,
-                                    pstride_of(ctx), ctx->a2max, ctx->dim3, !ctx->ising, ctx->nsl, ctx->d_obs,
+                                    pstride_of(ctx), ctx->a2max, ctx->dim3, tc, ctx->nsl, ctx->d_obs,
                                     dq_r, dq_w, false, 1, 0, ctx->ising ? 0 : 2));
     std::vector<double> h(need);
     NMC_HIP(hipMemcpyAsync(h.data(), ctx->d_obs, need * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1111,7 +1158,7 @@ namespace {
 // Enqueue one measurement of a prepared set into series slot `slot`
 // (launches only; the partials buffer must already be large enough).
 int sample_launches(nmc_ctx* ctx, Set* s, int slot) {
-    if (int rc = measure_launches(ctx, s)) return rc;
+    if (int rc = measure_launches(ctx, s, ctx->series_q)) return rc;
     NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, (int)s->list.size(), s->d_tb, s->d_te,
                                     ctx->d_partials, pstride_of(ctx), ctx->a2max, ctx->dim3, ctx->series_q, ctx->nsl,
                                     ctx->d_series, ctx->d_series_qr, ctx->d_series_qw, true, ctx->series_n, slot,
@@ -1856,6 +1903,7 @@ int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, i
             e = launch_block_acc_add(ctx->stream, net->acc + first[j], first[j + 1] - first[j],
                                      ctx->d_bacc + inst[j]);
     }
+    ctx->bacc_dirty = true;
     // later work on the network's stream (training reuses its buffers) waits
     // for this round
     if (e == hipSuccess) e = hipEventRecord(net->ready, ctx->stream);

@@ -101,7 +101,8 @@ struct MTile {
 
 // Shaped lattices are measured from a list of their present sites instead:
 // the same MTile then names kMeasureSites or fewer consecutive list entries
-// (row0 = first entry, nrows = count) of one colour of one instance.
+// (row0 = first entry, nrows = count) of one instance; each entry carries its
+// colour.
 constexpr int kMeasureSites = 1024;
 
 // Sweep work item over a list of non-empty quads (shaped lattices): entries
@@ -189,6 +190,14 @@ __device__ __forceinline__ SitePos site_of(const DevInst& in, uint32_t e, int c,
     p.q = (c + p.i0 + p.i1) & 1;
     p.i2 = 2 * p.k + p.q;
     return p;
+}
+
+// The one spin axis along which the axial DMI form carries D for bond slot g:
+// 3-D, the bond's own axis (Bloch); 2-D, the in-plane axis perpendicular to
+// the bond (Neel).
+template <bool DIM3>
+__host__ __device__ constexpr int dmi_axis(int g) {
+    return DIM3 ? (g >> 1) : 2 - (g >> 1);
 }
 
 // Element index (in the other colour) of the neighbour in stencil slot g, or

@@ -78,19 +78,25 @@ __device__ __forceinline__ double wave_sum(double v) {
 // C remainder semantics (truncating), as the reference's % on int.
 __device__ __forceinline__ int crem(int a, int b) { return a % b; }
 
-// One site's contributions to the observables: M (acc 0-2), subM(0) (3-5),
-// sub4M (6-8) and the two local-field sums s . (X +/- Y) (9-14), and for
-// Heisenberg spins the solid angles of its calc_TC triangles, without wrap
-// (t_ref) and with it (t_wrap).
-template <bool DIM3, bool PERIODIC, bool MASKED, bool ISING>
-__device__ __forceinline__ void measure_site(const DevInst& in, const void* __restrict__ pool,
-                                             const uint32_t* __restrict__ mpool, const Stencil& st, int c,
-                                             int64_t e, const SitePos& p, double* acc, double& t_ref,
-                                             double& t_wrap) {
+// Whether stencil slot g's neighbour lies inside the box without periodic
+// wrap (the reference's adj table of an open lattice).
+__device__ __forceinline__ bool link_in_box(const DevInst& in, const SitePos& p, int g) {
+    if (g < 2) return (g & 1) ? p.i0 + 1 < in.a0 : p.i0 > 0;
+    if (g < 4) return (g & 1) ? p.i1 + 1 < in.a1 : p.i1 > 0;
+    return (g & 1) ? p.i2 + 1 < in.n2 : p.i2 > 0;
+}
+
+// One site's contributions to the observable sums: M (acc 0-2), subM(0)
+// (3-5), sub4M (6-8) and the two local-field sums s . (X +/- Y) (9-14; with
+// DM = 0, Y = 0 and acc 12-14 are left to the caller, who copies 9-11).  DM is
+// the DMI form of the sweep kernels (0 none, 1 general, 2 axial: D of slot g
+// along dmi_axis(g)); the forms give the same sums, since the terms they
+// leave out are exact zeros.
+template <bool DIM3, bool PERIODIC, bool ISING, int DM>
+__device__ __forceinline__ void site_sums(const DevInst& in, const void* __restrict__ pool, const Stencil& st,
+                                          int c, int64_t e, const SitePos& p, double* acc) {
 #pragma clang fp contract(off)
     const int i0 = p.i0, i1 = p.i1, i2 = p.i2;
-    const int sx_slot = DIM3 ? 0 : 2;   // first stencil slot of the reference x axis
-    const int sy_slot = DIM3 ? 2 : 4;   // ... of the reference y axis
     double s0, s1, s2;
     load_spin<ISING>(pool, in, c, e, s0, s1, s2);
     acc[0] += s0; acc[1] += s1; acc[2] += s2;
@@ -112,6 +118,7 @@ __device__ __forceinline__ void measure_site(const DevInst& in, const void* __re
     }
     // local fields: X = sum J s_j, Y = sum D x s_j
     double X0 = 0, X1 = 0, X2 = 0, Y0 = 0, Y1 = 0, Y2 = 0;
+#pragma unroll
     for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
         const int ne = nbr_elem<PERIODIC>(in, p, g);
         if (ne < 0) continue;
@@ -119,49 +126,99 @@ __device__ __forceinline__ void measure_site(const DevInst& in, const void* __re
         load_spin<ISING>(pool, in, c ^ 1, ne, n0, n1, n2);
         X0 += st.Jd[g] * n0; X1 += st.Jd[g] * n1; X2 += st.Jd[g] * n2;
         const double* D = st.Dd[g];
-        Y0 += D[1] * n2 - D[2] * n1;
-        Y1 += D[2] * n0 - D[0] * n2;
-        Y2 += D[0] * n1 - D[1] * n0;
-    }
-    acc[9] += s0 * (X0 + Y0); acc[10] += s1 * (X1 + Y1); acc[11] += s2 * (X2 + Y2);
-    acc[12] += s0 * (X0 - Y0); acc[13] += s1 * (X1 - Y1); acc[14] += s2 * (X2 - Y2);
-    if (!ISING) {
-        // calc_TC triangles (i, -x, +y) and (i, +x, -y); q_ref without
-        // periodic wrap (the reference's adj), q_wrap with it.
-        // Where both unwrapped links exist the wrapped triangle is the
-        // same triangle, so its angle is computed once.
-        for (int tri = 0; tri < 2; ++tri) {
-            const int ga = tri == 0 ? sx_slot : sx_slot + 1;
-            const int gb = tri == 0 ? sy_slot + 1 : sy_slot;
-            const int na = nbr_elem<false>(in, p, ga);
-            const int nb = nbr_elem<false>(in, p, gb);
-            const bool inner = na >= 0 && nb >= 0;
-            int wa = na, wb = nb;
-            if (PERIODIC && !inner) {
-                wa = nbr_elem<true>(in, p, ga);
-                wb = nbr_elem<true>(in, p, gb);
-            }
-            for (int wrap = 0; wrap < (PERIODIC && !inner ? 2 : 1); ++wrap) {
-                const int ua = wrap ? wa : na, ub = wrap ? wb : nb;
-                if (ua < 0 || ub < 0) continue;
-                if (MASKED && (!mask_bit(mpool, in, c ^ 1, ua) || !mask_bit(mpool, in, c ^ 1, ub)))
-                    continue;
-                double a0, a1, a2, b0, b1, b2;
-                load_spin<false>(pool, in, c ^ 1, ua, a0, a1, a2);
-                load_spin<false>(pool, in, c ^ 1, ub, b0, b1, b2);
-                const double ang = solid_angle_d(s0, s1, s2, a0, a1, a2, b0, b1, b2);
-                if (wrap) t_wrap += ang;
-                else {
-                    t_ref += ang;
-                    if (inner) t_wrap += ang;
-                }
-            }
+        if (DM == 1) {
+            Y0 += D[1] * n2 - D[2] * n1;
+            Y1 += D[2] * n0 - D[0] * n2;
+            Y2 += D[0] * n1 - D[1] * n0;
+        } else if (DM == 2) {
+            const int a = dmi_axis<DIM3>(g);   // a constant once the loop is unrolled
+            if (a == 0) { Y1 += -(D[0] * n2); Y2 += D[0] * n1; }
+            if (a == 1) { Y0 += D[1] * n2; Y2 += -(D[1] * n0); }
+            if (a == 2) { Y0 += -(D[2] * n1); Y1 += D[2] * n0; }
         }
-        if (!PERIODIC) t_wrap = t_ref;
+    }
+    if (DM == 0) {
+        acc[9] += s0 * X0; acc[10] += s1 * X1; acc[11] += s2 * X2;
+    } else {
+        acc[9] += s0 * (X0 + Y0); acc[10] += s1 * (X1 + Y1); acc[11] += s2 * (X2 + Y2);
+        acc[12] += s0 * (X0 - Y0); acc[13] += s1 * (X1 - Y1); acc[14] += s2 * (X2 - Y2);
     }
 }
 
-template <bool DIM3, bool PERIODIC, bool MASKED, bool ISING>
+// The solid angles of one Heisenberg site's calc_TC triangles (i, -x, +y) and
+// (i, +x, -y): without periodic wrap into t_ref (the reference's adj), with
+// it into t_wrap.  On a periodic lattice the wrapped triangle is the
+// unwrapped one wherever both links lie in the box, so its angle is computed
+// once.  Kept apart from site_sums (a second pass over the tile) so that
+// neither pass holds the other's registers.
+template <bool DIM3, bool PERIODIC, bool MASKED>
+__device__ __forceinline__ void site_charge(const DevInst& in, const void* __restrict__ pool,
+                                            const uint32_t* __restrict__ mpool, int c, int64_t e,
+                                            const SitePos& p, double& t_ref, double& t_wrap) {
+#pragma clang fp contract(off)
+    constexpr int sx_slot = DIM3 ? 0 : 2;   // first stencil slot of the reference x axis
+    constexpr int sy_slot = DIM3 ? 2 : 4;   // ... of the reference y axis
+    double s0, s1, s2;
+    load_spin<false>(pool, in, c, e, s0, s1, s2);
+#pragma unroll
+    for (int tri = 0; tri < 2; ++tri) {
+        const int ga = tri == 0 ? sx_slot : sx_slot + 1;
+        const int gb = tri == 0 ? sy_slot + 1 : sy_slot;
+        const int na = nbr_elem<PERIODIC>(in, p, ga);
+        const int nb = nbr_elem<PERIODIC>(in, p, gb);
+        const bool inner = PERIODIC ? (link_in_box(in, p, ga) && link_in_box(in, p, gb)) : (na >= 0 && nb >= 0);
+        bool use = PERIODIC || inner;
+        if (MASKED && use) use = mask_bit(mpool, in, c ^ 1, na) && mask_bit(mpool, in, c ^ 1, nb);
+        if (!use) continue;
+        double a0, a1, a2, b0, b1, b2;
+        load_spin<false>(pool, in, c ^ 1, na, a0, a1, a2);
+        load_spin<false>(pool, in, c ^ 1, nb, b0, b1, b2);
+        const double ang = solid_angle_d(s0, s1, s2, a0, a1, a2, b0, b1, b2);
+        if (PERIODIC) {
+            t_wrap += ang;
+            if (inner) t_ref += ang;
+        } else {
+            t_ref += ang;
+        }
+    }
+    if (!PERIODIC) t_wrap = t_ref;
+}
+
+// Fixed-order workgroup reduction of the 15 scalars: wave sums, then the four
+// waves in order, into out[0..14].  Ends with the workgroup synchronised.
+__device__ __forceinline__ void block_reduce_out(double* acc, double* red, int wv, int lane,
+                                                 double* __restrict__ out) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int i = 0; i < 15; ++i) acc[i] = wave_sum(acc[i]);
+    if (lane == 0)
+        for (int i = 0; i < 15; ++i) red[wv * 17 + i] = acc[i];
+    __syncthreads();
+    if (threadIdx.x < 15) {
+        const int i = threadIdx.x;
+        out[i] = ((red[i] + red[17 + i]) + red[34 + i]) + red[51 + i];
+    }
+}
+
+// The same for the 2-D charges qr, qw into out[15], out[16] (red slots 15
+// and 16 of each wave, which block_reduce_out leaves alone).
+__device__ __forceinline__ void block_reduce_q2(double qr, double qw, double* red, int wv, int lane,
+                                                double* __restrict__ out) {
+#pragma clang fp contract(off)
+    qr = wave_sum(qr);
+    qw = wave_sum(qw);
+    if (lane == 0) {
+        red[wv * 17 + 15] = qr;
+        red[wv * 17 + 16] = qw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 15 || threadIdx.x == 16) {
+        const int i = threadIdx.x;
+        out[i] = ((red[i] + red[17 + i]) + red[34 + i]) + red[51 + i];
+    }
+}
+
+template <bool DIM3, bool PERIODIC, bool ISING, int DM, bool TC>
 __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ insts,
                                                  const MTile* __restrict__ tiles,
                                                  const void* __restrict__ pool,
@@ -173,7 +230,6 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
     const MTile tl = tiles[blockIdx.x];
     const DevInst& in = insts[tl.inst];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    constexpr bool TC = !ISING;
     constexpr bool TCS = TC && DIM3;   // per-slice charges in LDS
     const int half = in.half;
     const int G = half >= 64 ? 1 : min(gmax, 64 / half);
@@ -187,62 +243,59 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
         __syncthreads();
     }
     double* qbank = qs + (size_t)((wv * G + lrow) * 2) * a2max;
-    double acc[15];
-#pragma unroll
-    for (int i = 0; i < 15; ++i) acc[i] = 0.0;
-    double qr = 0.0, qw = 0.0;   // 2-D charges
-
-    for (int rbase = tl.row0 + wv * G; rbase < tl.row0 + tl.nrows; rbase += 4 * G) {
-        const int row = rbase + lrow;
-        if (lrow >= G || row >= tl.row0 + tl.nrows) continue;   // lanes beyond G * half idle
-        const int i0 = DIM3 ? row / in.a1 : 0;
-        const int i1 = DIM3 ? row - i0 * in.a1 : row;
-        for (int k = kbeg; k < half; k += kstep) {
-            for (int q2 = 0; q2 < 2; ++q2) {
-                const int i2 = 2 * k + q2;
-                if (i2 >= in.n2) continue;
-                const int c = (i0 + i1 + i2) & 1;
-                const int64_t e = (int64_t)row * in.half + k;
-                if (MASKED && !mask_bit(mpool, in, c, e)) continue;
-                SitePos p;
-                p.i0 = i0; p.i1 = i1; p.i2 = i2; p.k = k; p.q = q2; p.row = row;
-                double t_ref = 0.0, t_wrap = 0.0;
-                measure_site<DIM3, PERIODIC, MASKED, ISING>(in, pool, mpool, st, c, e, p, acc, t_ref, t_wrap);
-                if (TC) {
-                    if (TCS) {
-                        qbank[i2] += t_ref;
-                        qbank[a2max + i2] += t_wrap;
-                    } else {
-                        qr += t_ref;
-                        qw += t_wrap;
-                    }
+    // the tile's sites, a lane group's row and columns at a time
+    auto each_site = [&](auto&& f) {
+        for (int rbase = tl.row0 + wv * G; rbase < tl.row0 + tl.nrows; rbase += 4 * G) {
+            const int row = rbase + lrow;
+            if (lrow >= G || row >= tl.row0 + tl.nrows) continue;   // lanes beyond G * half idle
+            const int i0 = DIM3 ? row / in.a1 : 0;
+            const int i1 = DIM3 ? row - i0 * in.a1 : row;
+            for (int k = kbeg; k < half; k += kstep) {
+                for (int q2 = 0; q2 < 2; ++q2) {
+                    const int i2 = 2 * k + q2;
+                    if (i2 >= in.n2) continue;
+                    SitePos p;
+                    p.i0 = i0; p.i1 = i1; p.i2 = i2; p.k = k; p.q = q2; p.row = row;
+                    f((i0 + i1 + i2) & 1, (int64_t)row * in.half + k, p);
                 }
             }
         }
-    }
-    // fixed-order block reduction of the 15 scalars (+ 2-D charges)
-    const int nsc = 15 + ((TC && !DIM3) ? 2 : 0);
-    if (TC && !DIM3) { qr = wave_sum(qr); qw = wave_sum(qw); }
+    };
+    double* out = partials + (int64_t)blockIdx.x * pstride;
+    {   // pass 1: the 15 sums, reduced in a fixed order
+        double acc[15];
 #pragma unroll
-    for (int i = 0; i < 15; ++i) acc[i] = wave_sum(acc[i]);
-    if (lane == 0) {
-        for (int i = 0; i < 15; ++i) red[wv * 17 + i] = acc[i];
-        red[wv * 17 + 15] = qr;
-        red[wv * 17 + 16] = qw;
+        for (int i = 0; i < 15; ++i) acc[i] = 0.0;
+        each_site([&](int c, int64_t e, const SitePos& p) {
+            site_sums<DIM3, PERIODIC, ISING, DM>(in, pool, st, c, e, p, acc);
+        });
+        if (DM == 0) { acc[12] = acc[9]; acc[13] = acc[10]; acc[14] = acc[11]; }
+        block_reduce_out(acc, red, wv, lane, out);
+    }
+    if (!TC) return;
+    // pass 2: the charges (per slice in LDS for 3-D)
+    double qr = 0.0, qw = 0.0;
+    each_site([&](int c, int64_t e, const SitePos& p) {
+        double t_ref = 0.0, t_wrap = 0.0;
+        site_charge<DIM3, PERIODIC, false>(in, pool, mpool, c, e, p, t_ref, t_wrap);
+        if (TCS) {
+            qbank[p.i2] += t_ref;
+            qbank[a2max + p.i2] += t_wrap;
+        } else {
+            qr += t_ref;
+            qw += t_wrap;
+        }
+    });
+    if (!TCS) {
+        block_reduce_q2(qr, qw, red, wv, lane, out);
+        return;
     }
     __syncthreads();
-    double* out = partials + (int64_t)blockIdx.x * pstride;
-    if (threadIdx.x < nsc) {
-        const int i = threadIdx.x;
-        out[i] = ((red[i] + red[17 + i]) + red[34 + i]) + red[51 + i];
-    }
-    if (TCS) {
-        for (int i = threadIdx.x; i < 2 * in.a2; i += 256) {
-            const int which = i / in.a2, z = i - which * in.a2;
-            double v = 0.0;
-            for (int b = 0; b < 4 * G; ++b) v += qs[(b * 2 + which) * a2max + z];
-            out[15 + which * a2max + z] = v;
-        }
+    for (int i = threadIdx.x; i < 2 * in.a2; i += 256) {
+        const int which = i / in.a2, z = i - which * in.a2;
+        double v = 0.0;
+        for (int b = 0; b < 4 * G; ++b) v += qs[(b * 2 + which) * a2max + z];
+        out[15 + which * a2max + z] = v;
     }
 }
 
@@ -251,88 +304,79 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
 // e | colour << 31, in element order, so that a wavefront's loads stay
 // coalesced), so that no lane idles on the absent sites that fill most of a
 // grain's bounding box.  One workgroup takes up to kMeasureSites listed sites
-// of one colour of one instance (MTile {inst, first entry, count}) and writes
-// the same partials as k_measure.  3-D Heisenberg tiles keep each site's
-// solid angles and slice in LDS; each slice is then summed over the tile in
-// list order, a quarter of the tile per work-item.  Every sum runs in a fixed
+// of one instance (MTile {inst, first entry, count}) and writes
+// the same partials as k_measure.  3-D Heisenberg tiles with charges keep
+// each site's solid angles in LDS, grouped by slice: the host gives every
+// entry its rank in a stable sort of the tile by slice z, and zoff[z] the
+// first rank of slice z.  Work-item z then sums slice z over ranks
+// zoff[z] .. zoff[z + 1], i.e. in list order.  Every sum runs in a fixed
 // order.
 // ---------------------------------------------------------------------------
-template <bool DIM3, bool PERIODIC, bool ISING>
+template <bool DIM3, bool PERIODIC, bool ISING, int DM, bool TC>
 __global__ __launch_bounds__(256) void k_measure_sites(const DevInst* __restrict__ insts,
                                                        const MTile* __restrict__ tiles,
                                                        const uint32_t* __restrict__ sites,
+                                                       const uint16_t* __restrict__ rank,
+                                                       const uint16_t* __restrict__ zoff,
                                                        const void* __restrict__ pool,
                                                        const uint32_t* __restrict__ mpool, Stencil st,
                                                        double* __restrict__ partials, int pstride, int a2max) {
 #pragma clang fp contract(off)
-    constexpr bool TC = !ISING;
     constexpr bool TCS = TC && DIM3;
     constexpr int NV = TCS ? kMeasureSites : 1;
     __shared__ double red[4 * 17];
     __shared__ double vr[NV], vw[NV];
-    __shared__ int vz[NV];
-    extern __shared__ double qp[];   // [2][a2max][4] quarter sums per slice (3-D Heisenberg)
     const MTile tl = tiles[blockIdx.x];   // row0 = first list entry, nrows = count
     const DevInst& in = insts[tl.inst];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double acc[15];
+    double* out = partials + (int64_t)blockIdx.x * pstride;
+    {   // pass 1: the 15 sums, reduced in a fixed order
+        double acc[15];
 #pragma unroll
-    for (int i = 0; i < 15; ++i) acc[i] = 0.0;
-    double qr = 0.0, qw = 0.0;   // 2-D charges
+        for (int i = 0; i < 15; ++i) acc[i] = 0.0;
+        for (int j = threadIdx.x; j < tl.nrows; j += 256) {
+            const uint32_t ent = sites[(int64_t)tl.row0 + j];
+            const int c = (int)(ent >> 31);
+            const uint32_t e = ent & 0x7FFFFFFFu;
+            site_sums<DIM3, PERIODIC, ISING, DM>(in, pool, st, c, e, site_of(in, e, c, DIM3), acc);
+        }
+        if (DM == 0) { acc[12] = acc[9]; acc[13] = acc[10]; acc[14] = acc[11]; }
+        block_reduce_out(acc, red, wv, lane, out);
+    }
+    if (!TC) return;
+    // pass 2: the charges (3-D: per site into LDS at its slice rank)
+    double qr = 0.0, qw = 0.0;
     for (int j = threadIdx.x; j < tl.nrows; j += 256) {
         const uint32_t ent = sites[(int64_t)tl.row0 + j];
         const int c = (int)(ent >> 31);
         const uint32_t e = ent & 0x7FFFFFFFu;
-        const SitePos p = site_of(in, e, c, DIM3);
         double t_ref = 0.0, t_wrap = 0.0;
-        measure_site<DIM3, PERIODIC, true, ISING>(in, pool, mpool, st, c, e, p, acc, t_ref, t_wrap);
+        site_charge<DIM3, PERIODIC, true>(in, pool, mpool, c, e, site_of(in, e, c, DIM3), t_ref, t_wrap);
         if (TCS) {
-            vr[j] = t_ref;
-            vw[j] = t_wrap;
-            vz[j] = p.i2;
-        } else if (TC) {
+            const int r = rank[(int64_t)tl.row0 + j];
+            vr[r] = t_ref;
+            vw[r] = t_wrap;
+        } else {
             qr += t_ref;
             qw += t_wrap;
         }
     }
-    const int nsc = 15 + ((TC && !DIM3) ? 2 : 0);
-    if (TC && !DIM3) { qr = wave_sum(qr); qw = wave_sum(qw); }
-#pragma unroll
-    for (int i = 0; i < 15; ++i) acc[i] = wave_sum(acc[i]);
-    if (lane == 0) {
-        for (int i = 0; i < 15; ++i) red[wv * 17 + i] = acc[i];
-        red[wv * 17 + 15] = qr;
-        red[wv * 17 + 16] = qw;
+    if (!TCS) {
+        block_reduce_q2(qr, qw, red, wv, lane, out);
+        return;
     }
     __syncthreads();
-    double* out = partials + (int64_t)blockIdx.x * pstride;
-    if (threadIdx.x < nsc) {
-        const int i = threadIdx.x;
-        out[i] = ((red[i] + red[17 + i]) + red[34 + i]) + red[51 + i];
-    }
-    if (TCS) {
-        // slice z: four work-items scan a quarter of the tile each, then one
-        // adds the quarters in order
-        for (int i = threadIdx.x; i < 4 * a2max; i += 256) {
-            const int z = i >> 2, part = i & 3;
-            const int t0 = part * tl.nrows / 4, t1 = (part + 1) * tl.nrows / 4;
-            double sr = 0.0, sw = 0.0;
-            if (z < in.a2)
-                for (int t = t0; t < t1; ++t)
-                    if (vz[t] == z) {
-                        sr += vr[t];
-                        sw += vw[t];
-                    }
-            qp[i] = sr;
-            qp[4 * a2max + i] = sw;
+    // slice z by work-item z, over its entries in list order
+    const uint16_t* zo = zoff + (int64_t)blockIdx.x * (a2max + 1);
+    for (int z = threadIdx.x; z < a2max; z += 256) {
+        double sr = 0.0, sw = 0.0;
+        const int u1 = zo[z + 1];
+        for (int u = zo[z]; u < u1; ++u) {
+            sr += vr[u];
+            sw += vw[u];
         }
-        __syncthreads();
-        for (int z = threadIdx.x; z < a2max; z += 256) {
-            const double* r = qp + 4 * z;
-            const double* w = qp + 4 * a2max + 4 * z;
-            out[15 + z] = ((r[0] + r[1]) + r[2]) + r[3];
-            out[15 + a2max + z] = ((w[0] + w[1]) + w[2]) + w[3];
-        }
+        out[15 + z] = sr;
+        out[15 + a2max + z] = sw;
     }
 }
 
@@ -352,8 +396,21 @@ __global__ __launch_bounds__(256) void k_measure_finalize(
     const int tb = t_begin[li], te = t_end[li];
     const int nq = tc ? (dim3 ? 2 * a2max : 2) : 0;
     for (int i = threadIdx.x; i < 15 + nq; i += 256) {
-        double v = 0.0;
-        for (int t = tb; t < te; ++t) v += partials[(int64_t)t * pstride + i];
+        // tiles t = tb + 8m + r into eight running sums (independent loads in
+        // flight), added in a fixed order
+        const double* pp = partials + i;
+        double v8[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v8[r] = 0.0;
+        int t = tb;
+        for (; t + 8 <= te; t += 8) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v8[r] += pp[(int64_t)(t + r) * pstride];
+        }
+#pragma unroll
+        for (int r = 0; r < 7; ++r)
+            if (t + r < te) v8[r] += pp[(int64_t)(t + r) * pstride];
+        const double v = ((v8[0] + v8[1]) + (v8[2] + v8[3])) + ((v8[4] + v8[5]) + (v8[6] + v8[7]));
         if (i < 15) sum[i] = v;
         else if (tc) {
             const int j = i - 15;
@@ -614,22 +671,26 @@ __global__ __launch_bounds__(256) void k_build_weibull(int dim, int edge, const 
 // ---------------------------------------------------------------------------
 // Host-side launchers (declared in nmc_launch.h).
 // ---------------------------------------------------------------------------
-hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising,
-                          const DevInst* insts, const MTile* t, int ntiles, const void* pool,
+hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising, int dm,
+                          bool tc, const DevInst* insts, const MTile* t, int ntiles, const void* pool,
                           const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                           int a2max) {
     if (ntiles <= 0) return hipSuccess;
+    // shaped (masked) lattices are measured from site lists (k_measure_sites)
+    if (masked || dm < 0 || dm > 2) return hipErrorInvalidValue;
+    if (ising) { dm = 0; tc = false; }
     // per-slice charge banks: 4 waves x gmax lane groups x 2 x a2max doubles;
     // gmax = 64 / (a2max / 2) covers the longest rows exactly
     const int gmax = std::max(1, std::min(64, 128 / std::max(2, a2max)));
-    const size_t shm = (size_t)(((!ising && dim3) ? 8 * gmax * a2max : 0) + 4 * 17) * sizeof(double);
-#define NMC_MEAS(D3, PER, MSK, IS)                                                           \
-    if (dim3 == D3 && periodic == PER && masked == MSK && ising == IS)                       \
-        k_measure<D3, PER, MSK, IS><<<ntiles, 256, shm, stream>>>(insts, t, pool, mpool, st, \
-                                                                  partials, pstride, a2max, gmax);
-    // shaped (masked) lattices are measured from site lists (k_measure_sites)
-    if (masked) return hipErrorInvalidValue;
-#define NMC_MEAS_B(D3, PER) NMC_MEAS(D3, PER, false, false) NMC_MEAS(D3, PER, false, true)
+    const size_t shm = (size_t)((tc && dim3 ? 8 * gmax * a2max : 0) + 4 * 17) * sizeof(double);
+#define NMC_MEAS(D3, PER, IS, DMF, T)                                                                      \
+    if (dim3 == D3 && periodic == PER && ising == IS && dm == DMF && tc == T)                             \
+        k_measure<D3, PER, IS, DMF, T><<<ntiles, 256, shm, stream>>>(insts, t, pool, mpool, st, partials, \
+                                                                    pstride, a2max, gmax);
+#define NMC_MEAS_B(D3, PER)                                                              \
+    NMC_MEAS(D3, PER, true, 0, false)                                                    \
+    NMC_MEAS(D3, PER, false, 0, false) NMC_MEAS(D3, PER, false, 1, false) NMC_MEAS(D3, PER, false, 2, false) \
+    NMC_MEAS(D3, PER, false, 0, true) NMC_MEAS(D3, PER, false, 1, true) NMC_MEAS(D3, PER, false, 2, true)
     NMC_MEAS_B(true, true)
     NMC_MEAS_B(true, false)
     NMC_MEAS_B(false, true)
@@ -639,19 +700,28 @@ hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool mas
     return hipGetLastError();
 }
 
-hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, const DevInst* insts,
-                                const MTile* t, int ntiles, const uint32_t* sites, const void* pool,
+hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, int dm, bool tc,
+                                const DevInst* insts, const MTile* t, int ntiles, const uint32_t* sites,
+                                const uint16_t* rank, const uint16_t* zoff, const void* pool,
                                 const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                                 int a2max) {
     if (ntiles <= 0) return hipSuccess;
-    const size_t shm = (!ising && dim3) ? 8 * (size_t)a2max * sizeof(double) : 0;
-#define NMC_MEAS_S(D3, PER, IS)                                                                           \
-    if (dim3 == D3 && periodic == PER && ising == IS)                                                     \
-        k_measure_sites<D3, PER, IS><<<ntiles, 256, shm, stream>>>(insts, t, sites, pool, mpool, st, partials, \
-                                                                   pstride, a2max);
-    NMC_MEAS_S(true, true, false) NMC_MEAS_S(true, true, true) NMC_MEAS_S(true, false, false)
-    NMC_MEAS_S(true, false, true) NMC_MEAS_S(false, true, false) NMC_MEAS_S(false, true, true)
-    NMC_MEAS_S(false, false, false) NMC_MEAS_S(false, false, true)
+    if (dm < 0 || dm > 2) return hipErrorInvalidValue;
+    if (ising) { dm = 0; tc = false; }
+    if (tc && dim3 && (!rank || !zoff)) return hipErrorInvalidValue;
+#define NMC_MEAS_S(D3, PER, IS, DMF, T)                                                                    \
+    if (dim3 == D3 && periodic == PER && ising == IS && dm == DMF && tc == T)                             \
+        k_measure_sites<D3, PER, IS, DMF, T><<<ntiles, 256, 0, stream>>>(insts, t, sites, rank, zoff, pool, mpool, \
+                                                                        st, partials, pstride, a2max);
+#define NMC_MEAS_SB(D3, PER)                                                                   \
+    NMC_MEAS_S(D3, PER, true, 0, false)                                                        \
+    NMC_MEAS_S(D3, PER, false, 0, false) NMC_MEAS_S(D3, PER, false, 1, false) NMC_MEAS_S(D3, PER, false, 2, false) \
+    NMC_MEAS_S(D3, PER, false, 0, true) NMC_MEAS_S(D3, PER, false, 1, true) NMC_MEAS_S(D3, PER, false, 2, true)
+    NMC_MEAS_SB(true, true)
+    NMC_MEAS_SB(true, false)
+    NMC_MEAS_SB(false, true)
+    NMC_MEAS_SB(false, false)
+#undef NMC_MEAS_SB
 #undef NMC_MEAS_S
     return hipGetLastError();
 }

@@ -39,15 +39,23 @@ hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, 
                               int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                               unsigned long long* acc);
 
-hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising,
-                          const DevInst* insts, const MTile* tiles, int ntiles, const void* pool,
+// Observable partials of full boxes.  dm = DMI form of the stencil (0 none,
+// 1 general, 2 axial, as for the sweeps); tc = compute the topological
+// charges (Heisenberg only; without it the per-slice partials are not
+// written and the finalize must not read them).
+hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising, int dm,
+                          bool tc, const DevInst* insts, const MTile* tiles, int ntiles, const void* pool,
                           const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                           int a2max);
 
 // Shaped lattices: the same partials from lists of present sites
-// (e | colour << 31), tiles as described at kMeasureSites.
-hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, const DevInst* insts,
-                                const MTile* t, int ntiles, const uint32_t* sites, const void* pool,
+// (e | colour << 31), tiles as described at kMeasureSites.  3-D Heisenberg
+// with tc also needs each entry's rank in a stable sort of its tile by slice
+// (stored at the entry's list position) and each tile's a2max + 1 slice
+// offsets into that order (zoff).
+hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, int dm, bool tc,
+                                const DevInst* insts, const MTile* t, int ntiles, const uint32_t* sites,
+                                const uint16_t* rank, const uint16_t* zoff, const void* pool,
                                 const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                                 int a2max);
 

@@ -73,10 +73,6 @@ __device__ __forceinline__ bool mask_bit_s(const uint32_t* mw, const DevInst& in
 //     lie on x and carry D along y, slots 4, 5 on y with D along x).
 //     n x D then has two non-zero terms.  The host picks 2 only when every
 //     other component of every D is exactly zero.
-template <bool DIM3>
-__host__ __device__ constexpr int dmi_axis(int g) {
-    return DIM3 ? (g >> 1) : 2 - (g >> 1);
-}
 
 // Local field of stencil neighbour n added to v: J n (+ n x D, the
 // orientation dE_func uses, thermodynamics.cpp:45-46).

@@ -61,6 +61,8 @@ CASES = [
     (2, 16, 1, True, False, True),
     (2, 14, 1, False, True, True),
     (3, 8, 1, False, True, True),
+    (3, 18, 0, False, True, True),   # several measurement tiles per colour
+    (3, 16, 1, False, False, True),
 ]
 
 
@@ -101,6 +103,65 @@ def test_measure_matches_oracle(orc, bloch3d, case):
         assert np.allclose(obs[0], want, rtol=1e-12, atol=1e-9)
         assert np.allclose(qr[0], f.tcharge(s, wrap=False), rtol=1e-10, atol=1e-10)
         assert np.allclose(qw[0], f.tcharge(s, wrap=True), rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("dim,periodic,masked", [(2, 1, False), (2, 0, True), (3, 1, False), (3, 0, True),
+                                                 (3, 1, True)])
+def test_measure_general_dmi_matches_oracle(orc, bloch3d, dim, periodic, masked):
+    # DMI vectors off the lattice's one-axis pattern: the measurement's
+    # six-term form of n x D
+    rng = np.random.default_rng(401 + 10 * dim + periodic)
+    edge = 8 if dim == 3 else 14
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.3) if dim == 3 else neel2d(0.3)
+    D = np.asarray(D) + 0.1 * np.sign(np.asarray(D).sum(axis=1, keepdims=True))
+    locs = None
+    if masked:
+        locs = box_locs(dim, edge)
+        locs = locs[rng.random(len(locs)) < 0.75]
+    lat, f = make_pair(orc, dim, edge, periodic, bd, J, D, False, use_d=True, locs=locs)
+    s = f32(random_unit(f.n, rng))
+    lat.set_spins(0, s)
+    lat.set_params(0.7, 0.1)
+    obs, qr, qw = lat.measure()
+    want = np.concatenate([f.mag(s), f.submag(s), f.sub4mag(s),
+                           [f.energy(s, [0, 0, 0.1], use_d=True, convention=0),
+                            f.energy(s, [0, 0, 0.1], use_d=True, convention=1)]])
+    assert np.allclose(obs[0], want, rtol=1e-12, atol=1e-9)
+    assert abs(obs[0, 9] - obs[0, 10]) > 1e-3          # the DMI term is present
+    assert np.allclose(qr[0], f.tcharge(s, wrap=False), rtol=1e-10, atol=1e-10)
+    assert np.allclose(qw[0], f.tcharge(s, wrap=True), rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("dim,periodic,masked", [(3, 1, False), (2, 1, False), (3, 0, True)])
+def test_measure_without_charges_is_identical(orc, bloch3d, dim, periodic, masked):
+    # a measurement that asks for no charges skips the triangles: every other
+    # observable must be bitwise the one of the full measurement, directly and
+    # in a sample series
+    import ctypes as C
+    from nanomagmc_amd._lib import check, lib
+    rng = np.random.default_rng(77 + dim)
+    edge = 8 if dim == 3 else 16
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.3) if dim == 3 else neel2d(0.3)
+    locs = None
+    if masked:
+        locs = box_locs(dim, edge)
+        locs = locs[rng.random(len(locs)) < 0.7]
+    lat, f = make_pair(orc, dim, edge, periodic, bd, J, D, False, use_d=True, locs=locs, ninst=3)
+    lat.set_params([0.5, 1.0, 2.0], [0.0, 0.05, 0.1])
+    lat.randomize()
+    lat.equil(5)
+    full, _, _ = lat.measure()
+    bare = np.zeros_like(full)
+    check(lib().nmc_measure(lat._h, None, 0, bare.ctypes.data_as(C.POINTER(C.c_double)), None, None))
+    assert np.array_equal(full, bare)
+    series = []
+    for with_charge in (True, False):
+        lat.series_reset(2, with_charge=with_charge)
+        lat.sample(0)
+        lat.sample(1)
+        series.append(lat.get_series()[0])
+    assert np.array_equal(series[0], series[1])
+    assert np.array_equal(series[0][:, 0], full)
 
 
 def test_tcharge_skyrmion_fixture(orc, bloch3d):
