@@ -141,6 +141,19 @@ int nmc_randomize(nmc_ctx* ctx, const int* inst, int n);
  * nmc_randomize, nmc_measure, nmc_sample, nmc_run_samples,
  * nmc_accumulate_avgs, nmc_block_moves and nmc_block_swaps). */
 int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps);
+
+/* No counterpart in the reference (whose trials are uniform on the sphere):
+ * equilibration sweeps with the NMC_TRIAL_CONE half-angles adapted toward
+ * 50% acceptance on the device, without a host round trip per step.
+ * `nsweeps` sweeps as nmc_sweep, in chunks of `chunk` sweeps (the last one
+ * may be shorter); after each chunk every listed instance's half-angle is
+ * scaled by clamp(rate / 0.5, 0.5, 2) and clamped to [0.02, pi], where rate =
+ * accepted attempts / (sites * sweeps) over that chunk.  The same step on the
+ * host (nmc_reset_accepts, nmc_sweep, nmc_get_accepts, nmc_set_cone) gives
+ * the same lattices and angles.  Resets the accept counters first and leaves
+ * them at zero.  `half_angle` (may be NULL) receives the final angles, one
+ * per listed instance.  Heisenberg only (NMC_E_INVALID for Ising). */
+int nmc_sweep_adapt(nmc_ctx* ctx, const int* inst, int n, int nsweeps, int chunk, float* half_angle);
 /* Kernel path for nmc_sweep.  AUTO keeps every instance whose two colour
  * arrays fit in one workgroup's LDS resident there for whole sweeps (one
  * persistent workgroup per instance) and streams the others from HBM with

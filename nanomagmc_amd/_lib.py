@@ -23,7 +23,7 @@ OBS_NAMES = ["Mx", "My", "Mz", "subMx", "subMy", "subMz", "sub4Mx", "sub4My", "s
 # Every symbol include/nmc.h declares, in header order.
 EXPORTS = [
     "nmc_last_error", "nmc_create", "nmc_destroy", "nmc_get_stream", "nmc_sync", "nmc_num_sites", "nmc_num_slices",
-    "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_set_path", "nmc_set_lds_block", "nmc_get_accepts",
+    "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_sweep_adapt", "nmc_set_path", "nmc_set_lds_block", "nmc_get_accepts",
     "nmc_reset_accepts", "nmc_snapshot", "nmc_measure", "nmc_series_reset", "nmc_sample",
     "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_accumulate_avgs", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
     "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_net_create", "nmc_net_destroy",
@@ -76,6 +76,7 @@ def lib():
         "nmc_set_cone": (i, [vp, P(i), i, P(C.c_float)]),
         "nmc_randomize": (i, [vp, P(i), i]),
         "nmc_sweep": (i, [vp, P(i), i, i]),
+        "nmc_sweep_adapt": (i, [vp, P(i), i, i, i, P(C.c_float)]),
         "nmc_set_path": (i, [vp, i]),
         "nmc_set_lds_block": (i, [vp, i]),
         "nmc_get_accepts": (i, [vp, P(i), i, P(u64)]),

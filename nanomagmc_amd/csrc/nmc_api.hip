@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -81,6 +82,10 @@ struct Set {
     int* d_tb = nullptr;
     int* d_te = nullptr;
     int* d_lds = nullptr;    // instances swept LDS-resident (the rest are in d_sweep or d_qt)
+    int* d_atp = nullptr;      // cone adaptation (built on first use): per list entry, its range in d_ati ...
+    int* d_ati = nullptr;      // ... of streaming-tile counters (indices into d_tacc) ...
+    double* d_ans = nullptr;   // ... and its present sites
+    bool adapt_ready = false;
     int n_lds = 0;
     QTile* d_qt = nullptr;   // shaped Heisenberg lattices: colour-0 tiles, then colour-1 tiles
     int n_qt[2] = {0, 0};
@@ -102,6 +107,9 @@ struct Set {
         d_msites = nullptr; d_mrank = nullptr; d_mzoff = nullptr;
         (void)hipFree(d_tacc); (void)hipFree(d_qt); (void)hipFree(d_qlist);
         (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb); (void)hipFree(d_cs_q);
+        (void)hipFree(d_atp); (void)hipFree(d_ati); (void)hipFree(d_ans);
+        d_atp = nullptr; d_ati = nullptr; d_ans = nullptr;
+        adapt_ready = false;
         d_tacc = nullptr; d_qt = nullptr; d_qlist = nullptr;
         d_cl = nullptr; d_cs_e = nullptr; d_cs_nb = nullptr; d_cs_q = nullptr;
         d_list = nullptr; d_sweep = nullptr; d_init = nullptr; d_mt = nullptr;
@@ -172,6 +180,7 @@ struct nmc_ctx {
     size_t max_lds = 65536;   // dynamic LDS one workgroup may use on this device
     // instantiated sample-loop graphs keyed by (instance list, nsamples, sweeps)
     std::map<std::tuple<std::vector<int>, int, int>, hipGraphExec_t> graphs;
+    bool graphs_on = true;   // NMC_NO_GRAPHS=1 in the environment: launch directly, capture nothing
     uint64_t block_round = 0;   // learned block-move rounds issued (RNG counter)
 
     void drop_graphs() {
@@ -231,6 +240,17 @@ void for_each_site(const nmc_ctx* c, int i, F f) {
 
 int check_inst(const nmc_ctx* c, int i) {
     if (i < 0 || i >= (int)c->din.size()) return fail(NMC_E_INVALID, "instance index " + std::to_string(i) + " out of range");
+    return NMC_OK;
+}
+
+// The device recomputes every cone_w from its half-angle; the host copy takes
+// the result back (the host copy is the one every later upload sends).
+int cone_from_device(nmc_ctx* c) {
+    NMC_HIP(launch_cone_w(c->stream, c->d_inst, (int)c->din.size()));
+    std::vector<DevInst> t(c->din.size());
+    NMC_HIP(hipMemcpyAsync(t.data(), c->d_inst, t.size() * sizeof(DevInst), hipMemcpyDeviceToHost, c->stream));
+    NMC_HIP(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < t.size(); ++i) c->din[i].cone_w = t[i].cone_w;
     return NMC_OK;
 }
 
@@ -643,6 +663,7 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
     if (cfg->ising && cfg->trial == NMC_TRIAL_CONE)
         return fail(NMC_E_INVALID, "cone trials need Heisenberg spins");
     auto c = std::make_unique<nmc_ctx>();
+    if (const char* ng = std::getenv("NMC_NO_GRAPHS")) c->graphs_on = std::atoi(ng) == 0;
     c->cfg = *cfg;
     c->dim3 = cfg->dim == 3;
     c->ising = cfg->ising != 0;
@@ -752,7 +773,8 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
         if (c->masked) mwords += 2 * d.mask_stride;
         d.tab_base = (int64_t)i * c->tabsize;
         d.inst_id = (uint32_t)i;
-        d.beta = 1.f; d.beta_d = 1.0; d.h = 0.f; d.h_d = 0.0; d.cone_w = 1.f;
+        d.beta = 1.f; d.beta_d = 1.0; d.h = 0.f; d.h_d = 0.0;
+        d.cone_th = (float)(M_PI / 2); d.cone_w = 1.f;   // cone_w is recomputed on the device
         const int64_t box = (int64_t)h.n[0] * h.n[1] * h.n[2];
         if (s.mask) h.mask.assign(s.mask, s.mask + box);
         h.nsites = h.mask.empty() ? box : (int64_t)std::count_if(h.mask.begin(), h.mask.end(), [](uint8_t v) { return v != 0; });
@@ -803,6 +825,7 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
     NMC_HIP(hipMemsetAsync(c->d_acc, 0, c->din.size() * sizeof(unsigned long long), c->stream));
     NMC_HIP(hipMemsetAsync(c->d_bacc, 0, c->din.size() * sizeof(unsigned long long), c->stream));
     if ((rc = upload_insts(c.get()))) return rc;
+    if ((rc = cone_from_device(c.get()))) return rc;
     // default up spins (set_default_spins) with beta = 1, h = 0
     std::vector<double> b1(ninst, 1.0), h0(ninst, 0.0);
     nmc_ctx* raw = c.release();
@@ -892,9 +915,10 @@ int nmc_set_cone(nmc_ctx* ctx, const int* inst, int n, const float* half_angle) 
         if (int rc = check_inst(ctx, i)) return rc;
         const float th = half_angle[j];
         if (!(th > 0.f) || th > (float)M_PI + 1e-6f) return fail(NMC_E_INVALID, "cone half-angle must be in (0, pi]");
-        ctx->din[i].cone_w = (float)(1.0 - std::cos((double)th));
+        ctx->din[i].cone_th = th;
     }
-    return upload_insts(ctx);
+    if (int rc = upload_insts(ctx)) return rc;
+    return cone_from_device(ctx);
 }
 
 int nmc_randomize(nmc_ctx* ctx, const int* inst, int n) {
@@ -1009,7 +1033,7 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
     // replays continue the same random streams.  Sets swept only in LDS
     // launch once per 256 sweeps anyway and skip the capture (~3 ms).
     const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
-    if (s->sweep_block == 64 && streamed && nsweeps >= 2 * kGraphSweeps) {
+    if (ctx->graphs_on && s->sweep_block == 64 && streamed && nsweeps >= 2 * kGraphSweeps) {
         auto key = std::make_tuple(s->list, -1, kGraphSweeps);
         auto it = ctx->graphs.find(key);
         if (it == ctx->graphs.end()) {
@@ -1029,6 +1053,92 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
         return sweep_launches(ctx, s, nsweeps % kGraphSweeps);
     }
     return sweep_launches(ctx, s, nsweeps);
+}
+
+namespace {
+// Per list entry of a set: its streaming-tile counters and its present sites
+// (for k_adapt_cone).
+int adapt_prepare(nmc_ctx* c, Set* s) {
+    std::vector<int> pos(c->din.size(), -1);
+    for (size_t j = 0; j < s->list.size(); ++j) pos[s->list[j]] = (int)j;
+    const size_t nl = s->list.size();
+    std::vector<int> tptr(nl + 1, 0), tidx(s->h_sweep.size());
+    for (const Tile& t : s->h_sweep) ++tptr[pos[t.inst] + 1];
+    for (size_t j = 0; j < nl; ++j) tptr[j + 1] += tptr[j];
+    std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+    for (size_t k = 0; k < s->h_sweep.size(); ++k) tidx[fill[pos[s->h_sweep[k].inst]]++] = (int)k;
+    std::vector<double> ns(nl);
+    for (size_t j = 0; j < nl; ++j) ns[j] = (double)c->hin[s->list[j]].nsites;
+    if (int rc = upload(c, &s->d_atp, tptr)) return rc;
+    if (int rc = upload(c, &s->d_ati, tidx)) return rc;
+    if (int rc = upload(c, &s->d_ans, ns)) return rc;
+    s->adapt_ready = true;
+    return NMC_OK;
+}
+
+// `c` sweeps of a set, then one adaptation step.
+int adapt_chunk(nmc_ctx* ctx, Set* s, int c) {
+    if (int rc = sweep_launches(ctx, s, c)) return rc;
+    NMC_HIP(launch_adapt_cone(ctx->stream, ctx->d_inst, s->d_list, (int)s->list.size(), s->d_atp, s->d_ati,
+                              s->d_tacc, ctx->d_acc, s->d_ans, c));
+    return NMC_OK;
+}
+}  // namespace
+
+int nmc_sweep_adapt(nmc_ctx* ctx, const int* inst, int n, int nsweeps, int chunk, float* half_angle) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (nsweeps < 0 || chunk <= 0) return fail(NMC_E_INVALID, "negative sweep count or chunk <= 0");
+    if (ctx->ising) return fail(NMC_E_INVALID, "cone adaptation needs Heisenberg spins");
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    if (!s->adapt_ready)
+        if (int rc = adapt_prepare(ctx, s)) return rc;
+    if (int rc = nmc_reset_accepts(ctx)) return rc;
+    // as nmc_sweep: small streamed sets replay one captured graph of about
+    // kGraphSweeps sweeps (gc whole chunks, each with its adaptation step)
+    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    const int gc = std::max(1, kGraphSweeps / chunk);
+    const int reps = nsweeps / (gc * chunk);
+    int done = 0;
+    if (ctx->graphs_on && s->sweep_block == 64 && streamed && reps >= 2) {
+        auto key = std::make_tuple(s->list, -2, chunk);
+        auto it = ctx->graphs.find(key);
+        if (it == ctx->graphs.end()) {
+            NMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = NMC_OK;
+            for (int k = 0; k < gc && rc == NMC_OK; ++k) rc = adapt_chunk(ctx, s, chunk);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc != NMC_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return fail(NMC_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            hipGraphExec_t x = nullptr;
+            e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(NMC_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            it = ctx->graphs.emplace(key, x).first;
+        }
+        for (int r = 0; r < reps; ++r) NMC_HIP(hipGraphLaunch(it->second, ctx->stream));
+        done = reps * gc * chunk;
+    }
+    while (done < nsweeps) {
+        const int c = std::min(chunk, nsweeps - done);
+        if (int rc = adapt_chunk(ctx, s, c)) return rc;
+        done += c;
+    }
+    // the adapted angles back into the host copy of the descriptors
+    std::vector<DevInst> t(ctx->din.size());
+    NMC_HIP(hipMemcpyAsync(t.data(), ctx->d_inst, t.size() * sizeof(DevInst), hipMemcpyDeviceToHost, ctx->stream));
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t j = 0; j < s->list.size(); ++j) {
+        const int i = s->list[j];
+        ctx->din[i].cone_th = t[i].cone_th;
+        ctx->din[i].cone_w = t[i].cone_w;
+    }
+    if (half_angle) {
+        const int cnt = inst ? n : (int)ctx->din.size();
+        for (int j = 0; j < cnt; ++j) half_angle[j] = ctx->din[inst ? inst[j] : j].cone_th;
+    }
+    return NMC_OK;
 }
 
 int nmc_set_path(nmc_ctx* ctx, int path) {
@@ -1184,6 +1294,13 @@ int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp
     Set* s;
     if (int rc = get_set(ctx, inst, n, &s)) return rc;
     if (int rc = ensure_partials(ctx, (size_t)s->n_mt * pstride_of(ctx))) return rc;
+    if (!ctx->graphs_on) {
+        for (int slot = 0; slot < nsamples; ++slot) {
+            if (int rc = sweep_launches(ctx, s, samp_sweeps)) return rc;
+            if (int rc = sample_launches(ctx, s, slot)) return rc;
+        }
+        return NMC_OK;
+    }
     auto key = std::make_tuple(s->list, nsamples, samp_sweeps);
     auto it = ctx->graphs.find(key);
     if (it == ctx->graphs.end()) {

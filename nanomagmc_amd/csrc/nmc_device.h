@@ -75,6 +75,7 @@ struct DevInst {
     int64_t tab_base;       // Ising: offset of the acceptance table
     uint32_t inst_id;       // RNG key component
     float beta, h, cone_w;  // beta = 1/kT ; field on h_ind ; 1 - cos(theta)
+    float cone_th;          // the cone half-angle theta (cone_w is computed from it on the device)
     double beta_d, h_d;
 };
 

@@ -453,6 +453,58 @@ __global__ __launch_bounds__(256) void k_measure_finalize(
 }
 
 // ---------------------------------------------------------------------------
+// Cone trials.  cone_w = 1 - cos(theta) is always computed here, on the
+// device, from the half-angle: by the set-cone path and by the adaptation
+// alike, so that both give the same bits for the same angle.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float cone_w_of(float th) { return (float)(1.0 - cos((double)th)); }
+
+__global__ void k_cone_w(DevInst* __restrict__ insts, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) insts[i].cone_w = cone_w_of(insts[i].cone_th);
+}
+
+// One adaptation step per listed instance (one workgroup each): its accepted
+// attempts since the last step (the LDS-path counter acc[inst] plus the
+// set's streaming-tile counters tacc[tidx[tptr[j] .. tptr[j + 1])]), which
+// are read and cleared, give the rate a / (sites * chunk); the half-angle is
+// scaled by rate / 0.5 clamped to [0.5, 2], then clamped to [0.02, pi].
+// chunk = 0 only clears the counters.  Each counter belongs to one instance,
+// so the workgroups never touch the same word; the counts are integers, so
+// the order of the sum does not matter.
+__global__ __launch_bounds__(256) void k_adapt_cone(DevInst* __restrict__ insts, const int* __restrict__ list,
+                                                    const int* __restrict__ tptr, const int* __restrict__ tidx,
+                                                    unsigned long long* __restrict__ tacc,
+                                                    unsigned long long* __restrict__ acc,
+                                                    const double* __restrict__ nsites, int chunk) {
+    __shared__ unsigned long long part[256];
+    const int j = blockIdx.x;
+    const int inst = list[j];
+    unsigned long long a = 0ull;
+    for (int k = tptr[j] + (int)threadIdx.x; k < tptr[j + 1]; k += 256) {
+        const int t = tidx[k];
+        a += tacc[t];
+        tacc[t] = 0ull;
+    }
+    part[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    a = part[0] + acc[inst];
+    acc[inst] = 0ull;
+    if (chunk <= 0) return;
+    DevInst& d = insts[inst];
+    const double rate = (double)a / fmax(1.0, nsites[j] * (double)chunk);
+    const double f = fmin(2.0, fmax(0.5, rate / 0.5));
+    const float th = (float)fmin(M_PI, fmax(0.02, (double)d.cone_th * f));
+    d.cone_th = th;
+    d.cone_w = cone_w_of(th);
+}
+
+// ---------------------------------------------------------------------------
 // Initial spins: up spin (set_default_spins, field_type.cpp:76-95) or a fresh
 // draw per site (all_rand, field_type.cpp:197-204); absent sites get zero.
 // ---------------------------------------------------------------------------
@@ -726,6 +778,20 @@ hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bo
     return hipGetLastError();
 }
 
+
+hipError_t launch_cone_w(hipStream_t stream, DevInst* insts, int n) {
+    if (n <= 0) return hipSuccess;
+    k_cone_w<<<(n + 255) / 256, 256, 0, stream>>>(insts, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_adapt_cone(hipStream_t stream, DevInst* insts, const int* list, int n, const int* tptr,
+                             const int* tidx, unsigned long long* tacc, unsigned long long* acc,
+                             const double* nsites, int chunk) {
+    if (n <= 0) return hipSuccess;
+    k_adapt_cone<<<n, 256, 0, stream>>>(insts, list, tptr, tidx, tacc, acc, nsites, chunk);
+    return hipGetLastError();
+}
 
 hipError_t launch_measure_finalize(hipStream_t stream, const DevInst* insts, const int* list, int n,
                                    const int* t_begin, const int* t_end, const double* partials,

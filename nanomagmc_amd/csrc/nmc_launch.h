@@ -59,6 +59,16 @@ hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bo
                                 const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                                 int a2max);
 
+// cone_w of instances 0 .. n-1 from their half-angles.
+hipError_t launch_cone_w(hipStream_t stream, DevInst* insts, int n);
+
+// One step of the cone half-angle adaptation for the n listed instances (see
+// k_adapt_cone); tptr/tidx list each one's streaming-tile counters, nsites its
+// present sites; chunk = sweeps since the last step (0: clear the counters).
+hipError_t launch_adapt_cone(hipStream_t stream, DevInst* insts, const int* list, int n, const int* tptr,
+                             const int* tidx, unsigned long long* tacc, unsigned long long* acc,
+                             const double* nsites, int chunk);
+
 hipError_t launch_measure_finalize(hipStream_t stream, const DevInst* insts, const int* list, int n,
                                    const int* t_begin, const int* t_end, const double* partials,
                                    int pstride, int a2max, bool dim3, bool tc, int nsl, double* obs,

@@ -164,6 +164,16 @@ class Lattices:
         a, n = _ilist(inst)
         check(lib().nmc_sweep(self._h, _iptr(a), n, int(sweeps)))
 
+    def equil_adapt(self, sweeps, chunk=10, inst=None):
+        """`sweeps` sweeps with the cone half-angles adapted on the device
+        after every `chunk` sweeps; returns the final half-angles."""
+        a, n = _ilist(inst)
+        cnt = n if a is not None else self.ninst
+        out = np.zeros(cnt, dtype=np.float32)
+        check(lib().nmc_sweep_adapt(self._h, _iptr(a), n, int(sweeps), int(chunk),
+                                    out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
     def set_path(self, path):
         """'auto' (LDS-resident sweeps where the lattice fits) or 'stream'."""
         check(lib().nmc_set_path(self._h, {"auto": 0, "stream": 1}[path]))

@@ -25,6 +25,7 @@
 #include <hdf5.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -544,27 +545,16 @@ int main(int argc, char** argv) {
     nmc_check(nmc_randomize(ctx, nullptr, 0), "nmc_randomize");
     // equilibration sweeps; cone half-angles adapt only here (detailed
     // balance holds within each fixed-angle stretch; sampling uses frozen angles)
+    // (every 10 sweeps: half-angle x clamp(rate / 0.5, 0.5, 2), on the device)
     auto equilibrate = [&](const std::vector<int>& ids, long sweeps) {
         if (!adapt) { nmc_check(nmc_sweep(ctx, ids.data(), (int)ids.size(), (int)sweeps), "nmc_sweep"); return; }
-        long done = 0;
-        std::vector<uint64_t> acc(ids.size());
-        while (done < sweeps) {
-            const int chunk = (int)std::min<long>(10, sweeps - done);
-            nmc_check(nmc_reset_accepts(ctx), "nmc_reset_accepts");
-            nmc_check(nmc_sweep(ctx, ids.data(), (int)ids.size(), chunk), "nmc_sweep");
-            nmc_check(nmc_get_accepts(ctx, ids.data(), (int)ids.size(), acc.data()), "nmc_get_accepts");
-            std::vector<float> th(ids.size());
-            for (size_t j = 0; j < ids.size(); ++j) {
-                int64_t n = 0;
-                nmc_num_sites(ctx, ids[j], &n);
-                const double rate = (double)acc[j] / std::max<double>(1.0, (double)n * chunk);
-                const double f = std::min(2.0, std::max(0.5, rate / 0.5));
-                cone[ids[j]] = (float)std::min(M_PI, std::max(0.02, cone[ids[j]] * f));
-                th[j] = cone[ids[j]];
-            }
-            nmc_check(nmc_set_cone(ctx, ids.data(), (int)ids.size(), th.data()), "nmc_set_cone");
-            done += chunk;
+        std::vector<float> th(ids.size());
+        const long piece = 10L * (INT_MAX / 10);
+        for (long done = 0; done < sweeps; done += piece) {
+            const int now = (int)std::min(piece, sweeps - done);
+            nmc_check(nmc_sweep_adapt(ctx, ids.data(), (int)ids.size(), now, 10, th.data()), "nmc_sweep_adapt");
         }
+        for (size_t j = 0; j < ids.size(); ++j) cone[ids[j]] = th[j];
     };
     // warm-up at (Tmax, H = 0) (main.cpp:130)
     equilibrate(bases, 20L * sim.Eq_steps);

@@ -328,6 +328,85 @@ def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, period
     assert np.array_equal(lats[0].accepts(), lats[1].accepts())
 
 
+@pytest.mark.parametrize("kind", ["stream", "lds", "grains"])
+def test_device_cone_adaptation_matches_host_steps(orc, bloch3d, kind):
+    # nmc_sweep_adapt adapts the half-angles on the device after every chunk;
+    # the same steps taken on the host (reset, sweep, read the accepts, new
+    # angle, nmc_set_cone) must give the same lattices and angles bit for bit.
+    # "stream": one 24^3 lattice (streamed, the captured-chunk path);
+    # "lds": LDS-resident lattices; "grains": compacted and streamed grains.
+    from nanomagmc_amd import Bond, Lattices
+    rng = np.random.default_rng(13)
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
+    bonds = [Bond(bd[k], J[k], D[k]) for k in range(len(J))]
+    if kind == "stream":
+        insts, periodic = [{"n": (24, 24, 24), "edge": 24}], True
+    elif kind == "lds":
+        insts, periodic = [{"n": (8, 8, 8), "edge": 8}] * 3, True
+    else:
+        insts, periodic = [], False
+        for n in (10, 7, 32):
+            insts.append({"n": (n,) * 3, "edge": n, "mask": (rng.random(n ** 3) < 0.75).astype(np.uint8)})
+    ni = len(insts)
+    betas = list(np.linspace(0.3, 3.0, ni))
+    lats = []
+    for _ in range(2):
+        lat = Lattices(3, False, periodic, bonds, insts, use_d=True, trial="cone", seed=57)
+        lat.set_params(betas, 0.05)
+        lat.set_cone([0.7] * ni)
+        lat.randomize()
+        lats.append(lat)
+    sweeps, chunk = 125, 10   # two captured graphs of six chunks (streamed set), then 5 sweeps
+    got = lats[0].equil_adapt(sweeps, chunk)
+    host = lats[1]
+    theta = np.full(ni, 0.7, dtype=np.float32)
+    sites = [host.num_sites(i) for i in range(ni)]
+    done = 0
+    while done < sweeps:
+        c = min(chunk, sweeps - done)
+        host.reset_accepts()
+        host.equil(c)
+        acc = host.accepts()
+        for i in range(ni):
+            rate = float(acc[i]) / max(1.0, float(sites[i]) * c)
+            f = min(2.0, max(0.5, rate / 0.5))
+            theta[i] = np.float32(min(math.pi, max(0.02, float(theta[i]) * f)))
+        host.set_cone(theta)
+        done += c
+    assert np.array_equal(got, theta)
+    assert len(set(got.tolist())) > 1 or ni == 1
+    for i in range(ni):
+        assert np.array_equal(lats[0].get_spins(i), host.get_spins(i)), i
+    assert np.all(lats[0].accepts() == 0)
+
+
+def test_graphs_and_direct_launches_agree(orc, bloch3d, monkeypatch):
+    # a small streamed set replays captured graphs (plain sweeps, adapted
+    # sweeps, the sample loop); NMC_NO_GRAPHS=1 launches the same kernels
+    # directly: lattices and series must agree bit for bit
+    from nanomagmc_amd import Bond, Lattices
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
+    bonds = [Bond(bd[k], J[k], D[k]) for k in range(len(J))]
+    runs = []
+    for off in ("0", "1"):
+        monkeypatch.setenv("NMC_NO_GRAPHS", off)
+        lat = Lattices(3, False, True, bonds, [{"n": (24, 24, 24), "edge": 24}] * 2, use_d=True, trial="cone",
+                       seed=61)
+        lat.set_params([0.5, 1.5], [0.0, 0.1])
+        lat.randomize()
+        lat.equil(150)
+        th = lat.equil_adapt(130, 10)
+        lat.series_reset(6, with_charge=True)
+        lat.run_samples(6, 3)
+        runs.append((lat, th, lat.get_series()))
+    monkeypatch.delenv("NMC_NO_GRAPHS")
+    assert np.array_equal(runs[0][1], runs[1][1])
+    for i in range(2):
+        assert np.array_equal(runs[0][0].get_spins(i), runs[1][0].get_spins(i)), i
+    for a, b in zip(runs[0][2], runs[1][2]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("dim,trial,table", [(3, "cone", "bloch"), (3, "sphere", "general"), (2, "sphere", "neel"),
                                               (2, "cone", "general")])
 def test_compacted_and_streamed_grains_agree(bloch3d, orc, dim, trial, table):
