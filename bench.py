@@ -116,16 +116,6 @@ def ising2d(trial, replicas=1):
 WORKLOADS = {"heis3d": heis3d, "grains": grains, "ising2d": ising2d}
 
 
-def adapt_cone(lat, theta, sweeps, sites):
-    """One adaptation round: sweep, read acceptance, scale half-angles."""
-    lat.reset_accepts()
-    lat.equil(sweeps)
-    acc = lat.accepts().astype(np.float64) / (sweeps * np.asarray(sites, dtype=np.float64))
-    theta = np.clip(theta * np.clip(acc / 0.5, 0.5, 2.0), 0.02, math.pi)
-    lat.set_cone(theta.astype(np.float32))
-    return theta, acc
-
-
 def copy_bandwidth(nbytes=1 << 30, reps=10):
     """Read + write bytes per second of device-to-device copies of a 1 GiB
     buffer (larger than the 256 MB Infinity Cache, so HBM-resident): the
@@ -276,18 +266,13 @@ def main():
         tuned = {"choice": list(choice), "seconds_per_candidate": timings}
     if args.path:
         lat.set_path(args.path)
-    # warm-up; cone half-angles adapt toward 50% acceptance, then freeze
-    theta = np.full(lat.ninst, 1.0)
+    # warm-up; cone half-angles adapt toward 50% acceptance every 10 sweeps
+    # (on the device), then freeze
     if trial == "cone":
-        lat.set_cone(theta.astype(np.float32))
-    done = 0
-    while done < args.warmup:
-        k = min(10, args.warmup - done)
-        if trial == "cone":
-            theta, _ = adapt_cone(lat, theta, k, sites)
-        else:
-            lat.equil(k)
-        done += k
+        lat.set_cone(np.full(lat.ninst, 1.0, dtype=np.float32))
+        lat.equil_adapt(args.warmup, 10)
+    else:
+        lat.equil(args.warmup)
     lat.reset_accepts()
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(lat.stream())

@@ -1328,20 +1328,27 @@ int nmc_get_series(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_
     if (!ctx || !obs) return fail(NMC_E_INVALID, "null argument");
     const int cnt = inst ? n : (int)ctx->din.size();
     const size_t ns = (size_t)ctx->series_n;
+    for (int j = 0; j < cnt; ++j)
+        if (int rc = check_inst(ctx, inst ? inst[j] : j)) return rc;
     NMC_HIP(hipStreamSynchronize(ctx->stream));
-    for (int j = 0; j < cnt; ++j) {
-        const int i = inst ? inst[j] : j;
-        if (int rc = check_inst(ctx, i)) return rc;
-        NMC_HIP(hipMemcpy(obs + (size_t)j * ns * 11, ctx->d_series + (size_t)i * ns * 11, ns * 11 * sizeof(double),
+    // one copy per run of consecutive instances (the series rows are in
+    // instance order)
+    const size_t qn = ns * ctx->nsl;
+    for (int j0 = 0; j0 < cnt;) {
+        const int i0 = inst ? inst[j0] : j0;
+        int j1 = j0 + 1;
+        while (j1 < cnt && (inst ? inst[j1] : j1) == i0 + (j1 - j0)) ++j1;
+        const size_t m = (size_t)(j1 - j0);
+        NMC_HIP(hipMemcpy(obs + (size_t)j0 * ns * 11, ctx->d_series + (size_t)i0 * ns * 11, m * ns * 11 * sizeof(double),
                           hipMemcpyDeviceToHost));
-        const size_t qn = ns * ctx->nsl;
         for (int w = 0; w < 2; ++w) {
             double* dst = w ? q_wrap : q_ref;
             if (!dst) continue;
             double* src = w ? ctx->d_series_qw : ctx->d_series_qr;
-            if (!ctx->series_q) std::fill(dst + j * qn, dst + (j + 1) * qn, 0.0);
-            else NMC_HIP(hipMemcpy(dst + j * qn, src + (size_t)i * qn, qn * sizeof(double), hipMemcpyDeviceToHost));
+            if (!ctx->series_q) std::fill(dst + j0 * qn, dst + j1 * qn, 0.0);
+            else NMC_HIP(hipMemcpy(dst + j0 * qn, src + (size_t)i0 * qn, m * qn * sizeof(double), hipMemcpyDeviceToHost));
         }
+        j0 = j1;
     }
     return NMC_OK;
 }

@@ -318,63 +318,97 @@ void write_slab(hid_t f, const char* name, int rank, const hsize_t* off, const h
 
 double norm4(double a, double b, double c) { return pow(pow(a, 2) + pow(b, 2) + pow(c, 2) + pow(0.0, 2), 0.5); }
 
-// print_TD_h5 (output.cpp:254-458) for one chain: obs[ns][11], q*[ns][nsl].
-void print_TD_h5(const StateOptions& st, const SimOptions& sim, int var1, int var2, int v2max, int latt,
-                 const double* obs, const double* qr, const double* qw, int nsl) {
-    int H, T;
-    h_t_of(sim.protocol, var1, var2, H, T);
-    const int ns = sim.N_samp;
+// print_TD_h5 (output.cpp:254-458) for every chain of one v2 step:
+// obs[a][ns][11] and q*[a][ns][nsl] for chain a = (lattice k, v1 index i) of
+// `active`.  The reference opens the file and writes one chain per call;
+// here the file is opened once per step and every dataset takes one
+// hyperslab per run of consecutive lattices with the same v1 index (a whole
+// v1 row in a fresh run).  The file's contents are the same.
+void print_TD_step(const StateOptions& st, const SimOptions& sim, int var2, int v2max, int nl, int v1n,
+                   const std::vector<int>& active, const double* obs, const double* qr, const double* qw, int nsl) {
+    const size_t ns = (size_t)sim.N_samp;
+    struct Chain { int i, k; size_t a; };
+    std::vector<Chain> ch;
+    for (size_t a = 0; a < active.size(); ++a)
+        ch.push_back({(active[a] - nl) % v1n, (active[a] - nl) / v1n, a});
+    std::sort(ch.begin(), ch.end(), [](const Chain& x, const Chain& y) { return x.i != y.i ? x.i < y.i : x.k < y.k; });
     hid_t f = H5Fopen(sim.outFile.c_str(), H5F_ACC_RDWR, H5P_DEFAULT);
-    hsize_t cnt[3] = {1, 1, (hsize_t)ns};
-    hsize_t off[3] = {(hsize_t)H, (hsize_t)T, (hsize_t)latt * ns};
-    std::vector<float> buf(ns);
-    auto put = [&](const char* name, auto fn) {
-        for (int s = 0; s < ns; ++s) buf[s] = (float)fn(obs + (size_t)s * NMC_NOBS);
-        write_slab(f, name, 3, off, cnt, buf.data());
-    };
-    // norm() over the 4-entry vector (functions.cpp:26-34); Ising keeps comp 0
-    put("/Fulldat/mags", [](const double* o) { return norm4(o[0], o[1], o[2]); });
-    put("/Fulldat/energies", [](const double* o) { return o[NMC_OBS_ER]; });
-    put("/Fulldat/sub_mags", [](const double* o) { return norm4(o[3], o[4], o[5]); });
-    put("/Fulldat/sub4_mags", [](const double* o) { return norm4(o[6], o[7], o[8]); });
-    put("/Fulldat/energies_s", [](const double* o) { return o[NMC_OBS_ES]; });
+    // the scalar series: norm() over the 4-entry vector (functions.cpp:26-34);
+    // Ising keeps comp 0 and has no component series
+    struct Series { const char* name; int kind; };   // kind < 0: norm of comps -kind-1.., else component
+    std::vector<Series> ser = {{"/Fulldat/mags", -1}, {"/Fulldat/energies", NMC_OBS_ER}, {"/Fulldat/sub_mags", -4},
+                               {"/Fulldat/sub4_mags", -7}, {"/Fulldat/energies_s", NMC_OBS_ES}};
     if (!st.isIsing) {
         const char* names[] = {"/Fulldat/mag_xs", "/Fulldat/mag_ys", "/Fulldat/mag_zs", "/Fulldat/sub_mag_xs",
                                "/Fulldat/sub_mag_ys", "/Fulldat/sub_mag_zs", "/Fulldat/sub4_mag_xs",
                                "/Fulldat/sub4_mag_ys", "/Fulldat/sub4_mag_zs"};
-        for (int c = 0; c < 9; ++c) {
-            for (int s = 0; s < ns; ++s) buf[s] = (float)obs[(size_t)s * NMC_NOBS + c];
-            write_slab(f, names[c], 3, off, cnt, buf.data());
-        }
+        for (int c = 0; c < 9; ++c) ser.push_back({names[c], c});
     }
-    if (!st.isIsing && !sim.distrib) {
-        const int edge = (int)st.edgeSize;
-        hsize_t c4[4] = {1, 1, 1, (hsize_t)ns};
-        for (int w = 0; w < 2; ++w) {
-            const double* q = w ? qw : qr;
-            for (int z = 0; z < edge; ++z) {
-                hsize_t o4[4] = {(hsize_t)H, (hsize_t)T, (hsize_t)z, (hsize_t)latt * ns};
-                // 2-D lattices have one charge slice; the other rows are 0
-                for (int s = 0; s < ns; ++s) buf[s] = z < nsl ? (float)q[(size_t)s * nsl + z] : 0.f;
-                write_slab(f, w ? "/Fulldat/top_chars_wrap" : "/Fulldat/top_chars", 4, o4, c4, buf.data());
-            }
-        }
-    }
-    if (var2 == v2max - 1) {    // checkpoint flag, same condition as output.cpp:429
-        hid_t ds = H5Dopen2(f, "/Complete", H5P_DEFAULT);
+    const bool charges = !st.isIsing && !sim.distrib;
+    const int edge = (int)st.edgeSize;
+    auto slab = [&](hid_t ds, int rank, const hsize_t* off, const hsize_t* cnt, hid_t type, const void* data) {
         hid_t sl = H5Dget_space(ds);
-        hsize_t c2[2] = {1, 1}, o2[2] = {(hsize_t)var1, (hsize_t)latt};
-        H5Sselect_hyperslab(sl, H5S_SELECT_SET, o2, NULL, c2, NULL);
-        hid_t mem = H5Screate_simple(2, c2, NULL);
-        int one = 1;
-        H5Dwrite(ds, H5T_NATIVE_INT, mem, sl, H5P_DEFAULT, &one);
+        H5Sselect_hyperslab(sl, H5S_SELECT_SET, off, NULL, cnt, NULL);
+        hid_t mem = H5Screate_simple(rank, cnt, NULL);
+        H5Dwrite(ds, type, mem, sl, H5P_DEFAULT, data);
         H5Sclose(mem);
         H5Sclose(sl);
-        H5Dclose(ds);
+    };
+    std::vector<hid_t> dss;
+    for (const Series& s : ser) dss.push_back(H5Dopen2(f, s.name, H5P_DEFAULT));
+    hid_t dtc[2] = {-1, -1};
+    if (charges)
+        for (int w = 0; w < 2; ++w) dtc[w] = H5Dopen2(f, w ? "/Fulldat/top_chars_wrap" : "/Fulldat/top_chars", H5P_DEFAULT);
+    hid_t dcomp = var2 == v2max - 1 ? H5Dopen2(f, "/Complete", H5P_DEFAULT) : -1;
+    std::vector<float> buf;
+    for (size_t r0 = 0; r0 < ch.size();) {
+        size_t r1 = r0 + 1;
+        while (r1 < ch.size() && ch[r1].i == ch[r0].i && ch[r1].k == ch[r1 - 1].k + 1) ++r1;
+        const size_t m = r1 - r0;
+        int H, T;
+        h_t_of(sim.protocol, ch[r0].i, var2, H, T);
+        const hsize_t off[3] = {(hsize_t)H, (hsize_t)T, (hsize_t)ch[r0].k * ns};
+        const hsize_t cnt[3] = {1, 1, (hsize_t)(m * ns)};
+        buf.resize(m * ns);
+        for (size_t d = 0; d < ser.size(); ++d) {
+            const int kind = ser[d].kind;
+            for (size_t r = 0; r < m; ++r)
+                for (size_t s = 0; s < ns; ++s) {
+                    const double* o = obs + (ch[r0 + r].a * ns + s) * NMC_NOBS;
+                    buf[r * ns + s] = (float)(kind < 0 ? norm4(o[-kind - 1], o[-kind], o[-kind + 1]) : o[kind]);
+                }
+            slab(dss[d], 3, off, cnt, H5T_NATIVE_FLOAT, buf.data());
+        }
+        if (charges) {
+            // [H][T][z][latt * ns]; 2-D lattices have one charge slice, the other rows are 0
+            const hsize_t o4[4] = {(hsize_t)H, (hsize_t)T, 0, (hsize_t)ch[r0].k * ns};
+            const hsize_t c4[4] = {1, 1, (hsize_t)edge, (hsize_t)(m * ns)};
+            buf.resize((size_t)edge * m * ns);
+            for (int w = 0; w < 2; ++w) {
+                const double* q = w ? qw : qr;
+                for (int z = 0; z < edge; ++z)
+                    for (size_t r = 0; r < m; ++r)
+                        for (size_t s = 0; s < ns; ++s)
+                            buf[((size_t)z * m + r) * ns + s] =
+                                z < nsl ? (float)q[(ch[r0 + r].a * ns + s) * nsl + z] : 0.f;
+                slab(dtc[w], 4, o4, c4, H5T_NATIVE_FLOAT, buf.data());
+            }
+        }
+        if (dcomp >= 0) {    // checkpoint flags, same condition as output.cpp:429
+            const hsize_t o2[2] = {(hsize_t)ch[r0].i, (hsize_t)ch[r0].k}, c2[2] = {1, (hsize_t)m};
+            std::vector<int> ones(m, 1);
+            slab(dcomp, 2, o2, c2, H5T_NATIVE_INT, ones.data());
+        }
+        for (size_t r = r0; r < r1; ++r)
+            std::cout << "\rPrinting H = " << H << ", T = " << T << ", lattice number " << ch[r].k << "              ";
+        std::cout << std::flush;
+        r0 = r1;
     }
+    for (hid_t d : dss) H5Dclose(d);
+    for (int w = 0; w < 2; ++w)
+        if (dtc[w] >= 0) H5Dclose(dtc[w]);
+    if (dcomp >= 0) H5Dclose(dcomp);
     H5Fclose(f);
-    std::cout << "\rPrinting H = " << H << ", T = " << T << ", lattice number " << latt << "              "
-              << std::flush;
 }
 
 // field_type::print (field_type.cpp:214-312): edge^d x (1|3) float array,
@@ -609,11 +643,11 @@ int main(int argc, char** argv) {
         }
         nmc_check(nmc_get_series(ctx, active.data(), (int)active.size(), obs.data(), tc ? qr.data() : nullptr,
                                  tc ? qw.data() : nullptr), "nmc_get_series");
+        print_TD_step(st, sim, j, pr.v2_size, nl, v1n, active, obs.data(), tc ? qr.data() : nullptr,
+                      tc ? qw.data() : nullptr, nsl);
         for (size_t a = 0; a < active.size(); ++a) {
             const int id = active[a];
             const int k = (id - nl) / v1n, i = (id - nl) % v1n;
-            print_TD_h5(st, sim, i, j, pr.v2_size, k, obs.data() + a * ns * NMC_NOBS,
-                        tc ? qr.data() + a * ns * nsl : nullptr, tc ? qw.data() + a * ns * nsl : nullptr, nsl);
             if (k == 0 && !sim.distrib) {
                 int H, T;
                 h_t_of(sim.protocol, i, j, H, T);

@@ -129,3 +129,24 @@ def test_weibull_grains_distributed(tmp_path):
     assert "/Fulldat/top_chars" not in sh and "/Av_Latt/T_0-H_0" not in sh
     m = h5_read(path, "/Fulldat/mags")
     assert np.all(m > 0)
+
+
+def test_series_blocks_sit_at_their_lattice(tmp_path):
+    # a step's chains are written one hyperslab per run of lattices: lattice
+    # 0's block of the component series must sum to its average lattice (the
+    # plain sum of its sampled spins, state::add_to_av), and the other
+    # lattices' blocks are other chains
+    d = make_case(tmp_path, ISISING=0, DMISTREN=0.3, INTERACTIONS="2d_nn_neel.dat", SIZE=10, NLATTS=3,
+                  temps=[1.5, 0.7], fields=[0.0, 0.1])
+    run(d)
+    path = os.path.join(d, "Output", "J_1-s_10-k_1-K_0.3-sh_s-per_1-pr_1.h5")
+    ns, nl = 5, 3
+    comps = [h5_read(path, "/Fulldat/mag_%ss" % c).reshape(2, 2, nl * ns) for c in "xyz"]
+    for H in range(2):
+        for T in range(2):
+            av = h5_read(path, "/Av_Latt/T_%d-H_%d" % (T, H)).reshape(-1, 3).sum(axis=0)
+            blk = np.array([c[H, T, :ns].sum() for c in comps])
+            assert np.allclose(blk, av, rtol=1e-4, atol=1e-2), (H, T, blk, av)
+            for k in (1, 2):
+                other = np.array([c[H, T, k * ns:(k + 1) * ns] for c in comps])
+                assert not np.array_equal(other, np.array([c[H, T, :ns] for c in comps]))
