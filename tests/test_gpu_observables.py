@@ -7,7 +7,11 @@ independent replicas, so tau_int is accounted for):
 * 2-D Heisenberg 6x6 periodic with Neel DMI and a field: <E_s>, <|M|>, C,
   chi and <|Q|> (plaquette charge with periodic wrap) against the oracle's
   random-site Metropolis chain (the reference's state::equil with
-  dE_func, Old/lib/state.cpp:171-203), run in FP64 on the host.
+  dE_func, Old/lib/state.cpp:171-203), run in FP64 on the host;
+* 3-D Heisenberg 4^3 periodic with Bloch DMI and a field, sampled with the
+  benchmark's cone trials (half-angles adapted on the device, then frozen):
+  <E_s>, <|M|>, C and chi against the same oracle chain (uniform-sphere
+  trials).
 
 Everything is seeded: device sweeps draw from Philox counters and the
 reductions have a fixed order, so the outcome is the same on every run.
@@ -126,3 +130,40 @@ def test_heisenberg_dmi_observables_match_oracle_chain(orc):
         print("%-5s device %.5f +- %.5f   oracle %.5f +- %.5f   (%.2f sigma)" % (k, m, e, mr, er, abs(m - mr) / sig))
         assert abs(m - mr) < 3 * sig, (k, m, e, mr, er)
     assert np.mean([p["absQ"] for p in ref]) > 0.05     # the charge does fluctuate at this T
+
+
+def test_heisenberg_3d_cone_observables_match_oracle_chain(orc, bloch3d):
+    # the benchmark's move: cone trials whose half-angles were adapted on the
+    # device (nmc_sweep_adapt) and then frozen, on a 3-D periodic lattice
+    # with Bloch DMI and a field; the oracle's chain uses the reference's
+    # uniform-sphere trials.  Both must sample the same distribution.
+    from nanomagmc_amd import Bond, Lattices
+    L, h, T = 4, 0.2, 1.2
+    beta = 1.0 / T
+    d, Jv, Dv = orc.parse_bonds(bloch3d, 3, 1.0, 0.3)
+    R, ns = 16, 6000
+    lat = Lattices(3, False, True, [Bond(d[k], Jv[k], Dv[k]) for k in range(len(Jv))],
+                   [{"n": (L, L, L), "edge": L}] * R, use_d=True, trial="cone", seed=0x4E4D4314)
+    lat.set_params([beta] * R, h)
+    lat.set_cone([1.0] * R)
+    lat.randomize()
+    th = lat.equil_adapt(400, 10)
+    assert np.all(th < 3.0)        # adapted away from the whole sphere at this T
+    lat.equil(100)
+    lat.series_reset(ns, with_charge=False)
+    lat.run_samples(ns, 5)
+    obs, _, _ = lat.get_series()
+    N = L ** 3
+    dev = [estimators(obs[i, :, 10], obs[i, :, 0:3], beta, N) for i in range(R)]
+    f = orc.Field(box_locs(3, L), 3, L, 1, d, Jv, Dv)
+    rng = np.random.default_rng(0x4E4D4315)
+    ref = []
+    for _ in range(8):
+        E, M, _ = oracle_chain(orc, f, [0.0, 0.0, h], beta, 2000, 5, rng)
+        ref.append(estimators(E, M, beta, N))
+    for k in ("E", "absM", "C", "chi"):
+        m, e = replica_stats([p[k] for p in dev])
+        mr, er = replica_stats([p[k] for p in ref])
+        sig = math.hypot(e, er)
+        print("%-5s device %.5f +- %.5f   oracle %.5f +- %.5f   (%.2f sigma)" % (k, m, e, mr, er, abs(m - mr) / sig))
+        assert abs(m - mr) < 3 * sig, (k, m, e, mr, er)
