@@ -187,18 +187,32 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const __bf16* __restrict__ X
     const int dy = tap / 3, dx = tap - dy * 3;
     const int n = lane & 31;
     const int mh = (lane >> 5) * 8;
+    // pixel mk + mh as (patch, row, column), advanced by carries instead of
+    // two divisions per loaded element (Hout >= 16, so a step of 16 pixels
+    // wraps at most one row)
+    const int HH = Hout * Hout;
+    int p0 = (mbeg + mh) / HH, r0 = mbeg + mh - p0 * HH;
+    int y0 = r0 / Hout, x0 = r0 - y0 * Hout;
     f32x16 acc = {};
     for (int mk = mbeg; mk < mend; mk += 16) {
         bf16x8 a = zero8(), b = zero8();
+        int pp = p0, yy = y0, xx = x0;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int m = mk + mh + j;
             if (m < mend) {
-                const int p = m / (Hout * Hout), r = m - p * Hout * Hout;
-                const int oy = r / Hout, ox = r - oy * Hout;
-                a[j] = X[(((size_t)p * Hin + oy + dy) * Hin + ox + dx) * Cin + ci];
+                a[j] = X[(((size_t)pp * Hin + yy + dy) * Hin + xx + dx) * Cin + ci];
                 b[j] = dY[(size_t)m * 32 + n];
             }
+            if (++xx == Hout) {
+                xx = 0;
+                if (++yy == Hout) { yy = 0; ++pp; }
+            }
+        }
+        x0 += 16;
+        if (x0 >= Hout) {
+            x0 -= Hout;
+            if (++y0 == Hout) { y0 = 0; ++p0; }
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
@@ -210,20 +224,34 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const __bf16* __restrict__ X
     }
 }
 
-// Sum the weight-gradient partials (fixed chunk order: deterministic).
+// Sum of the chunk partials v[c * stride], c < nchunks, in a fixed order:
+// chunk c into running sum c % 8 (eight independent loads in flight), then
+// the eight sums pairwise.  Deterministic.
+__device__ __forceinline__ float chunk_sum(const float* __restrict__ v, int nchunks, size_t stride) {
+    float r8[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) r8[r] = 0.f;
+    int c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) r8[r] += v[(size_t)(c + r) * stride];
+    }
+#pragma unroll
+    for (int r = 0; r < 7; ++r)
+        if (c + r < nchunks) r8[r] += v[(size_t)(c + r) * stride];
+    return ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
+}
+
+// Sum the weight-gradient partials (fixed order: deterministic).
 __global__ void k_grad_reduce(const float* __restrict__ Pw, int nchunks, int K, float* __restrict__ gW) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < K * 32) {
-        float s = 0.f;
-        for (int c = 0; c < nchunks; ++c) s += Pw[(size_t)c * K * 32 + i];
-        gW[i] = s;   // [k][n]
-    }
+    if (i < K * 32) gW[i] = chunk_sum(Pw + i, nchunks, (size_t)K * 32);   // [k][n]
 }
 
 // Bias gradient gB[n] = sum over m of dY[m][n] in two deterministic stages:
 // workgroup c sums rows [c * chunk, (c + 1) * chunk) (8 row groups x 32
 // channels, then a fixed-order sum of the groups) into Pb[c][n]; then the
-// chunk partials are summed in chunk order.
+// chunk partials are summed in a fixed order (chunk_sum).
 __global__ __launch_bounds__(256) void k_bias_partial(const __bf16* __restrict__ dY, int M, int chunk,
                                                       float* __restrict__ Pb) {
     __shared__ float part[8][33];
@@ -244,9 +272,7 @@ __global__ __launch_bounds__(256) void k_bias_partial(const __bf16* __restrict__
 __global__ void k_bias_grad(const float* __restrict__ Pb, int nchunks, float* __restrict__ gB) {
     const int n = threadIdx.x;
     if (n >= 32) return;
-    float t = 0.f;
-    for (int c = 0; c < nchunks; ++c) t += Pb[(size_t)c * 32 + n];
-    gB[n] = t;
+    gB[n] = chunk_sum(Pb + n, nchunks, 32);
 }
 
 // Adam on the fp32 master weights ([k][n] layout, columns >= nout frozen at
@@ -763,6 +789,7 @@ hipError_t launch_conv_dgrad(hipStream_t s, const __bf16* dY, int Hout, const __
 hipError_t launch_conv_wgrad(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* dY, int npatch,
                              int chunk, float* Pw, int* nchunks_out) {
     const int Hout = Hin - 2;
+    if (Hout < 16) return hipErrorInvalidValue;   // the kernel advances 16 pixels with one row carry
     const int M = npatch * Hout * Hout;
     const int nchunks = (M + chunk - 1) / chunk;
     const int tiles = nchunks * ((9 * Cin + 31) / 32);
@@ -799,11 +826,19 @@ __global__ void k_block_xy(int* __restrict__ bxy, int nbx, int nby, int off_x, i
 
 // A round's accepted blocks of one instance added to its running count (one
 // work-item, in block order: the only writer of *total in the launch).
-__global__ void k_block_acc_add(const int* __restrict__ acc, int nblocks, unsigned long long* __restrict__ total) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_block_acc_add(const int* __restrict__ acc, int nblocks,
+                                                       unsigned long long* __restrict__ total) {
+    // integer counts: the order of the sum does not matter
+    __shared__ unsigned long long part[256];
     unsigned long long t = 0;
-    for (int b = 0; b < nblocks; ++b) t += (unsigned long long)acc[b];
-    *total += t;
+    for (int b = threadIdx.x; b < nblocks; b += 256) t += (unsigned long long)acc[b];
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total += part[0];
 }
 
 hipError_t launch_block_xy(hipStream_t s, int* bxy, int nbx, int nby, int off_x, int off_y) {
@@ -813,7 +848,7 @@ hipError_t launch_block_xy(hipStream_t s, int* bxy, int nbx, int nby, int off_x,
 }
 
 hipError_t launch_block_acc_add(hipStream_t s, const int* acc, int nblocks, unsigned long long* total) {
-    k_block_acc_add<<<1, 64, 0, s>>>(acc, nblocks, total);
+    k_block_acc_add<<<1, 256, 0, s>>>(acc, nblocks, total);
     return hipGetLastError();
 }
 
