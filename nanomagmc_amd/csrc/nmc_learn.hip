@@ -168,59 +168,103 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const __bf16* __restrict__ d
 }
 
 // Partial weight gradients: chunk c of the reduction over output pixels m
-// writes Pw[c][k][n] for one 32-row slice of k.
-__global__ __launch_bounds__(256) void k_conv_wgrad(const __bf16* __restrict__ X, int Hin, int Cin,
-                                                    const __bf16* __restrict__ dY, int npatch, int chunk,
-                                                    float* __restrict__ Pw) {
+// (chunk pixels) for three 32-row K tiles of dW, one per wave.  Per step of
+// kWgStep pixels the workgroup gathers the im2col rows X(m, k) of its K range
+// and dY[m][n] with 16-byte loads (eight k, or eight n, of one pixel each),
+// and stores them k-major (Xt[k][m], Yt[n][m]) in LDS, two pixels per 32-bit
+// write.  A wave then reads each MFMA operand as one 16-byte LDS load: lane l
+// takes row l % 32 and the eight pixels 8 (l / 32) .. +7 of a 16-pixel
+// group.  Each tile therefore accumulates exactly the operands, in exactly
+// the pixel order, of a plain loop over the chunk 16 pixels at a time
+// (rows >= K repeat row K - 1 and are not stored; pixels >= the chunk's end
+// are zero).  Partials Pw[c][k][n].
+constexpr int kWgStep = 64;
+constexpr int kWgTiles = 3;   // K tiles (waves) per workgroup
+__global__ __launch_bounds__(64 * kWgTiles) void k_conv_wgrad(const __bf16* __restrict__ X, int Hin, int Cin,
+                                                              const __bf16* __restrict__ dY, int npatch, int chunk,
+                                                              float* __restrict__ Pw) {
+    constexpr int LD = kWgStep + 8;   // row stride of the staging arrays (bf16), 16-byte aligned rows
+    __shared__ __attribute__((aligned(16))) __bf16 Xt[32 * kWgTiles][LD];
+    __shared__ __attribute__((aligned(16))) __bf16 Yt[32][LD];
+    __shared__ int poff[kWgStep];     // element offset of pixel j's (patch, row, column) in X
     const int Hout = Hin - 2;
-    const int M = npatch * Hout * Hout;
+    const int HH = Hout * Hout;
+    const int M = npatch * HH;
     const int K = 9 * Cin;
     const int ktiles = (K + 31) / 32;   // K = 144 (first layer) leaves a half tile
+    const int ngroups = (ktiles + kWgTiles - 1) / kWgTiles;
+    const int c = blockIdx.x / ngroups, tg = blockIdx.x - c * ngroups;
+    const int k0 = tg * kWgTiles * 32;                 // first K row of this workgroup
+    const int krows = min(K, k0 + 32 * kWgTiles) - k0;   // rows staged (multiple of 8)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + wave;
-    const int c = tile / ktiles, tk = tile - c * ktiles;
-    const int mbeg = c * chunk;
-    if (mbeg >= M) return;
-    const int mend = min(M, mbeg + chunk);
-    const int krow = min(tk * 32 + (lane & 31), K - 1);   // rows >= K are computed, not stored
-    const int tap = krow / Cin, ci = krow - tap * Cin;
-    const int dy = tap / 3, dx = tap - dy * 3;
-    const int n = lane & 31;
-    const int mh = (lane >> 5) * 8;
-    // pixel mk + mh as (patch, row, column), advanced by carries instead of
-    // two divisions per loaded element (Hout >= 16, so a step of 16 pixels
-    // wraps at most one row)
-    const int HH = Hout * Hout;
-    int p0 = (mbeg + mh) / HH, r0 = mbeg + mh - p0 * HH;
-    int y0 = r0 / Hout, x0 = r0 - y0 * Hout;
+    const int tk = tg * kWgTiles + wave;               // this wave's K tile
+    const int mbeg = c * chunk, mend = min(M, mbeg + chunk);
+    const int nthr = 64 * kWgTiles;
+    const int cg = Cin / 8;                             // 8-channel groups per tap
     f32x16 acc = {};
-    for (int mk = mbeg; mk < mend; mk += 16) {
-        bf16x8 a = zero8(), b = zero8();
-        int pp = p0, yy = y0, xx = x0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int m = mk + mh + j;
+    for (int m0 = mbeg; m0 < mend; m0 += kWgStep) {
+        for (int j = threadIdx.x; j < kWgStep; j += nthr) {
+            const int m = m0 + j;
+            int off = -1;
             if (m < mend) {
-                a[j] = X[(((size_t)pp * Hin + yy + dy) * Hin + xx + dx) * Cin + ci];
-                b[j] = dY[(size_t)m * 32 + n];
+                const int p = m / HH, r = m - p * HH;
+                const int oy = r / Hout, ox = r - oy * Hout;
+                off = ((p * Hin + oy) * Hin + ox) * Cin;
             }
-            if (++xx == Hout) {
-                xx = 0;
-                if (++yy == Hout) { yy = 0; ++pp; }
+            poff[j] = off;
+        }
+        __syncthreads();
+        // X: items (pixel pair, 8-row group of the K range)
+        const int xg = krows / 8;
+        for (int it = threadIdx.x; it < (kWgStep / 2) * xg; it += nthr) {
+            const int jp = it / xg, g = it - jp * xg;
+            const int kk = k0 + 8 * g;                  // first row of the group
+            const int tap = kk / Cin, ci = kk - tap * Cin;
+            const int dy = tap / 3, dx = tap - dy * 3;
+            const int sh = (dy * Hin + dx) * Cin + ci;
+            bf16x8 v0 = zero8(), v1 = zero8();
+            const int o0 = poff[2 * jp], o1 = poff[2 * jp + 1];
+            if (o0 >= 0) v0 = *(const bf16x8*)(X + (size_t)o0 + sh);
+            if (o1 >= 0) v1 = *(const bf16x8*)(X + (size_t)o1 + sh);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                __bf16 pr[2] = {v0[e], v1[e]};
+                *(uint32_t*)&Xt[8 * g + e][2 * jp] = *(const uint32_t*)pr;
             }
         }
-        x0 += 16;
-        if (x0 >= Hout) {
-            x0 -= Hout;
-            if (++y0 == Hout) { y0 = 0; ++p0; }
+        // dY: items (pixel pair, 8-channel group)
+        for (int it = threadIdx.x; it < (kWgStep / 2) * 4; it += nthr) {
+            const int jp = it >> 2, g = it & 3;
+            bf16x8 v0 = zero8(), v1 = zero8();
+            const int m = m0 + 2 * jp;
+            if (m < mend) v0 = *(const bf16x8*)(dY + (size_t)m * 32 + 8 * g);
+            if (m + 1 < mend) v1 = *(const bf16x8*)(dY + (size_t)(m + 1) * 32 + 8 * g);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                __bf16 pr[2] = {v0[e], v1[e]};
+                *(uint32_t*)&Yt[8 * g + e][2 * jp] = *(const uint32_t*)pr;
+            }
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        __syncthreads();
+        if (tk < ktiles) {
+            const int krow = min(tk * 32 + (lane & 31), K - 1) - k0;
+            const int h = (lane >> 5) * 8;
+#pragma unroll
+            for (int s = 0; s < kWgStep; s += 16) {
+                if (m0 + s >= mend) break;
+                const bf16x8 a = *(const bf16x8*)&Xt[krow][s + h];
+                const bf16x8 b = *(const bf16x8*)&Yt[lane & 31][s + h];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
     }
+    if (tk >= ktiles) return;
     float* out = Pw + (size_t)c * K * 32;
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const int kr = tk * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (kr < K) out[(size_t)kr * 32 + n] = acc[v];
+        if (kr < K) out[(size_t)kr * 32 + (lane & 31)] = acc[v];
     }
 }
 
@@ -258,9 +302,17 @@ __global__ __launch_bounds__(256) void k_bias_partial(const __bf16* __restrict__
     const int c = blockIdx.x;
     const int n = threadIdx.x & 31, g = threadIdx.x >> 5;
     const int mbeg = c * chunk, mend = min(M, mbeg + chunk);
-    float s = 0.f;
-    for (int m = mbeg + g; m < mend; m += 8) s += (float)dY[(size_t)m * 32 + n];
-    part[g][n] = s;
+    // rows mbeg + g + 8 i into running sum i % 4 (independent loads in flight)
+    float s4[4] = {0.f, 0.f, 0.f, 0.f};
+    int m = mbeg + g;
+    for (; m + 24 < mend; m += 32) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s4[r] += (float)dY[(size_t)(m + 8 * r) * 32 + n];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        if (m + 8 * r < mend) s4[r] += (float)dY[(size_t)(m + 8 * r) * 32 + n];
+    part[g][n] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
     __syncthreads();
     if (threadIdx.x < 32) {
         float t = 0.f;
@@ -789,11 +841,11 @@ hipError_t launch_conv_dgrad(hipStream_t s, const __bf16* dY, int Hout, const __
 hipError_t launch_conv_wgrad(hipStream_t s, const __bf16* X, int Hin, int Cin, const __bf16* dY, int npatch,
                              int chunk, float* Pw, int* nchunks_out) {
     const int Hout = Hin - 2;
-    if (Hout < 16) return hipErrorInvalidValue;   // the kernel advances 16 pixels with one row carry
+    if (Hout < 1 || Cin % 8 != 0 || (32 * kWgTiles) % Cin != 0) return hipErrorInvalidValue;   // staging groups
     const int M = npatch * Hout * Hout;
     const int nchunks = (M + chunk - 1) / chunk;
-    const int tiles = nchunks * ((9 * Cin + 31) / 32);
-    k_conv_wgrad<<<(tiles + 3) / 4, 256, 0, s>>>(X, Hin, Cin, dY, npatch, chunk, Pw);
+    const int ngroups = ((9 * Cin + 31) / 32 + kWgTiles - 1) / kWgTiles;
+    k_conv_wgrad<<<nchunks * ngroups, 64 * kWgTiles, 0, s>>>(X, Hin, Cin, dY, npatch, chunk, Pw);
     *nchunks_out = nchunks;
     return hipGetLastError();
 }
