@@ -48,7 +48,7 @@ __device__ __forceinline__ bf16x8 zero8() {
 // activations resident in LDS (two regions used in turn: input -> R0,
 // layer 1 -> R1, layer 2 -> R0, layer 3 -> R1), so HBM sees only the input
 // patch, the weights (L2-resident) and the network output.  Each layer is an
-// implicit GEMM over 32 x 32 output tiles (wave w takes tiles w, w + 4, ...)
+// implicit GEMM over 32 x 32 output tiles (wave w of eight takes tiles w, w + 8, ...)
 // with fp32 accumulation; hidden activations are rounded to bf16 after bias
 // and ReLU, the last layer's output stays fp32.  The hidden activations are
 // also written to global memory when the backward pass needs them.
@@ -57,27 +57,35 @@ struct NetW {
     const float* b[4];
 };
 
+constexpr int kFwdWaves = 8;   // waves per patch workgroup
+
 template <int HIN, int CIN, bool LAST>
 __device__ __forceinline__ void fused_layer(const __bf16* X, const __bf16* __restrict__ W,
                                             const float* __restrict__ bias, __bf16* Y, __bf16* __restrict__ Yg,
                                             float* __restrict__ Yf, int wave, int lane) {
-    constexpr int HOUT = HIN - 2, M = HOUT * HOUT, K = 9 * CIN, TILES = (M + 31) / 32;
+    constexpr int HOUT = HIN - 2, M = HOUT * HOUT, K = 9 * CIN, TILES = (M + 31) / 32, KS = K / 16;
     const int n = lane & 31, kh = (lane >> 5) * 8;
-    for (int tile = wave; tile < TILES; tile += 4) {
+    if (wave >= TILES) return;
+    // the layer's weight operands, loaded once (all in flight together) and
+    // kept in registers for every tile of this wave
+    bf16x8 bw[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) bw[s] = *(const bf16x8*)(W + (size_t)n * K + 16 * s + kh);
+    for (int tile = wave; tile < TILES; tile += kFwdWaves) {
         const int m0 = tile * 32;
         const int m = m0 + (lane & 31);
         const bool mv = m < M;
         int oy = 0, ox = 0;
         if (mv) { oy = m / HOUT; ox = m - oy * HOUT; }
         f32x16 acc = {};
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            const int k = k0 + kh;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 16 * s + kh;
             const int tap = k / CIN, ci = k - tap * CIN;
             const int dy = tap / 3, dx = tap - dy * 3;
             bf16x8 a = zero8();
             if (mv) a = *(const bf16x8*)(X + ((oy + dy) * HIN + ox + dx) * CIN + ci);
-            const bf16x8 bv = *(const bf16x8*)(W + (size_t)n * K + k);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[s], acc, 0, 0, 0);
         }
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
@@ -98,7 +106,7 @@ __device__ __forceinline__ void fused_layer(const __bf16* X, const __bf16* __res
 constexpr int kR0 = 20 * 20 * 32;   // bf16 elements: input patch (24*24*16) and layer-2 output
 constexpr int kR1 = 22 * 22 * 32;   // layer-1 output and layer-3 output (18*18*32)
 
-__global__ __launch_bounds__(256) void k_net_fwd(const __bf16* __restrict__ X0, int npatch, NetW w,
+__global__ __launch_bounds__(64 * kFwdWaves) void k_net_fwd(const __bf16* __restrict__ X0, int npatch, NetW w,
                                                  __bf16* __restrict__ y1g, __bf16* __restrict__ y2g,
                                                  __bf16* __restrict__ y3g, float* __restrict__ O) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -108,7 +116,7 @@ __global__ __launch_bounds__(256) void k_net_fwd(const __bf16* __restrict__ X0, 
     if (p >= npatch) return;
     const uint4* src = (const uint4*)(X0 + (size_t)p * (24 * 24 * 16));
     uint4* dst = (uint4*)R0;
-    for (int i = threadIdx.x; i < 24 * 24 * 16 / 8; i += 256) dst[i] = src[i];
+    for (int i = threadIdx.x; i < 24 * 24 * 16 / 8; i += 64 * kFwdWaves) dst[i] = src[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     fused_layer<24, 16, false>(R0, w.w[0], w.b[0], R1, y1g ? y1g + (size_t)p * (22 * 22 * 32) : nullptr, nullptr,
@@ -826,7 +834,7 @@ hipError_t launch_net_fwd(hipStream_t s, const __bf16* X0, int npatch, const __b
     if (npatch <= 0) return hipSuccess;
     NetW w;
     for (int l = 0; l < 4; ++l) { w.w[l] = W[l]; w.b[l] = b[l]; }
-    k_net_fwd<<<npatch, 256, (size_t)(kR0 + kR1) * sizeof(__bf16), s>>>(X0, npatch, w, y1, y2, y3, O);
+    k_net_fwd<<<npatch, 64 * kFwdWaves, (size_t)(kR0 + kR1) * sizeof(__bf16), s>>>(X0, npatch, w, y1, y2, y3, O);
     return hipGetLastError();
 }
 
