@@ -677,3 +677,11 @@ def test_invalid_arguments_raise(orc):
         Lattices(2, True, True, bonds, [{"n": (5, 5), "edge": 5}])   # odd periodic edge
     with pytest.raises(NmcError):
         Lattices(2, True, False, [Bond((2, 0), 1.0)], [{"n": (4, 4), "edge": 4}])
+    with pytest.raises(NmcError):
+        lat2.equil_adapt(10, 10)           # cone adaptation needs Heisenberg spins
+    heis = Lattices(2, False, True, bonds, [{"n": (4, 4), "edge": 4}], trial="cone")
+    with pytest.raises(NmcError):
+        heis.equil_adapt(10, 0)            # chunk must be > 0
+    with pytest.raises(NmcError):
+        heis.set_cone([0.0])               # half-angle outside (0, pi]
+    assert heis.equil_adapt(0, 10).shape == (1,)
