@@ -70,8 +70,28 @@ __device__ __forceinline__ void load_spin(const void* pool, const DevInst& in, i
     }
 }
 
+// v of lane l moved through the data-parallel-primitive crossbar (`ctrl` one
+// of the DPP patterns below); lanes outside row_mask read 0.0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the wave, complete in lane 63, in a fixed order: pairs and quads
+// (quad permutations), half rows (row_half_mirror), rows of 16
+// (row_mirror), then row 0 into row 1 and row 2 into row 3 (row_bcast15),
+// then rows 0-1 into rows 2-3 (row_bcast31).  Register moves only, no LDS.
 __device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+#pragma clang fp contract(off)
+    v += dpp_f64<0xB1, 0xF>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E, 0xF>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141, 0xF>(v);   // row_half_mirror
+    v += dpp_f64<0x140, 0xF>(v);   // row_mirror
+    v += dpp_f64<0x142, 0xA>(v);   // row_bcast15
+    v += dpp_f64<0x143, 0xC>(v);   // row_bcast31
     return v;
 }
 
@@ -191,7 +211,7 @@ __device__ __forceinline__ void block_reduce_out(double* acc, double* red, int w
 #pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 15; ++i) acc[i] = wave_sum(acc[i]);
-    if (lane == 0)
+    if (lane == 63)
         for (int i = 0; i < 15; ++i) red[wv * 17 + i] = acc[i];
     __syncthreads();
     if (threadIdx.x < 15) {
@@ -207,7 +227,7 @@ __device__ __forceinline__ void block_reduce_q2(double qr, double qw, double* re
 #pragma clang fp contract(off)
     qr = wave_sum(qr);
     qw = wave_sum(qw);
-    if (lane == 0) {
+    if (lane == 63) {
         red[wv * 17 + 15] = qr;
         red[wv * 17 + 16] = qw;
     }
