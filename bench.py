@@ -244,8 +244,8 @@ def main():
         print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 5}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: the engine runs independent chains per GPU and this "
-              "build does not launch multi-GPU runs (use --gpus 1).", file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs independent chains per GPU and "
+              "multi-GPU runs are not launched (use --gpus 1).", file=sys.stderr)
         sys.exit(2)
     import torch
     trial = args.trial or ("cone" if args.workload == "heis3d" else "sphere")

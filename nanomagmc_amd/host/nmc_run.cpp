@@ -224,7 +224,7 @@ void mkds(hid_t f, const char* name, int rank, const hsize_t* dims, hid_t type) 
     H5Sclose(sp);
 }
 
-// create_h5_file (output.cpp:80-252), plus the build's extra datasets
+// create_h5_file (output.cpp:80-252), plus two extra datasets
 // /Fulldat/energies_s (the energy the sampler's dE belongs to) and
 // /Fulldat/top_chars_wrap (charge counting triangles across a periodic wrap).
 void create_h5_file(const StateOptions& st, const SimOptions& sim, const std::vector<float>& Ts,
@@ -652,7 +652,7 @@ int main(int argc, char** argv) {
                 int H, T;
                 h_t_of(sim.protocol, i, j, H, T);
                 // av_latt_name has no PROTOCOL 4 case (print_latt.cpp:11-21):
-                // its H/T stay uninitialised there; this build uses PROTOCOL 2's
+                // its H/T stay uninitialised there; PROTOCOL 2's naming is used
                 const std::string an = "/Av_Latt/T_" + std::to_string(T) + "-H_" + std::to_string(H);
                 const std::string sn = "/Sing_Latt/T_" + std::to_string(T) + "-H_" + std::to_string(H);
                 std::vector<double> avg(latts[0].box_index.size() * 3), sp(avg.size());
