@@ -130,6 +130,7 @@ struct nmc_ctx {
     int zslots = 4;          // stencil slots used (4 in 2-D, 6 in 3-D)
     int dm = 0;              // DMI form of the sweep kernels (dmi_axis in nmc_device.h), from the fp32 D
     int dm_meas = 0;         // the same form from the fp64 D of the measurement
+    bool bond_sym = false;   // J equal and D opposite for every bond and its reverse: per-bond energy sums
     int tabsize = 0;         // Ising acceptance-table entries per instance
     int nsl = 1;             // topological-charge slices
     int a2max = 2;
@@ -632,11 +633,11 @@ int pstride_of(const nmc_ctx* c) {
 int measure_launches(nmc_ctx* c, Set* s, bool tc) {
     const int ps = pstride_of(c);
     if (c->masked)
-        NMC_HIP(launch_measure_sites(c->stream, c->dim3, c->periodic, c->ising, c->dm_meas, tc, c->d_inst, s->d_mt,
+        NMC_HIP(launch_measure_sites(c->stream, c->dim3, c->periodic, c->ising, c->dm_meas, tc, c->bond_sym, c->d_inst, s->d_mt,
                                      s->n_mt, s->d_msites, s->d_mrank, s->d_mzoff, c->pool, c->mpool, c->st,
                                      c->d_partials, ps, c->a2max));
     else
-        NMC_HIP(launch_measure(c->stream, c->dim3, c->periodic, c->masked, c->ising, c->dm_meas, tc, c->d_inst,
+        NMC_HIP(launch_measure(c->stream, c->dim3, c->periodic, c->masked, c->ising, c->dm_meas, tc, c->bond_sym, c->d_inst,
                                s->d_mt, s->n_mt, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
     return NMC_OK;
 }
@@ -717,6 +718,14 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
             }
         }
         c->dm_meas = !anyd ? 0 : (axiald ? 2 : 1);
+    }
+    {
+        bool sym = true;
+        for (int g = c->dim3 ? 0 : 2; g < 6; g += 2) {
+            sym = sym && c->st.Jd[g] == c->st.Jd[g + 1];
+            for (int a = 0; a < 3; ++a) sym = sym && c->st.Dd[g][a] == -c->st.Dd[g + 1][a];
+        }
+        c->bond_sym = sym;
     }
     for (int g = 0, p = 1; g < 6; ++g) {
         if (!c->dim3 && g < 2) { c->st.pow3[g] = 0; continue; }
@@ -1232,7 +1241,7 @@ int nmc_measure(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_ref
     if (int rc = run_measure(ctx, s, tc)) return rc;
     NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, cnt, s->d_tb, s->d_te, ctx->d_partials,
                                     pstride_of(ctx), ctx->a2max, ctx->dim3, tc, ctx->nsl, ctx->d_obs,
-                                    dq_r, dq_w, false, 1, 0, ctx->ising ? 0 : 2));
+                                    dq_r, dq_w, false, 1, 0, ctx->ising ? 0 : 2, ctx->bond_sym));
     std::vector<double> h(need);
     NMC_HIP(hipMemcpyAsync(h.data(), ctx->d_obs, need * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     NMC_HIP(hipStreamSynchronize(ctx->stream));
@@ -1272,7 +1281,7 @@ int sample_launches(nmc_ctx* ctx, Set* s, int slot) {
     NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, (int)s->list.size(), s->d_tb, s->d_te,
                                     ctx->d_partials, pstride_of(ctx), ctx->a2max, ctx->dim3, ctx->series_q, ctx->nsl,
                                     ctx->d_series, ctx->d_series_qr, ctx->d_series_qw, true, ctx->series_n, slot,
-                                    ctx->ising ? 0 : 2));
+                                    ctx->ising ? 0 : 2, ctx->bond_sym));
     return NMC_OK;
 }
 }  // namespace

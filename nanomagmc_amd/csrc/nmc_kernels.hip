@@ -112,7 +112,14 @@ __device__ __forceinline__ bool link_in_box(const DevInst& in, const SitePos& p,
 // the DMI form of the sweep kernels (0 none, 1 general, 2 axial: D of slot g
 // along dmi_axis(g)); the forms give the same sums, since the terms they
 // leave out are exact zeros.
-template <bool DIM3, bool PERIODIC, bool ISING, int DM>
+//
+// BOND (tables with J equal and D opposite for a bond and its reverse): the
+// local-field sums are taken per bond instead, over the + slots only: acc 9
+// gets sum_bonds J s.n + s.(D x n) and acc 10 the same with -D.  Summed over
+// the lattice these are half of sum_a acc[9 + a] and sum_a acc[12 + a] of
+// the per-site form (each bond is seen from both ends there), so the energy
+// is the same; acc 11-14 stay 0.
+template <bool DIM3, bool PERIODIC, bool ISING, int DM, bool BOND>
 __device__ __forceinline__ void site_sums(const DevInst& in, const void* __restrict__ pool, const Stencil& st,
                                           int c, int64_t e, const SitePos& p, double* acc) {
 #pragma clang fp contract(off)
@@ -135,6 +142,28 @@ __device__ __forceinline__ void site_sums(const DevInst& in, const void* __restr
             (crem(possum, 4) == 2 && crem(posdiff, 4) == 2 && crem(kz, 4) == 2)) {
             acc[6] += s0; acc[7] += s1; acc[8] += s2;
         }
+    }
+    if (BOND) {
+        double bE = 0.0, bS = 0.0;
+#pragma unroll
+        for (int g = DIM3 ? 1 : 3; g < 6; g += 2) {
+            const int ne = nbr_elem<PERIODIC>(in, p, g);
+            if (ne < 0) continue;
+            double n0, n1, n2;
+            load_spin<ISING>(pool, in, c ^ 1, ne, n0, n1, n2);
+            const double x = st.Jd[g] * ((s0 * n0 + s1 * n1) + s2 * n2);
+            double y = 0.0;
+            if (DM != 0) {
+                const double* D = st.Dd[g];
+                const double y0 = D[1] * n2 - D[2] * n1, y1 = D[2] * n0 - D[0] * n2, y2 = D[0] * n1 - D[1] * n0;
+                y = (s0 * y0 + s1 * y1) + s2 * y2;
+            }
+            bE += x + y;
+            bS += x - y;
+        }
+        acc[9] += bE;
+        acc[10] += bS;
+        return;
     }
     // local fields: X = sum J s_j, Y = sum D x s_j
     double X0 = 0, X1 = 0, X2 = 0, Y0 = 0, Y1 = 0, Y2 = 0;
@@ -238,7 +267,7 @@ __device__ __forceinline__ void block_reduce_q2(double qr, double qw, double* re
     }
 }
 
-template <bool DIM3, bool PERIODIC, bool ISING, int DM, bool TC>
+template <bool DIM3, bool PERIODIC, bool ISING, int DM, bool TC, bool BOND>
 __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ insts,
                                                  const MTile* __restrict__ tiles,
                                                  const void* __restrict__ pool,
@@ -287,9 +316,9 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
 #pragma unroll
         for (int i = 0; i < 15; ++i) acc[i] = 0.0;
         each_site([&](int c, int64_t e, const SitePos& p) {
-            site_sums<DIM3, PERIODIC, ISING, DM>(in, pool, st, c, e, p, acc);
+            site_sums<DIM3, PERIODIC, ISING, DM, BOND>(in, pool, st, c, e, p, acc);
         });
-        if (DM == 0) { acc[12] = acc[9]; acc[13] = acc[10]; acc[14] = acc[11]; }
+        if (!BOND && DM == 0) { acc[12] = acc[9]; acc[13] = acc[10]; acc[14] = acc[11]; }
         block_reduce_out(acc, red, wv, lane, out);
     }
     if (!TC) return;
@@ -332,7 +361,7 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
 // zoff[z] .. zoff[z + 1], i.e. in list order.  Every sum runs in a fixed
 // order.
 // ---------------------------------------------------------------------------
-template <bool DIM3, bool PERIODIC, bool ISING, int DM, bool TC>
+template <bool DIM3, bool PERIODIC, bool ISING, int DM, bool TC, bool BOND>
 __global__ __launch_bounds__(256) void k_measure_sites(const DevInst* __restrict__ insts,
                                                        const MTile* __restrict__ tiles,
                                                        const uint32_t* __restrict__ sites,
@@ -358,9 +387,9 @@ __global__ __launch_bounds__(256) void k_measure_sites(const DevInst* __restrict
             const uint32_t ent = sites[(int64_t)tl.row0 + j];
             const int c = (int)(ent >> 31);
             const uint32_t e = ent & 0x7FFFFFFFu;
-            site_sums<DIM3, PERIODIC, ISING, DM>(in, pool, st, c, e, site_of(in, e, c, DIM3), acc);
+            site_sums<DIM3, PERIODIC, ISING, DM, BOND>(in, pool, st, c, e, site_of(in, e, c, DIM3), acc);
         }
-        if (DM == 0) { acc[12] = acc[9]; acc[13] = acc[10]; acc[14] = acc[11]; }
+        if (!BOND && DM == 0) { acc[12] = acc[9]; acc[13] = acc[10]; acc[14] = acc[11]; }
         block_reduce_out(acc, red, wv, lane, out);
     }
     if (!TC) return;
@@ -406,7 +435,7 @@ __global__ __launch_bounds__(256) void k_measure_finalize(
     const DevInst* __restrict__ insts, const int* __restrict__ list, const int* __restrict__ t_begin,
     const int* __restrict__ t_end, const double* __restrict__ partials, int pstride, int a2max,
     int dim3, int tc, int nsl, double* __restrict__ obs, double* __restrict__ qref,
-    double* __restrict__ qwrap, int row_by_inst, int64_t row_mul, int64_t row_add, int hidx) {
+    double* __restrict__ qwrap, int row_by_inst, int64_t row_mul, int64_t row_add, int hidx, int bond) {
 #pragma clang fp contract(off)
     __shared__ double sum[15];
     const int li = blockIdx.x;
@@ -461,14 +490,21 @@ __global__ __launch_bounds__(256) void k_measure_finalize(
         for (int i = 0; i < 9; ++i) o[i] = sum[i];
         double H[3] = {0.0, 0.0, 0.0};
         H[hidx] = in.h_d;
-        // E_func: res[a] = -H[a] M[a] - 0.5 Vsum[a]; E = res[0] + res[1] + res[2]
-        double r[3], q[3];
-        for (int a = 0; a < 3; ++a) {
-            r[a] = -H[a] * sum[a] - 0.5 * sum[9 + a];
-            q[a] = -H[a] * sum[a] - 0.5 * sum[12 + a];
+        if (bond) {
+            // per-bond sums: E = -H.M - sum over bonds (each bond once)
+            const double hm = (-H[0] * sum[0] + -H[1] * sum[1]) + -H[2] * sum[2];
+            o[9] = hm - sum[9];
+            o[10] = hm - sum[10];
+        } else {
+            // E_func: res[a] = -H[a] M[a] - 0.5 Vsum[a]; E = res[0] + res[1] + res[2]
+            double r[3], q[3];
+            for (int a = 0; a < 3; ++a) {
+                r[a] = -H[a] * sum[a] - 0.5 * sum[9 + a];
+                q[a] = -H[a] * sum[a] - 0.5 * sum[12 + a];
+            }
+            o[9] = r[0] + r[1] + r[2];
+            o[10] = q[0] + q[1] + q[2];
         }
-        o[9] = r[0] + r[1] + r[2];
-        o[10] = q[0] + q[1] + q[2];
     }
 }
 
@@ -744,7 +780,7 @@ __global__ __launch_bounds__(256) void k_build_weibull(int dim, int edge, const 
 // Host-side launchers (declared in nmc_launch.h).
 // ---------------------------------------------------------------------------
 hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising, int dm,
-                          bool tc, const DevInst* insts, const MTile* t, int ntiles, const void* pool,
+                          bool tc, bool bond, const DevInst* insts, const MTile* t, int ntiles, const void* pool,
                           const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                           int a2max) {
     if (ntiles <= 0) return hipSuccess;
@@ -755,24 +791,26 @@ hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool mas
     // gmax = 64 / (a2max / 2) covers the longest rows exactly
     const int gmax = std::max(1, std::min(64, 128 / std::max(2, a2max)));
     const size_t shm = (size_t)((tc && dim3 ? 8 * gmax * a2max : 0) + 4 * 17) * sizeof(double);
-#define NMC_MEAS(D3, PER, IS, DMF, T)                                                                      \
-    if (dim3 == D3 && periodic == PER && ising == IS && dm == DMF && tc == T)                             \
-        k_measure<D3, PER, IS, DMF, T><<<ntiles, 256, shm, stream>>>(insts, t, pool, mpool, st, partials, \
-                                                                    pstride, a2max, gmax);
-#define NMC_MEAS_B(D3, PER)                                                              \
-    NMC_MEAS(D3, PER, true, 0, false)                                                    \
-    NMC_MEAS(D3, PER, false, 0, false) NMC_MEAS(D3, PER, false, 1, false) NMC_MEAS(D3, PER, false, 2, false) \
-    NMC_MEAS(D3, PER, false, 0, true) NMC_MEAS(D3, PER, false, 1, true) NMC_MEAS(D3, PER, false, 2, true)
+#define NMC_MEAS(D3, PER, IS, DMF, T, BD)                                                                \
+    if (dim3 == D3 && periodic == PER && ising == IS && dm == DMF && tc == T && bond == BD)             \
+        k_measure<D3, PER, IS, DMF, T, BD><<<ntiles, 256, shm, stream>>>(insts, t, pool, mpool, st, partials, \
+                                                                        pstride, a2max, gmax);
+#define NMC_MEAS_T(D3, PER, BD)                                                                          \
+    NMC_MEAS(D3, PER, true, 0, false, BD)                                                                \
+    NMC_MEAS(D3, PER, false, 0, false, BD) NMC_MEAS(D3, PER, false, 1, false, BD) NMC_MEAS(D3, PER, false, 2, false, BD) \
+    NMC_MEAS(D3, PER, false, 0, true, BD) NMC_MEAS(D3, PER, false, 1, true, BD) NMC_MEAS(D3, PER, false, 2, true, BD)
+#define NMC_MEAS_B(D3, PER) NMC_MEAS_T(D3, PER, false) NMC_MEAS_T(D3, PER, true)
     NMC_MEAS_B(true, true)
     NMC_MEAS_B(true, false)
     NMC_MEAS_B(false, true)
     NMC_MEAS_B(false, false)
 #undef NMC_MEAS_B
+#undef NMC_MEAS_T
 #undef NMC_MEAS
     return hipGetLastError();
 }
 
-hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, int dm, bool tc,
+hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, int dm, bool tc, bool bond,
                                 const DevInst* insts, const MTile* t, int ntiles, const uint32_t* sites,
                                 const uint16_t* rank, const uint16_t* zoff, const void* pool,
                                 const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
@@ -781,23 +819,24 @@ hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bo
     if (dm < 0 || dm > 2) return hipErrorInvalidValue;
     if (ising) { dm = 0; tc = false; }
     if (tc && dim3 && (!rank || !zoff)) return hipErrorInvalidValue;
-#define NMC_MEAS_S(D3, PER, IS, DMF, T)                                                                    \
-    if (dim3 == D3 && periodic == PER && ising == IS && dm == DMF && tc == T)                             \
-        k_measure_sites<D3, PER, IS, DMF, T><<<ntiles, 256, 0, stream>>>(insts, t, sites, rank, zoff, pool, mpool, \
-                                                                        st, partials, pstride, a2max);
-#define NMC_MEAS_SB(D3, PER)                                                                   \
-    NMC_MEAS_S(D3, PER, true, 0, false)                                                        \
-    NMC_MEAS_S(D3, PER, false, 0, false) NMC_MEAS_S(D3, PER, false, 1, false) NMC_MEAS_S(D3, PER, false, 2, false) \
-    NMC_MEAS_S(D3, PER, false, 0, true) NMC_MEAS_S(D3, PER, false, 1, true) NMC_MEAS_S(D3, PER, false, 2, true)
+#define NMC_MEAS_S(D3, PER, IS, DMF, T, BD)                                                                   \
+    if (dim3 == D3 && periodic == PER && ising == IS && dm == DMF && tc == T && bond == BD)                  \
+        k_measure_sites<D3, PER, IS, DMF, T, BD><<<ntiles, 256, 0, stream>>>(insts, t, sites, rank, zoff, pool,  \
+                                                                            mpool, st, partials, pstride, a2max);
+#define NMC_MEAS_ST(D3, PER, BD)                                                                         \
+    NMC_MEAS_S(D3, PER, true, 0, false, BD)                                                              \
+    NMC_MEAS_S(D3, PER, false, 0, false, BD) NMC_MEAS_S(D3, PER, false, 1, false, BD) NMC_MEAS_S(D3, PER, false, 2, false, BD) \
+    NMC_MEAS_S(D3, PER, false, 0, true, BD) NMC_MEAS_S(D3, PER, false, 1, true, BD) NMC_MEAS_S(D3, PER, false, 2, true, BD)
+#define NMC_MEAS_SB(D3, PER) NMC_MEAS_ST(D3, PER, false) NMC_MEAS_ST(D3, PER, true)
     NMC_MEAS_SB(true, true)
     NMC_MEAS_SB(true, false)
     NMC_MEAS_SB(false, true)
     NMC_MEAS_SB(false, false)
 #undef NMC_MEAS_SB
+#undef NMC_MEAS_ST
 #undef NMC_MEAS_S
     return hipGetLastError();
 }
-
 
 hipError_t launch_cone_w(hipStream_t stream, DevInst* insts, int n) {
     if (n <= 0) return hipSuccess;
@@ -817,11 +856,11 @@ hipError_t launch_measure_finalize(hipStream_t stream, const DevInst* insts, con
                                    const int* t_begin, const int* t_end, const double* partials,
                                    int pstride, int a2max, bool dim3, bool tc, int nsl, double* obs,
                                    double* qref, double* qwrap, bool row_by_inst, int64_t row_mul,
-                                   int64_t row_add, int hidx) {
+                                   int64_t row_add, int hidx, bool bond) {
     if (n <= 0) return hipSuccess;
     k_measure_finalize<<<n, 256, 0, stream>>>(insts, list, t_begin, t_end, partials, pstride, a2max,
                                               dim3 ? 1 : 0, tc ? 1 : 0, nsl, obs, qref, qwrap,
-                                              row_by_inst ? 1 : 0, row_mul, row_add, hidx);
+                                              row_by_inst ? 1 : 0, row_mul, row_add, hidx, bond ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -42,9 +42,11 @@ hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, 
 // Observable partials of full boxes.  dm = DMI form of the stencil (0 none,
 // 1 general, 2 axial, as for the sweeps); tc = compute the topological
 // charges (Heisenberg only; without it the per-slice partials are not
-// written and the finalize must not read them).
+// written and the finalize must not read them); bond = local-field sums per
+// bond (tables with J equal and D opposite for a bond and its reverse; the
+// finalize must be told the same).
 hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool masked, bool ising, int dm,
-                          bool tc, const DevInst* insts, const MTile* tiles, int ntiles, const void* pool,
+                          bool tc, bool bond, const DevInst* insts, const MTile* tiles, int ntiles, const void* pool,
                           const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                           int a2max);
 
@@ -54,7 +56,7 @@ hipError_t launch_measure(hipStream_t stream, bool dim3, bool periodic, bool mas
 // (stored at the entry's list position) and each tile's a2max + 1 slice
 // offsets into that order (zoff).
 hipError_t launch_measure_sites(hipStream_t stream, bool dim3, bool periodic, bool ising, int dm, bool tc,
-                                const DevInst* insts, const MTile* t, int ntiles, const uint32_t* sites,
+                                bool bond, const DevInst* insts, const MTile* t, int ntiles, const uint32_t* sites,
                                 const uint16_t* rank, const uint16_t* zoff, const void* pool,
                                 const uint32_t* mpool, const Stencil& st, double* partials, int pstride,
                                 int a2max);
@@ -73,7 +75,7 @@ hipError_t launch_measure_finalize(hipStream_t stream, const DevInst* insts, con
                                    const int* t_begin, const int* t_end, const double* partials,
                                    int pstride, int a2max, bool dim3, bool tc, int nsl, double* obs,
                                    double* qref, double* qwrap, bool row_by_inst, int64_t row_mul,
-                                   int64_t row_add, int hidx);
+                                   int64_t row_add, int hidx, bool bond);
 
 hipError_t launch_init_spins(hipStream_t stream, bool ising, const DevInst* insts, const Tile* tiles,
                              int ntiles, void* pool, const uint32_t* mpool, int colour, bool random,

@@ -132,6 +132,31 @@ def test_measure_general_dmi_matches_oracle(orc, bloch3d, dim, periodic, masked)
     assert np.allclose(qw[0], f.tcharge(s, wrap=True), rtol=1e-10, atol=1e-10)
 
 
+@pytest.mark.parametrize("dim,periodic,masked", [(2, 1, False), (2, 0, True), (3, 1, False), (3, 0, True)])
+def test_measure_asymmetric_table_matches_oracle(orc, bloch3d, dim, periodic, masked):
+    # a bond and its reverse with different J and D (the interaction file
+    # lists each direction on its own): the per-site form of the local-field
+    # sums; symmetric tables take the per-bond form, tested everywhere else
+    rng = np.random.default_rng(503 + 10 * dim + periodic)
+    edge = 8 if dim == 3 else 14
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.3) if dim == 3 else neel2d(0.3)
+    J = np.asarray(J, dtype=np.float64) * (1.0 + 0.3 * np.arange(len(J)))
+    D = np.asarray(D) * (1.0 + 0.2 * np.arange(len(J)))[:, None]
+    locs = None
+    if masked:
+        locs = box_locs(dim, edge)
+        locs = locs[rng.random(len(locs)) < 0.75]
+    lat, f = make_pair(orc, dim, edge, periodic, bd, J, D, False, use_d=True, locs=locs)
+    s = f32(random_unit(f.n, rng))
+    lat.set_spins(0, s)
+    lat.set_params(0.7, 0.1)
+    obs, _, _ = lat.measure()
+    want = np.concatenate([f.mag(s), f.submag(s), f.sub4mag(s),
+                           [f.energy(s, [0, 0, 0.1], use_d=True, convention=0),
+                            f.energy(s, [0, 0, 0.1], use_d=True, convention=1)]])
+    assert np.allclose(obs[0], want, rtol=1e-12, atol=1e-9)
+
+
 @pytest.mark.parametrize("dim,periodic,masked", [(3, 1, False), (2, 1, False), (3, 0, True)])
 def test_measure_without_charges_is_identical(orc, bloch3d, dim, periodic, masked):
     # a measurement that asks for no charges skips the triangles: every other
