@@ -37,83 +37,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-NN3 = [(0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)]
-
-
-def heis3d(trial, replicas=1):
-    """64 instances of the 64^3 periodic Heisenberg lattice (32 T x 2 h),
-    times `replicas` independent copies of each chain."""
-    from nanomagmc_amd import Bond, Lattices
-    L = 64
-    temps = list(np.linspace(2.0, 0.8, 32))
-    fields = [0.0, 0.05]
-    bonds = [Bond(d, 1.0) for d in NN3]
-    points = [(h, T) for h in fields for T in temps] * replicas
-    insts = [{"n": (L, L, L), "edge": L} for _ in points]
-    lat = Lattices(3, False, True, bonds, insts, use_j=True, use_d=False, trial=trial, seed=0x4E4D4302)
-    lat.set_params([1.0 / T for h, T in points], [h for h, T in points])
-    lat.randomize()
-    sites = [L ** 3] * lat.ninst
-    desc = ("3-D cubic Heisenberg L=64 periodic, J=1, D=0, h_z in {0,0.05} x 32 T 2.0->0.8 = 64 instances%s, "
-            "one checkerboard sweep per step, %s trials"
-            % ("" if replicas == 1 else " x %d replicas" % replicas, trial))
-    return lat, sites, 36, desc
-
-
-def grains(trial, replicas=1, site_range=None):
-    """512 Weibull spheres (LATTSHAPE x, open): radius ~ log-normal (mean 10,
-    sd 2.5, truncated to [4, 20]), Weibull factor 8, J=1, Bloch D=0.2,
-    h_z in {0,0.02,0.05,0.1} x 16 T 2.0->0.1 = 32768 instances.  With
-    site_range = (lo, hi) only the grains with lo <= present sites <= hi are
-    kept (to time the LDS-resident and streamed parts of the batch apart)."""
-    from nanomagmc_amd import Bond, Lattices, build_weibull, crop_to_box
-    ngr, edge, wb = 512, 45, 8.0
-    am, asd = 10.0, 2.5
-    lmean = math.log(am * am / math.sqrt(asd * asd + am * am))
-    lsd = math.sqrt(math.log(1 + asd * asd / (am * am)))
-    rng = np.random.default_rng(0x4E4D4304)
-    sizes = np.clip(rng.lognormal(lmean, lsd, ngr), 4.0, 20.0)
-    masks = build_weibull(3, edge, sizes, wb, seed=0x4E4D4304)
-    temps = list(np.linspace(2.0, 0.1, 16))
-    fields = [0.0, 0.02, 0.05, 0.1]
-    insts, sites, betas, hs = [], [], [], []
-    for g in range(ngr):
-        n, lo, sub = crop_to_box(masks[g], 3, edge)
-        nk = int(sub.sum())
-        if site_range is not None and not site_range[0] <= nk <= site_range[1]:
-            continue
-        for h in fields:
-            for T in temps:
-                insts.append({"n": tuple(n), "origin": tuple(lo), "edge": edge, "mask": sub})
-                sites.append(nk)
-                betas.append(1.0 / T)
-                hs.append(h)
-    bonds = [Bond(d, 1.0, tuple(0.2 * x for x in d)) for d in NN3]
-    lat = Lattices(3, False, False, bonds, insts, use_j=True, use_d=True, trial=trial, seed=0x4E4D4304)
-    lat.set_params(betas, hs)
-    lat.randomize()
-    desc = ("512 Weibull grains (radius log-normal 10+-2.5 in [4,20], factor 8, open) x 4 h x 16 T = "
-            "32768 instances, Heisenberg J=1 + Bloch D=0.2, one sweep of every instance per step, %s trials"
-            % trial)
-    if site_range is not None:
-        desc += "; only the %d instances of grains with %d..%d present sites" % (len(insts), site_range[0],
-                                                                                 site_range[1])
-    return lat, sites, 36, desc
-
-
-def ising2d(trial, replicas=1):
-    from nanomagmc_amd import Bond, Lattices
-    L = 64
-    temps = list(np.linspace(3.5, 1.5, 21))
-    bonds = [Bond((0, 1), 1.0), Bond((0, -1), 1.0), Bond((1, 0), 1.0), Bond((-1, 0), 1.0)]
-    lat = Lattices(2, True, True, bonds, [{"n": (L, L), "edge": L}] * len(temps), seed=0x4E4D4301)
-    lat.set_params([1.0 / T for T in temps], 0.0)
-    lat.randomize()
-    desc = "2-D square Ising L=64 periodic, J=1, h=0, 21 T 3.5->1.5, one checkerboard sweep per step"
-    return lat, [L * L] * lat.ninst, 3, desc
-
-
-WORKLOADS = {"heis3d": heis3d, "grains": grains, "ising2d": ising2d}
 
 
 def copy_bandwidth(nbytes=1 << 30, reps=10):
@@ -216,12 +139,38 @@ def ess_leg():
             "s_per_sample": r["s_per_sample"], "temps": rows}
 
 
+def learn_profile():
+    """The workload of the learned-kernel profile: a 512 x 512 skyrmion
+    lattice pair (J = 1, Neel D = 0.3, h = 0.06), 2e4 collected blocks, 50
+    Adam steps at batch 256, then 20 rounds each of vMF block moves and
+    texture swaps on both lattices.  Prints its wall time."""
+    from nanomagmc_amd import Lattices
+    from nanomagmc_amd.learn import Net, block_moves, block_swaps, neel_bonds
+    t0 = time.perf_counter()
+    lat = Lattices(2, False, True, neel_bonds(0.3), [{"n": (512, 512), "edge": 512}] * 2, use_j=True,
+                   use_d=True, seed=3)
+    lat.set_params([5.0, 5.0], 0.06)
+    lat.randomize()
+    lat.equil(200)
+    net = Net(seed=1)
+    for k in range(20):
+        lat.equil(5)
+        net.collect(lat, 0, 1000, seed=k)
+    net.train(50, batch=256, lr=1e-3, seed=2)
+    for k in range(20):
+        block_moves(lat, net, [0, 1], k % 32, (3 * k) % 32)
+        block_swaps(lat, net, [0, 1], k % 32, (5 * k) % 32)
+    lat.sync()
+    print(json.dumps({"learn_profile": "2 x 512^2 skyrmion lattices, 2e4 blocks, 50 Adam steps, 20 vMF + 20 swap "
+                                       "rounds", "wall_s": time.perf_counter() - t0}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="heis3d", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="heis3d", choices=["grains", "heis3d", "ising2d"])
     ap.add_argument("--trial", default=None, choices=["cone", "sphere"])
     ap.add_argument("--tune", action="store_true", help="autotune the sweep path before timing")
     ap.add_argument("--path", default=None, choices=["auto", "stream"], help="force a sweep path")
@@ -231,9 +180,14 @@ def main():
                     help="grains: keep only grains with LO..HI present sites")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the effective-samples leg")
+    ap.add_argument("--learn-profile", action="store_true",
+                    help="only run the learned-proposal workload profiled in profiles/learned_kernels_stats.md")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.learn_profile:
+        learn_profile()
+        return
     if args.copy_calibration:
         import torch
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -256,6 +210,7 @@ def main():
             sys.exit("--grain-sites applies to --workload grains")
         lo, hi = (int(x) for x in args.grain_sites.split(":"))
         extra["site_range"] = (lo, hi)
+    from nanomagmc_amd.workloads import WORKLOADS
     lat, sites, bpu, desc = WORKLOADS[args.workload](trial, replicas=args.replicas, **extra)
     setup_s = time.perf_counter() - t_setup
     total_sites = int(np.sum(sites))

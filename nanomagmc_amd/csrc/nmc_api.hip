@@ -12,6 +12,8 @@
 #include <numeric>
 #include <random>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -178,7 +180,8 @@ struct nmc_ctx {
 
     int path = NMC_PATH_AUTO;
     int lds_block = 0;        // 0 = derived from the largest LDS-resident instance
-    size_t max_lds = 65536;   // dynamic LDS one workgroup may use on this device
+    size_t max_lds = 65536;     // dynamic LDS one k_sweep_lds workgroup may use on this device
+    size_t max_lds_c = 65536;   // the same for k_sweep_lds_c
     // instantiated sample-loop graphs keyed by (instance list, nsamples, sweeps)
     std::map<std::tuple<std::vector<int>, int, int>, hipGraphExec_t> graphs;
     bool graphs_on = true;   // NMC_NO_GRAPHS=1 in the environment: launch directly, capture nothing
@@ -320,7 +323,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             int64_t np = 0;
             const uint32_t* mw = c->h_mask.data() + d.mask_base;
             for (int64_t w = 0; w < 2 * d.mask_stride; ++w) np += __builtin_popcount(mw[w]);
-            fits_compact = (size_t)(np + 1) * 12 + kStreamLdsReserve <= c->max_lds && 3 * (np + 1) <= 65535;
+            fits_compact = (size_t)(np + 1) * 12 + kStreamLdsReserve <= c->max_lds_c && 3 * (np + 1) <= 65535;
         }
         // both colour arrays, then (masked lattices) both colours' presence bits
         const size_t bytes = (size_t)(c->ising ? 2 * d.Sp : 6 * d.Sp * (int64_t)sizeof(float)) +
@@ -801,6 +804,19 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
         NMC_HIP(hipGetDevice(&dev));
         NMC_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         c->max_lds = (size_t)std::max(0, v);
+        // the LDS-resident kernels' dynamic-LDS limits, once per device and
+        // here rather than at launch: sweeps are also launched under capture
+        static std::mutex mu;
+        static std::map<int, std::pair<int, int>> done;
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = done.find(dev);
+        if (it == done.end()) {
+            int a = 0, b = 0;
+            NMC_HIP(set_lds_limits(v, &a, &b));
+            it = done.emplace(dev, std::make_pair(a, b)).first;
+        }
+        c->max_lds = (size_t)it->second.first;
+        c->max_lds_c = (size_t)it->second.second;
     }
     NMC_HIP(hipStreamCreate(&c->stream));
     NMC_HIP(hipStreamCreate(&c->aux));

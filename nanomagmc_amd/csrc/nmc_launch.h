@@ -32,6 +32,13 @@ hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool peri
                             const Stencil& st, int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                             unsigned long long* acc);
 
+// Sets the dynamic-LDS limit of every LDS-resident sweep kernel on the
+// current device to `shm` (the device's LDS per workgroup, bytes) less the
+// kernel's static LDS; call before their first launch and outside capture.
+// *lds_lim / *ldsc_lim receive the smallest limit of k_sweep_lds and of
+// k_sweep_lds_c.
+hipError_t set_lds_limits(int shm, int* lds_lim, int* ldsc_lim);
+
 // Compacted LDS-resident sweeps of shaped Heisenberg lattices (see CInst).
 hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,
                               const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,

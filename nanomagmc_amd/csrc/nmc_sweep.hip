@@ -695,10 +695,7 @@ static hipError_t lds_one(hipStream_t stream, int grid, int block, size_t shm, c
                           int n, void* pool, const uint32_t* mpool, const uint32_t* tab, const uint64_t* sweeps,
                           const Stencil& st, int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                           unsigned long long* acc) {
-    auto k = k_sweep_lds<IS, D3, PER, MSK, UD, CN>;
-    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    if (e != hipSuccess) return e;
-    k<<<grid, block, shm, stream>>>(insts, list, n, pool, mpool, tab, sweeps, st, nsweeps, s_off, seed_lo, seed_hi,
+    k_sweep_lds<IS, D3, PER, MSK, UD, CN><<<grid, block, shm, stream>>>(insts, list, n, pool, mpool, tab, sweeps, st, nsweeps, s_off, seed_lo, seed_hi,
                                    acc);
     return hipGetLastError();
 }
@@ -708,12 +705,50 @@ static hipError_t lds_c_one(hipStream_t stream, int grid, int block, size_t shm,
                             const CInst* cl, int n, float* pool, const uint32_t* cs_e, const uint2* cs_q,
                             const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st, int nsweeps,
                             uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc) {
-    auto k = k_sweep_lds_c<D3, UD, CN>;
-    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    if (e != hipSuccess) return e;
-    k<<<grid, block, shm, stream>>>(insts, cl, n, pool, cs_e, cs_q, cs_nb, sweeps, st, nsweeps, s_off, seed_lo,
+    k_sweep_lds_c<D3, UD, CN><<<grid, block, shm, stream>>>(insts, cl, n, pool, cs_e, cs_q, cs_nb, sweeps, st, nsweeps, s_off, seed_lo,
                                    seed_hi, acc);
     return hipGetLastError();
+}
+
+// Dynamic-LDS limit of every instance of the two LDS-resident kernels: the
+// device's LDS per workgroup less the kernel's own static LDS.  The attribute
+// is not a stream operation, so it is set once per device when a context is
+// created, never from a launcher: launchers also run while the context's
+// stream is being captured into a graph.  Returns the smallest limit over the
+// instances of each kernel.
+static hipError_t set_one_limit(const void* k, int shm, int& lim) {
+    hipFuncAttributes a{};
+    hipError_t e = hipFuncGetAttributes(&a, k);
+    if (e != hipSuccess) return e;
+    const int dyn = shm - (int)a.sharedSizeBytes;
+    if (dyn <= 0) return hipErrorInvalidValue;
+    e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    lim = lim < dyn ? lim : dyn;
+    return e;
+}
+
+hipError_t set_lds_limits(int shm, int* lds_lim, int* ldsc_lim) {
+    hipError_t e = hipSuccess;
+    int* lim = ldsc_lim;
+    *lds_lim = *ldsc_lim = shm;
+#define NMC_ATTR(K) \
+    if (e == hipSuccess) e = set_one_limit((const void*)(K), shm, *lim);
+#define NMC_ATTR_C(D3, UD) NMC_ATTR((k_sweep_lds_c<D3, UD, false>)) NMC_ATTR((k_sweep_lds_c<D3, UD, true>))
+    NMC_ATTR_C(true, 0) NMC_ATTR_C(true, 1) NMC_ATTR_C(true, 2)
+    NMC_ATTR_C(false, 0) NMC_ATTR_C(false, 1) NMC_ATTR_C(false, 2)
+    lim = lds_lim;
+#define NMC_ATTR_L(D3, PER, MSK)                                                                     \
+    NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 0, false>)) NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 0, true>)) \
+    NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 1, false>)) NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 1, true>)) \
+    NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 2, false>)) NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 2, true>)) \
+    NMC_ATTR((k_sweep_lds<true, D3, PER, MSK, 0, false>))
+    NMC_ATTR_L(true, true, false) NMC_ATTR_L(true, true, true) NMC_ATTR_L(true, false, false)
+    NMC_ATTR_L(true, false, true) NMC_ATTR_L(false, true, false) NMC_ATTR_L(false, true, true)
+    NMC_ATTR_L(false, false, false) NMC_ATTR_L(false, false, true)
+#undef NMC_ATTR_L
+#undef NMC_ATTR_C
+#undef NMC_ATTR
+    return e;
 }
 
 hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "../../include/nmc.h"
+#include "grain_sizes.h"
 
 namespace {
 
@@ -144,13 +145,6 @@ std::vector<float> load_list(const std::string& path) {
     double cur;
     while (in >> cur) v.push_back((float)cur);
     return v;
-}
-
-// AtoLn (functions.cpp:56-62)
-void AtoLn(double amean, double asd, double& lmean, double& lsd) {
-    double av = pow(asd, 2), am2 = pow(amean, 2);
-    lmean = log(am2 / pow(av + am2, 0.5));
-    lsd = pow(log(1 + av / am2), 0.5);
 }
 
 struct Protocol {               // set_protocol (protocol.cpp:6-59)
@@ -490,15 +484,8 @@ int main(int argc, char** argv) {
         }
     }
     // lattice sizes (main.cpp:116-117) and shapes (state::init_lattice)
-    std::mt19937_64 size_rng(sim.seed ^ 0x5A5A5A5Aull);
-    std::lognormal_distribution<double> lnd(sim.lmean, sim.lsd > 0 ? sim.lsd : 1e-300);
-    std::vector<double> sizes(sim.N_latts);
-    for (int k = 0; k < sim.N_latts; ++k) {
-        double s = sim.distrib ? lnd(size_rng) : sim.amean;
-        if (sim.distrib && sim.maxsize > 0) s = std::min(s, sim.maxsize);
-        if (sim.distrib && sim.minsize > 0) s = std::max(s, sim.minsize);
-        sizes[k] = s;
-    }
+    const std::vector<double> sizes =
+        draw_grain_sizes(sim.seed, sim.distrib, sim.amean, sim.lmean, sim.lsd, sim.N_latts, sim.minsize, sim.maxsize);
     std::vector<Lattice> latts(sim.N_latts);
     size_t box = 1;
     for (int i = 0; i < dim; ++i) box *= edge;

@@ -263,29 +263,3 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
         row["gain_E"] = {nm: row[nm]["E"]["ess_per_s"] / row["plain"]["E"]["ess_per_s"] for nm in methods}
         out["temps"].append(row)
     return out
-
-
-def _profile_workload():
-    """The workload of the learned-kernel profile: a 512 x 512 skyrmion
-    lattice pair, 2e4 collected blocks, 50 Adam steps at batch 256, then 20
-    rounds each of vMF block moves and texture swaps on both lattices."""
-    from .engine import Lattices
-    lat = Lattices(2, False, True, neel_bonds(0.3), [{"n": (512, 512), "edge": 512}] * 2, use_j=True,
-                   use_d=True, seed=3)
-    lat.set_params([5.0, 5.0], 0.06)
-    lat.randomize()
-    lat.equil(200)
-    net = Net(seed=1)
-    for k in range(20):
-        lat.equil(5)
-        net.collect(lat, 0, 1000, seed=k)
-    net.train(50, batch=256, lr=1e-3, seed=2)
-    for k in range(20):
-        block_moves(lat, net, [0, 1], k % 32, (3 * k) % 32)
-        block_swaps(lat, net, [0, 1], k % 32, (5 * k) % 32)
-    lat.sync()
-    print("done", flush=True)
-
-
-if __name__ == "__main__":
-    _profile_workload()

@@ -23,21 +23,9 @@ import pytest
 
 from tests import exact
 from tests.fixtures import box_locs, neel2d
+from tests.gpu_util import estimators, oracle_chain, replica_stats
 
 pytestmark = pytest.mark.gpu
-
-
-def replica_stats(per_replica):
-    """Mean over replicas and its standard error."""
-    x = np.asarray(per_replica, dtype=np.float64)
-    return x.mean(), x.std(ddof=1) / math.sqrt(x.size)
-
-
-def estimators(E, M, beta, N):
-    """Per-replica <E>, <|M|>, C = beta^2 var(E) / N, chi = beta (<M^2> - <|M|>^2) / N."""
-    aM = np.abs(M) if M.ndim == 1 else np.linalg.norm(M, axis=-1)
-    return {"E": E.mean(), "absM": aM.mean(), "C": beta * beta * E.var() / N,
-            "chi": beta * ((aM * aM).mean() - aM.mean() ** 2) / N}
 
 
 @pytest.mark.parametrize("T", [1.8, 2.4, 3.5])
@@ -62,34 +50,6 @@ def test_ising_4x4_observables_match_exact_enumeration(T):
         m, err = replica_stats([p[k] for p in per])
         print("T=%.2f %-5s device %.5f +- %.5f exact %.5f" % (T, k, m, err, want[k]))
         assert abs(m - want[k]) < 3 * err, (T, k, m, err, want[k])
-
-
-def oracle_chain(orc, f, H, beta, nsamples, sweeps, rng, burn=200):
-    """The reference's random-site Metropolis on the host in FP64: uniform
-    sphere trials, accept iff dE <= 0 or -dE*beta > log(u).  Returns the
-    E_s, M and Q_wrap series."""
-    n = f.n
-    s = rng.normal(size=(n, 3))
-    s /= np.linalg.norm(s, axis=1, keepdims=True)
-
-    def attempts(k):
-        z = rng.uniform(-1, 1, k)
-        ph = rng.uniform(0, 2 * math.pi, k)
-        r = np.sqrt(1 - z * z)
-        t = np.stack([r * np.cos(ph), r * np.sin(ph), z], axis=1)
-        return rng.integers(0, n, k), t, rng.uniform(size=k)
-
-    for _ in range(burn):
-        f.equil_replay(s, H, beta, *attempts(sweeps * n), use_d=True)
-    E = np.zeros(nsamples)
-    M = np.zeros((nsamples, 3))
-    Q = np.zeros(nsamples)
-    for k in range(nsamples):
-        f.equil_replay(s, H, beta, *attempts(sweeps * n), use_d=True)
-        E[k] = f.energy(s, H, use_d=True, convention=1)
-        M[k] = f.mag(s)
-        Q[k] = f.tcharge(s, wrap=True)[0]
-    return E, M, Q
 
 
 def test_heisenberg_dmi_observables_match_oracle_chain(orc):

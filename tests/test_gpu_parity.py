@@ -226,11 +226,21 @@ def test_delta_e_matches_oracle(orc, bloch3d, case):
     if ising:
         assert np.array_equal(got, want)          # bit-exact
     else:
-        # fp32 local field: 1e-5 relative to the local scale |H+V| |s - t|
+        # fp32 local field: within 1e-5 of the site's own scale |H + V| |s - t|,
+        # V = sum_j J_j s_j + s_j x D_j from the oracle's neighbour lists (the
+        # field dE_func dots with s - t, thermodynamics.cpp:51-69); the second
+        # term is fp32 rounding of the ~z terms of V where they cancel
         for i in range(n):
             j = sites[i]
-            scale = (abs(h) + 6 * 1.5) * np.linalg.norm(s[j] - trials[i])
-            assert abs(got[i] - want[i]) <= 1e-5 * max(scale, 1e-30)
+            V = np.array([0.0, 0.0, h])
+            terms = abs(h)
+            for c in range(f.ncount[j]):
+                k, nb = f.nchoice[j, c], s[f.neigh[j, c]]
+                V += f.J[k] * nb + (np.cross(nb, f.D[k]) if use_d else 0.0)
+                terms += abs(f.J[k]) + (np.linalg.norm(f.D[k]) if use_d else 0.0)
+            ds = np.linalg.norm(s[j] - trials[i])
+            tol = 1e-5 * np.linalg.norm(V) * ds + 4 * 2.0 ** -24 * terms * ds
+            assert abs(got[i] - want[i]) <= tol, (i, got[i], want[i], tol)
 
 
 # ------------------------------------------------ accept decisions (Ising) ---

@@ -1,6 +1,12 @@
-"""Pins the network oracle (oracle/netref.py): its analytic gradient against
-central finite differences in FP64, and the vMF log-density against
-numerical integration over the sphere.  CPU only."""
+"""Pins the network oracle (oracle/netref.py), CPU only:
+* forward(): every layer against torch's conv2d with the stated weight
+  layout, and one pixel against a nested loop over the 3 x 3 window;
+* bf16(): torch's bfloat16 rounding (nearest even, ties both ways);
+* loss_grad(): central finite differences in FP64;
+* logq(): numerical integration over the sphere;
+* swap_proposal(): an involution (a rotation by pi per site) that maps
+  texture A onto B, B a skyrmion more than A by the C oracle's charge, and
+  both textures stationary points of the C oracle's energy E_s."""
 import math
 
 import numpy as np
@@ -63,3 +69,133 @@ def test_vmf_density_normalised(netref):
         assert abs(np.trapezoid(f, th) - 1) < 1e-6
         mc = np.trapezoid(f * np.cos(th), th)
         assert abs(mc - (1 / math.tanh(kappa) - 1 / kappa)) < 1e-6
+
+
+def test_forward_is_a_textbook_convolution(netref):
+    # each layer of forward() against torch's conv2d (NCHW, weight
+    # [Cout][Cin][dy][dx] = W[(dy * 3 + dx) * Cin + ci][Cout]) + bias + ReLU,
+    # with dy along the patch's first axis; and one output pixel of the first
+    # layer against a nested loop over the 3 x 3 window
+    import torch
+    import torch.nn.functional as F
+    rng = np.random.default_rng(11)
+    p, patches, _ = random_case(netref, rng, n=3)
+    p = p + rng.normal(scale=0.05, size=p.shape)    # also the unused channels of layer 0
+    xs, out = netref.forward(p, patches)
+    X = np.zeros((3, 24, 24, 16))
+    X[..., :4] = patches
+    assert np.array_equal(xs[0], X)
+    layers = netref.unpack(p)
+    cur = torch.from_numpy(X).permute(0, 3, 1, 2)
+    for l, (W, b) in enumerate(layers):
+        cin = netref.CIN[l]
+        w = torch.from_numpy(W.reshape(3, 3, cin, 32).transpose(3, 2, 0, 1).copy())
+        y = F.conv2d(cur, w, torch.from_numpy(b))
+        if l < 3:
+            y = torch.relu(y)
+            assert np.allclose(xs[l + 1], y.permute(0, 2, 3, 1).numpy(), rtol=1e-12, atol=1e-12), l
+        cur = y
+    assert np.allclose(out, cur.permute(0, 2, 3, 1).numpy(), rtol=1e-12, atol=1e-12)
+    W0, b0 = layers[0]
+    for (n, u, v, o) in ((0, 0, 0, 0), (1, 21, 3, 7), (2, 10, 17, 31)):
+        acc = b0[o]
+        for dy in range(3):
+            for dx in range(3):
+                for ci in range(16):
+                    acc += X[n, u + dy, v + dx, ci] * W0[(dy * 3 + dx) * 16 + ci, o]
+        assert abs(xs[1][n, u, v, o] - max(acc, 0.0)) < 1e-12
+
+
+def test_bf16_rounding_matches_torch(netref):
+    # round to nearest even, including exact ties in both directions,
+    # subnormals, signed zeros and overflow to infinity
+    import torch
+    rng = np.random.default_rng(5)
+    vals = [rng.normal(size=4000).astype(np.float32), (rng.normal(size=1000) * 1e-39).astype(np.float32),
+            np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, 3.3895314e38, -3.3895314e38], dtype=np.float32)]
+    # exact ties: bf16 value + half an ulp, with even and odd kept mantissas
+    base = rng.normal(size=500).astype(np.float32).view(np.uint32) & np.uint32(0xFFFF0000)
+    vals.append((base | np.uint32(0x8000)).view(np.float32))
+    vals.append(((base | np.uint32(0x10000)) | np.uint32(0x8000)).view(np.float32))
+    x = np.concatenate(vals)
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    got = netref.bf16(x)
+    assert np.array_equal(got, want)
+    assert np.array_equal(np.signbit(got), np.signbit(want))
+    ties = np.concatenate(vals[-2:])
+    assert np.any(netref.bf16(ties) > ties.astype(np.float64)) and np.any(netref.bf16(ties) < ties.astype(np.float64))
+
+
+def _swap_case(netref, orc, L=32, bx=5, by=9, D=0.3, h=0.06, iters=80, seed=3):
+    from tests.fixtures import box_locs, neel2d
+    rng = np.random.default_rng(seed)
+    s = rng.normal(size=(L * L, 3))
+    s[:, 2] += 2.0                       # a ferromagnetic background, as at low T
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+    d, Jv, Dv = neel2d(D)
+    J6, D6 = np.zeros(6), np.zeros((6, 3))
+    slot = {(-1, 0): 2, (1, 0): 3, (0, -1): 4, (0, 1): 5}
+    for k in range(4):
+        g = slot[(int(d[k][0]), int(d[k][1]))]
+        J6[g], D6[g] = Jv[k], Dv[k]
+    mu = rng.normal(scale=0.2, size=(16, 16, 3))   # a mean texture along the background
+    mu[..., 2] += 1.0
+    f = orc.Field(box_locs(2, L), 2, L, 1, d, Jv, Dv)
+    return s, J6, D6, mu, f, (L, bx, by, D, h, iters)
+
+
+def test_swap_proposal_is_an_involution_mapping_a_to_b(netref, orc):
+    s, J6, D6, mu, f, (L, bx, by, D, h, iters) = _swap_case(netref, orc)
+    xp, A, B = netref.swap_proposal(s, L, bx, by, mu, J6, D6, h, iters, 5.0)
+    block = (np.arange(bx, bx + 16)[:, None] % L) * L + (np.arange(by, by + 16)[None, :] % L)
+    x = s[block]
+    # a rotation by pi about a unit axis: unit spins stay unit, and applying
+    # the move twice returns the block
+    assert np.allclose(np.linalg.norm(xp, axis=-1), 1.0, atol=1e-12)
+    s2 = s.copy()
+    s2[block] = xp
+    xpp, A2, B2 = netref.swap_proposal(s2, L, bx, by, mu, J6, D6, h, iters, 5.0)
+    assert np.array_equal(A2, A) and np.array_equal(B2, B)     # textures depend on the surroundings only
+    assert np.allclose(xpp, x, atol=1e-12)
+    # the move maps texture A onto texture B and back
+    s3 = s.copy()
+    s3[block] = A
+    assert np.allclose(netref.swap_proposal(s3, L, bx, by, mu, J6, D6, h, iters, 5.0)[0], B, atol=1e-12)
+    s3[block] = B
+    assert np.allclose(netref.swap_proposal(s3, L, bx, by, mu, J6, D6, h, iters, 5.0)[0], A, atol=1e-12)
+    # the two textures differ by one skyrmion (the lattice's plaquette charge
+    # with the block set to B exceeds the one with A by one, by the oracle's
+    # calc_TC with periodic wrap) and both keep unit length
+    s3[block] = A
+    qa = f.tcharge(s3, wrap=True)[0]
+    s3[block] = B
+    qb = f.tcharge(s3, wrap=True)[0]
+    assert abs(abs(qb - qa) - 1.0) < 1e-9, (qa, qb)
+    assert np.allclose(np.linalg.norm(A, axis=-1), 1.0) and np.allclose(np.linalg.norm(B, axis=-1), 1.0)
+    assert np.allclose(np.linalg.norm(A, axis=-1), 1.0) and np.allclose(np.linalg.norm(B, axis=-1), 1.0)
+
+
+def test_swap_textures_are_relaxed_to_the_oracle_energy_minimum(netref, orc):
+    # with the block's one-site ring fixed, A and B are relaxed to a stationary
+    # point of the sampler's energy E_s: every block spin is parallel to its
+    # local field -grad E_s, computed here by central differences of the C
+    # oracle's energy (independent of the relaxation's own field)
+    s, J6, D6, mu, f, (L, bx, by, D, h, iters) = _swap_case(netref, orc, iters=400)
+    _, A, B = netref.swap_proposal(s, L, bx, by, mu, J6, D6, h, iters, 5.0)
+    block = (np.arange(bx, bx + 16)[:, None] % L) * L + (np.arange(by, by + 16)[None, :] % L)
+    H = [0.0, 0.0, h]
+    for T in (A, B):
+        st = s.copy()
+        st[block] = T
+        worst = 0.0
+        for u in range(0, 16, 3):
+            for v in range(0, 16, 2):
+                i = block[u, v]
+                F = np.zeros(3)
+                for a in range(3):
+                    sp, sm = st.copy(), st.copy()
+                    sp[i, a] += 1e-5
+                    sm[i, a] -= 1e-5
+                    F[a] = -(f.energy(sp, H, use_d=True, convention=1) - f.energy(sm, H, use_d=True, convention=1)) / 2e-5
+                worst = max(worst, np.linalg.norm(np.cross(T[u, v], F)) / np.linalg.norm(F))
+        assert worst < 1e-4, worst
