@@ -183,6 +183,21 @@ int nmc_snapshot(nmc_ctx* ctx, const int* src, const int* dst, int n);
 int nmc_measure(nmc_ctx* ctx, const int* inst, int n, double* obs,
                 double* q_ref, double* q_wrap);
 
+/* No counterpart in the reference: nmc_measure for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels, same results bit for bit;
+ * it calls no hipMalloc/hipFree.  The per-tile partial sums and the
+ * finalized observables live in a 16 MiB scratch array in device memory
+ * that the library holds on every GPU it runs on until the process ends; a
+ * list needing more (n_tiles * stride + n * (11 + 2 * nslices) doubles)
+ * returns NMC_E_INVALID.  The instance list must have been prepared by an
+ * earlier call with the same list (nmc_create prepares inst == NULL;
+ * nmc_set_path / nmc_set_lds_block discard every prepared list), otherwise
+ * NMC_E_INVALID.  All calls on one GPU share the scratch: calls that overlap
+ * in time on one GPU (several contexts or threads) race. */
+int nmc_measure_noalloc(nmc_ctx* ctx, const int* inst, int n, double* obs,
+                        double* q_ref, double* q_wrap);
+
 /* Sample series kept in HBM: nmc_series_reset sizes the buffer for
  * `nsamples` samples of every instance (with_charge = 0 skips the per-slice
  * topological charges, as the reference does for distributed runs,

@@ -24,7 +24,7 @@ OBS_NAMES = ["Mx", "My", "Mz", "subMx", "subMy", "subMz", "sub4Mx", "sub4My", "s
 EXPORTS = [
     "nmc_last_error", "nmc_create", "nmc_destroy", "nmc_get_stream", "nmc_sync", "nmc_num_sites", "nmc_num_slices",
     "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_sweep_adapt", "nmc_set_path", "nmc_set_lds_block", "nmc_get_accepts",
-    "nmc_reset_accepts", "nmc_snapshot", "nmc_measure", "nmc_series_reset", "nmc_sample",
+    "nmc_reset_accepts", "nmc_snapshot", "nmc_measure", "nmc_measure_noalloc", "nmc_series_reset", "nmc_sample",
     "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_accumulate_avgs", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
     "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_net_create", "nmc_net_destroy",
     "nmc_net_num_params", "nmc_net_get_params", "nmc_net_set_params", "nmc_net_collect",
@@ -83,6 +83,7 @@ def lib():
         "nmc_reset_accepts": (i, [vp]),
         "nmc_snapshot": (i, [vp, P(i), P(i), i]),
         "nmc_measure": (i, [vp, P(i), i, P(d), P(d), P(d)]),
+        "nmc_measure_noalloc": (i, [vp, P(i), i, P(d), P(d), P(d)]),
         "nmc_series_reset": (i, [vp, i, i]),
         "nmc_sample": (i, [vp, P(i), i, i]),
         "nmc_get_series": (i, [vp, P(i), i, P(d), P(d), P(d)]),

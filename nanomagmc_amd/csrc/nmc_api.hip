@@ -124,6 +124,35 @@ constexpr int kGraphSweeps = 64;      // sweeps per captured graph of a small se
 constexpr size_t kStreamLdsReserve = 1024;   // LDS a compacted lattice leaves free on its CU
 constexpr int kMeasureTile = 1024;    // colour elements per measurement tile (one workgroup)
 
+// Scratch of nmc_measure_noalloc: the measurement's per-tile partial sums
+// followed by the finalized observables of the call.  Every GPU the library
+// runs on holds its own copy for the life of the process.  Host code never
+// uses the array's name as an address: each device's copy is reached through
+// the address hipGetSymbolAddress gives for it (scratch_address below).
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int scratch_address(int dev, double** out) {
+    static std::mutex mu;
+    static std::map<int, double*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it == addr.end()) {
+        double* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return NMC_E_HIP;
+        }
+        it = addr.emplace(dev, p).first;
+    }
+    *out = it->second;
+    return NMC_OK;
+}
+
 }  // namespace
 
 struct nmc_ctx {
@@ -182,6 +211,7 @@ struct nmc_ctx {
     int lds_block = 0;        // 0 = derived from the largest LDS-resident instance
     size_t max_lds = 65536;     // dynamic LDS one k_sweep_lds workgroup may use on this device
     size_t max_lds_c = 65536;   // the same for k_sweep_lds_c
+    double* scratch = nullptr;  // this device's copy of the static scratch (nmc_measure_noalloc)
     // instantiated sample-loop graphs keyed by (instance list, nsamples, sweeps)
     std::map<std::tuple<std::vector<int>, int, int>, hipGraphExec_t> graphs;
     bool graphs_on = true;   // NMC_NO_GRAPHS=1 in the environment: launch directly, capture nothing
@@ -633,21 +663,23 @@ int pstride_of(const nmc_ctx* c) {
 
 // The measurement launches of a set (no allocation: the partials buffer must
 // already hold n_mt tiles).  tc: compute the topological charges too.
-int measure_launches(nmc_ctx* c, Set* s, bool tc) {
+// partials: n_mt * pstride_of(c) doubles (the context's buffer, or the
+// static scratch of nmc_measure_noalloc).
+int measure_launches(nmc_ctx* c, Set* s, bool tc, double* partials) {
     const int ps = pstride_of(c);
     if (c->masked)
         NMC_HIP(launch_measure_sites(c->stream, c->dim3, c->periodic, c->ising, c->dm_meas, tc, c->bond_sym, c->d_inst, s->d_mt,
                                      s->n_mt, s->d_msites, s->d_mrank, s->d_mzoff, c->pool, c->mpool, c->st,
-                                     c->d_partials, ps, c->a2max));
+                                     partials, ps, c->a2max));
     else
         NMC_HIP(launch_measure(c->stream, c->dim3, c->periodic, c->masked, c->ising, c->dm_meas, tc, c->bond_sym, c->d_inst,
-                               s->d_mt, s->n_mt, c->pool, c->mpool, c->st, c->d_partials, ps, c->a2max));
+                               s->d_mt, s->n_mt, c->pool, c->mpool, c->st, partials, ps, c->a2max));
     return NMC_OK;
 }
 
 int run_measure(nmc_ctx* c, Set* s, bool tc) {
     if (int rc = ensure_partials(c, (size_t)s->n_mt * pstride_of(c))) return rc;
-    return measure_launches(c, s, tc);
+    return measure_launches(c, s, tc, c->d_partials);
 }
 
 }  // namespace
@@ -817,6 +849,11 @@ int nmc_create(const nmc_config* cfg, const nmc_bond* bonds, int nbonds,
         }
         c->max_lds = (size_t)it->second.first;
         c->max_lds_c = (size_t)it->second.second;
+    }
+    {
+        int dev = 0;
+        NMC_HIP(hipGetDevice(&dev));
+        if (int rc = scratch_address(dev, &c->scratch)) return rc;
     }
     NMC_HIP(hipStreamCreate(&c->stream));
     NMC_HIP(hipStreamCreate(&c->aux));
@@ -1268,6 +1305,54 @@ int nmc_measure(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_ref
     return NMC_OK;
 }
 
+namespace {
+// The cached work lists of an instance list (NULL = all instances), found
+// without building a key: nullptr if the list was never prepared.
+Set* find_set(nmc_ctx* ctx, const int* inst, int n) {
+    const size_t cnt = inst ? (size_t)(n > 0 ? n : 0) : ctx->din.size();
+    for (auto& kv : ctx->sets) {
+        const std::vector<int>& key = kv.first;
+        if (key.size() != cnt) continue;
+        bool same = true;
+        for (size_t j = 0; j < cnt && same; ++j) same = key[j] == (inst ? inst[j] : (int)j);
+        if (same) return kv.second.get();
+    }
+    return nullptr;
+}
+}  // namespace
+
+int nmc_measure_noalloc(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_ref, double* q_wrap) {
+    if (!ctx || !obs) return fail(NMC_E_INVALID, "null argument");
+    if (inst && n <= 0) return fail(NMC_E_INVALID, "empty instance list");
+    Set* s = find_set(ctx, inst, n);
+    if (!s)
+        return fail(NMC_E_INVALID, "nmc_measure_noalloc: this instance list has no work lists yet (they are built "
+                                   "by any earlier call with the same list, e.g. nmc_measure or nmc_sweep)");
+    const size_t cnt = s->list.size();
+    const size_t parts = (size_t)s->n_mt * pstride_of(ctx);
+    const size_t outs = cnt * (11 + 2 * (size_t)ctx->nsl);
+    if (parts + outs > kScratchDoubles)
+        return fail(NMC_E_INVALID, "nmc_measure_noalloc: the list needs " + std::to_string((parts + outs) * 8) +
+                                       " bytes of scratch, more than its " + std::to_string(kScratchBytes) +
+                                       " (use nmc_measure)");
+    double* dobs = ctx->scratch + parts;
+    double* dq_r = dobs + cnt * 11;
+    double* dq_w = dq_r + cnt * ctx->nsl;
+    const bool tc = !ctx->ising && (q_ref || q_wrap);
+    if (int rc = measure_launches(ctx, s, tc, ctx->scratch)) return rc;
+    NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, (int)cnt, s->d_tb, s->d_te, ctx->scratch,
+                                    pstride_of(ctx), ctx->a2max, ctx->dim3, tc, ctx->nsl, dobs, dq_r, dq_w, false, 1,
+                                    0, ctx->ising ? 0 : 2, ctx->bond_sym));
+    const size_t qn = cnt * ctx->nsl;
+    NMC_HIP(hipMemcpyAsync(obs, dobs, cnt * 11 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (q_ref && tc) NMC_HIP(hipMemcpyAsync(q_ref, dq_r, qn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (q_wrap && tc) NMC_HIP(hipMemcpyAsync(q_wrap, dq_w, qn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NMC_HIP(hipStreamSynchronize(ctx->stream));
+    if (q_ref && !tc) std::fill(q_ref, q_ref + qn, 0.0);
+    if (q_wrap && !tc) std::fill(q_wrap, q_wrap + qn, 0.0);
+    return NMC_OK;
+}
+
 int nmc_series_reset(nmc_ctx* ctx, int nsamples, int with_charge) {
     if (!ctx || nsamples < 0) return fail(NMC_E_INVALID, "bad argument");
     NMC_HIP(hipStreamSynchronize(ctx->stream));
@@ -1293,7 +1378,7 @@ namespace {
 // Enqueue one measurement of a prepared set into series slot `slot`
 // (launches only; the partials buffer must already be large enough).
 int sample_launches(nmc_ctx* ctx, Set* s, int slot) {
-    if (int rc = measure_launches(ctx, s, ctx->series_q)) return rc;
+    if (int rc = measure_launches(ctx, s, ctx->series_q, ctx->d_partials)) return rc;
     NMC_HIP(launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_list, (int)s->list.size(), s->d_tb, s->d_te,
                                     ctx->d_partials, pstride_of(ctx), ctx->a2max, ctx->dim3, ctx->series_q, ctx->nsl,
                                     ctx->d_series, ctx->d_series_qr, ctx->d_series_qw, true, ctx->series_n, slot,
