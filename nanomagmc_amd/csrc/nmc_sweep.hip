@@ -54,9 +54,9 @@ __device__ __forceinline__ bool mask_bit_s(const uint32_t* mw, const DevInst& in
 }
 
 // Heisenberg updates are done four colour elements at a time (a "quad",
-// e0 = 4q .. 4q+3): three Philox4x32-10 blocks with counters 3q, 3q+1, 3q+2
-// give the 12 words the quad needs, element m taking words 3m, 3m+1
-// (trial) and 3m+2 (acceptance uniform).
+// e0 = 4q .. 4q+3).  Colour element e draws two words from the Philox4x32-10
+// block with counter e >> 1: words 2 (e & 1) and 2 (e & 1) + 1.  So the quad
+// needs the blocks 2q (elements 4q, 4q+1) and 2q + 1 (elements 4q+2, 4q+3).
 
 // Every sweep kernel must round a site's update identically (the LDS-resident
 // and streaming paths are compared bit for bit), so the two helpers below fix
@@ -98,15 +98,17 @@ __device__ __forceinline__ void add_bond(const Stencil& st, int g, float nx, flo
 }
 
 // Trial spin, dE_func and the Metropolis test for one site with current spin
-// s and local field v, from Philox words wa (polar), wb (azimuth) and wc
-// (acceptance).  Writes the trial spin to t and returns the decision.
+// s and local field v, from the site's two Philox words: the top 20 bits of
+// wa give the polar uniform, those of wb the azimuth, and the low 12 bits of
+// both the 24-bit acceptance uniform.  Writes the trial spin to t and returns
+// the decision.
 template <bool CONE>
 __device__ __forceinline__ bool heis_decide(const DevInst& in, float sx, float sy, float sz, float vx, float vy,
-                                            float vz, uint32_t wa, uint32_t wb, uint32_t wc, float& tx, float& ty,
-                                            float& tz) {
+                                            float vz, uint32_t wa, uint32_t wb, float& tx, float& ty, float& tz) {
 #pragma clang fp contract(off)
-    const float ua = u24_open(wa);
-    const float uphi = u24(wb);
+    const float ua = ((float)(wa >> 12) + 0.5f) * (1.0f / 1048576.0f);   // (0, 1), centred on the 2^-20 grid
+    const float uphi = (float)(wb >> 12) * (1.0f / 1048576.0f);          // [0, 1)
+    const float uacc = (float)(((wa & 0xFFFu) << 12) | (wb & 0xFFFu)) * (1.0f / 16777216.0f);
     const float cphi = __builtin_amdgcn_cosf(uphi);   // cos(2 pi u): v_cos_f32 takes revolutions
     const float sphi = __builtin_amdgcn_sinf(uphi);
     if (!CONE) {
@@ -136,7 +138,7 @@ __device__ __forceinline__ bool heis_decide(const DevInst& in, float sx, float s
     }
     // dE_func: (H + Vsum) . (s_old - s_trial), H = (0, 0, h)
     const float dE = __builtin_fmaf(vx, sx - tx, __builtin_fmaf(vy, sy - ty, (in.h + vz) * (sz - tz)));
-    return (dE <= 0.f) || (u24(wc) < __expf(-in.beta * dE));
+    return (dE <= 0.f) || (uacc < __expf(-in.beta * dE));
 }
 
 template <class T>
@@ -166,15 +168,14 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
         MASKED ? (mw[colour * (uint32_t)in.mask_stride + (e0 >> 5)] >> (e0 & 31)) & 0xFu : 0xFu;
     // a quad with no present site (outside a grain) costs nothing
     if (MASKED && qmask == 0) return 0;
-    uint32_t w[12];
+    uint32_t w[8];
     {
-        // element j uses words 3j .. 3j + 2: block 0 serves j = 0, 1, block 1
-        // j = 1, 2, block 2 j = 2, 3; a block no present element uses is skipped
-        const uint32_t q3 = (e0 >> 2) * 3u;
+        // element j uses words 2j, 2j + 1: block e0 / 2 serves j = 0, 1, block
+        // e0 / 2 + 1 j = 2, 3; a block no present element uses is skipped
+        const uint32_t q2 = e0 >> 1;
         const uint32_t slo = (uint32_t)sweep, shi = (uint32_t)(sweep >> 32), k1 = seed_hi + in.inst_id;
-        if (qmask & 3u) philox4x32_10(q3, (uint32_t)colour, slo, shi, seed_lo, k1, w);
-        if (qmask & 6u) philox4x32_10(q3 + 1, (uint32_t)colour, slo, shi, seed_lo, k1, w + 4);
-        if (qmask & 12u) philox4x32_10(q3 + 2, (uint32_t)colour, slo, shi, seed_lo, k1, w + 8);
+        if (qmask & 3u) philox4x32_10(q2, (uint32_t)colour, slo, shi, seed_lo, k1, w);
+        if (qmask & 12u) philox4x32_10(q2 + 1, (uint32_t)colour, slo, shi, seed_lo, k1, w + 4);
     }
     const uint32_t S = (uint32_t)in.S;
     int nacc = 0;
@@ -234,8 +235,7 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
             if (MASKED && present) present = mask_bit_s(mw, in, colour, e0 + j);
             if (!present) continue;
             float tx, ty, tz;
-            if (heis_decide<CONE>(in, cx, cy, cz, vx[j], vy[j], vz[j], w[3 * j], w[3 * j + 1], w[3 * j + 2], tx, ty,
-                                  tz)) {
+            if (heis_decide<CONE>(in, cx, cy, cz, vx[j], vy[j], vz[j], w[2 * j], w[2 * j + 1], tx, ty, tz)) {
                 ox[j] = tx; oy[j] = ty; oz[j] = tz;
                 ++nacc;
             }
@@ -264,7 +264,7 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
             }
         }
         float tx, ty, tz;
-        if (heis_decide<CONE>(in, sx, sy, sz, vx, vy, vz, w[3 * j], w[3 * j + 1], w[3 * j + 2], tx, ty, tz)) {
+        if (heis_decide<CONE>(in, sx, sy, sz, vx, vy, vz, w[2 * j], w[2 * j + 1], tx, ty, tz)) {
             st32(own_x, e, tx);
             st32(own_y, e, ty);
             st32(own_z, e, tz);
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
 // indices, and stores them back.  A work-item takes one quad with a present
 // site: it draws the quad's Philox blocks once (only those its present
 // elements use) and updates the present elements, element e0 + m with words
-// 3m .. 3m + 2 of the three blocks and its bonds in the same order as
+// 2m, 2m + 1 of the two blocks and its bonds in the same order as
 // k_sweep_heis, so both kernels produce the same lattice.  Quads are grouped
 // by presence pattern, so a wavefront mostly takes the same branches.
 // ---------------------------------------------------------------------------
@@ -494,47 +494,40 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
 // (bit m: element e0 + m is present)}; the present elements have
 // consecutive slots.  NB holds, per slot, the float indices of its six
 // neighbours (16 bits each, three aligned words).  Element m uses words
-// 3m .. 3m + 2 of the quad's blocks 3q, 3q + 1, 3q + 2: block 0 serves
-// m = 0, 1, block 1 m = 1, 2 and block 2 m = 2, 3, and only the blocks a
-// present element uses are drawn.
+// 2m, 2m + 1 of the quad's blocks 2q (m = 0, 1) and 2q + 1 (m = 2, 3), and
+// only the blocks a present element uses are drawn.
 template <bool DIM3, int DM, bool CONE>
 __device__ __forceinline__ int cquad_update(uint2 qr, const uint16_t* __restrict__ NB, float* ls,
                                             const DevInst& in, const Stencil& st, uint32_t c, uint32_t slo,
                                             uint32_t shi, uint32_t k0, uint32_t k1) {
-    const uint32_t msk = qr.y >> 16, q3 = qr.x * 3u, b0 = qr.y & 0xFFFFu;
-    uint32_t w[12];
-    if (msk & 3u) philox4x32_10(q3, c, slo, shi, k0, k1, w);
-    if (msk & 6u) philox4x32_10(q3 + 1u, c, slo, shi, k0, k1, w + 4);
-    if (msk & 12u) philox4x32_10(q3 + 2u, c, slo, shi, k0, k1, w + 8);
-    // every read before any write: the elements are one colour, so no update
-    // here changes a spin another element of the quad reads
-    float sx[4], sy[4], sz[4], vx[4], vy[4], vz[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        if (!((msk >> m) & 1u)) continue;
-        const uint32_t b = b0 + 3u * __builtin_popcount(msk & ((1u << m) - 1u));
-        // the slot's six neighbour indices: 12 B at byte 4 b
-        const uint32_t* nbw = (const uint32_t*)(NB + 2 * b);
-        const uint32_t nw[3] = {nbw[0], nbw[1], nbw[2]};
-        sx[m] = ls[b]; sy[m] = ls[b + 1]; sz[m] = ls[b + 2];
-        vx[m] = 0.f; vy[m] = 0.f; vz[m] = 0.f;
-#pragma unroll
-        for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-            const uint32_t o = (nw[g >> 1] >> (16 * (g & 1))) & 0xFFFFu;
-            add_bond<DM, DIM3>(st, g, ls[o], ls[o + 1], ls[o + 2], vx[m], vy[m], vz[m]);
-        }
-    }
+    const uint32_t msk = qr.y >> 16, q2 = qr.x * 2u;
+    uint32_t w[8];
+    if (msk & 3u) philox4x32_10(q2, c, slo, shi, k0, k1, w);
+    if (msk & 12u) philox4x32_10(q2 + 1u, c, slo, shi, k0, k1, w + 4);
+    // The elements are one colour and read only the other colour, so each
+    // present element is read, updated and written in turn; b walks their
+    // consecutive slots.
+    uint32_t b = qr.y & 0xFFFFu;
     int nacc = 0;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         if (!((msk >> m) & 1u)) continue;
+        // the slot's six neighbour indices: 12 B at byte 4 b
+        const uint32_t* nbw = (const uint32_t*)(NB + 2 * b);
+        const uint32_t nw[3] = {nbw[0], nbw[1], nbw[2]};
+        const float sx = ls[b], sy = ls[b + 1], sz = ls[b + 2];
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+        for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+            const uint32_t o = (nw[g >> 1] >> (16 * (g & 1))) & 0xFFFFu;
+            add_bond<DM, DIM3>(st, g, ls[o], ls[o + 1], ls[o + 2], vx, vy, vz);
+        }
         float tx, ty, tz;
-        if (heis_decide<CONE>(in, sx[m], sy[m], sz[m], vx[m], vy[m], vz[m], w[3 * m], w[3 * m + 1], w[3 * m + 2],
-                              tx, ty, tz)) {
-            const uint32_t b = b0 + 3u * __builtin_popcount(msk & ((1u << m) - 1u));
+        if (heis_decide<CONE>(in, sx, sy, sz, vx, vy, vz, w[2 * m], w[2 * m + 1], tx, ty, tz)) {
             ls[b] = tx; ls[b + 1] = ty; ls[b + 2] = tz;
             ++nacc;
         }
+        b += 3u;
     }
     return nacc;
 }

@@ -80,9 +80,15 @@ def test_heisenberg_dmi_observables_match_oracle_chain(orc):
         st = estimators(E, M, beta, N)
         st["absQ"] = np.abs(Q).mean()
         ref.append(st)
-    # the plaquette charge on a torus is an integer for every configuration
-    # (to ~1e-7 here: the stored fp32 spins are unit vectors only to 1e-7)
-    assert np.allclose(qw[:, :, 0], np.round(qw[:, :, 0]), atol=1e-5)
+    # the plaquette charge on a torus is an integer for unit spins; the stored
+    # fp32 spins are unit vectors to ~1e-7, which moves a triangle's solid
+    # angle by ~1e-7 / |s_i + s_j|, so a sample holding a nearly antiparallel
+    # neighbour pair (|s_i + s_j| ~ 1e-3, about one per 10^5 samples here)
+    # is off by more than 1e-5.  (The charge of given spins matches the
+    # oracle's FP64 calc_TC to 1e-10: test_measure_matches_oracle.)
+    off = np.abs(qw[:, :, 0] - np.round(qw[:, :, 0]))
+    print("charge: %d of %d samples off an integer by > 1e-5, max %.3g" % ((off > 1e-5).sum(), off.size, off.max()))
+    assert (off > 1e-5).sum() <= 1e-4 * off.size and off.max() < 1e-2
     for k in ("E", "absM", "C", "chi", "absQ"):
         m, e = replica_stats([p[k] for p in dev])
         mr, er = replica_stats([p[k] for p in ref])
