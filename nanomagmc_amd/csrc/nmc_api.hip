@@ -92,6 +92,12 @@ struct Set {
     QTile* d_qt = nullptr;   // shaped Heisenberg lattices: colour-0 tiles, then colour-1 tiles
     int n_qt[2] = {0, 0};
     uint32_t* d_qlist = nullptr;   // their non-empty quads (first colour element of each)
+    SpanInst* d_si = nullptr;      // shaped lattices with open boundaries streamed from span copies: records ...
+    STile* d_stl = nullptr;        // ... tiles, colour 0 then colour 1 ...
+    int n_st[2] = {0, 0};
+    uint2* d_slist = nullptr;      // ... quads holding a present site {e0, packed offset} ...
+    int2* d_rowtab = nullptr;      // ... row tables ...
+    float* d_span = nullptr;       // ... and the packed copies (zeroed when created)
     CInst* d_cl = nullptr;         // shaped Heisenberg lattices swept compacted in LDS
     int n_cl = 0;
     uint32_t* d_cs_e = nullptr;    // their present sites (element index) ...
@@ -110,6 +116,9 @@ struct Set {
         (void)hipFree(d_tacc); (void)hipFree(d_qt); (void)hipFree(d_qlist);
         (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb); (void)hipFree(d_cs_q);
         (void)hipFree(d_atp); (void)hipFree(d_ati); (void)hipFree(d_ans);
+        (void)hipFree(d_si); (void)hipFree(d_stl); (void)hipFree(d_slist); (void)hipFree(d_rowtab);
+        (void)hipFree(d_span);
+        d_si = nullptr; d_stl = nullptr; d_slist = nullptr; d_rowtab = nullptr; d_span = nullptr;
         d_atp = nullptr; d_ati = nullptr; d_ans = nullptr;
         adapt_ready = false;
         d_tacc = nullptr; d_qt = nullptr; d_qlist = nullptr;
@@ -337,7 +346,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     const bool quarter = stream_elems < (int64_t)1024 * full;
     const int stile = quarter ? full / 4 : full;
     s->sweep_block = quarter ? 64 : 256;
-    std::vector<int> lds, qinst, cinst;
+    std::vector<int> lds, qinst, cinst, sinst;
     std::vector<int64_t> mbase;   // shaped lattices: first measured site of each instance in the list
     int64_t msite_total = 0;
     int64_t lds_work = 0;
@@ -365,7 +374,10 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             s->lds_bytes = std::max(s->lds_bytes, bytes);
             lds_work = std::max<int64_t>(lds_work, c->ising ? (d.S + 1) / 2 : (d.S + 3) / 4);
         } else if (quad_lists) {
-            qinst.push_back(i);
+            // open boundaries: the packed span copy (rows hold whole quads,
+            // a2 being padded to a multiple of 8); periodic: the quad list
+            if (!c->periodic && d.half % 4 == 0 && d.half / 4 <= 65535) sinst.push_back(i);
+            else qinst.push_back(i);
         } else {
             for (int64_t e = 0; e < d.S; e += stile) sw.push_back(Tile{i, (int)e});
         }
@@ -458,8 +470,60 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         }
         s->n_qt[col] = (int)qt.size() - (col ? s->n_qt[0] : 0);
     }
-    for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});   // accept counters follow the plain tiles
-    const int n_plain = (int)sw.size() - (int)qt.size();
+    const int n_plain = (int)sw.size();   // accept counters: plain tiles, then quad tiles, then span tiles
+    // streamed shaped lattices with open boundaries: per colour, each row's
+    // kept quads (first to last holding a present site) packed back to back,
+    // the row table, and the quads holding a present site (colour-major, 256
+    // to a tile)
+    std::vector<SpanInst> si(sinst.size());
+    std::vector<int2> rowtab;
+    std::vector<uint2> sl;
+    std::vector<STile> stl;
+    {
+        std::vector<std::vector<uint2>> quads(2 * sinst.size());
+        int64_t span_floats = 0;
+        for (size_t j = 0; j < sinst.size(); ++j) {
+            const DevInst& d = c->din[sinst[j]];
+            const int rows = d.a0 * d.a1, nq = d.half / 4;
+            for (int col = 0; col < 2; ++col) {
+                const uint32_t* mw = c->h_mask.data() + d.mask_base + (int64_t)col * d.mask_stride;
+                auto nib = [&](int64_t e) { return (mw[e >> 5] >> (e & 31)) & 0xFu; };
+                si[j].rt[col] = (int64_t)rowtab.size();
+                std::vector<uint2>& qv = quads[2 * j + col];
+                int64_t P = 0;
+                for (int r = 0; r < rows; ++r) {
+                    const int64_t e_row = (int64_t)r * d.half;
+                    int lo = -1, hi = -1;
+                    for (int q = 0; q < nq; ++q)
+                        if (nib(e_row + 4 * q)) { if (lo < 0) lo = q; hi = q + 1; }
+                    if (lo < 0) { rowtab.push_back(make_int2((int)P, 0)); continue; }
+                    rowtab.push_back(make_int2((int)P, lo | (hi << 16)));
+                    for (int q = lo; q < hi; ++q)
+                        if (nib(e_row + 4 * q)) qv.push_back(make_uint2((uint32_t)(e_row + 4 * q), (uint32_t)(P + 4 * (q - lo))));
+                    P += 4 * (hi - lo);
+                }
+                si[j].stride[col] = std::max<int64_t>(64, (P + 63) / 64 * 64);
+                si[j].base[col] = span_floats;
+                span_floats += 3 * si[j].stride[col];
+            }
+        }
+        for (int col = 0; col < 2; ++col) {
+            for (size_t j = 0; j < sinst.size(); ++j) {
+                const std::vector<uint2>& qv = quads[2 * j + col];
+                const size_t q0 = sl.size();
+                sl.insert(sl.end(), qv.begin(), qv.end());
+                if (sl.size() > (size_t)INT32_MAX) return fail(NMC_E_UNSUPPORTED, "too many quads in one sweep set");
+                for (size_t q = 0; q < qv.size(); q += 256)
+                    stl.push_back(STile{sinst[j], (int)j, (int)(q0 + q), (int)std::min<size_t>(256, qv.size() - q)});
+            }
+            s->n_st[col] = (int)stl.size() - (col ? s->n_st[0] : 0);
+        }
+        if (span_floats > 0) {
+            int rc = dalloc(&s->d_span, (size_t)span_floats);
+            if (rc) return rc;
+            NMC_HIP(hipMemsetAsync(s->d_span, 0, (size_t)span_floats * sizeof(float), c->stream));
+        }
+    }
     // compacted shaped lattices: present sites per colour, neighbour indices;
     // instances are independent, so their tables are filled in parallel
     std::vector<CInst> cl(cinst.size());
@@ -567,9 +631,13 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                                   : (int)std::min<int64_t>(1024, std::max<int64_t>(64, (cmax_quads + 63) / 64 * 64));
     if ((rc = upload(c, &s->d_qt, qt))) return rc;
     if ((rc = upload(c, &s->d_qlist, ql))) return rc;
-    sw.resize(n_plain);
     if ((rc = upload(c, &s->d_sweep, sw))) return rc;
     for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});
+    for (const STile& t : stl) sw.push_back(Tile{t.inst, t.q0});
+    if ((rc = upload(c, &s->d_si, si))) return rc;
+    if ((rc = upload(c, &s->d_stl, stl))) return rc;
+    if ((rc = upload(c, &s->d_slist, sl))) return rc;
+    if ((rc = upload(c, &s->d_rowtab, rowtab))) return rc;
     if ((rc = upload(c, &s->d_init, ini))) return rc;
     if ((rc = upload(c, &s->d_mt, mt))) return rc;
     if ((rc = upload(c, &s->d_msites, msites))) return rc;
@@ -1038,9 +1106,22 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
                              std::min(s->n_lds, 65535), s->lds_block, s->lds_bytes, ctx->d_inst, s->d_lds, s->n_lds,
                              ctx->pool, ctx->mpool, ctx->d_tab, ctx->d_sweeps, ctx->st,
                              std::min(kLdsSweepChunk, nsweeps - k), (uint32_t)k, slo, shi, ctx->d_acc);
-    const bool streaming = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    // span copies: filled before the passes, written back after them
+    const int nst = s->n_st[0] + s->n_st[1];
+    for (int col = 0; col < 2 && nst > 0 && nsweeps > 0 && e == hipSuccess; ++col)
+        e = launch_span_copy(ctx->stream, true, ctx->d_inst, s->d_si, s->d_stl + (col ? s->n_st[0] : 0), s->n_st[col],
+                             s->d_slist, (float*)ctx->pool, s->d_span, col);
+    const bool streaming = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0 || nst > 0;
+    const int st_acc = s->n_sweep + s->n_qt[0] + s->n_qt[1];   // first span tile's accept counter
     for (int k = 0; k < nsweeps && streaming && e == hipSuccess; ++k)
         for (int col = 0; col < 2 && e == hipSuccess; ++col) {
+            if (s->n_st[col] > 0) {
+                const int off = col ? s->n_st[0] : 0;
+                e = launch_sweep_span(ctx->stream, ctx->dim3, ctx->dm, ctx->cone, true, ctx->d_inst, s->d_si,
+                                      s->d_stl + off, s->n_st[col], s->d_slist, s->d_rowtab, s->d_span, ctx->mpool,
+                                      ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc + st_acc + off);
+                if (e != hipSuccess) continue;
+            }
             if (s->n_qt[col] > 0) {
                 const int off = col ? s->n_qt[0] : 0;
                 e = launch_sweep_heis_q(ctx->stream, ctx->dim3, ctx->periodic, ctx->dm, ctx->cone, true,
@@ -1059,11 +1140,14 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
                                       s->sweep_block, ctx->d_inst, s->d_sweep, s->n_sweep, (float*)ctx->pool, ctx->mpool,
                                       ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi, s->d_tacc);
         }
+    for (int col = 0; col < 2 && nst > 0 && nsweeps > 0 && e == hipSuccess; ++col)
+        e = launch_span_copy(ctx->stream, false, ctx->d_inst, s->d_si, s->d_stl + (col ? s->n_st[0] : 0),
+                             s->n_st[col], s->d_slist, (float*)ctx->pool, s->d_span, col);
     return e;
 }
 
 int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
-    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1] > 0;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     NMC_HIP(hipStreamIsCapturing(ctx->stream, &cap));
     const bool side = s->n_cl > 0 && streamed && nsweeps > 0 && cap == hipStreamCaptureStatusNone;
@@ -1094,7 +1178,7 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
     // relative and its last node advances the device sweep counters, so
     // replays continue the same random streams.  Sets swept only in LDS
     // launch once per 256 sweeps anyway and skip the capture (~3 ms).
-    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1] > 0;
     if (ctx->graphs_on && s->sweep_block == 64 && streamed && nsweeps >= 2 * kGraphSweeps) {
         auto key = std::make_tuple(s->list, -1, kGraphSweeps);
         auto it = ctx->graphs.find(key);
@@ -1158,7 +1242,7 @@ int nmc_sweep_adapt(nmc_ctx* ctx, const int* inst, int n, int nsweeps, int chunk
     if (int rc = nmc_reset_accepts(ctx)) return rc;
     // as nmc_sweep: small streamed sets replay one captured graph of about
     // kGraphSweeps sweeps (gc whole chunks, each with its adaptation step)
-    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0;
+    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1] > 0;
     const int gc = std::max(1, kGraphSweeps / chunk);
     const int reps = nsweeps / (gc * chunk);
     int done = 0;

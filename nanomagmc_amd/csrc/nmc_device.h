@@ -115,6 +115,28 @@ struct QTile {
     int qn;
 };
 
+// Streamed shaped lattices with open boundaries are swept from a packed
+// working copy ("spans"): per colour, each row (i0, i1) keeps only its quads
+// from the first to the last one holding a present site, rows back to back,
+// x, y and z in arrays of `stride` floats each.  Row r of colour c is
+// rowtab[rt[c] + r] = {float offset of its first kept quad, first kept quad |
+// end quad << 16}; every quad outside [first, end) of its row is empty.  The
+// copy is filled from the colour-split arrays before the sweeps of a call
+// and written back after them.
+struct SpanInst {
+    int64_t base[2];    // float offset of colour c's x array in the span pool
+    int64_t stride[2];  // floats per component array of colour c
+    int64_t rt[2];      // first row-table entry of colour c
+};
+
+// Work item of a span pass: entries q0 .. q0 + qn - 1 (qn <= 256) of the
+// span quad list, all of one instance (span record `span`) and one colour;
+// each entry is {first colour element e0 of a quad holding a present site,
+// float offset of that quad in its colour's packed arrays}.
+struct STile {
+    int inst, span, q0, qn;
+};
+
 // Shaped instance in compacted form for LDS-resident sweeps: its present
 // sites, colour 0 first (n0 of them) then colour 1 (n1), stored from site0 on
 // in the per-set site tables (element index; six neighbours, each the float

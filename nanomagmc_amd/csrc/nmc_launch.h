@@ -19,6 +19,18 @@ hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, int
                                int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                                unsigned long long* acc);
 
+// Shaped lattices with open boundaries, swept from their packed span copy
+// (see SpanInst); the copy is filled (to_span) and written back by
+// launch_span_copy over the same tiles.
+hipError_t launch_sweep_span(hipStream_t stream, bool dim3, int dm, bool cone, bool count, const DevInst* insts,
+                             const SpanInst* spans, const STile* tiles, int ntiles, const uint2* qlist,
+                             const int2* rowtab, float* span, const uint32_t* mpool, const uint64_t* sweeps,
+                             const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                             unsigned long long* acc);
+hipError_t launch_span_copy(hipStream_t stream, bool to_span, const DevInst* insts, const SpanInst* spans,
+                            const STile* tiles, int ntiles, const uint2* qlist, float* pool, float* span,
+                            int colour);
+
 hipError_t launch_sweep_ising(hipStream_t stream, bool dim3, bool periodic, bool masked,
                               bool count, int block, const DevInst* insts, const Tile* tiles, int ntiles,
                               int8_t* pool, const uint32_t* mpool, const uint32_t* tab,

@@ -381,6 +381,159 @@ __global__ __launch_bounds__(256) void k_sweep_heis_q(const DevInst* __restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Span passes: shaped lattices with open boundaries, streamed from their
+// packed span copy (SpanInst).  A bounding box of a grain is about half
+// empty, and the quad kernel above still moved every 128-byte line that a
+// listed quad or its neighbour rows touch, absent sites included.  The packed
+// rows hold only their kept quads, so the lines a pass moves are nearly all
+// present sites.  span_quad does heis_quad's vector-path arithmetic on the
+// same values -- a neighbour quad outside its row's kept range is empty and
+// reads as zero, as the absent sites of the box do -- so a span pass gives the
+// same lattice bit for bit.
+// ---------------------------------------------------------------------------
+
+// The quad q of row `row` of one colour's packed arrays: its float offset,
+// or -1 if it lies outside the row's kept range (an empty quad).
+__device__ __forceinline__ int span_at(const int2* __restrict__ rt, int row, int q) {
+    const int2 e = rt[row];
+    const int lo = e.y & 0xFFFF, hi = (int)((uint32_t)e.y >> 16);
+    return (q >= lo && q < hi) ? e.x + 4 * (q - lo) : -1;
+}
+
+template <bool DIM3, int DM, bool CONE>
+__device__ __forceinline__ int span_quad(const DevInst& in, const SpanInst& sp, const Stencil& st,
+                                         const int2* __restrict__ rowtab, float* span, const uint32_t* mw,
+                                         uint32_t e0, uint32_t o, int colour, uint64_t sweep, uint32_t seed_lo,
+                                         uint32_t seed_hi) {
+    const uint32_t qmask = (mw[colour * (uint32_t)in.mask_stride + (e0 >> 5)] >> (e0 & 31)) & 0xFu;
+    uint32_t w[8];
+    {
+        const uint32_t q2 = e0 >> 1;
+        const uint32_t slo = (uint32_t)sweep, shi = (uint32_t)(sweep >> 32), k1 = seed_hi + in.inst_id;
+        if (qmask & 3u) philox4x32_10(q2, (uint32_t)colour, slo, shi, seed_lo, k1, w);
+        if (qmask & 12u) philox4x32_10(q2 + 1, (uint32_t)colour, slo, shi, seed_lo, k1, w + 4);
+    }
+    const int c2 = colour ^ 1;
+    float* own_x = span + sp.base[colour];
+    float* own_y = own_x + sp.stride[colour];
+    float* own_z = own_y + sp.stride[colour];
+    const float* oth_x = span + sp.base[c2];
+    const float* oth_y = oth_x + sp.stride[c2];
+    const float* oth_z = oth_y + sp.stride[c2];
+    const int2* rt = rowtab + sp.rt[c2];   // the other colour's rows
+    const SitePos p = site_of(in, e0, colour, DIM3);
+    const int qi = p.k >> 2;
+    const float4 sx = ld4(own_x, o), sy = ld4(own_y, o), sz = ld4(own_z, o);
+    float vx[4] = {0.f, 0.f, 0.f, 0.f}, vy[4] = {0.f, 0.f, 0.f, 0.f}, vz[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = DIM3 ? 0 : 2; g < 4; ++g) {
+        int i0 = p.i0, i1 = p.i1;
+        bool ok;
+        if (g < 2) { i0 += (g & 1) ? 1 : -1; ok = i0 >= 0 && i0 < in.a0; }
+        else { i1 += (g & 1) ? 1 : -1; ok = i1 >= 0 && i1 < in.a1; }
+        const int u = ok ? span_at(rt, i0 * in.a1 + i1, qi) : -1;
+        float4 nx = make_float4(0.f, 0.f, 0.f, 0.f), ny = nx, nz = nx;
+        if (u >= 0) { nx = ld4(oth_x, (uint32_t)u); ny = ld4(oth_y, (uint32_t)u); nz = ld4(oth_z, (uint32_t)u); }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) add_bond<DM, DIM3>(st, g, f4(nx, j), f4(ny, j), f4(nz, j), vx[j], vy[j], vz[j]);
+    }
+    // axis 2 within the row: the same quad of the other colour, and the
+    // element just outside it (the last of quad qi - 1 or the first of qi + 1)
+    {
+        const int row = (int)p.row;
+        const int ua = span_at(rt, row, qi);
+        float4 ax = make_float4(0.f, 0.f, 0.f, 0.f), ay = ax, az = ax;
+        if (ua >= 0) { ax = ld4(oth_x, (uint32_t)ua); ay = ld4(oth_y, (uint32_t)ua); az = ld4(oth_z, (uint32_t)ua); }
+        const int ue = p.q == 0 ? span_at(rt, row, qi - 1) : span_at(rt, row, qi + 1);
+        float ex = 0.f, ey = 0.f, ez = 0.f;
+        if (ue >= 0) {
+            const uint32_t u = (uint32_t)ue + (p.q == 0 ? 3u : 0u);
+            ex = ld32(oth_x, u); ey = ld32(oth_y, u); ez = ld32(oth_z, u);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float mx, my, mz, qx, qy, qz;
+            if (p.q == 0) {
+                mx = j ? f4(ax, j - 1) : ex; my = j ? f4(ay, j - 1) : ey; mz = j ? f4(az, j - 1) : ez;
+                qx = f4(ax, j); qy = f4(ay, j); qz = f4(az, j);
+            } else {
+                mx = f4(ax, j); my = f4(ay, j); mz = f4(az, j);
+                qx = j < 3 ? f4(ax, j + 1) : ex; qy = j < 3 ? f4(ay, j + 1) : ey; qz = j < 3 ? f4(az, j + 1) : ez;
+            }
+            add_bond<DM, DIM3>(st, 4, mx, my, mz, vx[j], vy[j], vz[j]);
+            add_bond<DM, DIM3>(st, 5, qx, qy, qz, vx[j], vy[j], vz[j]);
+        }
+    }
+    float ox[4], oy[4], oz[4];
+    int nacc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float cx = f4(sx, j), cy = f4(sy, j), cz = f4(sz, j);
+        ox[j] = cx; oy[j] = cy; oz[j] = cz;
+        if (!((qmask >> j) & 1u)) continue;
+        float tx, ty, tz;
+        if (heis_decide<CONE>(in, cx, cy, cz, vx[j], vy[j], vz[j], w[2 * j], w[2 * j + 1], tx, ty, tz)) {
+            ox[j] = tx; oy[j] = ty; oz[j] = tz;
+            ++nacc;
+        }
+    }
+    st4(own_x, o, make_float4(ox[0], ox[1], ox[2], ox[3]));
+    st4(own_y, o, make_float4(oy[0], oy[1], oy[2], oy[3]));
+    st4(own_z, o, make_float4(oz[0], oz[1], oz[2], oz[3]));
+    return nacc;
+}
+
+template <bool DIM3, int DM, bool CONE, bool COUNT>
+__global__ __launch_bounds__(256) void k_sweep_span(const DevInst* __restrict__ insts,
+                                                    const SpanInst* __restrict__ spans,
+                                                    const STile* __restrict__ tiles, int ntiles,
+                                                    const uint2* __restrict__ qlist,
+                                                    const int2* __restrict__ rowtab, float* __restrict__ span,
+                                                    const uint32_t* __restrict__ mpool,
+                                                    const uint64_t* __restrict__ sweeps, Stencil st, int colour,
+                                                    uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                                                    unsigned long long* __restrict__ acc) {
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const STile tl = tiles[t];
+    const DevInst& in = insts[tl.inst];
+    int nacc = 0;
+    if ((int)threadIdx.x < tl.qn) {
+        const uint2 q = qlist[tl.q0 + (int)threadIdx.x];
+        nacc = span_quad<DIM3, DM, CONE>(in, spans[tl.span], st, rowtab, span, mpool + in.mask_base, q.x, q.y,
+                                         colour, sweeps[tl.inst] + s_off, seed_lo, seed_hi);
+    }
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        tile_count(nacc, acc, t);
+    }
+}
+
+// Copies the listed quads of one colour between the colour-split arrays and
+// the packed span copy (TO_SPAN: into the copy).  Quads of a kept range that
+// hold no present site are never listed: they stay zero in the copy, which is
+// zeroed when it is created.
+template <bool TO_SPAN>
+__global__ __launch_bounds__(256) void k_span_copy(const DevInst* __restrict__ insts,
+                                                   const SpanInst* __restrict__ spans,
+                                                   const STile* __restrict__ tiles, const uint2* __restrict__ qlist,
+                                                   float* __restrict__ pool, float* __restrict__ span, int colour) {
+    const STile tl = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tl.qn) return;
+    const DevInst& in = insts[tl.inst];
+    const SpanInst& sp = spans[tl.span];
+    const uint2 q = qlist[tl.q0 + (int)threadIdx.x];
+    float* d = pool + in.base + (int64_t)colour * 3 * in.Sp;
+    float* s = span + sp.base[colour];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float* da = d + (int64_t)a * in.Sp;
+        float* sa = s + (int64_t)a * sp.stride[colour];
+        if (TO_SPAN) st4(sa, q.y, ld4(da, q.x));
+        else st4(da, q.x, ld4(sa, q.y));
+    }
+}
+
 template <bool DIM3, bool PERIODIC, bool MASKED, bool COUNT>
 __global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__ insts,
                                                      const Tile* __restrict__ tiles, int ntiles,
@@ -653,6 +806,35 @@ hipError_t launch_sweep_heis_q(hipStream_t stream, bool dim3, bool periodic, int
     NMC_HEISQ_B(false, true)
 #undef NMC_HEISQ_B
 #undef NMC_HEISQ_D
+    return hipGetLastError();
+}
+
+#define NMC_SPAN_CASE(D3, UD, CN, CT)                                                                     \
+    if (dim3 == D3 && dm == UD && cone == CN && count == CT)                                               \
+        k_sweep_span<D3, UD, CN, CT><<<ntiles, 256, 0, stream>>>(insts, spans, tiles, ntiles, qlist, rowtab, span, \
+                                                                mpool, sweeps, st, colour, s_off, seed_lo, seed_hi, acc);
+
+hipError_t launch_sweep_span(hipStream_t stream, bool dim3, int dm, bool cone, bool count, const DevInst* insts,
+                             const SpanInst* spans, const STile* tiles, int ntiles, const uint2* qlist,
+                             const int2* rowtab, float* span, const uint32_t* mpool, const uint64_t* sweeps,
+                             const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                             unsigned long long* acc) {
+    if (ntiles <= 0) return hipSuccess;
+#define NMC_SPAN_D(D3, UD)                                                                              \
+    NMC_SPAN_CASE(D3, UD, false, false) NMC_SPAN_CASE(D3, UD, false, true) NMC_SPAN_CASE(D3, UD, true, false) \
+    NMC_SPAN_CASE(D3, UD, true, true)
+    NMC_SPAN_D(true, 0) NMC_SPAN_D(true, 1) NMC_SPAN_D(true, 2)
+    NMC_SPAN_D(false, 0) NMC_SPAN_D(false, 1) NMC_SPAN_D(false, 2)
+#undef NMC_SPAN_D
+    return hipGetLastError();
+}
+
+hipError_t launch_span_copy(hipStream_t stream, bool to_span, const DevInst* insts, const SpanInst* spans,
+                            const STile* tiles, int ntiles, const uint2* qlist, float* pool, float* span,
+                            int colour) {
+    if (ntiles <= 0) return hipSuccess;
+    if (to_span) k_span_copy<true><<<ntiles, 256, 0, stream>>>(insts, spans, tiles, qlist, pool, span, colour);
+    else k_span_copy<false><<<ntiles, 256, 0, stream>>>(insts, spans, tiles, qlist, pool, span, colour);
     return hipGetLastError();
 }
 
