@@ -16,6 +16,7 @@
 #include <set>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/nmc.h"
@@ -314,6 +315,36 @@ int upload(nmc_ctx* c, T** dptr, const std::vector<T>& v) {
     return NMC_OK;
 }
 
+// Shaped instances of identical geometry (extents and presence bits) share
+// their work tables: the 64 (T, h) chains of one grain hold one table, which
+// the workgroups sweeping them then read from L2.  rep[j] is the index into
+// `ids` of the first instance with the geometry of ids[j].
+std::vector<int> geometry_reps(const nmc_ctx* c, const std::vector<int>& ids) {
+    std::unordered_map<uint64_t, std::vector<int>> seen;
+    std::vector<int> rep(ids.size());
+    for (size_t j = 0; j < ids.size(); ++j) {
+        const DevInst& d = c->din[ids[j]];
+        const uint32_t* mw = c->h_mask.data() + d.mask_base;
+        uint64_t h = 1469598103934665603ull;   // FNV-1a
+        auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+        mix((uint64_t)d.a0); mix((uint64_t)d.a1); mix((uint64_t)d.a2); mix((uint64_t)d.n2);
+        for (int64_t w = 0; w < 2 * d.mask_stride; ++w) mix(mw[w]);
+        std::vector<int>& cand = seen[h];
+        int r = -1;
+        for (int k : cand) {
+            const DevInst& e = c->din[ids[k]];
+            if (e.a0 == d.a0 && e.a1 == d.a1 && e.a2 == d.a2 && e.n2 == d.n2 && e.mask_stride == d.mask_stride &&
+                std::memcmp(c->h_mask.data() + e.mask_base, mw, (size_t)(2 * d.mask_stride) * sizeof(uint32_t)) == 0) {
+                r = k;
+                break;
+            }
+        }
+        if (r < 0) { cand.push_back((int)j); r = (int)j; }
+        rep[j] = r;
+    }
+    return rep;
+}
+
 // Work decomposition for an instance list (cached per distinct list).
 int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     std::vector<int> list;
@@ -480,9 +511,20 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     std::vector<uint2> sl;
     std::vector<STile> stl;
     {
+        const std::vector<int> srep = geometry_reps(c, sinst);
         std::vector<std::vector<uint2>> quads(2 * sinst.size());
         int64_t span_floats = 0;
         for (size_t j = 0; j < sinst.size(); ++j) {
+            if (srep[j] != (int)j) {   // an identical instance: its row tables and quads, a copy of its own
+                const SpanInst& r = si[srep[j]];
+                for (int col = 0; col < 2; ++col) {
+                    si[j].rt[col] = r.rt[col];
+                    si[j].stride[col] = r.stride[col];
+                    si[j].base[col] = span_floats;
+                    span_floats += 3 * si[j].stride[col];
+                }
+                continue;
+            }
             const DevInst& d = c->din[sinst[j]];
             const int rows = d.a0 * d.a1, nq = d.half / 4;
             for (int col = 0; col < 2; ++col) {
@@ -507,11 +549,17 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                 span_floats += 3 * si[j].stride[col];
             }
         }
+        std::vector<size_t> qstart(2 * sinst.size(), 0);
         for (int col = 0; col < 2; ++col) {
             for (size_t j = 0; j < sinst.size(); ++j) {
-                const std::vector<uint2>& qv = quads[2 * j + col];
-                const size_t q0 = sl.size();
-                sl.insert(sl.end(), qv.begin(), qv.end());
+                const size_t rj = (size_t)srep[j];
+                const std::vector<uint2>& qv = quads[2 * rj + col];
+                size_t q0 = qstart[2 * rj + col];
+                if (rj == j) {
+                    q0 = sl.size();
+                    qstart[2 * j + col] = q0;
+                    sl.insert(sl.end(), qv.begin(), qv.end());
+                }
                 if (sl.size() > (size_t)INT32_MAX) return fail(NMC_E_UNSUPPORTED, "too many quads in one sweep set");
                 for (size_t q = 0; q < qv.size(); q += 256)
                     stl.push_back(STile{sinst[j], (int)j, (int)(q0 + q), (int)std::min<size_t>(256, qv.size() - q)});
@@ -527,8 +575,14 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     // compacted shaped lattices: present sites per colour, neighbour indices;
     // instances are independent, so their tables are filled in parallel
     std::vector<CInst> cl(cinst.size());
+    const std::vector<int> crep = geometry_reps(c, cinst);
     int64_t total_sites = 0, total_quads = 0, cmax_quads = 0;
     for (size_t q = 0; q < cinst.size(); ++q) {
+        if (crep[q] != (int)q) {   // an identical instance: the same tables
+            cl[q] = cl[crep[q]];
+            cl[q].inst = cinst[q];
+            continue;
+        }
         const DevInst& d = c->din[cinst[q]];
         const uint32_t* mw = c->h_mask.data() + d.mask_base;
         int cnt[2] = {0, 0}, nq[2] = {0, 0};
@@ -549,6 +603,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     auto fill = [&](size_t qb, size_t qe) {
         std::vector<int> slot[2];
         for (size_t q = qb; q < qe; ++q) {
+            if (crep[q] != (int)q) continue;   // filled with its representative
             const CInst& ci = cl[q];
             const DevInst& d = c->din[ci.inst];
             const uint32_t* mw = c->h_mask.data() + d.mask_base;
