@@ -167,6 +167,22 @@ int nmc_set_path(nmc_ctx* ctx, int path);
  * depend on it. */
 int nmc_set_lds_block(nmc_ctx* ctx, int block);
 
+/* Work-items per tile of the streaming passes over colour-split boxes: 0
+ * picks by set size (64-work-item quarter tiles when a set has fewer than
+ * 1024 full tiles, else 256), 64 or 256 force one.  Set by the autotuner;
+ * results do not depend on it.  No reference counterpart. */
+int nmc_set_stream_block(nmc_ctx* ctx, int block);
+
+/* The kernel family that sweeps each listed instance when the list is swept
+ * (no reference counterpart; for measurement and tuning).  kind[i] is one of
+ * the NMC_KIND_* values below. */
+#define NMC_KIND_STREAM 0        /* streaming colour passes over the box (k_sweep_heis, k_sweep_ising) */
+#define NMC_KIND_QUADS 1         /* streaming passes over the non-empty quads (periodic shaped lattices) */
+#define NMC_KIND_SPANS 2         /* streaming passes over a packed copy of the occupied row spans (open shaped) */
+#define NMC_KIND_LDS 3           /* whole sweeps with the box resident in one workgroup's LDS */
+#define NMC_KIND_LDS_COMPACT 4   /* whole sweeps with the present sites compacted in LDS (open shaped) */
+int nmc_sweep_kinds(nmc_ctx* ctx, const int* inst, int n, int* kind);
+
 /* Accepted-attempt counters since the last reset (one per instance). */
 int nmc_get_accepts(nmc_ctx* ctx, const int* inst, int n, uint64_t* out);
 int nmc_reset_accepts(nmc_ctx* ctx);

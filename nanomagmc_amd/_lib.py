@@ -17,13 +17,16 @@ NMC_E_UNSUPPORTED = -4
 NMC_TRIAL_SPHERE = 0
 NMC_TRIAL_CONE = 1
 NMC_NOBS = 11
+# sweep kernel families (nmc_sweep_kinds)
+NMC_KIND_STREAM, NMC_KIND_QUADS, NMC_KIND_SPANS, NMC_KIND_LDS, NMC_KIND_LDS_COMPACT = range(5)
+KIND_NAMES = ["stream", "quads", "spans", "lds", "lds_compact"]
 
 OBS_NAMES = ["Mx", "My", "Mz", "subMx", "subMy", "subMz", "sub4Mx", "sub4My", "sub4Mz", "E_r", "E_s"]
 
 # Every symbol include/nmc.h declares, in header order.
 EXPORTS = [
     "nmc_last_error", "nmc_create", "nmc_destroy", "nmc_get_stream", "nmc_sync", "nmc_num_sites", "nmc_num_slices",
-    "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_sweep_adapt", "nmc_set_path", "nmc_set_lds_block", "nmc_get_accepts",
+    "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_sweep_adapt", "nmc_set_path", "nmc_set_lds_block", "nmc_set_stream_block", "nmc_sweep_kinds", "nmc_get_accepts",
     "nmc_reset_accepts", "nmc_snapshot", "nmc_measure", "nmc_measure_noalloc", "nmc_series_reset", "nmc_sample",
     "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_accumulate_avgs", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
     "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_net_create", "nmc_net_destroy",
@@ -83,6 +86,8 @@ def lib():
         "nmc_reset_accepts": (i, [vp]),
         "nmc_snapshot": (i, [vp, P(i), P(i), i]),
         "nmc_measure": (i, [vp, P(i), i, P(d), P(d), P(d)]),
+        "nmc_set_stream_block": (i, [vp, i]),
+        "nmc_sweep_kinds": (i, [vp, P(i), i, P(i)]),
         "nmc_measure_noalloc": (i, [vp, P(i), i, P(d), P(d), P(d)]),
         "nmc_series_reset": (i, [vp, i, i]),
         "nmc_sample": (i, [vp, P(i), i, i]),

@@ -73,6 +73,7 @@ struct Set {
     Tile* d_sweep = nullptr;
     int n_sweep = 0;
     int sweep_block = 256;   // work-items per streaming tile (64 for small sets)
+    bool small = false;      // too few elements for 1024 full tiles: launch-bound, long calls replay graphs
     std::vector<Tile> h_sweep;                // host copy (tile -> instance)
     unsigned long long* d_tacc = nullptr;     // accepted attempts per streaming tile
     Tile* d_init = nullptr;
@@ -108,6 +109,7 @@ struct Set {
     int c_block = 256;
     size_t lds_bytes = 0;    // dynamic LDS of the largest of them
     int lds_block = 256;
+    std::vector<int> kind;   // NMC_KIND_* of each list entry
 
     void release() {
         (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
@@ -219,6 +221,7 @@ struct nmc_ctx {
 
     int path = NMC_PATH_AUTO;
     int lds_block = 0;        // 0 = derived from the largest LDS-resident instance
+    int stream_block = 0;     // work-items per streaming tile: 0 = by set size, 64 (quarter tiles) or 256
     size_t max_lds = 65536;     // dynamic LDS one k_sweep_lds workgroup may use on this device
     size_t max_lds_c = 65536;   // the same for k_sweep_lds_c
     double* scratch = nullptr;  // this device's copy of the static scratch (nmc_measure_noalloc)
@@ -374,7 +377,8 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     int64_t stream_elems = 0;
     for (int i : list) stream_elems += c->din[i].S;
     const int full = c->ising ? kIsingTile : kSweepTile;
-    const bool quarter = stream_elems < (int64_t)1024 * full;
+    s->small = stream_elems < (int64_t)1024 * full;
+    const bool quarter = c->stream_block ? c->stream_block == 64 : s->small;
     const int stile = quarter ? full / 4 : full;
     s->sweep_block = quarter ? 64 : 256;
     std::vector<int> lds, qinst, cinst, sinst;
@@ -383,7 +387,9 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     int64_t lds_work = 0;
     const bool quad_lists = c->masked && !c->ising;
     const bool compact = quad_lists && !c->periodic && c->path == NMC_PATH_AUTO;
-    for (int i : list) {
+    s->kind.assign(list.size(), NMC_KIND_STREAM);
+    for (size_t li = 0; li < list.size(); ++li) {
+        const int i = list[li];
         const DevInst& d = c->din[i];
         bool fits_compact = false;
         if (compact) {
@@ -400,15 +406,17 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                              (c->masked ? (size_t)(2 * d.mask_stride * 4 + 15) / 16 * 16 : 0);
         if (fits_compact) {
             cinst.push_back(i);
+            s->kind[li] = NMC_KIND_LDS_COMPACT;
         } else if (c->path == NMC_PATH_AUTO && bytes <= c->max_lds) {
             lds.push_back(i);
+            s->kind[li] = NMC_KIND_LDS;
             s->lds_bytes = std::max(s->lds_bytes, bytes);
             lds_work = std::max<int64_t>(lds_work, c->ising ? (d.S + 1) / 2 : (d.S + 3) / 4);
         } else if (quad_lists) {
             // open boundaries: the packed span copy (rows hold whole quads,
             // a2 being padded to a multiple of 8); periodic: the quad list
-            if (!c->periodic && d.half % 4 == 0 && d.half / 4 <= 65535) sinst.push_back(i);
-            else qinst.push_back(i);
+            if (!c->periodic && d.half % 4 == 0 && d.half / 4 <= 65535) { sinst.push_back(i); s->kind[li] = NMC_KIND_SPANS; }
+            else { qinst.push_back(i); s->kind[li] = NMC_KIND_QUADS; }
         } else {
             for (int64_t e = 0; e < d.S; e += stile) sw.push_back(Tile{i, (int)e});
         }
@@ -1203,9 +1211,10 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
 
 int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
     const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1] > 0;
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    NMC_HIP(hipStreamIsCapturing(ctx->stream, &cap));
-    const bool side = s->n_cl > 0 && streamed && nsweeps > 0 && cap == hipStreamCaptureStatusNone;
+    // also under capture: the auxiliary stream joins the capture through the
+    // fork event and is joined back before the capture ends, so the graph
+    // holds the compacted and streamed sweeps as parallel branches
+    const bool side = s->n_cl > 0 && streamed && nsweeps > 0;
     if (side) {
         NMC_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
         NMC_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
@@ -1234,7 +1243,7 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
     // replays continue the same random streams.  Sets swept only in LDS
     // launch once per 256 sweeps anyway and skip the capture (~3 ms).
     const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1] > 0;
-    if (ctx->graphs_on && s->sweep_block == 64 && streamed && nsweeps >= 2 * kGraphSweeps) {
+    if (ctx->graphs_on && s->small && streamed && nsweeps >= 2 * kGraphSweeps) {
         auto key = std::make_tuple(s->list, -1, kGraphSweeps);
         auto it = ctx->graphs.find(key);
         if (it == ctx->graphs.end()) {
@@ -1301,7 +1310,7 @@ int nmc_sweep_adapt(nmc_ctx* ctx, const int* inst, int n, int nsweeps, int chunk
     const int gc = std::max(1, kGraphSweeps / chunk);
     const int reps = nsweeps / (gc * chunk);
     int done = 0;
-    if (ctx->graphs_on && s->sweep_block == 64 && streamed && reps >= 2) {
+    if (ctx->graphs_on && s->small && streamed && reps >= 2) {
         auto key = std::make_tuple(s->list, -2, chunk);
         auto it = ctx->graphs.find(key);
         if (it == ctx->graphs.end()) {
@@ -1358,6 +1367,23 @@ int nmc_set_lds_block(nmc_ctx* ctx, int block) {
     if (block == ctx->lds_block) return NMC_OK;
     if (int rc = release_sets(ctx)) return rc;
     ctx->lds_block = block;
+    return NMC_OK;
+}
+
+int nmc_set_stream_block(nmc_ctx* ctx, int block) {
+    if (!ctx) return fail(NMC_E_INVALID, "null context");
+    if (block != 0 && block != 64 && block != 256) return fail(NMC_E_INVALID, "stream block must be 0, 64 or 256");
+    if (block == ctx->stream_block) return NMC_OK;
+    if (int rc = release_sets(ctx)) return rc;
+    ctx->stream_block = block;
+    return NMC_OK;
+}
+
+int nmc_sweep_kinds(nmc_ctx* ctx, const int* inst, int n, int* kind) {
+    if (!ctx || !kind) return fail(NMC_E_INVALID, "null argument");
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    std::copy(s->kind.begin(), s->kind.end(), kind);
     return NMC_OK;
 }
 

@@ -174,6 +174,18 @@ class Lattices:
                                     out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
 
+    def set_stream_block(self, block):
+        """Work-items per streaming tile: 0 (by set size), 64 or 256."""
+        check(lib().nmc_set_stream_block(self._h, int(block)))
+
+    def sweep_kinds(self, inst=None):
+        """NMC_KIND_* of the kernel family sweeping each listed instance."""
+        a, n = _ilist(inst)
+        cnt = n if a is not None else self.ninst
+        out = np.zeros(cnt, dtype=np.int32)
+        check(lib().nmc_sweep_kinds(self._h, _iptr(a), n, _iptr(out)))
+        return out
+
     def set_path(self, path):
         """'auto' (LDS-resident sweeps where the lattice fits) or 'stream'."""
         check(lib().nmc_set_path(self._h, {"auto": 0, "stream": 1}[path]))

@@ -45,3 +45,27 @@ def mean_err(x):
     x = np.asarray(x, dtype=np.float64)
     tau, _ = tau_int(x)
     return x.mean(), math.sqrt(2.0 * tau * x.var(ddof=1) / x.size), tau
+
+
+def tau_int_batch(X, c=6.0):
+    """tau_int of every row of X (chains x samples) at once, the same
+    estimator as tau_int: tau(W) = 1/2 + sum_{t=1}^{W} rho(t) at the first
+    W >= c tau(W).  Returns (tau, closed): closed is False where no window
+    closed before the end of the series (tau is then a lower bound)."""
+    X = np.asarray(X, dtype=np.float64)
+    m, n = X.shape
+    X = X - X.mean(axis=1, keepdims=True)
+    v = (X * X).sum(axis=1) / n
+    f = np.fft.rfft(X, 2 * n, axis=1)
+    rho = np.fft.irfft(f * np.conj(f), axis=1)[:, :n]
+    safe = np.where(v > 0, v, 1.0)
+    rho = rho / (n * safe[:, None])
+    rho[v == 0] = 1.0                                  # a constant series counts as frozen, as in autocorr
+    cum = 0.5 + np.cumsum(rho[:, 1:], axis=1)          # tau(W), W = 1 .. n-1
+    W = np.arange(1, n)[None, :]
+    ok = W >= c * cum
+    has = ok.any(axis=1)
+    first = ok.argmax(axis=1)
+    tau = np.where(has, cum[np.arange(m), first], cum[:, -1])
+    tau = np.maximum(tau, 0.5)
+    return tau, has & (np.where(has, first + 1, n) < n / 5)

@@ -227,6 +227,27 @@ class Field:
         return rate, secs.value
 
 
+def bench_lib(out_dir):
+    """The restatement built with `make benchref` into out_dir (the timed CPU
+    baseline, compiled for the host it runs on); returns its
+    orc_bench_reference."""
+    subprocess.run(["make", "-s", "-C", HERE, "benchref", "OUT=" + os.path.abspath(out_dir)], check=True)
+    L = C.CDLL(os.path.join(os.path.abspath(out_dir), "libbenchref.so"))
+    P, i, d, u32, i64 = C.POINTER, C.c_int, C.c_double, C.c_uint32, C.c_int64
+    f = L.orc_bench_reference
+    f.argtypes = [i, P(i), P(i), P(i), i, P(d), P(d), i, i, i, d, d, i, i64, u32, P(d)]
+    f.restype = d
+    return f
+
+
+def bench_reference(fn, field, ising, h, beta, nthreads, attempts, use_j=True, use_d=False, seed=1):
+    """Field.bench_reference through another build of the restatement."""
+    secs = C.c_double()
+    rate = fn(field.n, *field._args(), int(ising), int(use_j), int(use_d), float(h), float(beta), int(nthreads),
+              int(attempts), int(seed), C.byref(secs))
+    return rate, secs.value
+
+
 def accept_threshold(dE, beta):
     return lib().orc_accept_threshold(float(dE), float(beta))
 

@@ -415,18 +415,28 @@ def test_device_cone_adaptation_matches_host_steps(orc, bloch3d, kind):
     assert np.all(lats[0].accepts() == 0)
 
 
-def test_graphs_and_direct_launches_agree(orc, bloch3d, monkeypatch):
+@pytest.mark.parametrize("kind", ["periodic", "grains"])
+def test_graphs_and_direct_launches_agree(orc, bloch3d, monkeypatch, kind):
     # a small streamed set replays captured graphs (plain sweeps, adapted
     # sweeps, the sample loop); NMC_NO_GRAPHS=1 launches the same kernels
-    # directly: lattices and series must agree bit for bit
+    # directly: lattices and series must agree bit for bit.  "grains": open
+    # grains swept compacted in LDS and from span copies, whose graphs hold
+    # the two as branches on two streams.
     from nanomagmc_amd import Bond, Lattices
     bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
     bonds = [Bond(bd[k], J[k], D[k]) for k in range(len(J))]
+    rng = np.random.default_rng(61)
+    grains = [{"n": (n,) * 3, "edge": n, "mask": (rng.random(n ** 3) < 0.75).astype(np.uint8)} for n in (9, 30)]
     runs = []
     for off in ("0", "1"):
         monkeypatch.setenv("NMC_NO_GRAPHS", off)
-        lat = Lattices(3, False, True, bonds, [{"n": (24, 24, 24), "edge": 24}] * 2, use_d=True, trial="cone",
-                       seed=61)
+        if kind == "periodic":
+            lat = Lattices(3, False, True, bonds, [{"n": (24, 24, 24), "edge": 24}] * 2, use_d=True, trial="cone",
+                           seed=61)
+        else:
+            lat = Lattices(3, False, False, bonds, grains, use_d=True, trial="cone", seed=61)
+            from nanomagmc_amd._lib import NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS
+            assert lat.sweep_kinds().tolist() == [NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS]
         lat.set_params([0.5, 1.5], [0.0, 0.1])
         lat.randomize()
         lat.equil(150)

@@ -46,3 +46,21 @@ def test_mean_err_matches_independent_case():
     x = rng.normal(size=100_000)
     m, e, tau = stats.mean_err(x)
     assert abs(m) < 5 * e and abs(e - 1 / math.sqrt(x.size)) < 0.05 / math.sqrt(x.size)
+
+
+def test_batched_tau_equals_the_loop():
+    from nanomagmc_amd.stats import ess, tau_int, tau_int_batch
+    rng = np.random.default_rng(3)
+    rows = []
+    for phi in (0.0, 0.5, 0.9, 0.99):
+        x = np.zeros(600)
+        for t in range(1, 600):
+            x[t] = phi * x[t - 1] + rng.normal()
+        rows.append(x)
+    rows.append(np.zeros(600))                       # constant series
+    X = np.array(rows)
+    tau, closed = tau_int_batch(X)
+    for i, x in enumerate(rows):
+        t, W = tau_int(x)
+        assert abs(tau[i] - t) < 1e-12 * max(1.0, t), (i, tau[i], t)
+        assert bool(closed[i]) == bool(ess(x)[2]), i
