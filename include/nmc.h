@@ -234,6 +234,15 @@ int nmc_get_series(nmc_ctx* ctx, const int* inst, int n, double* obs,
  * the series size of nmc_series_reset. */
 int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp_sweeps);
 
+/* The sample loop of one path point with the lattice average: for slot
+ * 0..nsamples-1, `samp_sweeps` sweeps, a measurement into that slot, then
+ * add_to_av (state.cpp:288-296) for each instance of avg_inst -- the loop of
+ * main.cpp:182-240 with its add_to_av at :236-239 -- recorded once as a HIP
+ * graph per (lists, nsamples, samp_sweeps) and replayed as one launch.  The
+ * same sums and series as nmc_run_samples + nmc_accumulate_avgs per slot. */
+int nmc_run_samples_avg(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp_sweeps,
+                        const int* avg_inst, int navg);
+
 /* state::add_to_av (state.cpp:288-296): sum the current spins of src into
  * the running lattice average of instance `dst_avg` slot; nmc_get_avg returns
  * it in site order (3 doubles per site), nmc_zero_avg is field_type::all_zero

@@ -28,7 +28,7 @@ EXPORTS = [
     "nmc_last_error", "nmc_create", "nmc_destroy", "nmc_get_stream", "nmc_sync", "nmc_num_sites", "nmc_num_slices",
     "nmc_set_params", "nmc_set_cone", "nmc_randomize", "nmc_sweep", "nmc_sweep_adapt", "nmc_set_path", "nmc_set_lds_block", "nmc_set_stream_block", "nmc_sweep_kinds", "nmc_get_accepts",
     "nmc_reset_accepts", "nmc_snapshot", "nmc_measure", "nmc_measure_noalloc", "nmc_series_reset", "nmc_sample",
-    "nmc_get_series", "nmc_run_samples", "nmc_accumulate_avg", "nmc_accumulate_avgs", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
+    "nmc_get_series", "nmc_run_samples", "nmc_run_samples_avg", "nmc_accumulate_avg", "nmc_accumulate_avgs", "nmc_get_avg", "nmc_zero_avg", "nmc_get_spins",
     "nmc_set_spins", "nmc_delta_e", "nmc_replay_ising", "nmc_net_create", "nmc_net_destroy",
     "nmc_net_num_params", "nmc_net_get_params", "nmc_net_set_params", "nmc_net_collect",
     "nmc_net_num_samples", "nmc_net_train", "nmc_net_loss_grad", "nmc_block_moves", "nmc_block_swaps", "nmc_get_block_accepts", "nmc_build_weibull",
@@ -86,6 +86,7 @@ def lib():
         "nmc_reset_accepts": (i, [vp]),
         "nmc_snapshot": (i, [vp, P(i), P(i), i]),
         "nmc_measure": (i, [vp, P(i), i, P(d), P(d), P(d)]),
+        "nmc_run_samples_avg": (i, [vp, P(i), i, i, i, P(i), i]),
         "nmc_set_stream_block": (i, [vp, i]),
         "nmc_sweep_kinds": (i, [vp, P(i), i, P(i)]),
         "nmc_measure_noalloc": (i, [vp, P(i), i, P(d), P(d), P(d)]),

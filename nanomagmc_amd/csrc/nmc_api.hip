@@ -1599,6 +1599,79 @@ int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp
     return NMC_OK;
 }
 
+namespace {
+int ensure_avg(nmc_ctx* ctx, int inst, double** out);
+}  // namespace
+
+int nmc_run_samples_avg(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp_sweeps, const int* avg_inst,
+                        int navg) {
+    if (!ctx || navg < 0 || (navg > 0 && !avg_inst)) return fail(NMC_E_INVALID, "bad argument");
+    if (navg == 0) return nmc_run_samples(ctx, inst, n, nsamples, samp_sweeps);
+    if (nsamples < 0 || nsamples > ctx->series_n) return fail(NMC_E_INVALID, "nsamples exceeds the series buffer");
+    if (samp_sweeps < 0) return fail(NMC_E_INVALID, "negative sweep count");
+    if (nsamples == 0) return NMC_OK;
+    Set* s;
+    if (int rc = get_set(ctx, inst, n, &s)) return rc;
+    if (int rc = ensure_partials(ctx, (size_t)s->n_mt * pstride_of(ctx))) return rc;
+    // the running sums and their tile list exist before anything is recorded
+    std::vector<int> alist(avg_inst, avg_inst + navg);
+    for (int i : alist) {
+        if (int rc = check_inst(ctx, i)) return rc;
+        double* a = nullptr;
+        if (int rc = ensure_avg(ctx, i, &a)) return rc;
+    }
+    {
+        std::vector<int> sorted(alist);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(NMC_E_INVALID, "nmc_run_samples_avg: an averaged instance is listed twice");
+    }
+    auto at = ctx->avg_tiles.find(alist);
+    if (at == ctx->avg_tiles.end()) {
+        std::vector<Tile> tiles;
+        for (int i : alist)
+            for (int64_t e = 0; e < ctx->din[i].Sp; e += 256) tiles.push_back(Tile{i, (int)e});
+        Tile* dt = nullptr;
+        if (int rc = upload(ctx, &dt, tiles)) return rc;
+        at = ctx->avg_tiles.emplace(alist, std::make_pair(dt, (int)tiles.size())).first;
+    }
+    NMC_HIP(hipStreamSynchronize(ctx->stream));   // the pointer table is current before capture
+    auto step = [&](int slot) -> int {
+        if (int rc = sweep_launches(ctx, s, samp_sweeps)) return rc;
+        if (int rc = sample_launches(ctx, s, slot)) return rc;
+        NMC_HIP(launch_accum_avg_tiles(ctx->stream, ctx->ising, ctx->d_inst, at->second.first, at->second.second,
+                                       ctx->pool, ctx->d_avgptr));
+        return NMC_OK;
+    };
+    if (!ctx->graphs_on) {
+        for (int slot = 0; slot < nsamples; ++slot)
+            if (int rc = step(slot)) return rc;
+        return NMC_OK;
+    }
+    // the key carries the averaged list after a -1 separator
+    std::vector<int> gkey(s->list);
+    gkey.push_back(-1);
+    gkey.insert(gkey.end(), alist.begin(), alist.end());
+    auto key = std::make_tuple(gkey, nsamples, samp_sweeps);
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        NMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = NMC_OK;
+        for (int slot = 0; slot < nsamples && rc == NMC_OK; ++slot) rc = step(slot);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != NMC_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(NMC_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(NMC_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        it = ctx->graphs.emplace(key, x).first;
+    }
+    NMC_HIP(hipGraphLaunch(it->second, ctx->stream));
+    return NMC_OK;
+}
+
 int nmc_get_series(nmc_ctx* ctx, const int* inst, int n, double* obs, double* q_ref, double* q_wrap) {
     if (!ctx || !obs) return fail(NMC_E_INVALID, "null argument");
     const int cnt = inst ? n : (int)ctx->din.size();

@@ -229,6 +229,13 @@ class Lattices:
         a, n = _ilist(inst)
         check(lib().nmc_run_samples(self._h, _iptr(a), n, int(nsamples), int(samp_sweeps)))
 
+    def run_samples_avg(self, nsamples, samp_sweeps, avg_inst, inst=None):
+        """nsamples x (samp_sweeps sweeps + sample + add_to_av of avg_inst) as
+        one replayed HIP graph."""
+        a, n = _ilist(inst)
+        b, m = _ilist(avg_inst)
+        check(lib().nmc_run_samples_avg(self._h, _iptr(a), n, int(nsamples), int(samp_sweeps), _iptr(b), m))
+
     def get_series(self, inst=None):
         a, n = _ilist(inst)
         cnt = n if a is not None else self.ninst

@@ -622,12 +622,11 @@ int main(int argc, char** argv) {
         if (!sim.distrib)
             for (int id : active)
                 if (id < curr(1, 0)) avg_ids.push_back(id);
-        for (int s = 0; s < sim.N_samp; ++s) {
-            nmc_check(nmc_sweep(ctx, active.data(), (int)active.size(), sim.Samp_steps), "nmc_sweep");
-            nmc_check(nmc_sample(ctx, active.data(), (int)active.size(), s), "nmc_sample");
-            if (!avg_ids.empty())
-                nmc_check(nmc_accumulate_avgs(ctx, avg_ids.data(), (int)avg_ids.size()), "nmc_accumulate_avgs");
-        }
+        // NSAMPS x (SAMPSWEEPS sweeps, measurement, add_to_av) as one
+        // replayed graph, recorded at the first v2 point (main.cpp:182-240)
+        nmc_check(nmc_run_samples_avg(ctx, active.data(), (int)active.size(), sim.N_samp, sim.Samp_steps,
+                                      avg_ids.empty() ? nullptr : avg_ids.data(), (int)avg_ids.size()),
+                  "nmc_run_samples_avg");
         nmc_check(nmc_get_series(ctx, active.data(), (int)active.size(), obs.data(), tc ? qr.data() : nullptr,
                                  tc ? qw.data() : nullptr), "nmc_get_series");
         print_TD_step(st, sim, j, pr.v2_size, nl, v1n, active, obs.data(), tc ? qr.data() : nullptr,

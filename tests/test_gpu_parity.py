@@ -730,3 +730,41 @@ def test_invalid_arguments_raise(orc):
     with pytest.raises(NmcError):
         heis.set_cone([0.0])               # half-angle outside (0, pi]
     assert heis.equil_adapt(0, 10).shape == (1,)
+
+
+@pytest.mark.parametrize("graphs", ["0", "1"])
+def test_sample_loop_with_averages_equals_the_calls(orc, bloch3d, monkeypatch, graphs):
+    # nmc_run_samples_avg (the drop-in driver's whole sample loop, one graph)
+    # against nmc_sweep + nmc_sample + nmc_accumulate_avgs per slot, on open
+    # grains (compacted and span-streamed) and a periodic lattice set
+    from nanomagmc_amd import Bond, Lattices
+    monkeypatch.setenv("NMC_NO_GRAPHS", "1" if graphs == "0" else "0")
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
+    bonds = [Bond(bd[k], J[k], D[k]) for k in range(len(J))]
+    rng = np.random.default_rng(71)
+    insts = [{"n": (n,) * 3, "edge": n, "mask": (rng.random(n ** 3) < 0.75).astype(np.uint8)} for n in (9, 30, 11)]
+    lats = []
+    for way in ("loop", "calls"):
+        lat = Lattices(3, False, False, bonds, insts, use_d=True, seed=71)
+        lat.set_params([0.5, 1.5, 3.0], [0.0, 0.1, 0.05])
+        lat.randomize()
+        lat.equil(20)
+        lat.series_reset(7, with_charge=True)
+        for i in (0, 2):
+            lat.zero_avg(i)
+        if way == "loop":
+            lat.run_samples_avg(7, 3, [0, 2])
+        else:
+            for slot in range(7):
+                lat.equil(3)
+                lat.sample(slot)
+                lat.accumulate_avgs([0, 2])
+        lats.append(lat)
+    a, b = lats
+    for x, y in zip(a.get_series(), b.get_series()):
+        assert np.array_equal(x, y)
+    for i in range(3):
+        assert np.array_equal(a.get_spins(i), b.get_spins(i)), i
+    for i in (0, 2):
+        assert np.array_equal(a.get_avg(i), b.get_avg(i)), i
+    assert np.abs(a.get_avg(0)).sum() > 0
