@@ -113,15 +113,18 @@ __device__ __forceinline__ bool heis_decide(const DevInst& in, float sx, float s
     const float sphi = __builtin_amdgcn_sinf(uphi);
     if (!CONE) {
         // uniform direction on S^2: z uniform in (-1, 1), azimuth uniform
+        // the hardware square root (v_sqrt_f32, about 1 ulp): the IEEE-exact
+        // one costs a refinement and denormal scaling of ~15 instructions,
+        // and the trial spin is rounded to fp32 anyway
         tz = __builtin_fmaf(2.f, ua, -1.f);
-        const float r = __builtin_sqrtf(fmaxf(0.f, __builtin_fmaf(-tz, tz, 1.f)));
+        const float r = __builtin_amdgcn_sqrtf(fmaxf(0.f, __builtin_fmaf(-tz, tz, 1.f)));
         tx = r * cphi;
         ty = r * sphi;
     } else {
         // uniform on the cap of half-angle theta around s
         // (orthonormal frame: Duff et al., JCGT 6(1), 2017)
         const float ca = __builtin_fmaf(-ua, in.cone_w, 1.f);
-        const float sa = __builtin_sqrtf(fmaxf(0.f, __builtin_fmaf(-ca, ca, 1.f)));
+        const float sa = __builtin_amdgcn_sqrtf(fmaxf(0.f, __builtin_fmaf(-ca, ca, 1.f)));
         const float sgn = sz >= 0.f ? 1.f : -1.f;
         const float a = -__builtin_amdgcn_rcpf(sgn + sz);   // |sgn + sz| >= 1
         const float b = sx * sy * a;
