@@ -314,33 +314,37 @@ def plain_ess(lat, nsamples=400, samp_sweeps=5):
 def ess_leg():
     """Effective samples/s of E and |Q| (Q_wrap, periodic plaquette sum):
     plain sweeps, sweeps + learned vMF block proposals, and sweeps + learned
-    texture swaps; tau_int in samples of 4 sweeps each, Sokal window c = 6
+    texture swaps, from starts every method shares (anneal, then swap-sampler
+    preparation of every chain); tau_int in samples of 4 sweeps each, over
+    each method's replicas together with the multi-chain autocorrelation
+    (chains stuck at different values are charged), Sokal window c = 6
     (`window_closed` false: the series is shorter than the window, tau is a
     lower bound and ESS an upper bound).  q_changes_per_ksweep counts the
-    samples whose integer charge differs from the previous one: a frozen
-    charge has a small tau_int without exploring anything."""
+    samples whose integer charge differs from the previous one."""
     from nanomagmc_amd.learn import compare_ess
     methods = ("vmf", "swap")
-    r = compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_sweeps=3000,
-                    collect_rounds=100, collect_blocks=1000, train_steps=2000, batch=256, nsamples=2000,
-                    methods=methods)
+    r = compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=4, anneal_sweeps=3000,
+                    collect_rounds=100, collect_blocks=1000, train_steps=2000, batch=256, eq_samples=10000,
+                    nsamples=4000, methods=methods)
     rows = []
     for row in r["temps"]:
         def pick(m, k):
             st = row[m][k]
-            return {"tau_int": st["tau_int"], "ess_per_s": st["ess_per_s"], "window_closed": st["window_closed"],
-                    "mean": st["mean"], "err": st["err"]}
+            return {"tau_int": st["tau_int"], "tau_int_per_chain": st["tau_int_per_chain"],
+                    "ess_per_s": st["ess_per_s"], "window_closed": st["window_closed"], "mean": st["mean"],
+                    "err": st["err"]}
         rows.append({"T": row["T"], "train_loss": row["loss"], "block_acceptance": row["acceptance"],
                      **{m: {"E": pick(m, "E"), "absQ": pick(m, "absQ"),
                             "q_changes_per_ksweep": row[m]["q_changes_per_ksweep"]} for m in ("plain",) + methods},
-                     "gain_absQ": row["gain_absQ"], "gain_E": row["gain_E"]})
+                     "gain_absQ": row["gain_absQ"], "gain_absQ_err": row["gain_absQ_err"],
+                     "gain_E": row["gain_E"], "gain_E_err": row["gain_E_err"]})
     return {"metric": "tau_int-normalised effective samples/s", "unit": "samples/s",
             "config": "2-D skyrmion L=512 periodic, J=1, Neel D=0.3, h_z=0.06, %d replicas per method and T, "
-                      "%d samples x 4 sweeps, learned: 1 round of 256 16x16 blocks per 4 sweeps "
-                      "(swap: %d relaxation steps, core radius %.1f)"
-                      % (r["replicas"], r["nsamples"], r["relax_iters"], r["core_radius"]),
+                      "%d swap samples of preparation, then %d samples x 4 sweeps; learned: 1 round of 256 16x16 "
+                      "blocks per lattice per 4 sweeps (swap: %d relaxation steps, core radius %.1f)"
+                      % (r["replicas"], r["eq_samples"], r["nsamples"], r["relax_iters"], r["core_radius"]),
             "train": {"blocks": r["train_blocks"], "steps": r["train_steps"], "batch": r["batch"],
-                      "seconds": r["train_s"]},
+                      "seconds": r["train_s"], "swap_loss": r["swap_loss"]},
             "s_per_sample": r["s_per_sample"], "temps": rows}
 
 
@@ -364,10 +368,10 @@ def learn_profile():
     net.train(50, batch=256, lr=1e-3, seed=2)
     for k in range(20):
         block_moves(lat, net, [0, 1], k % 32, (3 * k) % 32)
-        block_swaps(lat, net, [0, 1], k % 32, (5 * k) % 32)
+        block_swaps(lat, net, [0, 1], k % 32, (5 * k) % 32, relax_iters=40)
     lat.sync()
     print(json.dumps({"learn_profile": "2 x 512^2 skyrmion lattices, 2e4 blocks, 50 Adam steps, 20 vMF + 20 swap "
-                                       "rounds", "wall_s": time.perf_counter() - t0}), flush=True)
+                                       "rounds (40 relaxation steps)", "wall_s": time.perf_counter() - t0}), flush=True)
 
 
 # ---------------------------------------------------------------------- main

@@ -315,22 +315,27 @@ int nmc_net_loss_grad(nmc_net* net, const float* patches, const float* targets, 
  * counts the accepted blocks of instance inst[k].  With accepted, stats and
  * proposal all NULL the round is only enqueued (no synchronisation); the
  * accepted blocks also add to per-instance totals (nmc_get_block_accepts).
+ * The network sees the block's 4-site halo and, on the block, the Coons
+ * interpolation of its 1-site ring (never the block's own spins).
  * Test mode (both optional, blocks ordered by instance, then i, then j): stats gets
- * (dE_s, log q(x'), log q(x)) per block, proposal the 16 x 16 x 3 proposed
+ * (dE_s, log q(x'), log q(x), 0) per block, proposal the 16 x 16 x 3 proposed
  * spins (row = x offset, column = y offset). */
 int nmc_block_moves(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y,
                     int64_t* accepted, float* stats, float* proposal);
 /* One round of learned texture swaps on the same blocks.  Per block, from
- * its surroundings only: texture A = the network's mean directions relaxed
- * (relax_iters pairs of checkerboard Gauss-Seidel half-steps with
- * over-relaxation factor 1.85, the 1-site ring held fixed) to the energy
- * minimum; texture B = the same with a skyrmion of radius
- * core_radius in the block centre, relaxed likewise.  The move rotates every
- * block spin by pi about the bisector of A_i and B_i (an involution that maps
- * A to B and preserves the measure) and is accepted with
- * min(1, exp(-beta dE_s)).  No counterpart in the reference.  Test mode as
- * above: stats gets (dE_s, sum_i |A_i - B_i|^2, 0) per block, proposal the
- * rotated block. */
+ * its surroundings only: texture A = the Coons interpolation of the 1-site
+ * ring relaxed (relax_iters pairs of checkerboard Gauss-Seidel half-steps
+ * with over-relaxation factor 1.85, the ring held fixed) to the energy
+ * minimum; texture B = the same with a Neel skyrmion of radius core_radius
+ * centred on the seed site, relaxed likewise.  The seed site is the block
+ * site u, v in [4, 12) whose predicted mean alignment with the ring's mean
+ * direction m, A(kappa) (mu . m) with A(k) = coth k - 1/k, is lowest (first in
+ * row order on ties).  The move rotates every block spin by pi about the
+ * bisector of A_i and B_i (an involution that maps A to B and preserves the
+ * measure) and is accepted with min(1, exp(-beta dE_s)).  No counterpart in
+ * the reference.  Test mode as above: stats gets (dE_s, sum_i |A_i - B_i|^2,
+ * Q(B) - Q(A), seed site u * 16 + v) per block, Q the solid-angle charge over
+ * the block and its ring, proposal the rotated block. */
 int nmc_block_swaps(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, int off_y,
                     int relax_iters, float core_radius, int64_t* accepted, float* stats,
                     float* proposal);

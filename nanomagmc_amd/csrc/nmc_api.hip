@@ -2231,7 +2231,11 @@ int nmc_net_collect(nmc_net* net, nmc_ctx* ctx, int inst, int nblocks, uint64_t 
     }
     int* dbxy = nullptr;
     if (int rc = upload(ctx, &dbxy, bxy)) return rc;   // synchronises ctx: spins are current
-    hipError_t e = launch_gather_patches(ctx->stream, (const float*)ctx->pool, d.base, d.Sp, nx, ny, dbxy, nblocks,
+    RoundTab tab{};
+    tab.n = 1;
+    tab.nblocks = nblocks;
+    tab.in[0] = RoundInst{d.base, d.Sp, nx, ny, d.beta, d.h, d.inst_id, 0, inst, 0};
+    hipError_t e = launch_gather_patches(ctx->stream, (const float*)ctx->pool, tab, dbxy,
                                          net->data_x + (size_t)net->ndata * kPatchElems,
                                          net->data_t + (size_t)net->ndata * kTargetElems);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -2337,15 +2341,28 @@ int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, i
     }
     first[n] = total;
     if (int rc = net_reserve(net, total)) return rc;
+    // one launch per kernel for up to kRoundMaxInst instances; blocks keep
+    // the order (instance, then i, then j) across the launches
+    std::vector<RoundTab> tabs;
+    for (int j0 = 0; j0 < n; j0 += kRoundMaxInst) {
+        RoundTab tab{};
+        tab.n = std::min(kRoundMaxInst, n - j0);
+        for (int j = 0; j < tab.n; ++j) {
+            const DevInst& d = ctx->din[inst[j0 + j]];
+            tab.in[j] = RoundInst{d.base, d.Sp, d.a1, d.a2, d.beta, d.h, d.inst_id, first[j0 + j] - first[j0],
+                                  inst[j0 + j], 0};
+        }
+        tab.nblocks = first[j0 + tab.n] - first[j0];
+        tabs.push_back(tab);
+    }
     // the network's training work (its own stream) before the round reads it
     NMC_HIP(hipEventRecord(net->ready, net->stream));
     NMC_HIP(hipStreamWaitEvent(ctx->stream, net->ready, 0));
-    for (int j = 0; j < n; ++j) {
-        const DevInst& d = ctx->din[inst[j]];
-        NMC_HIP(launch_block_xy(ctx->stream, net->bxy + 2 * first[j], d.a1 / 32, d.a2 / 32, off_x, off_y));
-        NMC_HIP(launch_gather_patches(ctx->stream, (const float*)ctx->pool, d.base, d.Sp, d.a1, d.a2,
-                                      net->bxy + 2 * first[j], first[j + 1] - first[j],
-                                      net->x0 + (size_t)first[j] * kPatchElems, nullptr));
+    for (size_t c = 0; c < tabs.size(); ++c) {
+        const int f0 = first[c * kRoundMaxInst];
+        NMC_HIP(launch_block_xy(ctx->stream, net->bxy + 2 * f0, tabs[c], off_x, off_y));
+        NMC_HIP(launch_gather_patches(ctx->stream, (const float*)ctx->pool, tabs[c], net->bxy + 2 * f0,
+                                      net->x0 + (size_t)f0 * kPatchElems, nullptr));
     }
     hipStream_t keep = net->stream;
     net->stream = ctx->stream;
@@ -2355,25 +2372,21 @@ int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, i
     const uint64_t round = ++ctx->block_round;
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
     float *dstats = nullptr, *dprop = nullptr;
-    if (stats && (rc = dalloc(&dstats, (size_t)3 * total))) return rc;
+    if (stats && (rc = dalloc(&dstats, (size_t)4 * total))) return rc;
     if (proposal && (rc = dalloc(&dprop, (size_t)768 * total))) { (void)hipFree(dstats); return rc; }
     hipError_t e = hipSuccess;
-    for (int j = 0; j < n && e == hipSuccess; ++j) {
-        const DevInst& d = ctx->din[inst[j]];
-        float* sj = dstats ? dstats + 3 * first[j] : nullptr;
-        float* pj = dprop ? dprop + (size_t)768 * first[j] : nullptr;
+    for (size_t c = 0; c < tabs.size() && e == hipSuccess; ++c) {
+        const int f0 = first[c * kRoundMaxInst];
+        float* sj = dstats ? dstats + 4 * f0 : nullptr;
+        float* pj = dprop ? dprop + (size_t)768 * f0 : nullptr;
         if (swap)
-            e = launch_block_swap(ctx->stream, (float*)ctx->pool, d.base, d.Sp, d.a1, d.a2, net->bxy + 2 * first[j],
-                                  first[j + 1] - first[j], net->o + (size_t)first[j] * 256 * 32, ctx->st, d.beta,
-                                  d.h, relax_iters, core_radius, slo, shi, d.inst_id, round, net->acc + first[j], sj,
-                                  pj);
+            e = launch_block_swap(ctx->stream, (float*)ctx->pool, tabs[c], net->bxy + 2 * f0,
+                                  net->o + (size_t)f0 * 256 * 32, ctx->st, relax_iters, core_radius, slo, shi, round,
+                                  net->acc + f0, sj, pj);
         else
-            e = launch_block_mh(ctx->stream, (float*)ctx->pool, d.base, d.Sp, d.a1, d.a2, net->bxy + 2 * first[j],
-                                first[j + 1] - first[j], net->o + (size_t)first[j] * 256 * 32, ctx->st, d.beta, d.h,
-                                slo, shi, d.inst_id, round, net->acc + first[j], sj, pj);
-        if (e == hipSuccess)
-            e = launch_block_acc_add(ctx->stream, net->acc + first[j], first[j + 1] - first[j],
-                                     ctx->d_bacc + inst[j]);
+            e = launch_block_mh(ctx->stream, (float*)ctx->pool, tabs[c], net->bxy + 2 * f0,
+                                net->o + (size_t)f0 * 256 * 32, ctx->st, slo, shi, round, net->acc + f0, sj, pj);
+        if (e == hipSuccess) e = launch_block_acc_add(ctx->stream, net->acc + f0, tabs[c], ctx->d_bacc);
     }
     ctx->bacc_dirty = true;
     // later work on the network's stream (training reuses its buffers) waits
@@ -2386,7 +2399,7 @@ int block_round(nmc_ctx* ctx, nmc_net* net, const int* inst, int n, int off_x, i
     if (e == hipSuccess)
         e = hipMemcpyAsync(acc.data(), net->acc, total * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess && dstats)
-        e = hipMemcpyAsync(stats, dstats, (size_t)3 * total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        e = hipMemcpyAsync(stats, dstats, (size_t)4 * total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess && dprop)
         e = hipMemcpyAsync(proposal, dprop, (size_t)768 * total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

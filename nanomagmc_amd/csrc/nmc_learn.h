@@ -25,22 +25,42 @@ hipError_t launch_grad_reduce(hipStream_t s, const float* Pw, int nchunks, int K
 hipError_t launch_adam(hipStream_t s, float* w, float* mw, float* vw, const float* g, int K, int Cin, int nout,
                        float lr, float b1, float b2, float eps, float bc1, float bc2, float* b, float* mb, float* vb,
                        const float* gb, __bf16* W16, __bf16* Wd16);
-hipError_t launch_gather_patches(hipStream_t s, const float* pool, int64_t base, int64_t Sp, int nx, int ny,
-                                 const int* bxy, int npatch, __bf16* X, float* target);
-// Block corners of a round (nbx x nby blocks of 32 x 32 sites, offset
-// (off_x, off_y)) and the running per-instance count of accepted blocks.
-hipError_t launch_block_xy(hipStream_t s, int* bxy, int nbx, int nby, int off_x, int off_y);
-hipError_t launch_block_acc_add(hipStream_t s, const int* acc, int nblocks, unsigned long long* total);
+// One launch of a learned round (or of a patch gather) covers up to
+// kRoundMaxInst instances.  Block b of the launch belongs to the last slot j
+// with in[j].first <= b (slots in increasing `first`) and is block
+// b - in[j].first of that instance; every per-block array a launcher takes
+// (corners, patches, network outputs, accept flags, stats, proposals) is
+// indexed by the launch's block b.
+constexpr int kRoundMaxInst = 16;
+struct RoundInst {
+    int64_t base, Sp;    // the instance's spins in the pool (colour-split, components at stride Sp)
+    int nx, ny;          // lattice edges
+    float beta, h;
+    uint32_t inst_id;    // Philox stream of the instance
+    int first;           // first block of the instance in the launch
+    int slot;            // context index of the instance (running accept totals)
+    int pad_;
+};
+struct RoundTab {
+    int n, nblocks;
+    RoundInst in[kRoundMaxInst];
+};
+
+hipError_t launch_gather_patches(hipStream_t s, const float* pool, const RoundTab& tab, const int* bxy, __bf16* X,
+                                 float* target);
+// Block corners of a round (per instance (nx / 32) x (ny / 32) blocks of 32
+// x 32 sites, offset (off_x, off_y)) and the running per-instance counts of
+// accepted blocks (totals indexed by the slot's context index).
+hipError_t launch_block_xy(hipStream_t s, int* bxy, const RoundTab& tab, int off_x, int off_y);
+hipError_t launch_block_acc_add(hipStream_t s, const int* acc, const RoundTab& tab, unsigned long long* totals);
 hipError_t launch_head_loss(hipStream_t s, const float* O, const float* target, int npatch, float scale, __bf16* dO,
                             float* loss);
-hipError_t launch_block_mh(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
-                           int nblocks, const float* O, const Stencil& st, float beta, float h, uint32_t seed_lo,
-                           uint32_t seed_hi, uint32_t inst, uint64_t round, int* accepted, float* stats,
-                           float* proposal);
-hipError_t launch_block_swap(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
-                             int nblocks, const float* O, const Stencil& st, float beta, float h, int relax_iters,
-                             float core_radius, uint32_t seed_lo, uint32_t seed_hi, uint32_t inst, uint64_t round,
-                             int* accepted, float* stats, float* proposal);
+hipError_t launch_block_mh(hipStream_t s, float* pool, const RoundTab& tab, const int* bxy, const float* O,
+                           const Stencil& st, uint32_t seed_lo, uint32_t seed_hi, uint64_t round, int* accepted,
+                           float* stats, float* proposal);
+hipError_t launch_block_swap(hipStream_t s, float* pool, const RoundTab& tab, const int* bxy, const float* O,
+                             const Stencil& st, int relax_iters, float core_radius, uint32_t seed_lo,
+                             uint32_t seed_hi, uint64_t round, int* accepted, float* stats, float* proposal);
 hipError_t launch_copy_rows(hipStream_t s, const void* src, void* dst, const int* idx, int n, int64_t row_bytes);
 hipError_t launch_put_patches(hipStream_t s, const float* in, int n, __bf16* X);
 

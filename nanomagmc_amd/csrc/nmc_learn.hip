@@ -369,8 +369,7 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ mw, float* __r
 // Patches, the vMF head and the Metropolis-Hastings block move (2-D).
 // ---------------------------------------------------------------------------
 struct Lat2 {   // one 2-D Heisenberg instance in the colour-split layout
-    const float* c0;   // colour 0, components x|y|z at stride Sp
-    const float* c1;
+    const float* c0;   // colour 0, components x|y|z at stride Sp; colour 1 follows at 3 Sp
     int64_t Sp;
     int nx, ny, half;
 };
@@ -378,33 +377,84 @@ struct Lat2 {   // one 2-D Heisenberg instance in the colour-split layout
 __device__ __forceinline__ void spin_at(const Lat2& L, int x, int y, float& sx, float& sy, float& sz) {
     x = (x % L.nx + L.nx) % L.nx;
     y = (y % L.ny + L.ny) % L.ny;
-    const float* c = ((x + y) & 1) ? L.c1 : L.c0;
-    const int64_t e = (int64_t)x * L.half + (y >> 1);
-    sx = c[e]; sy = c[L.Sp + e]; sz = c[2 * L.Sp + e];
+    // the colour as an offset, not a choice between two pointers (a pointer
+    // select on a captured aggregate becomes an indexed private-memory load)
+    const int64_t e = (int64_t)x * L.half + (y >> 1) + (((x + y) & 1) ? 3 * L.Sp : 0);
+    sx = L.c0[e]; sy = L.c0[L.Sp + e]; sz = L.c0[2 * L.Sp + e];
+}
+
+// Coons (transfinite) interpolation of block site (u, v) of a 16 x 16 block
+// from its one-site ring: R(a, b) gives the ring spin at (a, b) with a or b
+// in {-1, 16} (corners included).  With s = (u + 1) / 17, t = (v + 1) / 17,
+//   P = (1-s) R(-1,v) + s R(16,v) + (1-t) R(u,-1) + t R(u,16)
+//       - [(1-s)(1-t) R(-1,-1) + (1-s) t R(-1,16) + s (1-t) R(16,-1) + s t R(16,16)],
+// normalised (|P|^2 < 1e-12 gives +z).  A function of the ring only.
+template <class Ring>
+__device__ __forceinline__ void coons_site(int u, int v, Ring R, float& px, float& py, float& pz) {
+    const float s = (float)(u + 1) * (1.f / 17.f), t = (float)(v + 1) * (1.f / 17.f);
+    const float3 l = R(-1, v), r = R(16, v), b = R(u, -1), e = R(u, 16);
+    const float3 c00 = R(-1, -1), c01 = R(-1, 16), c10 = R(16, -1), c11 = R(16, 16);
+    const float w00 = (1.f - s) * (1.f - t), w01 = (1.f - s) * t, w10 = s * (1.f - t), w11 = s * t;
+    px = (1.f - s) * l.x + s * r.x + (1.f - t) * b.x + t * e.x - (w00 * c00.x + w01 * c01.x + w10 * c10.x + w11 * c11.x);
+    py = (1.f - s) * l.y + s * r.y + (1.f - t) * b.y + t * e.y - (w00 * c00.y + w01 * c01.y + w10 * c10.y + w11 * c11.y);
+    pz = (1.f - s) * l.z + s * r.z + (1.f - t) * b.z + t * e.z - (w00 * c00.z + w01 * c01.z + w10 * c10.z + w11 * c11.z);
+    const float n2 = px * px + py * py + pz * pz;
+    if (n2 < 1e-12f) { px = 0.f; py = 0.f; pz = 1.f; return; }
+    const float inv = 1.f / sqrtf(n2);
+    px *= inv; py *= inv; pz *= inv;
 }
 
 // Patch p of block (bx, by): 24 x 24 x 16 bf16 NHWC; channels 0..2 spins of
-// the halo (block interior zeroed), channel 3 = 1 on the halo, rest 0.
-// Optional target: the block's 16 x 16 x 3 spins.
-__global__ void k_gather_patches(const float* __restrict__ pool, int64_t base, int64_t Sp, int nx, int ny,
-                                 const int* __restrict__ bxy, int npatch, __bf16* __restrict__ X,
-                                 float* __restrict__ target) {
+// the 4-site halo and, on the block, the Coons interpolation of its one-site
+// ring (so every block site sees its surroundings through the network's
+// radius-4 window, and the block's own spins never enter); channel 3 = 1 on
+// the halo, rest 0.  Optional target: the block's 16 x 16 x 3 spins.
+__device__ __forceinline__ int round_slot(const RoundTab& tab, int b) {
+    int j = 0;
+    for (int k = 1; k < tab.n; ++k)
+        if (tab.in[k].first <= b) j = k;
+    return j;
+}
+
+__device__ __forceinline__ Lat2 round_lat(const float* pool, const RoundInst& ri) {
+    return Lat2{pool + ri.base, ri.Sp, ri.nx, ri.ny, ri.ny / 2};
+}
+
+__global__ __launch_bounds__(256) void k_gather_patches(const float* __restrict__ pool, RoundTab tab,
+                                                        const int* __restrict__ bxy, __bf16* __restrict__ X,
+                                                        float* __restrict__ target) {
+    __shared__ float Ps[3][576];
     const int p = blockIdx.x;
-    if (p >= npatch) return;
-    Lat2 L{pool + base, pool + base + 3 * Sp, Sp, nx, ny, ny / 2};
+    if (p >= tab.nblocks) return;
+    const Lat2 L = round_lat(pool, tab.in[round_slot(tab, p)]);
     const int bx = bxy[2 * p], by = bxy[2 * p + 1];
-    for (int t = threadIdx.x; t < 24 * 24; t += blockDim.x) {
+    for (int t = threadIdx.x; t < 576; t += blockDim.x) {
         const int u = t / 24, v = t - u * 24;   // patch row (x) and column (y)
         float sx, sy, sz;
         spin_at(L, bx - 4 + u, by - 4 + v, sx, sy, sz);
+        Ps[0][t] = sx; Ps[1][t] = sy; Ps[2][t] = sz;
+    }
+    __syncthreads();
+    auto ring = [&](int a, int b) {
+        const int i = (a + 4) * 24 + (b + 4);
+        return make_float3(Ps[0][i], Ps[1][i], Ps[2][i]);
+    };
+    for (int t = threadIdx.x; t < 576; t += blockDim.x) {
+        const int u = t / 24, v = t - u * 24;
         const bool inner = u >= 4 && u < 20 && v >= 4 && v < 20;
         __bf16* o = X + ((size_t)p * 576 + t) * 16;
 #pragma unroll
         for (int c = 0; c < 16; ++c) o[c] = (__bf16)0.f;
-        if (!inner) { o[0] = (__bf16)sx; o[1] = (__bf16)sy; o[2] = (__bf16)sz; o[3] = (__bf16)1.f; }
-        else if (target) {
-            float* tg = target + ((size_t)p * 256 + (u - 4) * 16 + (v - 4)) * 3;
-            tg[0] = sx; tg[1] = sy; tg[2] = sz;
+        if (!inner) {
+            o[0] = (__bf16)Ps[0][t]; o[1] = (__bf16)Ps[1][t]; o[2] = (__bf16)Ps[2][t]; o[3] = (__bf16)1.f;
+        } else {
+            float px, py, pz;
+            coons_site(u - 4, v - 4, ring, px, py, pz);
+            o[0] = (__bf16)px; o[1] = (__bf16)py; o[2] = (__bf16)pz;
+            if (target) {
+                float* tg = target + ((size_t)p * 256 + (u - 4) * 16 + (v - 4)) * 3;
+                tg[0] = Ps[0][t]; tg[1] = Ps[1][t]; tg[2] = Ps[2][t];
+            }
         }
     }
 }
@@ -478,30 +528,33 @@ __global__ void k_head_loss(const float* __restrict__ O, const float* __restrict
 // One Metropolis-Hastings block move per workgroup (256 work-items = the 16 x
 // 16 block sites).  x' ~ prod vMF(mu_i, kappa_i); dE is the change of E_s
 // over every bond touching the block; the block is written back if accepted.
-__global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, int64_t base, int64_t Sp, int nx, int ny,
-                                                  const int* __restrict__ bxy, int nblocks,
-                                                  const float* __restrict__ O, Stencil st, float beta, float h,
-                                                  uint32_t seed_lo, uint32_t seed_hi, uint32_t inst, uint64_t round,
-                                                  int* __restrict__ accepted, float* __restrict__ stats,
-                                                  float* __restrict__ proposal) {
+__global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, RoundTab tab, const int* __restrict__ bxy,
+                                                  const float* __restrict__ O, Stencil st, uint32_t seed_lo,
+                                                  uint32_t seed_hi, uint64_t round, int* __restrict__ accepted,
+                                                  float* __restrict__ stats, float* __restrict__ proposal) {
     __shared__ float nx_s[256], ny_s[256], nz_s[256];
     __shared__ float red[3][4];
     __shared__ int acc_s;
     const int b = blockIdx.x;
-    if (b >= nblocks) return;
+    if (b >= tab.nblocks) return;
+    const RoundInst& ri = tab.in[round_slot(tab, b)];
+    const int lb = b - ri.first;   // the block's index within its instance (its Philox counter)
+    const float beta = ri.beta, h = ri.h;
+    const int nx = ri.nx, ny = ri.ny;
+    const int64_t Sp = ri.Sp;
     const int t = threadIdx.x;
     const int u = t >> 4, v = t & 15;
     const int bx = bxy[2 * b], by = bxy[2 * b + 1];
-    float* c0 = pool + base;
-    float* c1 = pool + base + 3 * Sp;
-    Lat2 L{c0, c1, Sp, nx, ny, ny / 2};
+    float* c0 = pool + ri.base;
+    float* c1 = pool + ri.base + 3 * Sp;
+    Lat2 L{c0, Sp, nx, ny, ny / 2};
     const int X = bx + u, Y = by + v;
     float sx, sy, sz;
     spin_at(L, X, Y, sx, sy, sz);
     const VMF f = vmf_unit(O + ((size_t)b * 256 + t) * 32);
     uint32_t w[4];
-    philox4x32_10((uint32_t)(b * 256 + t), (uint32_t)(kPurposeBlock << 8), (uint32_t)round,
-                  (uint32_t)(round >> 32), seed_lo, seed_hi + inst, w);
+    philox4x32_10((uint32_t)(lb * 256 + t), (uint32_t)(kPurposeBlock << 8), (uint32_t)round,
+                  (uint32_t)(round >> 32), seed_lo, seed_hi + ri.inst_id, w);
     // sample the vMF: cos angle w = 1 + log(u + (1 - u) e^{-2k}) / k
     const float k = f.kappa;
     const float u1 = u24_open(w[0]);
@@ -568,7 +621,7 @@ __global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, int6
         const float ua = u24_open(w[2]);
         acc_s = (loga >= 0.f || logf(ua) < loga) ? 1 : 0;
         accepted[b] = acc_s;
-        if (stats) { stats[3 * b] = DE; stats[3 * b + 1] = LQN; stats[3 * b + 2] = LQO; }
+        if (stats) { stats[4 * b] = DE; stats[4 * b + 1] = LQN; stats[4 * b + 2] = LQO; stats[4 * b + 3] = 0.f; }
     }
     __syncthreads();
     if (acc_s) {
@@ -581,87 +634,112 @@ __global__ __launch_bounds__(256) void k_block_mh(float* __restrict__ pool, int6
 
 // ---------------------------------------------------------------------------
 // Learned texture swaps.  Per 16 x 16 block, from the frozen surroundings
-// only: texture A = the network's mean directions relaxed to the energy
-// minimum with the 1-site ring around the block held fixed; texture B = the
-// same start with a skyrmion (core against the ring's mean direction) in the
-// block centre, relaxed likewise.  The move is x'_i = T_i x_i, T_i the
-// rotation by pi about the bisector of A_i and B_i: T_i A_i = B_i, T_i T_i =
-// 1 and T_i preserves the measure on S^2, so the move is a deterministic
-// involution and exact Metropolis accepts it with min(1, exp(-beta dE_s)).
-// Thermal deviations from A are carried to the same deviations from B, so
-// the energy cost is essentially E(B) - E(A): the block can gain or lose a
+// only: texture A = the Coons interpolation of the 1-site ring around the
+// block, relaxed to the energy minimum with the ring held fixed; texture B =
+// the same start with a Neel skyrmion (core against the ring's mean
+// direction) centred on the seed site, relaxed likewise.  The network picks
+// the seed site: of the sites u, v in [4, 12), the one whose predicted mean
+// alignment with the ring direction m, E_q[s . m] = A(kappa) (mu . m) with
+// A(k) = coth k - 1/k, is lowest -- where the surroundings make a skyrmion
+// core most likely.  The move is x'_i = T_i x_i, T_i the rotation by pi
+// about the bisector of A_i and B_i: T_i A_i = B_i, T_i T_i = 1 and T_i
+// preserves the measure on S^2, so the move is a deterministic involution
+// and exact Metropolis accepts it with min(1, exp(-beta dE_s)).  Thermal
+// deviations from A are carried to the same deviations from B, so the
+// energy cost is essentially E(B) - E(A): the block can gain or lose a
 // skyrmion in one step.
 // ---------------------------------------------------------------------------
 constexpr int kRing = 18;   // 16 x 16 block + 1-site ring
+constexpr int kSeedLo = 4;  // seed sites u, v in [kSeedLo, kSeedLo + 8)
 
 __device__ __forceinline__ int ring_idx(int u, int v) { return (u + 1) * kRing + (v + 1); }
 
 // Over-relaxation factor of the texture relaxation: with it the 16 x 16
 // block converges in ~80 half-step pairs instead of ~500.
 constexpr float kRelaxOmega = 1.85f;
+constexpr int kSwapThreads = 128;   // wave 0 relaxes texture A, wave 1 texture B
 
-// One over-relaxed checkerboard Gauss-Seidel half-step on the sites of parity
-// `par` (global parity of X + Y): s_i <- normalise(s_i + omega (F_i - s_i)),
-// F_i the unit local field (H + sum_j J s_j + s_j x D) / |..| that minimises
-// the site's energy.
-__device__ __forceinline__ void relax_half(float* S, int u, int v, int par, int X, int Y, const Stencil& st,
-                                           float h) {
-    if (((X + Y) & 1) != par) return;
+// Orders a wave's LDS writes before its lanes' next reads without a
+// workgroup barrier: the wave runs in lockstep, the release fence waits for
+// its LDS writes to complete, and the fences keep the compiler from moving
+// LDS accesses across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Block site k (0..3) of lane l in the texture relaxation: k >> 1 is the
+// half-step that updates it (global parity (bx + by + u + v) & 1 = k >> 1),
+// so every lane updates two sites per half-step and the indices are static.
+__device__ __forceinline__ void swap_site(int l, int k, int pxy, int& u, int& v) {
+    const int j = l + 64 * (k & 1);   // rank among the 128 sites of that parity
+    u = j >> 3;
+    v = 2 * (j & 7) + ((u + pxy + (k >> 1)) & 1);
+}
+
+// Site (u, v)'s over-relaxed Gauss-Seidel update s <- normalise(s + omega
+// (F - s)), F the unit local field (H + sum_j J s_j + s_j x D) / |..| that
+// minimises the site's energy; a vanishing field or step keeps s.
+__device__ __forceinline__ void relax_site(const float* S, int u, int v, const Stencil& st, float h, float& cx,
+                                           float& cy, float& cz) {
+    constexpr int RR = kRing * kRing;
     const int du[4] = {0, 0, 1, -1}, dv[4] = {1, -1, 0, 0};
     const int gs[4] = {5, 4, 3, 2};
     float vx = 0.f, vy = 0.f, vz = h;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int r = ring_idx(u + du[q], v + dv[q]);
-        const float nx = S[r], ny = S[kRing * kRing + r], nz = S[2 * kRing * kRing + r];
+        const float nx = S[r], ny = S[RR + r], nz = S[2 * RR + r];
         const int g = gs[q];
         vx += st.J[g] * nx + (ny * st.D[g][2] - nz * st.D[g][1]);
         vy += st.J[g] * ny + (nz * st.D[g][0] - nx * st.D[g][2]);
         vz += st.J[g] * nz + (nx * st.D[g][1] - ny * st.D[g][0]);
     }
+    // the hardware reciprocal square root: the iteration's fixed point does
+    // not depend on how each step is rounded
     const float n2 = vx * vx + vy * vy + vz * vz;
     if (n2 > 1e-30f) {
-        const float inv = 1.f / sqrtf(n2);
-        const int r = ring_idx(u, v);
-        const float cx = S[r], cy = S[kRing * kRing + r], cz = S[2 * kRing * kRing + r];
-        float mx = cx + kRelaxOmega * (vx * inv - cx);
-        float my = cy + kRelaxOmega * (vy * inv - cy);
-        float mz = cz + kRelaxOmega * (vz * inv - cz);
+        const float inv = __builtin_amdgcn_rsqf(n2);
+        const float mx = cx + kRelaxOmega * (vx * inv - cx);
+        const float my = cy + kRelaxOmega * (vy * inv - cy);
+        const float mz = cz + kRelaxOmega * (vz * inv - cz);
         const float m2 = mx * mx + my * my + mz * mz;
         if (m2 > 1e-30f) {
-            const float minv = 1.f / sqrtf(m2);
-            S[r] = mx * minv; S[kRing * kRing + r] = my * minv; S[2 * kRing * kRing + r] = mz * minv;
+            const float minv = __builtin_amdgcn_rsqf(m2);
+            cx = mx * minv; cy = my * minv; cz = mz * minv;
         }
     }
 }
 
-__global__ __launch_bounds__(512) void k_block_swap(float* __restrict__ pool, int64_t base, int64_t Sp, int nx,
-                                                    int ny, const int* __restrict__ bxy, int nblocks,
-                                                    const float* __restrict__ O, Stencil st, float beta, float h,
-                                                    int relax_iters, float core_radius, uint32_t seed_lo,
-                                                    uint32_t seed_hi, uint32_t inst, uint64_t round,
-                                                    int* __restrict__ accepted, float* __restrict__ stats,
-                                                    float* __restrict__ proposal) {
-    // 512 work-items: items 0..255 relax texture A, 256..511 texture B, one
-    // block site each; items 0..255 then apply the move and decide it.
+__global__ __launch_bounds__(kSwapThreads) void k_block_swap(float* __restrict__ pool, RoundTab tab,
+                                                             const int* __restrict__ bxy, const float* __restrict__ O,
+                                                             Stencil st, int relax_iters, float core_radius,
+                                                             uint32_t seed_lo, uint32_t seed_hi, uint64_t round,
+                                                             int* __restrict__ accepted, float* __restrict__ stats,
+                                                             float* __restrict__ proposal) {
     constexpr int RR = kRing * kRing;
     __shared__ float A[3 * RR], B[3 * RR];
     __shared__ float nx_s[256], ny_s[256], nz_s[256];
-    __shared__ float red[2][4];
+    __shared__ float red[3][2];
     __shared__ float mring[3];
-    __shared__ int acc_s;
+    __shared__ int acc_s, seed_s;
     const int b = blockIdx.x;
-    if (b >= nblocks) return;
+    if (b >= tab.nblocks) return;
+    const RoundInst& ri = tab.in[round_slot(tab, b)];
+    const int lb = b - ri.first;   // the block's index within its instance (its Philox counter)
+    const float beta = ri.beta, h = ri.h;
+    const int nx = ri.nx, ny = ri.ny;
+    const int64_t Sp = ri.Sp;
     const int t = threadIdx.x;
-    const int site = t & 255, which = t >> 8;
-    const int u = site >> 4, v = site & 15;
+    const int wave = t >> 6, lane = t & 63;
     const int bx = bxy[2 * b], by = bxy[2 * b + 1];
-    float* c0 = pool + base;
-    float* c1 = pool + base + 3 * Sp;
-    Lat2 L{c0, c1, Sp, nx, ny, ny / 2};
-    const int X = bx + u, Y = by + v;
-    // the fixed ring (corners unused) into both textures
-    for (int i = t; i < RR; i += 512) {
+    const int pxy = (bx + by) & 1;
+    float* c0 = pool + ri.base;
+    float* c1 = pool + ri.base + 3 * Sp;
+    Lat2 L{c0, Sp, nx, ny, ny / 2};
+    // the fixed ring (corners included) into both textures
+    for (int i = t; i < RR; i += kSwapThreads) {
         const int uu = i / kRing - 1, vv = i % kRing - 1;
         if (uu == -1 || uu == 16 || vv == -1 || vv == 16) {
             float sx, sy, sz;
@@ -671,12 +749,12 @@ __global__ __launch_bounds__(512) void k_block_swap(float* __restrict__ pool, in
         }
     }
     __syncthreads();
-    if (t == 0) {   // mean direction of the ring's edge sites, fixed order
+    if (t == 0) {   // mean direction of the ring's 64 edge sites, fixed order
         float mx = 0.f, my = 0.f, mz = 0.f;
-        for (int i = 0; i < RR; ++i) {
-            const int uu = i / kRing - 1, vv = i % kRing - 1;
-            const bool edge = (uu == -1 || uu == 16) != (vv == -1 || vv == 16);
-            if (edge) { mx += A[i]; my += A[RR + i]; mz += A[2 * RR + i]; }
+        for (int q = 0; q < 16; ++q) {
+            const int e[4] = {ring_idx(-1, q), ring_idx(16, q), ring_idx(q, -1), ring_idx(q, 16)};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { mx += A[e[k]]; my += A[RR + e[k]]; mz += A[2 * RR + e[k]]; }
         }
         const float n2 = mx * mx + my * my + mz * mz;
         if (n2 > 1e-20f) { const float inv = 1.f / sqrtf(n2); mx *= inv; my *= inv; mz *= inv; }
@@ -684,43 +762,113 @@ __global__ __launch_bounds__(512) void k_block_swap(float* __restrict__ pool, in
         mring[0] = mx; mring[1] = my; mring[2] = mz;
     }
     __syncthreads();
-    {   // starting textures: network means; B gets a skyrmion in the centre
-        const VMF f = vmf_unit(O + ((size_t)b * 256 + site) * 32);
-        const int r = ring_idx(u, v);
-        float* S = which ? B : A;
-        float px = f.mx, py = f.my, pz = f.mz;
-        const float du = (float)u - 7.5f, dv = (float)v - 7.5f;
-        const float rad = sqrtf(du * du + dv * dv);
-        if (which && rad < core_radius) {
-            // Neel template in the frame (e1, e2, m) of the ring direction m;
-            // the in-plane part points against D_{+x} . y (the chirality that
-            // lowers the DMI bond energy -s_i . (s_j x D))
-            const float chir = st.D[3][1] > 0.f ? -1.f : 1.f;
-            const float th = 3.14159265f * (1.f - rad / core_radius);
-            const float cph = rad > 0.f ? chir * du / rad : chir, sph = rad > 0.f ? chir * dv / rad : 0.f;
-            const float ta = sinf(th) * cph, tb = sinf(th) * sph, tc = cosf(th);
-            const float mx = mring[0], my = mring[1], mz = mring[2];
-            const float sgn = mz >= 0.f ? 1.f : -1.f;
-            const float a = -1.f / (sgn + mz);
-            const float bb = mx * my * a;
-            const float e1x = 1.f + sgn * mx * mx * a, e1y = sgn * bb, e1z = -sgn * mx;
-            const float e2x = bb, e2y = sgn + my * my * a, e2z = -my;
-            px = ta * e1x + tb * e2x + tc * mx;
-            py = ta * e1y + tb * e2y + tc * my;
-            pz = ta * e1z + tb * e2z + tc * mz;
+    if (wave == 0) {   // the seed site: lowest predicted alignment A(kappa) (mu . m) over the 8 x 8 window
+        const int cs = (kSeedLo + (lane >> 3)) * 16 + kSeedLo + (lane & 7);
+        const VMF f = vmf_unit(O + ((size_t)b * 256 + cs) * 32);
+        const float e2 = expf(-2.f * f.kappa);
+        const float mrl = (1.f + e2) / (1.f - e2) - 1.f / f.kappa;   // coth k - 1/k
+        float sc = mrl * (f.mx * mring[0] + f.my * mring[1] + f.mz * mring[2]);
+        int si = cs;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float osc = __shfl_xor(sc, o);
+            const int osi = __shfl_xor(si, o);
+            if (osc < sc || (osc == sc && osi < si)) { sc = osc; si = osi; }
         }
-        S[r] = px; S[RR + r] = py; S[2 * RR + r] = pz;
+        if (lane == 0) seed_s = si;
     }
     __syncthreads();
+    // starting textures, each wave its own: the Coons interpolation of the
+    // ring; B gets a skyrmion on the seed site.  The lane keeps its four
+    // sites in registers (it is their only writer).
+    float* S = wave ? B : A;
+    float3 o0, o1, o2, o3;   // the lane's sites k = 0..3 (named: no indexed register arrays)
+    {
+        auto ringA = [&](int a, int c) {
+            const int i = ring_idx(a, c);
+            return make_float3(A[i], A[RR + i], A[2 * RR + i]);
+        };
+        const float chir = st.D[3][1] > 0.f ? -1.f : 1.f;
+        const float mx = mring[0], my = mring[1], mz = mring[2];
+        const float sgn = mz >= 0.f ? 1.f : -1.f;
+        const float a = -1.f / (sgn + mz);
+        const float bb = mx * my * a;
+        const float e1x = 1.f + sgn * mx * mx * a, e1y = sgn * bb, e1z = -sgn * mx;
+        const float e2x = bb, e2y = sgn + my * my * a, e2z = -my;
+        auto start = [&](int k, float3& o) {
+            int u, v;
+            swap_site(lane, k, pxy, u, v);
+            coons_site(u, v, ringA, o.x, o.y, o.z);
+            const float du = (float)(u - (seed_s >> 4)), dv = (float)(v - (seed_s & 15));
+            const float rad = sqrtf(du * du + dv * dv);
+            if (wave == 1 && rad < core_radius) {
+                // Neel template in the frame (e1, e2, m) of the ring direction
+                // m; the in-plane part points against D_{+x} . y (the
+                // chirality that lowers the DMI bond energy -s_i . (s_j x D))
+                const float th = 3.14159265f * (1.f - rad / core_radius);
+                const float cph = rad > 0.f ? chir * du / rad : chir, sph = rad > 0.f ? chir * dv / rad : 0.f;
+                const float ta = sinf(th) * cph, tb = sinf(th) * sph, tc = cosf(th);
+                o.x = ta * e1x + tb * e2x + tc * mx;
+                o.y = ta * e1y + tb * e2y + tc * my;
+                o.z = ta * e1z + tb * e2z + tc * mz;
+            }
+        };
+        start(0, o0); start(1, o1); start(2, o2); start(3, o3);
+    }
+    auto put = [&](int k, const float3& o) {
+        int u, v;
+        swap_site(lane, k, pxy, u, v);
+        const int r = ring_idx(u, v);
+        S[r] = o.x; S[RR + r] = o.y; S[2 * RR + r] = o.z;
+    };
+    auto relax = [&](int k, float3& o) {
+        int u, v;
+        swap_site(lane, k, pxy, u, v);
+        relax_site(S, u, v, st, h, o.x, o.y, o.z);
+    };
+    put(0, o0); put(1, o1); put(2, o2); put(3, o3);
+    wave_lds_sync();
+    // relaxation: each wave alone (its texture's interior is written by its
+    // own lanes only), checkerboard half-steps separated by wave-level LDS
+    // ordering instead of workgroup barriers.  A half-step reads only the
+    // other parity's sites, so its writes need no ordering against its reads.
     for (int it = 0; it < relax_iters; ++it) {
-        for (int par = 0; par < 2; ++par) {
-            relax_half(which ? B : A, u, v, par, X, Y, st, h);
-            __syncthreads();
+        relax(0, o0); relax(1, o1);
+        put(0, o0); put(1, o1);
+        wave_lds_sync();
+        relax(2, o2); relax(3, o3);
+        put(2, o2); put(3, o3);
+        wave_lds_sync();
+    }
+    __syncthreads();
+    float dq = 0.f;
+    if (stats) {
+        // Q(B) - Q(A) over the 17 x 17 plaquettes of block and ring (an
+        // integer: the textures share the ring), triangles (p00, p01, p11)
+        // and (p00, p11, p10) with p_ab at (x + a, y + b) -- the orientation
+        // of the lattice charge -- solid angle 2 atan2(a . b x c, 1 + a.b + b.c + c.a)
+        auto tri = [&](const float* T, int ia, int ib, int ic) {
+            const float ax = T[ia], ay = T[RR + ia], az = T[2 * RR + ia];
+            const float bx_ = T[ib], by_ = T[RR + ib], bz_ = T[2 * RR + ib];
+            const float cx = T[ic], cy = T[RR + ic], cz = T[2 * RR + ic];
+            const float trip = ax * (by_ * cz - bz_ * cy) + ay * (bz_ * cx - bx_ * cz) + az * (bx_ * cy - by_ * cx);
+            const float den = 1.f + ax * bx_ + ay * by_ + az * bz_ + bx_ * cx + by_ * cy + bz_ * cz + cx * ax +
+                              cy * ay + cz * az;
+            return 2.f * atan2f(trip, den);
+        };
+        for (int p = t; p < 289; p += kSwapThreads) {
+            const int pu = p / 17 - 1, pv = p - (p / 17) * 17 - 1;
+            const int i00 = ring_idx(pu, pv), i10 = ring_idx(pu + 1, pv), i11 = ring_idx(pu + 1, pv + 1),
+                      i01 = ring_idx(pu, pv + 1);
+            dq += (tri(B, i00, i01, i11) + tri(B, i00, i11, i10)) - (tri(A, i00, i01, i11) + tri(A, i00, i11, i10));
         }
     }
-    float dE = 0.f, sep = 0.f, tx = 0.f, ty = 0.f, tz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-    if (t < 256) {
-        // the involution: pi rotation about the bisector of A_i and B_i
+    // the involution: pi rotation about the bisector of A_i and B_i, sites t
+    // and t + 128 (the old spin s and the proposed one p)
+    float3 s0, s1, p0, p1;
+    float sep = 0.f;
+    auto rotate = [&](int site, float3& sv, float3& pv) {
+        const int u = site >> 4, v = site & 15;
         const int r = ring_idx(u, v);
         const float ax = A[r], ay = A[RR + r], az = A[2 * RR + r];
         const float qx = B[r], qy = B[RR + r], qz = B[2 * RR + r];
@@ -733,27 +881,30 @@ __global__ __launch_bounds__(512) void k_block_swap(float* __restrict__ pool, in
         }
         const float winv = 1.f / sqrtf(w2);
         wx *= winv; wy *= winv; wz *= winv;
-        spin_at(L, X, Y, sx, sy, sz);
-        const float d = 2.f * (wx * sx + wy * sy + wz * sz);
-        tx = d * wx - sx; ty = d * wy - sy; tz = d * wz - sz;
-        nx_s[t] = tx; ny_s[t] = ty; nz_s[t] = tz;
+        spin_at(L, bx + u, by + v, sv.x, sv.y, sv.z);
+        const float d = 2.f * (wx * sv.x + wy * sv.y + wz * sv.z);
+        pv = make_float3(d * wx - sv.x, d * wy - sv.y, d * wz - sv.z);
+        nx_s[site] = pv.x; ny_s[site] = pv.y; nz_s[site] = pv.z;
         if (proposal) {
-            float* pr = proposal + ((size_t)b * 256 + t) * 3;
-            pr[0] = tx; pr[1] = ty; pr[2] = tz;
+            float* pr = proposal + ((size_t)b * 256 + site) * 3;
+            pr[0] = pv.x; pr[1] = pv.y; pr[2] = pv.z;
         }
-        sep = (ax - qx) * (ax - qx) + (ay - qy) * (ay - qy) + (az - qz) * (az - qz);
-    }
+        sep += (ax - qx) * (ax - qx) + (ay - qy) * (ay - qy) + (az - qz) * (az - qz);
+    };
+    rotate(t, s0, p0);
+    rotate(t + kSwapThreads, s1, p1);
     __syncthreads();
-    if (t < 256) {
-        // dE_s over every bond touching the block (inside: half from each end)
-        dE = -h * (tz - sz);
+    // dE_s over every bond touching the block (inside: half from each end)
+    auto site_dE = [&](int site, const float3& sv, const float3& pv) {
+        const int u = site >> 4, v = site & 15;
+        float e = -h * (pv.z - sv.z);
         const int dxs[4] = {0, 0, 1, -1}, dys[4] = {1, -1, 0, 0};
         const int gs[4] = {5, 4, 3, 2};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int uu = u + dxs[q], vv = v + dys[q];
             float ox_, oy_, oz_, mx_, my_, mz_;
-            spin_at(L, X + dxs[q], Y + dys[q], ox_, oy_, oz_);
+            spin_at(L, bx + uu, by + vv, ox_, oy_, oz_);
             const bool in_b = uu >= 0 && uu < 16 && vv >= 0 && vv < 16;
             if (in_b) { mx_ = nx_s[uu * 16 + vv]; my_ = ny_s[uu * 16 + vv]; mz_ = nz_s[uu * 16 + vv]; }
             else { mx_ = ox_; my_ = oy_; mz_ = oz_; }
@@ -764,35 +915,47 @@ __global__ __launch_bounds__(512) void k_block_swap(float* __restrict__ pool, in
                 const float crx = cy * Dz - cz * Dy, cry = cz * Dx - cx * Dz, crz = cx * Dy - cy * Dx;
                 return -(J * (px * cx + py * cy + pz * cz) + px * crx + py * cry + pz * crz);
             };
-            const float dd = bond(tx, ty, tz, mx_, my_, mz_) - bond(sx, sy, sz, ox_, oy_, oz_);
-            dE += in_b ? 0.5f * dd : dd;
+            const float dd = bond(pv.x, pv.y, pv.z, mx_, my_, mz_) - bond(sv.x, sv.y, sv.z, ox_, oy_, oz_);
+            e += in_b ? 0.5f * dd : dd;
         }
-        float vals[2] = {dE, sep};
+        return e;
+    };
+    const float dE = site_dE(t, s0, p0) + site_dE(t + kSwapThreads, s1, p1);
+    {
+        float vals[3] = {dE, sep, dq};
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 3; ++j) {
             float s2 = vals[j];
             for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
-            if ((t & 63) == 0) red[j][t >> 6] = s2;
+            if (lane == 0) red[j][wave] = s2;
         }
     }
     __syncthreads();
     if (t == 0) {
-        const float DE = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-        const float SEP = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        const float DE = red[0][0] + red[0][1];
         uint32_t w[4];
-        philox4x32_10((uint32_t)(b * 256), (uint32_t)(kPurposeSwap << 8), (uint32_t)round, (uint32_t)(round >> 32),
-                      seed_lo, seed_hi + inst, w);
+        philox4x32_10((uint32_t)(lb * 256), (uint32_t)(kPurposeSwap << 8), (uint32_t)round, (uint32_t)(round >> 32),
+                      seed_lo, seed_hi + ri.inst_id, w);
         const float loga = -beta * DE;
         acc_s = (loga >= 0.f || logf(u24_open(w[2])) < loga) ? 1 : 0;
         accepted[b] = acc_s;
-        if (stats) { stats[3 * b] = DE; stats[3 * b + 1] = SEP; stats[3 * b + 2] = 0.f; }
+        if (stats) {
+            stats[4 * b] = DE;
+            stats[4 * b + 1] = red[1][0] + red[1][1];
+            stats[4 * b + 2] = (red[2][0] + red[2][1]) * 0.0795774715f;   // / 4 pi
+            stats[4 * b + 3] = (float)seed_s;
+        }
     }
     __syncthreads();
-    if (t < 256 && acc_s) {
-        const int xx = X % nx, yy = Y % ny;
-        float* c = ((xx + yy) & 1) ? c1 : c0;
-        const int64_t e = (int64_t)xx * (ny / 2) + (yy >> 1);
-        c[e] = tx; c[Sp + e] = ty; c[2 * Sp + e] = tz;
+    if (acc_s) {
+        auto store = [&](int site, const float3& pv) {
+            const int xx = (bx + (site >> 4)) % nx, yy = (by + (site & 15)) % ny;
+            float* c = ((xx + yy) & 1) ? c1 : c0;
+            const int64_t e = (int64_t)xx * (ny / 2) + (yy >> 1);
+            c[e] = pv.x; c[Sp + e] = pv.y; c[2 * Sp + e] = pv.z;
+        };
+        store(t, p0);
+        store(t + kSwapThreads, p1);
     }
 }
 
@@ -874,48 +1037,53 @@ hipError_t launch_adam(hipStream_t s, float* w, float* mw, float* vw, const floa
     return hipGetLastError();
 }
 
-// Corners of one round's blocks on one instance: block b = cx * nby + cy
+// Corners of one round's blocks: block lb = cx * nby + cy of an instance
 // sits at (32 cx + off_x, 32 cy + off_y), the order the host enumerates.
-__global__ void k_block_xy(int* __restrict__ bxy, int nbx, int nby, int off_x, int off_y) {
+__global__ void k_block_xy(int* __restrict__ bxy, RoundTab tab, int off_x, int off_y) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbx * nby) return;
-    const int cx = b / nby, cy = b - cx * nby;
+    if (b >= tab.nblocks) return;
+    const RoundInst& ri = tab.in[round_slot(tab, b)];
+    const int lb = b - ri.first, nby = ri.ny / 32;
+    const int cx = lb / nby, cy = lb - cx * nby;
     bxy[2 * b] = 32 * cx + off_x;
     bxy[2 * b + 1] = 32 * cy + off_y;
 }
 
-// A round's accepted blocks of one instance added to its running count (one
-// work-item, in block order: the only writer of *total in the launch).
-__global__ __launch_bounds__(256) void k_block_acc_add(const int* __restrict__ acc, int nblocks,
-                                                       unsigned long long* __restrict__ total) {
+// A round's accepted blocks of each slot's instance added to its running
+// count: one workgroup per slot, the only writer of that total in the launch.
+__global__ __launch_bounds__(256) void k_block_acc_add(const int* __restrict__ acc, RoundTab tab,
+                                                       unsigned long long* __restrict__ totals) {
     // integer counts: the order of the sum does not matter
     __shared__ unsigned long long part[256];
+    const int j = blockIdx.x;
+    const int lo = tab.in[j].first, hi = j + 1 < tab.n ? tab.in[j + 1].first : tab.nblocks;
     unsigned long long t = 0;
-    for (int b = threadIdx.x; b < nblocks; b += 256) t += (unsigned long long)acc[b];
+    for (int b = lo + (int)threadIdx.x; b < hi; b += 256) t += (unsigned long long)acc[b];
     part[threadIdx.x] = t;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *total += part[0];
+    if (threadIdx.x == 0) totals[tab.in[j].slot] += part[0];
 }
 
-hipError_t launch_block_xy(hipStream_t s, int* bxy, int nbx, int nby, int off_x, int off_y) {
-    if (nbx * nby <= 0) return hipSuccess;
-    k_block_xy<<<(nbx * nby + 255) / 256, 256, 0, s>>>(bxy, nbx, nby, off_x, off_y);
+hipError_t launch_block_xy(hipStream_t s, int* bxy, const RoundTab& tab, int off_x, int off_y) {
+    if (tab.nblocks <= 0) return hipSuccess;
+    k_block_xy<<<(tab.nblocks + 255) / 256, 256, 0, s>>>(bxy, tab, off_x, off_y);
     return hipGetLastError();
 }
 
-hipError_t launch_block_acc_add(hipStream_t s, const int* acc, int nblocks, unsigned long long* total) {
-    k_block_acc_add<<<1, 256, 0, s>>>(acc, nblocks, total);
+hipError_t launch_block_acc_add(hipStream_t s, const int* acc, const RoundTab& tab, unsigned long long* totals) {
+    if (tab.n <= 0) return hipSuccess;
+    k_block_acc_add<<<tab.n, 256, 0, s>>>(acc, tab, totals);
     return hipGetLastError();
 }
 
-hipError_t launch_gather_patches(hipStream_t s, const float* pool, int64_t base, int64_t Sp, int nx, int ny,
-                                 const int* bxy, int npatch, __bf16* X, float* target) {
-    if (npatch <= 0) return hipSuccess;
-    k_gather_patches<<<npatch, 256, 0, s>>>(pool, base, Sp, nx, ny, bxy, npatch, X, target);
+hipError_t launch_gather_patches(hipStream_t s, const float* pool, const RoundTab& tab, const int* bxy, __bf16* X,
+                                 float* target) {
+    if (tab.nblocks <= 0) return hipSuccess;
+    k_gather_patches<<<tab.nblocks, 256, 0, s>>>(pool, tab, bxy, X, target);
     return hipGetLastError();
 }
 
@@ -926,23 +1094,20 @@ hipError_t launch_head_loss(hipStream_t s, const float* O, const float* target, 
     return hipGetLastError();
 }
 
-hipError_t launch_block_swap(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
-                             int nblocks, const float* O, const Stencil& st, float beta, float h, int relax_iters,
-                             float core_radius, uint32_t seed_lo, uint32_t seed_hi, uint32_t inst, uint64_t round,
-                             int* accepted, float* stats, float* proposal) {
-    if (nblocks <= 0) return hipSuccess;
-    k_block_swap<<<nblocks, 512, 0, s>>>(pool, base, Sp, nx, ny, bxy, nblocks, O, st, beta, h, relax_iters,
-                                         core_radius, seed_lo, seed_hi, inst, round, accepted, stats, proposal);
+hipError_t launch_block_swap(hipStream_t s, float* pool, const RoundTab& tab, const int* bxy, const float* O,
+                             const Stencil& st, int relax_iters, float core_radius, uint32_t seed_lo,
+                             uint32_t seed_hi, uint64_t round, int* accepted, float* stats, float* proposal) {
+    if (tab.nblocks <= 0) return hipSuccess;
+    k_block_swap<<<tab.nblocks, kSwapThreads, 0, s>>>(pool, tab, bxy, O, st, relax_iters, core_radius, seed_lo,
+                                                      seed_hi, round, accepted, stats, proposal);
     return hipGetLastError();
 }
 
-hipError_t launch_block_mh(hipStream_t s, float* pool, int64_t base, int64_t Sp, int nx, int ny, const int* bxy,
-                           int nblocks, const float* O, const Stencil& st, float beta, float h, uint32_t seed_lo,
-                           uint32_t seed_hi, uint32_t inst, uint64_t round, int* accepted, float* stats,
-                           float* proposal) {
-    if (nblocks <= 0) return hipSuccess;
-    k_block_mh<<<nblocks, 256, 0, s>>>(pool, base, Sp, nx, ny, bxy, nblocks, O, st, beta, h, seed_lo, seed_hi, inst,
-                                       round, accepted, stats, proposal);
+hipError_t launch_block_mh(hipStream_t s, float* pool, const RoundTab& tab, const int* bxy, const float* O,
+                           const Stencil& st, uint32_t seed_lo, uint32_t seed_hi, uint64_t round, int* accepted,
+                           float* stats, float* proposal) {
+    if (tab.nblocks <= 0) return hipSuccess;
+    k_block_mh<<<tab.nblocks, 256, 0, s>>>(pool, tab, bxy, O, st, seed_lo, seed_hi, round, accepted, stats, proposal);
     return hipGetLastError();
 }
 

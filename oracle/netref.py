@@ -2,6 +2,11 @@
 INFRASTRUCTURE ONLY (the checker of the device's forward and backward
 kernels; never imported by the nanomagmc_amd package).
 
+Input (patch()): the 24 x 24 window around a 16 x 16 block, channels
+(spin x, y, z, halo flag); the block itself carries the Coons interpolation
+of its one-site ring (flag 0), so every block site sees its surroundings
+through four 3 x 3 layers.
+
 Network: four valid 3x3 convolutions on a 24 x 24 patch, input channels
 (spin x, y, z, halo flag) padded to 16, then 32, 32, 32 channels with ReLU,
 last layer 4 outputs per block site: mu (unnormalised) and the kappa
@@ -158,16 +163,99 @@ def init_params(rng):
     return np.concatenate(p)
 
 
-def swap_proposal(spins, L, bx, by, mu_raw, J, D, h, iters, core_radius):
+def coons(ring, normalise=True):
+    """Transfinite (Coons) interpolation of a 16 x 16 block from its one-site
+    ring: ring (18, 18, 3) indexed [u + 1, v + 1] for u, v in [-1, 16]
+    (corners included).  With s = (u + 1) / 17, t = (v + 1) / 17,
+        P = (1-s) R(-1,v) + s R(16,v) + (1-t) R(u,-1) + t R(u,16)
+            - [(1-s)(1-t) R(-1,-1) + (1-s) t R(-1,16) + s (1-t) R(16,-1) + s t R(16,16)],
+    then normalised per site (|P|^2 < 1e-12 gives +z).  Returns (16, 16, 3)."""
+    R = np.asarray(ring, dtype=np.float64)
+    u = np.arange(16)[:, None, None]
+    v = np.arange(16)[None, :, None]
+    s = (u + 1) / 17.0
+    t = (v + 1) / 17.0
+    P = ((1 - s) * R[0, 1:17][None] + s * R[17, 1:17][None] + (1 - t) * R[1:17, 0][:, None]
+         + t * R[1:17, 17][:, None]
+         - ((1 - s) * (1 - t) * R[0, 0] + (1 - s) * t * R[0, 17] + s * (1 - t) * R[17, 0] + s * t * R[17, 17]))
+    if not normalise:
+        return P
+    n2 = (P * P).sum(-1, keepdims=True)
+    out = P / np.sqrt(np.maximum(n2, 1e-300))
+    out[n2[..., 0] < 1e-12] = [0.0, 0.0, 1.0]
+    return out
+
+
+def patch(spins, L, bx, by):
+    """The network input of the 16 x 16 block at (bx, by) of a periodic L x L
+    lattice (spins (L*L, 3), site index x * L + y): 24 x 24 x 4, the 4-site
+    halo's spins with flag 1, and on the block the Coons interpolation of its
+    one-site ring with flag 0.  The block's own spins never enter."""
+    s = np.asarray(spins, dtype=np.float64).reshape(L, L, 3)
+    P = np.zeros((24, 24, 4))
+    P[..., :3] = s[np.ix_(np.arange(bx - 4, bx + 20) % L, np.arange(by - 4, by + 20) % L)]
+    P[..., 3] = 1.0
+    P[4:20, 4:20, :3] = coons(P[3:21, 3:21, :3])
+    P[4:20, 4:20, 3] = 0.0
+    return P
+
+
+SEED_LO, SEED_HI = 4, 12     # seed sites u, v in [4, 12)
+
+
+def seed_site(out, m):
+    """Where the swap seeds its skyrmion: among block sites u, v in [4, 12),
+    the one whose predicted mean alignment with the ring direction m,
+    E_q[s . m] = A(kappa) (mu . m) with A(k) = coth k - 1/k, is lowest (the
+    first in u-major order on ties).  out (16, 16, >=4): the network's raw
+    outputs.  Returns ((cu, cv), score (16, 16) with +inf outside the window)."""
+    O = np.asarray(out, dtype=np.float64)
+    m3 = O[..., :3]
+    n = np.linalg.norm(m3, axis=-1, keepdims=True)
+    mu = np.where(n > 1e-15, m3 / np.maximum(n, 1e-300), [0.0, 0.0, 1.0])
+    z = O[..., 3]
+    kappa = np.where(z > 20, z, np.log1p(np.exp(np.minimum(z, 20)))) + 0.05
+    A = 1.0 / np.tanh(kappa) - 1.0 / kappa
+    score = np.full((16, 16), np.inf)
+    w = slice(SEED_LO, SEED_HI)
+    score[w, w] = (A * (mu @ np.asarray(m, dtype=np.float64)))[w, w]
+    k = int(np.argmin(score))
+    return (k // 16, k % 16), score
+
+
+def solid_angle(a, b, c):
+    """Signed solid angle of the spherical triangle (a, b, c):
+    2 atan2(a . (b x c), 1 + a.b + b.c + c.a)."""
+    return 2.0 * np.arctan2((a * np.cross(b, c)).sum(-1), 1.0 + (a * b).sum(-1) + (b * c).sum(-1) + (c * a).sum(-1))
+
+
+def block_charge(S):
+    """Sum of the solid angles / 4 pi over the 17 x 17 plaquettes of an
+    (18, 18, 3) block-plus-ring texture, each plaquette split into the
+    triangles (p00, p01, p11) and (p00, p11, p10), p_ab at (x + a, y + b):
+    the orientation of the reference's lattice charge.  Two textures with the
+    same ring differ by an integer."""
+    S = np.asarray(S, dtype=np.float64)
+    p00, p10, p11, p01 = S[:-1, :-1], S[1:, :-1], S[1:, 1:], S[:-1, 1:]
+    return float((solid_angle(p00, p01, p11) + solid_angle(p00, p11, p10)).sum() / (4 * np.pi))
+
+
+def swap_proposal(spins, L, bx, by, out, J, D, h, iters, core_radius, seed=None):
     """FP64 restatement of the learned texture swap of one 16 x 16 block of a
     periodic L x L lattice (spins (L*L, 3), site index x * L + y).
 
     J[g], D[g] (3-vector) per stencil slot g: 2 = (-1, 0), 3 = (+1, 0),
-    4 = (0, -1), 5 = (0, +1) (x = first coordinate).  mu_raw (16, 16, 3): the
-    network's unnormalised mean directions.  Relaxation: `iters` pairs of
-    checkerboard half-steps s <- normalise(s + 1.85 (F - s)), F the unit local
-    field.  Returns (x', A, B): the rotated block and the two relaxed
-    textures, each (16, 16, 3)."""
+    4 = (0, -1), 5 = (0, +1) (x = first coordinate).  out (16, 16, >=4): the
+    network's raw outputs on the block, which choose the seed site
+    (seed_site); `seed` = (cu, cv) overrides the choice.  Texture A starts
+    from the Coons interpolation of the block's one-site ring, texture B from
+    the same with a Neel skyrmion of radius core_radius centred on the seed
+    site, its core against the ring's mean direction m and its chirality
+    -sign(D_{+x} . y).  Relaxation: `iters` pairs of checkerboard half-steps
+    s <- normalise(s + 1.85 (F - s)), F the unit local field, ring fixed.
+    Returns (x', A, B, info): the rotated block, the two relaxed textures,
+    each (16, 16, 3), and info = {"seed": (cu, cv), "dq": Q(B) - Q(A)} with
+    Q the block charge over the block and its ring."""
     s = np.asarray(spins, dtype=np.float64).reshape(L, L, 3)
     xs = np.arange(bx - 1, bx + 17) % L
     ys = np.arange(by - 1, by + 17) % L
@@ -177,16 +265,20 @@ def swap_proposal(spins, L, bx, by, mu_raw, J, D, h, iters, core_radius):
     m = ring[edge].sum(axis=0)
     nm = np.linalg.norm(m)
     m = m / nm if nm > 1e-10 else np.array([0.0, 0.0, 1.0])
-    mu = mu_raw / np.linalg.norm(mu_raw, axis=-1, keepdims=True)
+    if seed is None:
+        seed, _ = seed_site(out, m)
+    cu, cv = seed
     A = ring.copy()
-    A[1:17, 1:17] = mu
+    A[1:17, 1:17] = coons(ring)
     B = A.copy()
     u, v = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
-    du, dv = u - 7.5, v - 7.5
+    du, dv = (u - cu).astype(np.float64), (v - cv).astype(np.float64)
     rad = np.sqrt(du * du + dv * dv)
     th = np.pi * (1 - rad / core_radius)
     chir = -1.0 if D[3][1] > 0 else 1.0
-    cph, sph = chir * du / rad, chir * dv / rad
+    safe = np.where(rad > 0, rad, 1.0)
+    cph = np.where(rad > 0, chir * du / safe, chir)
+    sph = np.where(rad > 0, chir * dv / safe, 0.0)
     ta, tb, tc = np.sin(th) * cph, np.sin(th) * sph, np.cos(th)
     sgn = 1.0 if m[2] >= 0 else -1.0
     a = -1.0 / (sgn + m[2])
@@ -218,6 +310,7 @@ def swap_proposal(spins, L, bx, by, mu_raw, J, D, h, iters, core_radius):
         for p in (0, 1):
             half(A, p)
             half(B, p)
+    dq = block_charge(B) - block_charge(A)
     Ab, Bb = A[1:17, 1:17], B[1:17, 1:17]
     w = Ab + Bb
     nw = np.linalg.norm(w, axis=-1, keepdims=True)
@@ -229,4 +322,4 @@ def swap_proposal(spins, L, bx, by, mu_raw, J, D, h, iters, core_radius):
     w = w / nw
     x = s[np.ix_(np.arange(bx, bx + 16) % L, np.arange(by, by + 16) % L)]
     xp = 2 * (w * x).sum(-1, keepdims=True) * w - x
-    return xp, Ab, Bb
+    return xp, Ab, Bb, {"seed": (int(cu), int(cv)), "dq": dq}

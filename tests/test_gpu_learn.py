@@ -28,19 +28,14 @@ def last_bias_offset():
 
 
 def patches_of(spins, L, blocks):
-    """The device's network input for blocks at (bx, by): 24 x 24 x 4 with the
-    16 x 16 interior masked, and the interior targets."""
+    """The device's network input for blocks at (bx, by) (the oracle's
+    patch(): halo spins and the block's interpolated ring) and the block
+    targets."""
+    from oracle import netref
     s = spins.reshape(L, L, 3)
-    P = np.zeros((len(blocks), 24, 24, 4))
-    T = np.zeros((len(blocks), 256, 3))
-    for i, (bx, by) in enumerate(blocks):
-        xs = (np.arange(bx - 4, bx + 20) % L)
-        ys = (np.arange(by - 4, by + 20) % L)
-        patch = s[np.ix_(xs, ys)]
-        P[i, ..., :3] = patch
-        P[i, ..., 3] = 1.0
-        T[i] = patch[4:20, 4:20].reshape(256, 3)
-        P[i, 4:20, 4:20, :] = 0.0
+    P = np.stack([netref.patch(spins, L, bx, by) for bx, by in blocks])
+    T = np.stack([s[np.ix_(np.arange(bx, bx + 16) % L, np.arange(by, by + 16) % L)].reshape(256, 3)
+                  for bx, by in blocks])
     return P, T
 
 
@@ -208,24 +203,53 @@ def test_training_lowers_the_loss_and_keeps_the_physics():
     print("loss %.3f -> %.3f, learned block acceptance %.3f" % (first, last, acc / (n * 4)))
 
 
-def test_learned_rounds_against_plain_sweeps_tau_and_ess():
-    # the comparison protocol at a reduced size (L = 128, T = 0.3): every
-    # method runs, tau_int of E and |Q| and the ESS/s ratios are reported
+@pytest.fixture(scope="module")
+def learned_comparison():
+    # the comparison protocol at a reduced size: L = 128, T = 0.3, 8 replicas
+    # per method, 20000 swap samples of preparation, 4000 samples of 4 sweeps
     from nanomagmc_amd.learn import compare_ess
-    r = compare_ess(L=128, temps=(0.3,), replicas=2, anneal_sweeps=2000, collect_rounds=20, collect_blocks=500,
-                    train_steps=300, batch=128, nsamples=800, timing_samples=50, log=print)
+    r = compare_ess(L=128, temps=(0.3,), replicas=8, anneal_sweeps=2000, collect_rounds=20, collect_blocks=500,
+                    train_steps=300, batch=128, eq_samples=20000, nsamples=4000, timing_samples=100, log=print)
     row = r["temps"][0]
-    print(row)
+    print(r["s_per_sample"], row)
+    return r, row
+
+
+def test_learned_rounds_against_plain_sweeps_tau_and_ess(learned_comparison):
+    # what the learned rounds do that can be measured at this size, each
+    # against its error: the training loss falls; E's windows close and
+    # E's gain is in the measured band (0.24 +- 0.02 for swaps, 0.33 +- 0.03
+    # for vMF: a round costs more than it decorrelates E); the observables
+    # agree with plain sweeps; and swaps change the topological charge more
+    # often per sweep than plain sweeps (0.23 against 0.05 per 1000 sweeps)
+    r, row = learned_comparison
     assert r["train_blocks"] == 10_000
     assert row["loss"][1] < row["loss"][0]
     for m in ("vmf", "swap"):
         assert 0.0 <= row["acceptance"][m] <= 1.0
         assert r["s_per_sample"][m] > r["s_per_sample"]["plain"] > 0
-        assert math.isfinite(row["gain_absQ"][m]) and math.isfinite(row["gain_E"][m])
+    assert row["acceptance"]["swap"] > 0.05
     for m in ("plain", "vmf", "swap"):
+        assert row[m]["E"]["window_closed"], m
         for key in ("E", "absQ"):
             st = row[m][key]
             assert math.isfinite(st["mean"]) and st["tau_int"] >= 0.5 and st["ess_per_s"] > 0
+            d = st["mean"] - row["plain"][key]["mean"]
+            assert abs(d) < 4 * math.hypot(st["err"], row["plain"][key]["err"]) + 1e-9, (m, key, d)
+    for m in ("vmf", "swap"):
+        g, e = row["gain_E"][m], row["gain_E_err"][m]
+        assert 0.08 < g < 1.0 and e < 0.3 * g, (m, g, e)
+    assert row["swap"]["q_changes_per_ksweep"] > 2 * row["plain"]["q_changes_per_ksweep"]
+
+
+@pytest.mark.xfail(reason="the >= 3x target for |Q| is not reached: at T <= 0.3 no method's |Q| series "
+                          "decorrelates within the run (the multi-chain windows do not close), and a swap "
+                          "round costs about 4x a plain sample of 4 sweeps", strict=False)
+def test_learned_gain_target_for_abs_q(learned_comparison):
+    _, row = learned_comparison
+    for m in ("plain", "swap"):
+        assert row[m]["absQ"]["window_closed"], m
+    assert row["gain_absQ"]["swap"] - 2 * row["gain_absQ_err"]["swap"] >= 3.0
 
 
 def stencil_slots(D):
@@ -252,7 +276,11 @@ def constant_net(mu, kappa=1.0):
 
 
 def test_block_swaps_match_oracle(orc):
-    from nanomagmc_amd.learn import block_swaps
+    # a random network (so the seed sites differ between blocks) on a noisy
+    # ferromagnetic lattice: the seed site, the rotated block, |A - B|^2, the
+    # block charge difference, dE_s and every accept decision against the
+    # FP64 oracle
+    from nanomagmc_amd.learn import Net, block_swaps
     from oracle import netref
     L, D, h, T = 64, 0.3, 0.06, 0.3
     lat = lattice(L, D)
@@ -262,11 +290,11 @@ def test_block_swaps_match_oracle(orc):
     s[:, 2] = 1.0
     s += 0.25 * rng.normal(size=s.shape)
     s /= np.linalg.norm(s, axis=1, keepdims=True)
-    # one block starts near a relaxed skyrmion, so both directions of the swap occur
     before = s.astype(np.float32).astype(np.float64)
     lat.set_spins(0, before)
-    mu = (0.1, -0.05, 1.0)
-    net = constant_net(mu)
+    p = netref.init_params(rng).astype(np.float32)
+    net = Net(seed=6)
+    net.set_params(p)
     off = (3, 21)
     iters, R0 = 80, 5.0
     acc, stats, prop = block_swaps(lat, net, [0], *off, relax_iters=iters, core_radius=R0, diagnostics=True)
@@ -277,14 +305,26 @@ def test_block_swaps_match_oracle(orc):
     H = [0, 0, h]
     E0 = f.energy(before, H, use_d=True, convention=1)
     blocks = [(32 * i + off[0], 32 * j + off[1]) for i in range(2) for j in range(2)]
-    mu_raw = np.broadcast_to(np.asarray(mu, dtype=np.float64), (16, 16, 3))
-    naccepted = 0
+    P, _ = patches_of(before, L, blocks)
+    _, O = netref.forward(p.astype(np.float64), P, use_bf16=True)
+    naccepted, seeds, created = 0, set(), 0
     for b, (bx, by) in enumerate(blocks):
-        xp, A, B = netref.swap_proposal(before, L, bx, by, mu_raw, J6, D6, h, iters, R0)
+        ring = before.reshape(L, L, 3)[np.ix_(np.arange(bx - 1, bx + 17) % L, np.arange(by - 1, by + 17) % L)]
+        m = ring[0, 1:17].sum(0) + ring[17, 1:17].sum(0) + ring[1:17, 0].sum(0) + ring[1:17, 17].sum(0)
+        m /= np.linalg.norm(m)
+        (cu, cv), score = netref.seed_site(O[b], m)
+        dev = int(stats[b, 3])
+        # the device's choice is the oracle's, or ties with it to the device's fp32/bf16 rounding
+        assert dev == 16 * cu + cv or score[dev // 16, dev % 16] - score[cu, cv] < 2e-3, (b, dev, (cu, cv))
+        seeds.add(dev)
+        xp, A, B, info = netref.swap_proposal(before, L, bx, by, O[b], J6, D6, h, iters, R0,
+                                              seed=(dev // 16, dev % 16))
         assert np.abs(prop[b] - xp).max() < 2e-3, (b, np.abs(prop[b] - xp).max())
         sep = ((A - B) ** 2).sum()
         assert abs(stats[b, 1] - sep) <= 1e-2 * sep + 1e-3, (b, stats[b, 1], sep)
-        assert sep > 10.0          # B holds a skyrmion that A does not
+        assert abs(stats[b, 2] - info["dq"]) < 1e-3 and abs(info["dq"] - round(info["dq"])) < 1e-6, \
+            (b, stats[b, 2], info["dq"])
+        created += abs(round(info["dq"])) == 1
         idx = ((np.arange(bx, bx + 16)[:, None] % L) * L + (np.arange(by, by + 16)[None, :] % L)).reshape(-1)
         trial = before.copy()
         trial[idx] = prop[b].reshape(256, 3)
@@ -300,6 +340,7 @@ def test_block_swaps_match_oracle(orc):
         else:
             assert np.array_equal(after[idx], before[idx])
     assert acc[0] == naccepted
+    assert len(seeds) > 1 and created >= 2
 
 
 def test_block_swap_is_an_involution():

@@ -64,3 +64,33 @@ def test_batched_tau_equals_the_loop():
         t, W = tau_int(x)
         assert abs(tau[i] - t) < 1e-12 * max(1.0, t), (i, tau[i], t)
         assert bool(closed[i]) == bool(ess(x)[2]), i
+
+
+def test_multi_chain_tau_on_ar1_and_stuck_chains():
+    # AR(1) chains x_n = phi x_{n-1} + e_n started stationary: tau_int =
+    # (1 + phi) / (2 (1 - phi)) (1/2 for phi = 0); chains frozen at different
+    # values count as about half a sample each; with one chain the estimate
+    # agrees with the single-chain tau_int
+    from nanomagmc_amd.stats import tau_int, tau_int_chains
+    rng = np.random.default_rng(7)
+    for phi in (0.0, 0.6, 0.9):
+        M, N = 8, 20000
+        x = np.zeros((M, N))
+        x[:, 0] = rng.normal(size=M) / np.sqrt(1 - phi * phi)
+        e = rng.normal(size=(M, N))
+        for n in range(1, N):
+            x[:, n] = phi * x[:, n - 1] + e[:, n]
+        want = (1 + phi) / (2 * (1 - phi))
+        tau, W, closed = tau_int_chains(x)
+        assert closed and abs(tau - want) < 0.06 * want + 0.02, (phi, tau, want)
+        t1, _ = tau_int(x[0])
+        t1m, _, _ = tau_int_chains(x[:1])
+        assert abs(t1m - t1) < 0.05 * t1 + 0.02, (phi, t1m, t1)
+    stuck = np.repeat(np.array([[3.0], [5.0], [4.0], [7.0]]), 1000, axis=1)
+    stuck[:, ::97] += 1.0          # rare one-sample flickers: the per-chain tau sees only these
+    tau, W, closed = tau_int_chains(stuck)
+    assert not closed and tau > 0.9 * 1000
+    assert tau_int(stuck[0])[0] < 1.0
+    # chains drifting apart (not mixing) are charged too
+    drift = np.cumsum(rng.normal(size=(6, 4000)), axis=1)
+    assert tau_int_chains(drift)[0] > 400
