@@ -243,9 +243,11 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
                 ++nacc;
             }
         }
-        st4(own_x, e0, make_float4(ox[0], ox[1], ox[2], ox[3]));
-        st4(own_y, e0, make_float4(oy[0], oy[1], oy[2], oy[3]));
-        st4(own_z, e0, make_float4(oz[0], oz[1], oz[2], oz[3]));
+        if (nacc) {   // a quad with nothing accepted is unchanged: its lines stay clean
+            st4(own_x, e0, make_float4(ox[0], ox[1], ox[2], ox[3]));
+            st4(own_y, e0, make_float4(oy[0], oy[1], oy[2], oy[3]));
+            st4(own_z, e0, make_float4(oz[0], oz[1], oz[2], oz[3]));
+        }
         return nacc;
     }
 #pragma unroll
@@ -481,9 +483,11 @@ __device__ __forceinline__ int span_quad(const DevInst& in, const SpanInst& sp, 
             ++nacc;
         }
     }
-    st4(own_x, o, make_float4(ox[0], ox[1], ox[2], ox[3]));
-    st4(own_y, o, make_float4(oy[0], oy[1], oy[2], oy[3]));
-    st4(own_z, o, make_float4(oz[0], oz[1], oz[2], oz[3]));
+    if (nacc) {   // a quad with nothing accepted is unchanged: its lines stay clean
+        st4(own_x, o, make_float4(ox[0], ox[1], ox[2], ox[3]));
+        st4(own_y, o, make_float4(oy[0], oy[1], oy[2], oy[3]));
+        st4(own_z, o, make_float4(oz[0], oz[1], oz[2], oz[3]));
+    }
     return nacc;
 }
 
