@@ -105,8 +105,13 @@ struct Set {
     uint32_t* d_cs_e = nullptr;    // their present sites (element index) ...
     uint16_t* d_cs_nb = nullptr;   // ... and six neighbour float indices per site
     uint2* d_cs_q = nullptr;       // their quads with a present site
-    size_t c_lds_bytes = 0;
-    int c_block = 256;
+    // size classes of the compacted lattices: d_cl[off .. off + n) get
+    // workgroups of `block` work-items and `lds` bytes of dynamic LDS
+    struct CClass {
+        int off, n, block;
+        size_t lds;
+    };
+    std::vector<CClass> cclass;
     size_t lds_bytes = 0;    // dynamic LDS of the largest of them
     int lds_block = 256;
     std::vector<int> kind;   // NMC_KIND_* of each list entry
@@ -584,7 +589,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     // instances are independent, so their tables are filled in parallel
     std::vector<CInst> cl(cinst.size());
     const std::vector<int> crep = geometry_reps(c, cinst);
-    int64_t total_sites = 0, total_quads = 0, cmax_quads = 0;
+    int64_t total_sites = 0, total_quads = 0;
     for (size_t q = 0; q < cinst.size(); ++q) {
         if (crep[q] != (int)q) {   // an identical instance: the same tables
             cl[q] = cl[crep[q]];
@@ -603,7 +608,6 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         cl[q] = CInst{cinst[q], cnt[0], cnt[1], total_sites, nq[0], nq[1], total_quads};
         total_sites += cnt[0] + cnt[1];
         total_quads += nq[0] + nq[1];
-        cmax_quads = std::max<int64_t>(cmax_quads, std::max(nq[0], nq[1]));
     }
     std::vector<uint32_t> cs_e((size_t)total_sites);
     std::vector<uint16_t> cs_nb((size_t)total_sites * 6);
@@ -681,6 +685,33 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         fill(0, nq / nt);
         for (auto& th : pool) th.join();
     }
+    // size classes: the compacted lattices sorted by their larger colour's
+    // quad count into (0, 128], (128, 256], (256, 512], (512, inf), each
+    // class one launch whose workgroup covers its largest lattice's quads
+    // (up to 1024 work-items) and whose LDS fits its largest lattice, so a
+    // small grain does not hold the largest grain's waves and LDS on its CU.
+    // A forced LDS workgroup size keeps one class.
+    {
+        auto cls = [&](const CInst& ci) {
+            if (c->lds_block > 0) return 0;
+            const int m = std::max(ci.nq0, ci.nq1);
+            return m <= 128 ? 0 : m <= 256 ? 1 : m <= 512 ? 2 : 3;
+        };
+        std::stable_sort(cl.begin(), cl.end(), [&](const CInst& a, const CInst& b) { return cls(a) < cls(b); });
+        for (size_t q = 0; q < cl.size();) {
+            const int k = cls(cl[q]);
+            Set::CClass cc{(int)q, 0, 0, 64};
+            int mq = 0;
+            for (; q < cl.size() && cls(cl[q]) == k; ++q) {
+                const CInst& ci = cl[q];
+                mq = std::max(mq, std::max(ci.nq0, ci.nq1));
+                cc.lds = std::max(cc.lds, (size_t)(ci.n0 + ci.n1 + 1) * 3 * sizeof(float));
+                ++cc.n;
+            }
+            cc.block = c->lds_block > 0 ? c->lds_block : (int)std::min(1024, std::max(64, (mq + 63) / 64 * 64));
+            s->cclass.push_back(cc);
+        }
+    }
     int rc;
     if ((rc = upload(c, &s->d_list, list))) return rc;
     if ((rc = upload(c, &s->d_cl, cl))) return rc;
@@ -688,10 +719,6 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     if ((rc = upload(c, &s->d_cs_nb, cs_nb))) return rc;
     if ((rc = upload(c, &s->d_cs_q, cs_q))) return rc;
     s->n_cl = (int)cl.size();
-    for (const CInst& ci : cl)
-        s->c_lds_bytes = std::max(s->c_lds_bytes, std::max<size_t>(64, (size_t)(ci.n0 + ci.n1 + 1) * 3 * sizeof(float)));
-    s->c_block = c->lds_block > 0 ? c->lds_block
-                                  : (int)std::min<int64_t>(1024, std::max<int64_t>(64, (cmax_quads + 63) / 64 * 64));
     if ((rc = upload(c, &s->d_qt, qt))) return rc;
     if ((rc = upload(c, &s->d_qlist, ql))) return rc;
     if ((rc = upload(c, &s->d_sweep, sw))) return rc;
@@ -1156,10 +1183,13 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
     hipError_t e = hipSuccess;
     for (int k = 0; k < nsweeps && s->n_cl > 0 && e == hipSuccess; k += kLdsSweepChunk)
-        e = launch_sweep_lds_c(cstream, ctx->dim3, ctx->dm, ctx->cone, std::min(s->n_cl, 65535), s->c_block,
-                               s->c_lds_bytes, ctx->d_inst, s->d_cl, s->n_cl, (float*)ctx->pool, s->d_cs_e,
-                               s->d_cs_q, s->d_cs_nb, ctx->d_sweeps, ctx->st, std::min(kLdsSweepChunk, nsweeps - k),
-                               (uint32_t)k, slo, shi, ctx->d_acc);
+        for (size_t q = 0; q < s->cclass.size() && e == hipSuccess; ++q) {
+            const Set::CClass& cc = s->cclass[q];
+            e = launch_sweep_lds_c(cstream, ctx->dim3, ctx->dm, ctx->cone, std::min(cc.n, 65535), cc.block, cc.lds,
+                                   ctx->d_inst, s->d_cl + cc.off, cc.n, (float*)ctx->pool, s->d_cs_e, s->d_cs_q,
+                                   s->d_cs_nb, ctx->d_sweeps, ctx->st, std::min(kLdsSweepChunk, nsweeps - k),
+                                   (uint32_t)k, slo, shi, ctx->d_acc);
+        }
     if (side) {
         const hipError_t j = hipEventRecord(ctx->ev_join, ctx->aux);
         if (e == hipSuccess) e = j;
