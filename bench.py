@@ -324,7 +324,7 @@ def ess_leg():
     from nanomagmc_amd.learn import compare_ess
     methods = ("vmf", "swap")
     r = compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=4, anneal_sweeps=3000,
-                    collect_rounds=100, collect_blocks=1000, train_steps=2000, batch=256, eq_samples=10000,
+                    collect_rounds=10000, collect_blocks=10, train_steps=2000, batch=256, eq_samples=10000,
                     nsamples=4000, methods=methods)
     rows = []
     for row in r["temps"]:
@@ -343,8 +343,9 @@ def ess_leg():
                       "%d swap samples of preparation, then %d samples x 4 sweeps; learned: 1 round of 256 16x16 "
                       "blocks per lattice per 4 sweeps (swap: %d relaxation steps, core radius %.1f)"
                       % (r["replicas"], r["eq_samples"], r["nsamples"], r["relax_iters"], r["core_radius"]),
-            "train": {"blocks": r["train_blocks"], "steps": r["train_steps"], "batch": r["batch"],
-                      "seconds": r["train_s"], "swap_loss": r["swap_loss"]},
+            "train": {"blocks": r["train_blocks"], "configurations": r["train_configurations"],
+                      "steps": r["train_steps"], "batch": r["batch"], "seconds": r["train_s"],
+                      "swap_loss": r["swap_loss"]},
             "s_per_sample": r["s_per_sample"], "temps": rows}
 
 

@@ -139,7 +139,7 @@ def neel_bonds(D):
 
 
 def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_sweeps=3000,
-                collect_rounds=100, collect_blocks=1000, train_steps=2000, batch=256, lr=1e-3,
+                collect_rounds=10000, collect_blocks=10, train_steps=2000, batch=256, lr=1e-3,
                 eq_samples=20000, nsamples=4000, sweeps_per_round=4, timing_samples=100, methods=("vmf", "swap"),
                 relax_iters=40, core_radius=5.0, seed=0x4E4D4305, log=None):
     """Plain checkerboard sweeps against sweeps interleaved with learned block
@@ -154,8 +154,9 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
     context:
     1. all instances are annealed from random spins (T = 1 -> T over
        anneal_sweeps plain sweeps);
-    2. the networks are trained on collect_rounds x collect_blocks blocks of
-       the plain instances (the plain sampler's configurations);
+    2. the networks are trained on collect_blocks blocks from each of
+       collect_rounds configurations of the plain instances, 10 plain sweeps
+       apart (the plain sampler's configurations);
     3. every instance is brought towards equilibrium by eq_samples samples of
        the swap sampler, the faster of the two at moving the topological
        charge, so that every method starts stationary (any valid sampler may
@@ -270,6 +271,7 @@ def compare_ess(L=512, temps=(0.1, 0.2, 0.3), h=0.06, D=0.3, replicas=2, anneal_
     blocks_per_round = (L // 32) ** 2
     out = {"L": L, "h": h, "D": D, "replicas": R, "nsamples": nsamples, "eq_samples": eq_samples,
            "sweeps_per_round": sweeps_per_round, "train_blocks": collect_rounds * collect_blocks,
+           "train_configurations": collect_rounds,
            "train_steps": train_steps, "batch": batch, "train_s": train_s, "relax_iters": relax_iters,
            "core_radius": core_radius, "s_per_sample": cost, "swap_loss": swap_loss, "temps": []}
     for t, T in enumerate(temps):

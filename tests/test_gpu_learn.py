@@ -208,7 +208,7 @@ def learned_comparison():
     # the comparison protocol at a reduced size: L = 128, T = 0.3, 8 replicas
     # per method, 20000 swap samples of preparation, 4000 samples of 4 sweeps
     from nanomagmc_amd.learn import compare_ess
-    r = compare_ess(L=128, temps=(0.3,), replicas=8, anneal_sweeps=2000, collect_rounds=20, collect_blocks=500,
+    r = compare_ess(L=128, temps=(0.3,), replicas=8, anneal_sweeps=2000, collect_rounds=1000, collect_blocks=10,
                     train_steps=300, batch=128, eq_samples=20000, nsamples=4000, timing_samples=100, log=print)
     row = r["temps"][0]
     print(r["s_per_sample"], row)
