@@ -54,7 +54,7 @@ VALU_PEAK = 256 * 4 * 2.4e9 / 4
 BYTES = {"grains": 36.25, "heis3d": 36.0, "ising2d": 3.0}
 # kernel families (nmc_sweep_kinds order): kernel-name prefix, bound
 FAMILIES = [("k_sweep_heis<", "hbm"), ("k_sweep_heis_q<", "hbm"), ("k_sweep_span<", "hbm"), ("k_sweep_lds<", "valu"),
-            ("k_sweep_lds_c<", "valu")]
+            ("k_sweep_lds_c<", "valu"), ("k_sweep_cstream<", "hbm")]
 ISING_FAMILY = ("k_sweep_ising<", "hbm")
 LDS_CHUNK = 256   # sweeps per LDS-resident launch
 
@@ -126,8 +126,11 @@ def measure_counters(args, launch, steps=6, warmup=2):
     warm-up and timed dispatches."""
     if shutil.which("rocprofv3") is None:
         return None, "rocprofv3 not found"
+    # --families-apart: the counters are sampled per dispatch over the whole
+    # device, so the kernel families (which share the GPU on two streams in
+    # the timed run) are swept one after the other here
     child = workload_args(args) + ["--steps", str(steps), "--warmup", str(warmup), "--no-ess", "--no-cpu-baseline",
-                                   "--no-counters", "--no-tune", "--no-parts",
+                                   "--no-counters", "--no-tune", "--no-parts", "--families-apart",
                                    "--launch-config", "%s:%d:%d" % tuple(launch)]
     tmp = tempfile.mkdtemp(prefix="nmc_pmc_")
     try:
@@ -159,14 +162,24 @@ def measure_counters(args, launch, steps=6, warmup=2):
         hbm = [1024.0 * (a / ffac + b / wfac) for a, b in zip(f, w)]
         rec = {"dispatches": len(f)}
         if prefix.startswith("k_sweep_lds"):
-            # dispatch 0: the warm-up call, dispatch 1: the timed call
-            lo, hi = min(range(len(hbm)), key=lambda k: hbm[k]), max(range(len(hbm)), key=lambda k: hbm[k])
+            # the warm-up call, then the timed call, each one dispatch per
+            # size class of the family: per-call sums give the per-sweep
+            # slope and the per-call intercept (loading and storing spins)
+            half = len(hbm) // 2
+            if half == 0 or len(hbm) != 2 * half:
+                continue
+            rec["dispatches_per_call"] = half
             ds = steps - warmup
-            rec["hbm_bytes_per_sweep"] = (hbm[hi] - hbm[lo]) / ds
-            rec["hbm_bytes_per_call"] = hbm[lo] - warmup * rec["hbm_bytes_per_sweep"]
+            lo, hi = sum(hbm[:half]), sum(hbm[half:])
+            # the per-sweep bytes are a small difference of two large
+            # per-call figures (loading and storing the spins, and the first
+            # call's cold tables): noise can make it negative
+            rec["hbm_bytes_timed_call"] = hi
+            rec["hbm_bytes_per_sweep"] = max(0.0, (hi - lo) / ds)
             if len(v) == len(f):
-                rec["valu_per_sweep"] = (max(v) - min(v)) / ds
-                rec["valu_per_call"] = min(v) - warmup * rec["valu_per_sweep"]
+                vlo, vhi = sum(v[:half]), sum(v[half:])
+                rec["valu_per_sweep"] = (vhi - vlo) / ds
+                rec["valu_per_call"] = vlo - warmup * rec["valu_per_sweep"]
         else:
             rec["hbm_bytes_per_dispatch"] = float(np.mean(hbm))
             if v:
@@ -385,7 +398,7 @@ def main():
     ap.add_argument("--trial", default=None, choices=["cone", "sphere"])
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--tune-only", action="store_true", help="only run the autotuner and print its choice")
-    ap.add_argument("--path", default=None, choices=["auto", "stream"], help="force a sweep path")
+    ap.add_argument("--path", default=None, choices=["auto", "stream", "cstream"], help="force a sweep path")
     ap.add_argument("--launch-config", default=None, metavar="PATH:LDS:STREAM",
                     help="apply a launch shape (path, LDS workgroup size, streaming tile work-items)")
     ap.add_argument("--replicas", type=int, default=1,
@@ -396,6 +409,9 @@ def main():
     ap.add_argument("--no-ess", action="store_true", help="skip the effective-samples legs")
     ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter runs")
     ap.add_argument("--no-parts", action="store_true", help="skip timing each kernel family alone")
+    ap.add_argument("--families-apart", action="store_true",
+                    help="only sweep each kernel family alone (warm-up, then the steps), one family after the "
+                         "other: the workload of the counter runs")
     ap.add_argument("--learn-profile", action="store_true",
                     help="only run the learned-proposal workload profiled in profiles/learned_kernels_stats.md")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -457,6 +473,14 @@ def main():
         print(json.dumps({"choice": list(choice), "timings": timings}), flush=True)
         return
     autotune.apply(lat, launch)
+    if args.families_apart:
+        kinds = lat.sweep_kinds()
+        for k in sorted(set(kinds.tolist())):
+            ids = np.nonzero(kinds == k)[0]
+            lat.equil(args.warmup, inst=ids)
+            lat.equil(args.steps, inst=ids)
+            lat.sync()
+        return
     # warm-up; cone half-angles adapt toward 50% acceptance every 10 sweeps
     # (on the device), then freeze
     if trial == "cone":
@@ -488,6 +512,7 @@ def main():
             part["seconds"] = gpu_s
         parts.append(part)
     kc = (counters or {}).get("kernels", {})
+    kc_steps = (counters or {}).get("steps")
     for part in parts:
         if "seconds" not in part:
             continue
@@ -497,12 +522,17 @@ def main():
         part["alg_GBps"] = alg / per_sweep / 1e9
         c = kc.get(part["kernel"], {})
         if part["family"] in ("lds", "lds_compact"):
+            # a launch here is one chunk of up to LDS_CHUNK sweeps: one
+            # dispatch per size class, back to back on one stream
             launches = math.ceil(args.steps / LDS_CHUNK)
             part["launches"] = launches
+            if "dispatches_per_call" in c:
+                part["dispatches_per_launch"] = c["dispatches_per_call"]
             part["avg_launch_us"] = part["seconds"] / launches * 1e6
             sweeps_per_launch = args.steps / launches
             if "hbm_bytes_per_sweep" in c:
-                part["traffic"] = c["hbm_bytes_per_call"] + c["hbm_bytes_per_sweep"] * sweeps_per_launch
+                steps_c = kc_steps if kc_steps else sweeps_per_launch
+                part["traffic"] = c["hbm_bytes_timed_call"] + c["hbm_bytes_per_sweep"] * (sweeps_per_launch - steps_c)
             if "valu_per_sweep" in c:
                 part["valu_per_launch"] = c["valu_per_call"] + c["valu_per_sweep"] * sweeps_per_launch
                 part["valu_Ginst_per_s"] = part["valu_per_launch"] / (part["avg_launch_us"] * 1e-6) / 1e9
@@ -513,10 +543,17 @@ def main():
             part["launches"] = 2 * args.steps
             part["avg_launch_us"] = part["seconds"] / (2 * args.steps) * 1e6
             part["alg_bytes_per_launch"] = alg / 2
+            part["hbm_frac"] = part["alg_bytes_per_launch"] / (part["avg_launch_us"] * 1e-6) / HBM_PEAK
             if "hbm_bytes_per_dispatch" in c:
                 part["traffic"] = c["hbm_bytes_per_dispatch"]
             if "valu_per_dispatch" in c:
                 part["valu_per_launch"] = c["valu_per_dispatch"]
+                part["valu_Ginst_per_s"] = part["valu_per_launch"] / (part["avg_launch_us"] * 1e-6) / 1e9
+                part["valu_frac"] = part["valu_Ginst_per_s"] * 1e9 / VALU_PEAK
+            # a streaming pass is bound by whichever of HBM and VALU issue it
+            # uses the larger share of
+            if part.get("valu_frac", 0.0) > part["hbm_frac"]:
+                part["bound"] = "valu"
     timed = [p for p in parts if "seconds" in p]
     dom = max(timed, key=lambda p: p["seconds"]) if timed else None
     roof = {"step": {"alg_bytes": step_alg, "achieved": step_alg / (gpu_s / args.steps) / 1e9, "peak": HBM_PEAK / 1e9,
@@ -527,13 +564,18 @@ def main():
     if dom is not None:
         if dom["bound"] == "valu":
             ach = dom.get("valu_Ginst_per_s")
+            if dom["family"] in ("lds", "lds_compact"):
+                what = ("%s: %d instances, %d sites, %d sweeps per launch (%s dispatches, one per size class, back "
+                        "to back)" % (dom["kernel"], dom["instances"], dom["sites"],
+                                      round(args.steps / dom["launches"]), dom.get("dispatches_per_launch", "?")))
+            else:
+                what = ("%s: one colour pass of %d instances (%d updates)"
+                        % (dom["kernel"], dom["instances"], dom["sites"] // 2))
             roof.update({"bound": "valu", "achieved": ach, "peak": VALU_PEAK / 1e9, "unit": "Ginst/s",
                          "frac": None if ach is None else ach * 1e9 / VALU_PEAK,
                          "traffic": dom.get("traffic"),
-                         "kernel": "%s: %d instances, %d sites, %d sweeps per launch; achieved = wave64 VALU "
-                                   "instructions per launch (counters) / launch time (events)"
-                                   % (dom["kernel"], dom["instances"], dom["sites"],
-                                      round(args.steps / dom["launches"])),
+                         "kernel": what + "; achieved = wave64 VALU instructions per launch (counters, the family "
+                                          "swept alone) / launch time (events, the family alone)",
                          "avg_launch_us": dom["avg_launch_us"]})
         else:
             ach = dom["alg_bytes_per_launch"] / (dom["avg_launch_us"] * 1e-6)

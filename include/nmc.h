@@ -155,12 +155,17 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps);
  * per listed instance.  Heisenberg only (NMC_E_INVALID for Ising). */
 int nmc_sweep_adapt(nmc_ctx* ctx, const int* inst, int n, int nsweeps, int chunk, float* half_angle);
 /* Kernel path for nmc_sweep.  AUTO keeps every instance whose two colour
- * arrays fit in one workgroup's LDS resident there for whole sweeps (one
- * persistent workgroup per instance) and streams the others from HBM with
- * one launch per colour pass; STREAM forces streaming for all.  Both paths
- * draw the same random numbers, so they give the same trajectories. */
+ * arrays (open shaped lattices: whose present sites) fit in one workgroup's
+ * LDS resident there for whole sweeps (one persistent workgroup per
+ * instance) and streams the others from HBM with one launch per colour pass
+ * (open shaped lattices from compacted copies of their present sites);
+ * STREAM forces streaming for all (open shaped lattices from packed row
+ * spans); COMPACT_STREAM streams open shaped lattices from compacted copies
+ * and the others as STREAM.  All paths draw the same random numbers, so
+ * they give the same trajectories. */
 #define NMC_PATH_AUTO 0
 #define NMC_PATH_STREAM 1
+#define NMC_PATH_COMPACT_STREAM 2
 int nmc_set_path(nmc_ctx* ctx, int path);
 /* Work-items per LDS-resident workgroup (64..1024, multiple of 64); 0 picks
  * one from the largest instance.  Set by the autotuner; results do not
@@ -181,6 +186,8 @@ int nmc_set_stream_block(nmc_ctx* ctx, int block);
 #define NMC_KIND_SPANS 2         /* streaming passes over a packed copy of the occupied row spans (open shaped) */
 #define NMC_KIND_LDS 3           /* whole sweeps with the box resident in one workgroup's LDS */
 #define NMC_KIND_LDS_COMPACT 4   /* whole sweeps with the present sites compacted in LDS (open shaped) */
+#define NMC_KIND_COMPACT_STREAM 5 /* streaming passes over a compacted copy of the present sites (open shaped,
+                                     too large for LDS) */
 int nmc_sweep_kinds(nmc_ctx* ctx, const int* inst, int n, int* kind);
 
 /* Accepted-attempt counters since the last reset (one per instance). */

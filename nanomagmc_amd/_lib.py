@@ -18,8 +18,8 @@ NMC_TRIAL_SPHERE = 0
 NMC_TRIAL_CONE = 1
 NMC_NOBS = 11
 # sweep kernel families (nmc_sweep_kinds)
-NMC_KIND_STREAM, NMC_KIND_QUADS, NMC_KIND_SPANS, NMC_KIND_LDS, NMC_KIND_LDS_COMPACT = range(5)
-KIND_NAMES = ["stream", "quads", "spans", "lds", "lds_compact"]
+NMC_KIND_STREAM, NMC_KIND_QUADS, NMC_KIND_SPANS, NMC_KIND_LDS, NMC_KIND_LDS_COMPACT, NMC_KIND_COMPACT_STREAM = range(6)
+KIND_NAMES = ["stream", "quads", "spans", "lds", "lds_compact", "compact_stream"]
 
 OBS_NAMES = ["Mx", "My", "Mz", "subMx", "subMy", "subMz", "sub4Mx", "sub4My", "sub4Mz", "E_r", "E_s"]
 

@@ -2,8 +2,8 @@
 
 The engine sweeps an instance either LDS-resident (whole sweeps in one
 workgroup: the dense box or the compacted present sites) or by streaming
-colour passes, and three launch shapes are free: the path (AUTO, or
-streaming only), the workgroup size of the LDS-resident kernels and the tile
+colour passes, and three launch shapes are free: the path (AUTO, streaming
+only, or open shaped lattices streamed from compacted copies), the workgroup size of the LDS-resident kernels and the tile
 size of the streaming passes over colour-split boxes (64-work-item quarter
 tiles or 256-work-item full tiles).  Results do not depend on any of them
 (every path draws the same random numbers per site), so the tuner is free to
@@ -18,7 +18,7 @@ import time
 
 # (path, LDS workgroup size, streaming tile work-items); 0 = the engine's choice
 CANDIDATES = [("auto", 0, 0), ("auto", 256, 0), ("auto", 512, 0), ("auto", 1024, 0), ("auto", 0, 64),
-              ("auto", 0, 256), ("stream", 0, 0), ("stream", 0, 64), ("stream", 0, 256)]
+              ("auto", 0, 256), ("stream", 0, 0), ("stream", 0, 64), ("stream", 0, 256), ("cstream", 0, 0)]
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "nanomagmc_amd", "autotune.json")
 
 

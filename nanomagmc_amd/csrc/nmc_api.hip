@@ -104,7 +104,14 @@ struct Set {
     int n_cl = 0;
     uint32_t* d_cs_e = nullptr;    // their present sites (element index) ...
     uint16_t* d_cs_nb = nullptr;   // ... and six neighbour float indices per site
-    uint2* d_cs_q = nullptr;       // their quads with a present site
+    uint32_t* d_cs_p = nullptr;    // their pairs of colour elements with a present site
+    CInst* d_xl = nullptr;         // open shaped lattices streamed from compacted copies (tables as above) ...
+    int n_xl = 0;
+    XTile* d_xt = nullptr;         // ... pass tiles, colour 0 then colour 1 ...
+    int n_xt[2] = {0, 0};
+    XTile* d_xct = nullptr;        // ... copy tiles ...
+    int n_xct = 0;
+    float* d_cspin = nullptr;      // ... and the compacted copies (zeroed when created)
     // size classes of the compacted lattices: d_cl[off .. off + n) get
     // workgroups of `block` work-items and `lds` bytes of dynamic LDS
     struct CClass {
@@ -116,21 +123,28 @@ struct Set {
     int lds_block = 256;
     std::vector<int> kind;   // NMC_KIND_* of each list entry
 
+    // any instance swept by streaming passes (launches per colour pass)
+    bool streamed() const {
+        return n_sweep > 0 || n_qt[0] + n_qt[1] + n_st[0] + n_st[1] + n_xt[0] + n_xt[1] > 0;
+    }
+
     void release() {
         (void)hipFree(d_list); (void)hipFree(d_sweep); (void)hipFree(d_init);
         (void)hipFree(d_mt); (void)hipFree(d_tb); (void)hipFree(d_te); (void)hipFree(d_lds);
         (void)hipFree(d_msites); (void)hipFree(d_mrank); (void)hipFree(d_mzoff);
         d_msites = nullptr; d_mrank = nullptr; d_mzoff = nullptr;
         (void)hipFree(d_tacc); (void)hipFree(d_qt); (void)hipFree(d_qlist);
-        (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb); (void)hipFree(d_cs_q);
+        (void)hipFree(d_cl); (void)hipFree(d_cs_e); (void)hipFree(d_cs_nb); (void)hipFree(d_cs_p);
         (void)hipFree(d_atp); (void)hipFree(d_ati); (void)hipFree(d_ans);
         (void)hipFree(d_si); (void)hipFree(d_stl); (void)hipFree(d_slist); (void)hipFree(d_rowtab);
         (void)hipFree(d_span);
+        (void)hipFree(d_xl); (void)hipFree(d_xt); (void)hipFree(d_xct); (void)hipFree(d_cspin);
+        d_xl = nullptr; d_xt = nullptr; d_xct = nullptr; d_cspin = nullptr;
         d_si = nullptr; d_stl = nullptr; d_slist = nullptr; d_rowtab = nullptr; d_span = nullptr;
         d_atp = nullptr; d_ati = nullptr; d_ans = nullptr;
         adapt_ready = false;
         d_tacc = nullptr; d_qt = nullptr; d_qlist = nullptr;
-        d_cl = nullptr; d_cs_e = nullptr; d_cs_nb = nullptr; d_cs_q = nullptr;
+        d_cl = nullptr; d_cs_e = nullptr; d_cs_nb = nullptr; d_cs_p = nullptr;
         d_list = nullptr; d_sweep = nullptr; d_init = nullptr; d_mt = nullptr;
         d_tb = d_te = d_lds = nullptr;
     }
@@ -386,17 +400,17 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     const bool quarter = c->stream_block ? c->stream_block == 64 : s->small;
     const int stile = quarter ? full / 4 : full;
     s->sweep_block = quarter ? 64 : 256;
-    std::vector<int> lds, qinst, cinst, sinst;
+    std::vector<int> lds, qinst, cinst, sinst, xinst;
     std::vector<int64_t> mbase;   // shaped lattices: first measured site of each instance in the list
     int64_t msite_total = 0;
     int64_t lds_work = 0;
     const bool quad_lists = c->masked && !c->ising;
-    const bool compact = quad_lists && !c->periodic && c->path == NMC_PATH_AUTO;
+    const bool compact = quad_lists && !c->periodic && c->path != NMC_PATH_STREAM;
     s->kind.assign(list.size(), NMC_KIND_STREAM);
     for (size_t li = 0; li < list.size(); ++li) {
         const int i = list[li];
         const DevInst& d = c->din[i];
-        bool fits_compact = false;
+        bool fits_compact = false, fits_xstream = false;
         if (compact) {
             // present sites (12 B each) + the zero slot, compacted in LDS if
             // they fit with kStreamLdsReserve to spare, so that a streaming
@@ -404,7 +418,11 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             int64_t np = 0;
             const uint32_t* mw = c->h_mask.data() + d.mask_base;
             for (int64_t w = 0; w < 2 * d.mask_stride; ++w) np += __builtin_popcount(mw[w]);
-            fits_compact = (size_t)(np + 1) * 12 + kStreamLdsReserve <= c->max_lds_c && 3 * (np + 1) <= 65535;
+            fits_compact = c->path == NMC_PATH_AUTO &&
+                           (size_t)(np + 1) * 12 + kStreamLdsReserve <= c->max_lds_c && 3 * (np + 1) <= 65535;
+            // too large for LDS: a compacted copy in HBM, neighbours as
+            // 16-bit slots (the zero slot np included)
+            fits_xstream = np + 1 <= 65536;
         }
         // both colour arrays, then (masked lattices) both colours' presence bits
         const size_t bytes = (size_t)(c->ising ? 2 * d.Sp : 6 * d.Sp * (int64_t)sizeof(float)) +
@@ -417,6 +435,9 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             s->kind[li] = NMC_KIND_LDS;
             s->lds_bytes = std::max(s->lds_bytes, bytes);
             lds_work = std::max<int64_t>(lds_work, c->ising ? (d.S + 1) / 2 : (d.S + 3) / 4);
+        } else if (fits_xstream) {
+            xinst.push_back(i);
+            s->kind[li] = NMC_KIND_COMPACT_STREAM;
         } else if (quad_lists) {
             // open boundaries: the packed span copy (rows hold whole quads,
             // a2 being padded to a multiple of 8); periodic: the quad list
@@ -586,32 +607,45 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         }
     }
     // compacted shaped lattices: present sites per colour, neighbour indices;
-    // instances are independent, so their tables are filled in parallel
-    std::vector<CInst> cl(cinst.size());
-    const std::vector<int> crep = geometry_reps(c, cinst);
-    int64_t total_sites = 0, total_quads = 0;
-    for (size_t q = 0; q < cinst.size(); ++q) {
+    // instances are independent, so their tables are filled in parallel.
+    // The LDS-resident ones (cinst) come first, then the streamed ones
+    // (xinst), which also get a compacted spin copy each.
+    const size_t ncl = cinst.size();
+    std::vector<int> call(cinst);
+    call.insert(call.end(), xinst.begin(), xinst.end());
+    std::vector<CInst> cl(call.size());
+    const std::vector<int> crep = geometry_reps(c, call);
+    int64_t total_sites = 0, total_pairs = 0, total_cspin = 0;
+    for (size_t q = 0; q < call.size(); ++q) {
         if (crep[q] != (int)q) {   // an identical instance: the same tables
             cl[q] = cl[crep[q]];
-            cl[q].inst = cinst[q];
-            continue;
+            cl[q].inst = call[q];
+        } else {
+            const DevInst& d = c->din[call[q]];
+            const uint32_t* mw = c->h_mask.data() + d.mask_base;
+            int cnt[2] = {0, 0}, nd[2] = {0, 0}, ns[2] = {0, 0};
+            for (int col = 0; col < 2; ++col)
+                for (int64_t w = 0; w < d.mask_stride; ++w) {
+                    const uint32_t b = mw[col * d.mask_stride + w];
+                    cnt[col] += __builtin_popcount(b);
+                    for (int k = 0; k < 32; k += 2) {
+                        const uint32_t pr = (b >> k) & 3u;
+                        nd[col] += pr == 3u;
+                        ns[col] += pr == 1u || pr == 2u;
+                    }
+                }
+            cl[q] = CInst{call[q], cnt[0], cnt[1], total_sites, nd[0], ns[0], nd[1], ns[1], total_pairs, 0};
+            total_sites += cnt[0] + cnt[1];
+            total_pairs += nd[0] + ns[0] + nd[1] + ns[1];
         }
-        const DevInst& d = c->din[cinst[q]];
-        const uint32_t* mw = c->h_mask.data() + d.mask_base;
-        int cnt[2] = {0, 0}, nq[2] = {0, 0};
-        for (int col = 0; col < 2; ++col)
-            for (int64_t w = 0; w < d.mask_stride; ++w) {
-                const uint32_t b = mw[col * d.mask_stride + w];
-                cnt[col] += __builtin_popcount(b);
-                for (int k = 0; k < 32; k += 4) nq[col] += ((b >> k) & 0xFu) != 0;
-            }
-        cl[q] = CInst{cinst[q], cnt[0], cnt[1], total_sites, nq[0], nq[1], total_quads};
-        total_sites += cnt[0] + cnt[1];
-        total_quads += nq[0] + nq[1];
+        if (q >= ncl) {   // x, y, z per slot and the zero slot, 16-byte aligned
+            cl[q].spin0 = total_cspin;
+            total_cspin += (3 * (int64_t)(cl[q].n0 + cl[q].n1 + 1) + 3) / 4 * 4;
+        }
     }
     std::vector<uint32_t> cs_e((size_t)total_sites);
     std::vector<uint16_t> cs_nb((size_t)total_sites * 6);
-    std::vector<uint2> cs_q((size_t)total_quads);
+    std::vector<uint32_t> cs_p((size_t)total_pairs);
     auto fill = [&](size_t qb, size_t qe) {
         std::vector<int> slot[2];
         for (size_t q = qb; q < qe; ++q) {
@@ -621,34 +655,32 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             const uint32_t* mw = c->h_mask.data() + d.mask_base;
             uint32_t* E = cs_e.data() + ci.site0;
             uint16_t* NB = cs_nb.data() + 6 * ci.site0;
-            uint2* QD = cs_q.data() + ci.quad0;
-            int next = 0;
+            uint32_t* PD = cs_p.data() + ci.pair0;
             for (int col = 0; col < 2; ++col) {
+                // this colour's pairs (2k, 2k + 1): doubles then singles, each
+                // in element order; slots by work-item (S is even, so a pair
+                // never straddles a mask word)
                 slot[col].assign((size_t)d.S, -1);
-                for (int64_t e = 0; e < d.S; ++e)
-                    if ((mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 1u) {
-                        slot[col][e] = next;
-                        E[next++] = (uint32_t)e;
+                const int base = col ? ci.n0 : 0, nd = col ? ci.nd1 : ci.nd0;
+                int td = 0, ts = nd;
+                for (int64_t e = 0; e < d.S; e += 2) {
+                    const uint32_t pr = (mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 3u;
+                    if (pr == 3u) {
+                        slot[col][e] = base + td;
+                        slot[col][e + 1] = base + nd + td;
+                        PD[td++] = (uint32_t)(e >> 1);
+                    } else if (pr) {
+                        const int64_t ep = pr == 1u ? e : e + 1;
+                        slot[col][ep] = base + nd + ts;
+                        PD[ts++] = (uint32_t)(e >> 1) | (pr == 2u ? 0x80000000u : 0u);
                     }
-                // this colour's quads, counting-sorted by presence mask
-                // (S is a multiple of 4, so a quad never straddles a mask word)
-                int start[16] = {0};
-                auto nib = [&](int64_t e) { return (mw[col * d.mask_stride + (e >> 5)] >> (e & 31)) & 0xFu; };
-                for (int64_t e = 0; e < d.S; e += 4) ++start[nib(e)];
-                for (int m = 1, run = 0; m < 16; ++m) {
-                    const int t = start[m];
-                    start[m] = run;
-                    run += t;
                 }
-                for (int64_t e = 0; e < d.S; e += 4) {
-                    const uint32_t m = nib(e);
-                    if (!m) continue;
-                    const int first = slot[col][e + __builtin_ctz(m)];
-                    QD[start[m]++] = make_uint2((uint32_t)(e >> 2), 3u * (uint32_t)first | (m << 16));
-                }
-                QD += col ? ci.nq1 : ci.nq0;
+                for (int64_t e = 0; e < d.S; ++e)
+                    if (slot[col][e] >= 0) E[slot[col][e]] = (uint32_t)e;
+                PD += col ? ci.nd1 + ci.ns1 : ci.nd0 + ci.ns0;
             }
             const int zero = ci.n0 + ci.n1;
+            const int nbmul = q < ncl ? 3 : 1;   // LDS: float indices; streamed: slots
             for (int j = 0; j < zero; ++j) {
                 const int col = j < ci.n0 ? 0 : 1;
                 const int64_t e = E[j];
@@ -671,7 +703,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                         const int t = slot[col ^ 1][((int64_t)n0 * d.a1 + n1) * d.half + kk];
                         if (t >= 0) sv = t;
                     }
-                    NB[6 * j + g] = (uint16_t)(3 * sv);
+                    NB[6 * j + g] = (uint16_t)(nbmul * sv);
                 }
             }
         }
@@ -685,16 +717,37 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         fill(0, nq / nt);
         for (auto& th : pool) th.join();
     }
+    // the streamed ones: per colour, tiles of 256 pairs; copy tiles of 256
+    // slots
+    std::vector<CInst> xl(cl.begin() + (int64_t)ncl, cl.end());
+    cl.resize(ncl);
+    std::vector<XTile> xt, xct;
+    for (int col = 0; col < 2; ++col) {
+        for (size_t x = 0; x < xl.size(); ++x) {
+            const int np = col ? xl[x].nd1 + xl[x].ns1 : xl[x].nd0 + xl[x].ns0;
+            for (int t = 0; t < np; t += 256) xt.push_back(XTile{(int)x, t, std::min(256, np - t)});
+        }
+        s->n_xt[col] = (int)xt.size() - (col ? s->n_xt[0] : 0);
+    }
+    for (size_t x = 0; x < xl.size(); ++x) {
+        const int N = xl[x].n0 + xl[x].n1;
+        for (int t = 0; t < N; t += 256) xct.push_back(XTile{(int)x, t, std::min(256, N - t)});
+    }
+    if (total_cspin > 0) {
+        int rc = dalloc(&s->d_cspin, (size_t)total_cspin);
+        if (rc) return rc;
+        NMC_HIP(hipMemsetAsync(s->d_cspin, 0, (size_t)total_cspin * sizeof(float), c->stream));
+    }
     // size classes: the compacted lattices sorted by their larger colour's
-    // quad count into (0, 128], (128, 256], (256, 512], (512, inf), each
-    // class one launch whose workgroup covers its largest lattice's quads
+    // pair count into (0, 128], (128, 256], (256, 512], (512, inf), each
+    // class one launch whose workgroup covers its largest lattice's pairs
     // (up to 1024 work-items) and whose LDS fits its largest lattice, so a
     // small grain does not hold the largest grain's waves and LDS on its CU.
     // A forced LDS workgroup size keeps one class.
     {
         auto cls = [&](const CInst& ci) {
             if (c->lds_block > 0) return 0;
-            const int m = std::max(ci.nq0, ci.nq1);
+            const int m = std::max(ci.nd0 + ci.ns0, ci.nd1 + ci.ns1);
             return m <= 128 ? 0 : m <= 256 ? 1 : m <= 512 ? 2 : 3;
         };
         std::stable_sort(cl.begin(), cl.end(), [&](const CInst& a, const CInst& b) { return cls(a) < cls(b); });
@@ -704,7 +757,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             int mq = 0;
             for (; q < cl.size() && cls(cl[q]) == k; ++q) {
                 const CInst& ci = cl[q];
-                mq = std::max(mq, std::max(ci.nq0, ci.nq1));
+                mq = std::max(mq, std::max(ci.nd0 + ci.ns0, ci.nd1 + ci.ns1));
                 cc.lds = std::max(cc.lds, (size_t)(ci.n0 + ci.n1 + 1) * 3 * sizeof(float));
                 ++cc.n;
             }
@@ -717,13 +770,19 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     if ((rc = upload(c, &s->d_cl, cl))) return rc;
     if ((rc = upload(c, &s->d_cs_e, cs_e))) return rc;
     if ((rc = upload(c, &s->d_cs_nb, cs_nb))) return rc;
-    if ((rc = upload(c, &s->d_cs_q, cs_q))) return rc;
+    if ((rc = upload(c, &s->d_cs_p, cs_p))) return rc;
     s->n_cl = (int)cl.size();
     if ((rc = upload(c, &s->d_qt, qt))) return rc;
     if ((rc = upload(c, &s->d_qlist, ql))) return rc;
     if ((rc = upload(c, &s->d_sweep, sw))) return rc;
     for (const QTile& q : qt) sw.push_back(Tile{q.inst, q.q0});
     for (const STile& t : stl) sw.push_back(Tile{t.inst, t.q0});
+    for (const XTile& t : xt) sw.push_back(Tile{xl[t.x].inst, t.item0});
+    if ((rc = upload(c, &s->d_xl, xl))) return rc;
+    if ((rc = upload(c, &s->d_xt, xt))) return rc;
+    if ((rc = upload(c, &s->d_xct, xct))) return rc;
+    s->n_xl = (int)xl.size();
+    s->n_xct = (int)xct.size();
     if ((rc = upload(c, &s->d_si, si))) return rc;
     if ((rc = upload(c, &s->d_stl, stl))) return rc;
     if ((rc = upload(c, &s->d_slist, sl))) return rc;
@@ -1186,7 +1245,7 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
         for (size_t q = 0; q < s->cclass.size() && e == hipSuccess; ++q) {
             const Set::CClass& cc = s->cclass[q];
             e = launch_sweep_lds_c(cstream, ctx->dim3, ctx->dm, ctx->cone, std::min(cc.n, 65535), cc.block, cc.lds,
-                                   ctx->d_inst, s->d_cl + cc.off, cc.n, (float*)ctx->pool, s->d_cs_e, s->d_cs_q,
+                                   ctx->d_inst, s->d_cl + cc.off, cc.n, (float*)ctx->pool, s->d_cs_e, s->d_cs_p,
                                    s->d_cs_nb, ctx->d_sweeps, ctx->st, std::min(kLdsSweepChunk, nsweeps - k),
                                    (uint32_t)k, slo, shi, ctx->d_acc);
         }
@@ -1204,10 +1263,23 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
     for (int col = 0; col < 2 && nst > 0 && nsweeps > 0 && e == hipSuccess; ++col)
         e = launch_span_copy(ctx->stream, true, ctx->d_inst, s->d_si, s->d_stl + (col ? s->n_st[0] : 0), s->n_st[col],
                              s->d_slist, (float*)ctx->pool, s->d_span, col);
-    const bool streaming = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] > 0 || nst > 0;
+    // compacted copies likewise
+    if (s->n_xct > 0 && nsweeps > 0 && e == hipSuccess)
+        e = launch_cstream_copy(ctx->stream, true, ctx->d_inst, s->d_xl, s->d_xct, s->n_xct, s->d_cs_e,
+                                (float*)ctx->pool, s->d_cspin);
+    const bool streaming = s->streamed();
     const int st_acc = s->n_sweep + s->n_qt[0] + s->n_qt[1];   // first span tile's accept counter
+    const int xt_acc = st_acc + nst;                           // first compacted pass tile's
     for (int k = 0; k < nsweeps && streaming && e == hipSuccess; ++k)
         for (int col = 0; col < 2 && e == hipSuccess; ++col) {
+            if (s->n_xt[col] > 0) {
+                const int off = col ? s->n_xt[0] : 0;
+                e = launch_sweep_cstream(ctx->stream, ctx->dim3, ctx->dm, ctx->cone, true, ctx->d_inst, s->d_xl,
+                                         s->d_xt + off, s->n_xt[col], s->d_cs_p, s->d_cs_nb, s->d_cspin,
+                                         ctx->d_sweeps, ctx->st, col, (uint32_t)k, slo, shi,
+                                         s->d_tacc + xt_acc + off);
+                if (e != hipSuccess) continue;
+            }
             if (s->n_st[col] > 0) {
                 const int off = col ? s->n_st[0] : 0;
                 e = launch_sweep_span(ctx->stream, ctx->dim3, ctx->dm, ctx->cone, true, ctx->d_inst, s->d_si,
@@ -1236,11 +1308,14 @@ hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream,
     for (int col = 0; col < 2 && nst > 0 && nsweeps > 0 && e == hipSuccess; ++col)
         e = launch_span_copy(ctx->stream, false, ctx->d_inst, s->d_si, s->d_stl + (col ? s->n_st[0] : 0),
                              s->n_st[col], s->d_slist, (float*)ctx->pool, s->d_span, col);
+    if (s->n_xct > 0 && nsweeps > 0 && e == hipSuccess)
+        e = launch_cstream_copy(ctx->stream, false, ctx->d_inst, s->d_xl, s->d_xct, s->n_xct, s->d_cs_e,
+                                (float*)ctx->pool, s->d_cspin);
     return e;
 }
 
 int sweep_launches(nmc_ctx* ctx, Set* s, int nsweeps) {
-    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1] > 0;
+    const bool streamed = s->streamed();
     // also under capture: the auxiliary stream joins the capture through the
     // fork event and is joined back before the capture ends, so the graph
     // holds the compacted and streamed sweeps as parallel branches
@@ -1272,7 +1347,7 @@ int nmc_sweep(nmc_ctx* ctx, const int* inst, int n, int nsweeps) {
     // relative and its last node advances the device sweep counters, so
     // replays continue the same random streams.  Sets swept only in LDS
     // launch once per 256 sweeps anyway and skip the capture (~3 ms).
-    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1] > 0;
+    const bool streamed = s->streamed();
     if (ctx->graphs_on && s->small && streamed && nsweeps >= 2 * kGraphSweeps) {
         auto key = std::make_tuple(s->list, -1, kGraphSweeps);
         auto it = ctx->graphs.find(key);
@@ -1336,7 +1411,7 @@ int nmc_sweep_adapt(nmc_ctx* ctx, const int* inst, int n, int nsweeps, int chunk
     if (int rc = nmc_reset_accepts(ctx)) return rc;
     // as nmc_sweep: small streamed sets replay one captured graph of about
     // kGraphSweeps sweeps (gc whole chunks, each with its adaptation step)
-    const bool streamed = s->n_sweep > 0 || s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1] > 0;
+    const bool streamed = s->streamed();
     const int gc = std::max(1, kGraphSweeps / chunk);
     const int reps = nsweeps / (gc * chunk);
     int done = 0;
@@ -1383,7 +1458,8 @@ int nmc_sweep_adapt(nmc_ctx* ctx, const int* inst, int n, int nsweeps, int chunk
 
 int nmc_set_path(nmc_ctx* ctx, int path) {
     if (!ctx) return fail(NMC_E_INVALID, "null context");
-    if (path != NMC_PATH_AUTO && path != NMC_PATH_STREAM) return fail(NMC_E_INVALID, "unknown sweep path");
+    if (path != NMC_PATH_AUTO && path != NMC_PATH_STREAM && path != NMC_PATH_COMPACT_STREAM)
+        return fail(NMC_E_INVALID, "unknown sweep path");
     if (path == ctx->path) return NMC_OK;
     if (int rc = release_sets(ctx)) return rc;
     ctx->path = path;

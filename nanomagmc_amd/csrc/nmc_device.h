@@ -137,20 +137,32 @@ struct STile {
     int inst, span, q0, qn;
 };
 
-// Shaped instance in compacted form for LDS-resident sweeps: its present
-// sites, colour 0 first (n0 of them) then colour 1 (n1), stored from site0 on
-// in the per-set site tables (element index; six neighbours, each the float
-// index 3 x slot of a site of the other colour or of n0 + n1 = the zero
-// slot), and its quads with a present site, colour 0 first, from quad0 on in
-// the per-set quad table: {quad index e >> 2, 3 x first slot | 4-bit
-// presence mask << 16}, grouped by presence mask so that a wavefront mostly
-// sees one pattern.
+// Shaped instance in compacted form, for LDS-resident sweeps and for
+// streamed sweeps of a compacted copy in HBM: its present sites, colour 0
+// first (n0 of them) then colour 1 (n1), stored from site0 on in the per-set
+// site tables (element index; six neighbours, each a site of the other
+// colour or n0 + n1 = the zero slot: its float index 3 x slot for the LDS
+// kernel, its slot for the streamed one), and its pairs of colour elements
+// (2k, 2k + 1) with a present site,
+// colour 0 first, from pair0 on in the per-set pair table: per colour the
+// nd pairs with both elements present, then the ns with one, each entry
+// k | (the single's present element is 2k + 1) << 31.  Within a colour the
+// t-th double's elements have slots t and nd + t, the t-th single's element
+// (t counted from nd) nd + t, relative to the colour's first slot.
 struct CInst {
     int inst;
     int n0, n1;       // present sites of colour 0 and 1 (slots 0..n0-1, n0..n0+n1-1)
     int64_t site0;    // first slot in the set's site tables
-    int nq0, nq1;     // quads with a present site, per colour
-    int64_t quad0;    // first quad in the set's quad table
+    int nd0, ns0;     // colour 0: pairs with two present elements, with one
+    int nd1, ns1;     // colour 1: the same
+    int64_t pair0;    // first pair in the set's pair table
+    int64_t spin0;    // streamed: first float of its compacted spin copy (x, y, z per slot, then a zero slot)
+};
+
+// Tile of a streamed compacted pass or copy: items [item0, item0 + n) of
+// compacted instance x (pairs of one colour for a pass, slots for a copy).
+struct XTile {
+    int x, item0, n;
 };
 
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,

@@ -51,11 +51,25 @@ hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool peri
 // k_sweep_lds_c.
 hipError_t set_lds_limits(int shm, int* lds_lim, int* ldsc_lim);
 
+// One colour pass of the streamed compacted instances (see CInst, XTile):
+// ntiles tiles of up to 256 pairs of this colour; count = add each tile's
+// accepted attempts to acc[tile].
+hipError_t launch_sweep_cstream(hipStream_t stream, bool dim3, int dm, bool cone, bool count,
+                                const DevInst* insts, const CInst* xl, const XTile* tiles, int ntiles,
+                                const uint32_t* cs_p, const uint16_t* cs_nb, float* cspin, const uint64_t* sweeps,
+                                const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                                unsigned long long* acc);
+
+// Copies the tiled slots of streamed compacted instances between the
+// colour-split arrays and their compacted copies (to_copy: into the copies).
+hipError_t launch_cstream_copy(hipStream_t stream, bool to_copy, const DevInst* insts, const CInst* xl,
+                               const XTile* tiles, int ntiles, const uint32_t* cs_e, float* pool, float* cspin);
+
 // Compacted LDS-resident sweeps of shaped Heisenberg lattices (see CInst).
 hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,
                               const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
-                              const uint2* cs_q, const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st,
-                              int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                              const uint32_t* cs_p, const uint16_t* cs_nb, const uint64_t* sweeps,
+                              const Stencil& st, int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                               unsigned long long* acc);
 
 // Observable partials of full boxes.  dm = DMI form of the stencil (0 none,

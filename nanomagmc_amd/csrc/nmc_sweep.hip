@@ -641,55 +641,48 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
 // boundaries): one workgroup per instance loads only the present sites into
 // LDS (x, y, z of slot j at floats 3j .. 3j + 2, plus a zero slot for absent
 // neighbours), runs `nsweeps` sweeps there with precomputed neighbour
-// indices, and stores them back.  A work-item takes one quad with a present
-// site: it draws the quad's Philox blocks once (only those its present
-// elements use) and updates the present elements, element e0 + m with words
-// 2m, 2m + 1 of the two blocks and its bonds in the same order as
-// k_sweep_heis, so both kernels produce the same lattice.  Quads are grouped
-// by presence pattern, so a wavefront mostly takes the same branches.
+// indices, and stores them back.
+//
+// Work is split by pairs of colour elements (2k, 2k + 1), the two elements
+// that share Philox block k: a work-item draws the block once and updates the
+// pair's present elements, element 2k + m with words 2m, 2m + 1 and its bonds
+// in the same order as k_sweep_heis, so both kernels produce the same lattice.
+// Per colour the pairs with two present elements ("doubles", nd of them) come
+// first, then those with one ("singles"), each in element order.  Slots are
+// laid out by work-item, so that the t-th work item's sites sit at
+// consecutive slots across a wavefront (stride 12 B: no LDS bank conflicts
+// on the own-site accesses, and few on the neighbours', which mostly
+// advance in step):
+//   double t:  element 2k at slot base + t, element 2k + 1 at base + nd + t;
+//   single t:  its present element at slot base + nd + t;
+// with base = 0 for colour 0 and n0 for colour 1.  A pair entry is
+// k | (the single's present element is 2k + 1) << 31.
 // ---------------------------------------------------------------------------
 
-// Updates of one quad; returns the number accepted.  qr = {quad index q,
-// float index of its first present element (3 x slot) | presence mask << 16
-// (bit m: element e0 + m is present)}; the present elements have
-// consecutive slots.  NB holds, per slot, the float indices of its six
-// neighbours (16 bits each, three aligned words).  Element m uses words
-// 2m, 2m + 1 of the quad's blocks 2q (m = 0, 1) and 2q + 1 (m = 2, 3), and
-// only the blocks a present element uses are drawn.
-template <bool DIM3, int DM, bool CONE>
-__device__ __forceinline__ int cquad_update(uint2 qr, const uint16_t* __restrict__ NB, float* ls,
-                                            const DevInst& in, const Stencil& st, uint32_t c, uint32_t slo,
-                                            uint32_t shi, uint32_t k0, uint32_t k1) {
-    const uint32_t msk = qr.y >> 16, q2 = qr.x * 2u;
-    uint32_t w[8];
-    if (msk & 3u) philox4x32_10(q2, c, slo, shi, k0, k1, w);
-    if (msk & 12u) philox4x32_10(q2 + 1u, c, slo, shi, k0, k1, w + 4);
-    // The elements are one colour and read only the other colour, so each
-    // present element is read, updated and written in turn; b walks their
-    // consecutive slots.
-    uint32_t b = qr.y & 0xFFFFu;
-    int nacc = 0;
+// One site of the other-colour-reading update: slot j of the compacted spins
+// ls (x, y, z of slot i at floats 3i .. 3i + 2; in LDS or in HBM), Philox
+// words wa, wb.  NB holds per slot its six neighbours, 16 bits each in three
+// aligned words, as float indices (NBMUL = 1: 3 x slot) or as slots
+// (NBMUL = 3).  Returns 1 if the trial was accepted.
+template <bool DIM3, int DM, bool CONE, uint32_t NBMUL = 1>
+__device__ __forceinline__ int csite_update(uint32_t j, const uint32_t* __restrict__ NB, float* ls,
+                                            const DevInst& in, const Stencil& st, uint32_t wa, uint32_t wb) {
+    const uint32_t* nbw = NB + 3u * j;
+    const uint32_t nw[3] = {nbw[0], nbw[1], nbw[2]};
+    const uint32_t b = 3u * j;
+    const float sx = ls[b], sy = ls[b + 1], sz = ls[b + 2];
+    float vx = 0.f, vy = 0.f, vz = 0.f;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        if (!((msk >> m) & 1u)) continue;
-        // the slot's six neighbour indices: 12 B at byte 4 b
-        const uint32_t* nbw = (const uint32_t*)(NB + 2 * b);
-        const uint32_t nw[3] = {nbw[0], nbw[1], nbw[2]};
-        const float sx = ls[b], sy = ls[b + 1], sz = ls[b + 2];
-        float vx = 0.f, vy = 0.f, vz = 0.f;
-#pragma unroll
-        for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-            const uint32_t o = (nw[g >> 1] >> (16 * (g & 1))) & 0xFFFFu;
-            add_bond<DM, DIM3>(st, g, ls[o], ls[o + 1], ls[o + 2], vx, vy, vz);
-        }
-        float tx, ty, tz;
-        if (heis_decide<CONE>(in, sx, sy, sz, vx, vy, vz, w[2 * m], w[2 * m + 1], tx, ty, tz)) {
-            ls[b] = tx; ls[b + 1] = ty; ls[b + 2] = tz;
-            ++nacc;
-        }
-        b += 3u;
+    for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
+        const uint32_t o = NBMUL * ((nw[g >> 1] >> (16 * (g & 1))) & 0xFFFFu);
+        add_bond<DM, DIM3>(st, g, ls[o], ls[o + 1], ls[o + 2], vx, vy, vz);
     }
-    return nacc;
+    float tx, ty, tz;
+    if (heis_decide<CONE>(in, sx, sy, sz, vx, vy, vz, wa, wb, tx, ty, tz)) {
+        ls[b] = tx; ls[b + 1] = ty; ls[b + 2] = tz;
+        return 1;
+    }
+    return 0;
 }
 
 template <bool DIM3, int DM, bool CONE>
@@ -697,8 +690,8 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
                                                       const CInst* __restrict__ cl, int n,
                                                       float* __restrict__ pool,
                                                       const uint32_t* __restrict__ cs_e,
-                                                      const uint2* __restrict__ cs_q,
-                                                      const uint16_t* __restrict__ cs_nb,
+                                                      const uint32_t* __restrict__ cs_p,
+                                                      const uint32_t* __restrict__ cs_nb,
                                                       const uint64_t* __restrict__ sweeps, Stencil st, int nsweeps,
                                                       uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                                                       unsigned long long* __restrict__ acc) {
@@ -710,7 +703,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
         const DevInst& in = insts[ci.inst];
         const int N = ci.n0 + ci.n1;
         const uint32_t* E = cs_e + ci.site0;
-        const uint16_t* NB = cs_nb + 6 * ci.site0;
+        const uint32_t* NB = cs_nb + 3 * ci.site0;
         const int64_t Sp = in.Sp;
         for (int j = threadIdx.x; j < N; j += blockDim.x) {
             const float* p = pool + in.base + (j < ci.n0 ? 0 : 3 * Sp);
@@ -719,7 +712,6 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
         }
         if (threadIdx.x < 3) ls[3 * N + threadIdx.x] = 0.f;
         __syncthreads();
-        const uint2* Q = cs_q + ci.quad0;
         const uint64_t sweep0 = sweeps[ci.inst] + s_off;
         const uint32_t k1 = seed_hi + in.inst_id;
         int nacc = 0;
@@ -727,10 +719,20 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
             const uint64_t sw = sweep0 + (uint64_t)s;
             const uint32_t slo = (uint32_t)sw, shi = (uint32_t)(sw >> 32);
             for (int c = 0; c < 2; ++c) {
-                const int tb = c ? ci.nq0 : 0, te = c ? ci.nq0 + ci.nq1 : ci.nq0;
-                for (int t = tb + (int)threadIdx.x; t < te; t += blockDim.x)
-                    nacc += cquad_update<DIM3, DM, CONE>(Q[t], NB, ls, in, st, (uint32_t)c, slo, shi, seed_lo,
-                                                            k1);
+                const int nd = c ? ci.nd1 : ci.nd0, np = nd + (c ? ci.ns1 : ci.ns0);
+                const uint32_t* P = cs_p + ci.pair0 + (c ? ci.nd0 + ci.ns0 : 0);
+                const uint32_t base = c ? (uint32_t)ci.n0 : 0u;
+                for (int t = threadIdx.x; t < np; t += blockDim.x) {
+                    const uint32_t pe = P[t];
+                    const bool dbl = t < nd;
+                    uint32_t w[4];
+                    philox4x32_10(pe & 0x7FFFFFFFu, (uint32_t)c, slo, shi, seed_lo, k1, w);
+                    // doubles: element 2k at base + t; singles: at base + nd + t
+                    const bool hi = !dbl && (pe >> 31);
+                    nacc += csite_update<DIM3, DM, CONE>(base + (dbl ? t : nd + t), NB, ls, in, st,
+                                                         hi ? w[2] : w[0], hi ? w[3] : w[1]);
+                    if (dbl) nacc += csite_update<DIM3, DM, CONE>(base + nd + t, NB, ls, in, st, w[2], w[3]);
+                }
                 __syncthreads();
             }
         }
@@ -752,6 +754,78 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
             if (t) acc[ci.inst] += (unsigned long long)t;
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Streamed compacted passes: open shaped lattices whose present sites do not
+// fit in one workgroup's LDS.  Each is swept from a compacted copy in HBM
+// (x, y, z of slot j at floats 3j .. 3j + 2, then a zero slot) with the slot
+// layout, pair lists and neighbour tables of the LDS-resident kernel
+// (neighbours as slots).  One launch per colour pass over tiles of 256 pairs
+// of every listed instance; a work-item takes one pair exactly as
+// k_sweep_lds_c does, so both produce the same lattice.  Across a wavefront
+// the own sites, the table entries and most neighbours are consecutive
+// slots, so every load is coalesced.
+// ---------------------------------------------------------------------------
+template <bool DIM3, int DM, bool CONE, bool COUNT>
+__global__ __launch_bounds__(256) void k_sweep_cstream(const DevInst* __restrict__ insts,
+                                                       const CInst* __restrict__ xl,
+                                                       const XTile* __restrict__ tiles, int ntiles,
+                                                       const uint32_t* __restrict__ cs_p,
+                                                       const uint32_t* __restrict__ cs_nb,
+                                                       float* __restrict__ cspin,
+                                                       const uint64_t* __restrict__ sweeps, Stencil st, int colour,
+                                                       uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                                                       unsigned long long* __restrict__ acc) {
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const XTile tl = tiles[t];
+    const CInst ci = xl[tl.x];
+    const DevInst& in = insts[ci.inst];
+    int nacc = 0;
+    if ((int)threadIdx.x < tl.n) {
+        const int item = tl.item0 + (int)threadIdx.x;
+        const int nd = colour ? ci.nd1 : ci.nd0;
+        const uint32_t base = colour ? (uint32_t)ci.n0 : 0u;
+        const uint32_t pe = cs_p[ci.pair0 + (colour ? ci.nd0 + ci.ns0 : 0) + item];
+        const uint64_t sw = sweeps[ci.inst] + s_off;
+        uint32_t w[4];
+        philox4x32_10(pe & 0x7FFFFFFFu, (uint32_t)colour, (uint32_t)sw, (uint32_t)(sw >> 32), seed_lo,
+                      seed_hi + in.inst_id, w);
+        const bool dbl = item < nd;
+        const bool hi = !dbl && (pe >> 31);
+        const uint32_t* NB = cs_nb + 3 * ci.site0;
+        float* sp = cspin + ci.spin0;
+        nacc = csite_update<DIM3, DM, CONE, 3>(base + (uint32_t)(dbl ? item : nd + item), NB, sp, in, st,
+                                               hi ? w[2] : w[0], hi ? w[3] : w[1]);
+        if (dbl) nacc += csite_update<DIM3, DM, CONE, 3>(base + (uint32_t)(nd + item), NB, sp, in, st, w[2], w[3]);
+    }
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        tile_count(nacc, acc, t);
+    }
+}
+
+// Copies slots [item0, item0 + n) of compacted instance x between the
+// colour-split arrays and its compacted copy (TO_COPY: into the copy).
+template <bool TO_COPY>
+__global__ __launch_bounds__(256) void k_cstream_copy(const DevInst* __restrict__ insts,
+                                                      const CInst* __restrict__ xl,
+                                                      const XTile* __restrict__ tiles,
+                                                      const uint32_t* __restrict__ cs_e, float* __restrict__ pool,
+                                                      float* __restrict__ cspin) {
+    const XTile tl = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tl.n) return;
+    const CInst ci = xl[tl.x];
+    const DevInst& in = insts[ci.inst];
+    const int j = tl.item0 + (int)threadIdx.x;
+    const uint32_t e = cs_e[ci.site0 + j];
+    float* p = pool + in.base + (j < ci.n0 ? 0 : 3 * in.Sp);
+    float* c = cspin + ci.spin0 + 3 * (int64_t)j;
+    if (TO_COPY) {
+        c[0] = ld32(p, e); c[1] = ld32(p + in.Sp, e); c[2] = ld32(p + 2 * in.Sp, e);
+    } else {
+        st32(p, e, c[0]); st32(p + in.Sp, e, c[1]); st32(p + 2 * in.Sp, e, c[2]);
     }
 }
 
@@ -884,11 +958,14 @@ static hipError_t lds_one(hipStream_t stream, int grid, int block, size_t shm, c
 
 template <bool D3, int UD, bool CN>
 static hipError_t lds_c_one(hipStream_t stream, int grid, int block, size_t shm, const DevInst* insts,
-                            const CInst* cl, int n, float* pool, const uint32_t* cs_e, const uint2* cs_q,
+                            const CInst* cl, int n, float* pool, const uint32_t* cs_e, const uint32_t* cs_p,
                             const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st, int nsweeps,
                             uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc) {
-    k_sweep_lds_c<D3, UD, CN><<<grid, block, shm, stream>>>(insts, cl, n, pool, cs_e, cs_q, cs_nb, sweeps, st, nsweeps, s_off, seed_lo,
-                                   seed_hi, acc);
+    // six 16-bit neighbour indices per slot read as three 32-bit words
+    // (the table and every slot's entries are 4-byte aligned)
+    k_sweep_lds_c<D3, UD, CN><<<grid, block, shm, stream>>>(insts, cl, n, pool, cs_e, cs_p,
+                                                            (const uint32_t*)cs_nb, sweeps, st, nsweeps, s_off,
+                                                            seed_lo, seed_hi, acc);
     return hipGetLastError();
 }
 
@@ -935,13 +1012,13 @@ hipError_t set_lds_limits(int shm, int* lds_lim, int* ldsc_lim) {
 
 hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,
                               const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
-                              const uint2* cs_q, const uint16_t* cs_nb, const uint64_t* sweeps, const Stencil& st,
-                              int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                              const uint32_t* cs_p, const uint16_t* cs_nb, const uint64_t* sweeps,
+                              const Stencil& st, int nsweeps, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                               unsigned long long* acc) {
     if (n <= 0 || nsweeps <= 0) return hipSuccess;
 #define NMC_LDSC(D3, UD, CN)                          \
     if (dim3 == D3 && dm == UD && cone == CN)         \
-        return lds_c_one<D3, UD, CN>(stream, grid, block, shm, insts, cl, n, pool, cs_e, cs_q, cs_nb, sweeps, \
+        return lds_c_one<D3, UD, CN>(stream, grid, block, shm, insts, cl, n, pool, cs_e, cs_p, cs_nb, sweeps, \
                                      st, nsweeps, s_off, seed_lo, seed_hi, acc);
 #define NMC_LDSC_D(D3, UD) NMC_LDSC(D3, UD, false) NMC_LDSC(D3, UD, true)
     NMC_LDSC_D(true, 0) NMC_LDSC_D(true, 1) NMC_LDSC_D(true, 2)
@@ -949,6 +1026,38 @@ hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, 
 #undef NMC_LDSC_D
 #undef NMC_LDSC
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep_cstream(hipStream_t stream, bool dim3, int dm, bool cone, bool count,
+                                const DevInst* insts, const CInst* xl, const XTile* tiles, int ntiles,
+                                const uint32_t* cs_p, const uint16_t* cs_nb, float* cspin, const uint64_t* sweeps,
+                                const Stencil& st, int colour, uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
+                                unsigned long long* acc) {
+    if (ntiles <= 0) return hipSuccess;
+    // six 16-bit neighbour slots per slot read as three 32-bit words
+    const uint32_t* nb = (const uint32_t*)cs_nb;
+#define NMC_CS(D3, UD, CN, CT)                                                                                 \
+    if (dim3 == D3 && dm == UD && cone == CN && count == CT) {                                               \
+        k_sweep_cstream<D3, UD, CN, CT><<<ntiles, 256, 0, stream>>>(insts, xl, tiles, ntiles, cs_p, nb, cspin, \
+                                                                   sweeps, st, colour, s_off, seed_lo,         \
+                                                                   seed_hi, acc);                              \
+        return hipGetLastError();                                                                              \
+    }
+#define NMC_CS_D(D3, UD) NMC_CS(D3, UD, false, false) NMC_CS(D3, UD, false, true) \
+                         NMC_CS(D3, UD, true, false) NMC_CS(D3, UD, true, true)
+    NMC_CS_D(true, 0) NMC_CS_D(true, 1) NMC_CS_D(true, 2)
+    NMC_CS_D(false, 0) NMC_CS_D(false, 1) NMC_CS_D(false, 2)
+#undef NMC_CS_D
+#undef NMC_CS
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_cstream_copy(hipStream_t stream, bool to_copy, const DevInst* insts, const CInst* xl,
+                               const XTile* tiles, int ntiles, const uint32_t* cs_e, float* pool, float* cspin) {
+    if (ntiles <= 0) return hipSuccess;
+    if (to_copy) k_cstream_copy<true><<<ntiles, 256, 0, stream>>>(insts, xl, tiles, cs_e, pool, cspin);
+    else k_cstream_copy<false><<<ntiles, 256, 0, stream>>>(insts, xl, tiles, cs_e, pool, cspin);
+    return hipGetLastError();
 }
 
 hipError_t launch_sweep_lds(hipStream_t stream, bool ising, bool dim3, bool periodic, bool masked, int dm,

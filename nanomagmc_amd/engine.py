@@ -187,8 +187,9 @@ class Lattices:
         return out
 
     def set_path(self, path):
-        """'auto' (LDS-resident sweeps where the lattice fits) or 'stream'."""
-        check(lib().nmc_set_path(self._h, {"auto": 0, "stream": 1}[path]))
+        """'auto' (LDS-resident sweeps where the lattice fits), 'stream', or
+        'cstream' (open shaped lattices streamed from compacted copies)."""
+        check(lib().nmc_set_path(self._h, {"auto": 0, "stream": 1, "cstream": 2}[path]))
 
     def accepts(self, inst=None):
         a, n = _ilist(inst)

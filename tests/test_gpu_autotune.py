@@ -63,5 +63,7 @@ def test_choice_is_the_measured_fastest(orc, bloch3d, tmp_path):
     assert timings[autotune.key(choice)] == min(timings.values())
     # the chosen shape stays in force: the path it names decides the kernels
     kinds = set(lat.sweep_kinds().tolist())
-    from nanomagmc_amd._lib import NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS
-    assert kinds == ({NMC_KIND_SPANS} if choice[0] == "stream" else {NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS})
+    from nanomagmc_amd._lib import NMC_KIND_COMPACT_STREAM, NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS
+    want = {"stream": {NMC_KIND_SPANS}, "cstream": {NMC_KIND_COMPACT_STREAM},
+            "auto": {NMC_KIND_LDS_COMPACT, NMC_KIND_COMPACT_STREAM}}
+    assert kinds == want[choice[0]]

@@ -3,9 +3,11 @@ DMI) and the full-tile launches of the 3-D Heisenberg batch, against the
 oracle.  Needs an MI355X.
 
 * Sampled observables of single Weibull grains swept by the kernels the grain
-  batch uses -- compacted LDS-resident (AUTO) and streamed over the
-  non-empty quads (STREAM) -- and of a periodic masked lattice (the masked
-  LDS-resident kernel), against the oracle's FP64 random-site Metropolis
+  batch uses -- compacted LDS-resident (AUTO) and streamed from a compacted
+  copy in HBM (CSTREAM, the kernel of the batch's large grains) -- and by the
+  packed row-span streaming kernel (STREAM), and of a periodic masked
+  lattice (the masked LDS-resident kernel), against the oracle's FP64
+  random-site Metropolis
   chain on the same site list (the reference's state::equil with dE_func on
   a shape::weibull lattice, Old/lib/state.cpp:124-203, shape.cpp:40-58):
   <E_s>, <|M|>, C and chi at 3 sigma, errors from the spread of independent
@@ -72,8 +74,9 @@ def _table(orc, bloch3d, kind, D):
 
 
 @pytest.mark.parametrize("which,path,trial", [("bloch3d", "auto", "sphere"), ("bloch3d", "stream", "sphere"),
-                                              ("bloch3d", "auto", "cone"), ("neel2d", "auto", "sphere"),
-                                              ("neel2d", "stream", "sphere")])
+                                              ("bloch3d", "cstream", "sphere"), ("bloch3d", "auto", "cone"),
+                                              ("neel2d", "auto", "sphere"), ("neel2d", "stream", "sphere"),
+                                              ("neel2d", "cstream", "sphere")])
 def test_weibull_grain_observables_match_oracle_chain(orc, bloch3d, which, path, trial):
     from nanomagmc_amd import Bond, Lattices
     dim, edge, size, kind, D, h, T = GRAINS[which]

@@ -420,7 +420,7 @@ def test_graphs_and_direct_launches_agree(orc, bloch3d, monkeypatch, kind):
     # a small streamed set replays captured graphs (plain sweeps, adapted
     # sweeps, the sample loop); NMC_NO_GRAPHS=1 launches the same kernels
     # directly: lattices and series must agree bit for bit.  "grains": open
-    # grains swept compacted in LDS and from span copies, whose graphs hold
+    # grains swept compacted in LDS and from compacted copies, whose graphs hold
     # the two as branches on two streams.
     from nanomagmc_amd import Bond, Lattices
     bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
@@ -435,8 +435,8 @@ def test_graphs_and_direct_launches_agree(orc, bloch3d, monkeypatch, kind):
                            seed=61)
         else:
             lat = Lattices(3, False, False, bonds, grains, use_d=True, trial="cone", seed=61)
-            from nanomagmc_amd._lib import NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS
-            assert lat.sweep_kinds().tolist() == [NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS]
+            from nanomagmc_amd._lib import NMC_KIND_COMPACT_STREAM, NMC_KIND_LDS_COMPACT
+            assert lat.sweep_kinds().tolist() == [NMC_KIND_LDS_COMPACT, NMC_KIND_COMPACT_STREAM]
         lat.set_params([0.5, 1.5], [0.0, 0.1])
         lat.randomize()
         lat.equil(150)
@@ -456,9 +456,10 @@ def test_graphs_and_direct_launches_agree(orc, bloch3d, monkeypatch, kind):
                                               (2, "cone", "general")])
 def test_compacted_and_streamed_grains_agree(bloch3d, orc, dim, trial, table):
     # one set holding grains small enough for the compacted LDS sweep and one
-    # too large for it (streamed over its non-empty quads): the lattice and the
-    # accept counts must equal an all-streaming run bit for bit, for both DMI
-    # forms of the kernels
+    # too large for it (streamed from its compacted copy): the lattice and the
+    # accept counts must equal an all-streaming run over packed row spans,
+    # and one streaming every grain from compacted copies, bit for bit, for
+    # both DMI forms of the kernels
     from nanomagmc_amd import Bond, Lattices
     rng = np.random.default_rng(5 + dim)
     bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2) if dim == 3 else neel2d(0.2)
@@ -471,7 +472,7 @@ def test_compacted_and_streamed_grains_agree(bloch3d, orc, dim, trial, table):
         m = (rng.random(n ** dim) < 0.75).astype(np.uint8)
         insts.append({"n": (n,) * dim, "edge": n, "mask": m})
     runs = []
-    for path in ("auto", "stream"):
+    for path in ("auto", "stream", "cstream"):
         lat = Lattices(dim, False, False, bonds, insts, use_d=True, trial=trial, seed=91)
         lat.set_path(path)
         lat.set_params([0.7, 1.3, 2.0], [0.05, 0.0, -0.1])
@@ -481,9 +482,14 @@ def test_compacted_and_streamed_grains_agree(bloch3d, orc, dim, trial, table):
         lat.equil(5)
         lat.equil(260, inst=[0, 2])
         runs.append(lat)
-    for i in range(3):
-        assert np.array_equal(runs[0].get_spins(i), runs[1].get_spins(i)), i
-    assert np.array_equal(runs[0].accepts(), runs[1].accepts())
+    from nanomagmc_amd._lib import NMC_KIND_COMPACT_STREAM, NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS
+    assert runs[0].sweep_kinds().tolist() == [NMC_KIND_LDS_COMPACT, NMC_KIND_LDS_COMPACT, NMC_KIND_COMPACT_STREAM]
+    assert runs[1].sweep_kinds().tolist() == [NMC_KIND_SPANS] * 3
+    assert runs[2].sweep_kinds().tolist() == [NMC_KIND_COMPACT_STREAM] * 3
+    for r in runs[1:]:
+        for i in range(3):
+            assert np.array_equal(runs[0].get_spins(i), r.get_spins(i)), i
+        assert np.array_equal(runs[0].accepts(), r.accepts())
 
 
 @pytest.mark.parametrize("ising", [True, False])
@@ -736,7 +742,8 @@ def test_invalid_arguments_raise(orc):
 def test_sample_loop_with_averages_equals_the_calls(orc, bloch3d, monkeypatch, graphs):
     # nmc_run_samples_avg (the drop-in driver's whole sample loop, one graph)
     # against nmc_sweep + nmc_sample + nmc_accumulate_avgs per slot, on open
-    # grains (compacted and span-streamed) and a periodic lattice set
+    # grains (compacted in LDS and streamed from compacted copies) and a
+    # periodic lattice set
     from nanomagmc_amd import Bond, Lattices
     monkeypatch.setenv("NMC_NO_GRAPHS", "1" if graphs == "0" else "0")
     bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
