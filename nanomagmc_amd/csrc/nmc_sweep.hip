@@ -659,6 +659,11 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
 // k | (the single's present element is 2k + 1) << 31.
 // ---------------------------------------------------------------------------
 
+// Three floats moved as one 12-byte access.
+struct F3 {
+    float x, y, z;
+};
+
 // One site of the other-colour-reading update: slot j of the compacted spins
 // ls (x, y, z of slot i at floats 3i .. 3i + 2; in LDS or in HBM), Philox
 // words wa, wb.  NB holds per slot its six neighbours, 16 bits each in three
@@ -670,7 +675,13 @@ __device__ __forceinline__ int csite_update(uint32_t j, const uint32_t* __restri
     const uint32_t* nbw = NB + 3u * j;
     const uint32_t nw[3] = {nbw[0], nbw[1], nbw[2]};
     const uint32_t b = 3u * j;
-    const float sx = ls[b], sy = ls[b + 1], sz = ls[b + 2];
+    float sx, sy, sz;
+    if (NBMUL == 3) {   // HBM: one 12-byte load
+        const F3 v = *(const F3*)(ls + b);
+        sx = v.x; sy = v.y; sz = v.z;
+    } else {
+        sx = ls[b]; sy = ls[b + 1]; sz = ls[b + 2];
+    }
     float vx = 0.f, vy = 0.f, vz = 0.f;
 #pragma unroll
     for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
@@ -679,7 +690,8 @@ __device__ __forceinline__ int csite_update(uint32_t j, const uint32_t* __restri
     }
     float tx, ty, tz;
     if (heis_decide<CONE>(in, sx, sy, sz, vx, vy, vz, wa, wb, tx, ty, tz)) {
-        ls[b] = tx; ls[b + 1] = ty; ls[b + 2] = tz;
+        if (NBMUL == 3) *(F3*)(ls + b) = F3{tx, ty, tz};   // HBM: one 12-byte store
+        else { ls[b] = tx; ls[b + 1] = ty; ls[b + 2] = tz; }
         return 1;
     }
     return 0;
