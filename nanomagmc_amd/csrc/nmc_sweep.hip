@@ -106,7 +106,7 @@ template <bool CONE>
 __device__ __forceinline__ bool heis_decide(const DevInst& in, float sx, float sy, float sz, float vx, float vy,
                                             float vz, uint32_t wa, uint32_t wb, float& tx, float& ty, float& tz) {
 #pragma clang fp contract(off)
-    const float ua = ((float)(wa >> 12) + 0.5f) * (1.0f / 1048576.0f);   // (0, 1), centred on the 2^-20 grid
+    const float ka = (float)(wa >> 12);                                   // polar grid index, exact
     const float uphi = (float)(wb >> 12) * (1.0f / 1048576.0f);          // [0, 1)
     const float uacc = (float)(((wa & 0xFFFu) << 12) | (wb & 0xFFFu)) * (1.0f / 16777216.0f);
     const float cphi = __builtin_amdgcn_cosf(uphi);   // cos(2 pi u): v_cos_f32 takes revolutions
@@ -115,14 +115,18 @@ __device__ __forceinline__ bool heis_decide(const DevInst& in, float sx, float s
         // uniform direction on S^2: z uniform in (-1, 1), azimuth uniform
         // the hardware square root (v_sqrt_f32, about 1 ulp): the IEEE-exact
         // one costs a refinement and denormal scaling of ~15 instructions,
-        // and the trial spin is rounded to fp32 anyway
-        tz = __builtin_fmaf(2.f, ua, -1.f);
+        // and the trial spin is rounded to fp32 anyway.
+        // z = 2 ua - 1 with ua = (ka + 1/2) 2^-20, as one fused multiply-add
+        // of exact operands (2^-20 - 1 is exact in fp32): one rounding of the
+        // same real number
+        tz = __builtin_fmaf(ka, 1.0f / 524288.0f, 1.0f / 1048576.0f - 1.0f);
         const float r = __builtin_amdgcn_sqrtf(fmaxf(0.f, __builtin_fmaf(-tz, tz, 1.f)));
         tx = r * cphi;
         ty = r * sphi;
     } else {
         // uniform on the cap of half-angle theta around s
         // (orthonormal frame: Duff et al., JCGT 6(1), 2017)
+        const float ua = (ka + 0.5f) * (1.0f / 1048576.0f);   // (0, 1), centred on the 2^-20 grid
         const float ca = __builtin_fmaf(-ua, in.cone_w, 1.f);
         const float sa = __builtin_amdgcn_sqrtf(fmaxf(0.f, __builtin_fmaf(-ca, ca, 1.f)));
         const float sgn = sz >= 0.f ? 1.f : -1.f;
