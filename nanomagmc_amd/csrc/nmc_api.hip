@@ -112,6 +112,18 @@ struct Set {
     XTile* d_xct = nullptr;        // ... copy tiles ...
     int n_xct = 0;
     float* d_cspin = nullptr;      // ... and the compacted copies (zeroed when created)
+    // fused sample loops (every instance compacted): per site of a compacted
+    // instance, in the measurement's order (present sites in element order,
+    // colour 0 then 1), its slot; the streamed instances' measurement tiles,
+    // instance list and tile ranges; HBM scratch of the LDS-resident loop
+    uint16_t* d_cs_mslot = nullptr;
+    CMTile* d_xmt = nullptr;
+    int n_xmt = 0;
+    int* d_xlist = nullptr;
+    int* d_xtb = nullptr;
+    int* d_xte = nullptr;
+    double* d_sscr = nullptr;
+    bool fusable = false;
     // size classes of the compacted lattices: d_cl[off .. off + n) get
     // workgroups of `block` work-items and `lds` bytes of dynamic LDS
     struct CClass {
@@ -140,6 +152,10 @@ struct Set {
         (void)hipFree(d_span);
         (void)hipFree(d_xl); (void)hipFree(d_xt); (void)hipFree(d_xct); (void)hipFree(d_cspin);
         d_xl = nullptr; d_xt = nullptr; d_xct = nullptr; d_cspin = nullptr;
+        (void)hipFree(d_cs_mslot); (void)hipFree(d_xmt); (void)hipFree(d_xlist); (void)hipFree(d_xtb);
+        (void)hipFree(d_xte); (void)hipFree(d_sscr);
+        d_cs_mslot = nullptr; d_xmt = nullptr; d_xlist = nullptr; d_xtb = nullptr; d_xte = nullptr;
+        d_sscr = nullptr;
         d_si = nullptr; d_stl = nullptr; d_slist = nullptr; d_rowtab = nullptr; d_span = nullptr;
         d_atp = nullptr; d_ati = nullptr; d_ans = nullptr;
         adapt_ready = false;
@@ -228,6 +244,7 @@ struct nmc_ctx {
     std::vector<double*> h_avgptr;   // per instance: running sum (device) or null ...
     double** d_avgptr = nullptr;     // ... and its device copy, read by the batched accumulate
     std::map<std::vector<int>, std::pair<Tile*, int>> avg_tiles;   // tile lists per instance list
+    std::map<std::vector<int>, int*> avg_flags;   // fused sample loops: averaged instances as per-instance flags
     uint64_t init_epoch = 0;
     hipStream_t stream = nullptr;
     // Second stream (default priority) for compacted-grain sweeps that run
@@ -260,6 +277,7 @@ struct nmc_ctx {
         for (auto& kv : sets) kv.second->release();
         for (auto& kv : avg) (void)hipFree(kv.second);
         for (auto& kv : avg_tiles) (void)hipFree(kv.second.first);
+        for (auto& kv : avg_flags) (void)hipFree(kv.second);
         (void)hipFree(d_avgptr);
         (void)hipFree(d_inst); (void)hipFree(pool); (void)hipFree(mpool); (void)hipFree(d_tab);
         (void)hipFree(d_dtab); (void)hipFree(d_sweeps); (void)hipFree(d_acc); (void)hipFree(d_bacc);
@@ -646,6 +664,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     std::vector<uint32_t> cs_e((size_t)total_sites);
     std::vector<uint16_t> cs_nb((size_t)total_sites * 6);
     std::vector<uint32_t> cs_p((size_t)total_pairs);
+    std::vector<uint16_t> cs_mslot((size_t)total_sites);
     auto fill = [&](size_t qb, size_t qe) {
         std::vector<int> slot[2];
         for (size_t q = qb; q < qe; ++q) {
@@ -677,6 +696,11 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                 }
                 for (int64_t e = 0; e < d.S; ++e)
                     if (slot[col][e] >= 0) E[slot[col][e]] = (uint32_t)e;
+                // the measurement's order: this colour's present elements,
+                // ascending, after colour 0's (slots < 65536 here)
+                uint16_t* MS = cs_mslot.data() + ci.site0 + (col ? ci.n0 : 0);
+                for (int64_t e = 0; e < d.S; ++e)
+                    if (slot[col][e] >= 0) *MS++ = (uint16_t)slot[col][e];
                 PD += col ? ci.nd1 + ci.ns1 : ci.nd0 + ci.ns0;
             }
             const int zero = ci.n0 + ci.n1;
@@ -738,6 +762,28 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         if (rc) return rc;
         NMC_HIP(hipMemsetAsync(s->d_cspin, 0, (size_t)total_cspin * sizeof(float), c->stream));
     }
+    // fused sample loops: when every listed instance is compacted, the
+    // streamed ones are measured from their copies with the measurement's
+    // own tiles (the list entry's tiles tb .. te, cut as above)
+    std::vector<CMTile> xmt;
+    std::vector<int> xlist, xtb, xte;
+    s->fusable = !c->ising && c->masked && !c->periodic && (!cl.empty() || !xl.empty());
+    for (int k : s->kind) s->fusable = s->fusable && (k == NMC_KIND_LDS_COMPACT || k == NMC_KIND_COMPACT_STREAM);
+    if (s->fusable && !xl.empty()) {
+        std::unordered_map<int, int> li_of;
+        for (size_t li = 0; li < list.size(); ++li) li_of[list[li]] = (int)li;
+        for (size_t x = 0; x < xl.size(); ++x) {
+            const int li = li_of[xl[x].inst];
+            const int N = xl[x].n0 + xl[x].n1, nt = te[li] - tb[li];
+            for (int t = 0; t < nt; ++t) {
+                const int q0 = (int)((int64_t)N * t / nt), q1 = (int)((int64_t)N * (t + 1) / nt);
+                xmt.push_back(CMTile{(int)x, q0, q1 - q0, tb[li] + t});
+            }
+            xlist.push_back(xl[x].inst);
+            xtb.push_back(tb[li]);
+            xte.push_back(te[li]);
+        }
+    }
     // size classes: the compacted lattices sorted by their larger colour's
     // pair count into (0, 128], (128, 256], (256, 512], (512, inf), each
     // class one launch whose workgroup covers its largest lattice's pairs
@@ -779,6 +825,12 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
     for (const STile& t : stl) sw.push_back(Tile{t.inst, t.q0});
     for (const XTile& t : xt) sw.push_back(Tile{xl[t.x].inst, t.item0});
     if ((rc = upload(c, &s->d_xl, xl))) return rc;
+    if ((rc = upload(c, &s->d_cs_mslot, cs_mslot))) return rc;
+    if ((rc = upload(c, &s->d_xmt, xmt))) return rc;
+    if ((rc = upload(c, &s->d_xlist, xlist))) return rc;
+    if ((rc = upload(c, &s->d_xtb, xtb))) return rc;
+    if ((rc = upload(c, &s->d_xte, xte))) return rc;
+    s->n_xmt = (int)xmt.size();
     if ((rc = upload(c, &s->d_xt, xt))) return rc;
     if ((rc = upload(c, &s->d_xct, xct))) return rc;
     s->n_xl = (int)xl.size();
@@ -1658,6 +1710,93 @@ int sample_launches(nmc_ctx* ctx, Set* s, int slot) {
 }
 }  // namespace
 
+namespace {
+// Whether a set's sample loop can run fused (fused_sample_launches): every
+// instance compacted, no charges in the series, the bond-form measurement.
+bool fused_ok(const nmc_ctx* ctx, const Set* s) {
+    return s->fusable && !ctx->series_q && ctx->bond_sym && (ctx->dm == 0) == (ctx->dm_meas == 0);
+}
+
+// The sample loop of a fusable set: nsamples x (samp_sweeps sweeps + one
+// measurement into series slot 0 .. nsamples - 1, + add_to_av of the
+// instances flagged in d_avgflag, if given).  The LDS-resident grains run it
+// in one launch per size class (k_samples_lds_c) on the auxiliary stream;
+// the streamed grains, beside them, sweep and are measured from their
+// compacted copies (filled once before the loop, written back once after
+// it); every observable, average and lattice equals the step-by-step loop's
+// bit for bit.  Launches only, so it can be recorded into a graph.
+// The LDS-resident loop's scratch, allocated before anything is recorded.
+int fused_prepare(Set* s) {
+    if (s->n_cl > 0 && !s->d_sscr) {
+        int grid = 0;
+        for (const Set::CClass& cc : s->cclass) grid = std::max(grid, std::min(cc.n, 65535));
+        if (int rc = dalloc(&s->d_sscr, (size_t)grid * kSampleScratch)) return rc;
+    }
+    return NMC_OK;
+}
+
+int fused_sample_launches(nmc_ctx* ctx, Set* s, int nsamples, int samp_sweeps, const int* d_avgflag) {
+    const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
+    const bool side = s->n_cl > 0 && s->n_xl > 0;
+    if (side) {
+        NMC_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+        NMC_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
+    }
+    const hipStream_t ls = side ? ctx->aux : ctx->stream;
+    hipError_t e = hipSuccess;
+    for (size_t q = 0; q < s->cclass.size() && e == hipSuccess; ++q) {
+        const Set::CClass& cc = s->cclass[q];
+        const int block = (std::max(256, cc.block) + 255) / 256 * 256;
+        e = launch_samples_lds_c(ls, ctx->dim3, ctx->dm, ctx->cone, std::min(cc.n, 65535), block, cc.lds,
+                                 ctx->d_inst, s->d_cl + cc.off, cc.n, (float*)ctx->pool, s->d_cs_e, s->d_cs_p,
+                                 s->d_cs_nb, s->d_cs_mslot, ctx->d_sweeps, ctx->st, nsamples, samp_sweeps, slo, shi,
+                                 ctx->d_acc, ctx->d_series, ctx->series_n, 2, s->d_sscr, ctx->d_avgptr, d_avgflag);
+    }
+    if (side) {
+        const hipError_t j = hipEventRecord(ctx->ev_join, ctx->aux);
+        if (e == hipSuccess) e = j;
+    }
+    if (s->n_xl > 0 && e == hipSuccess) {
+        const int xt_acc = s->n_sweep + s->n_qt[0] + s->n_qt[1] + s->n_st[0] + s->n_st[1];
+        e = launch_cstream_copy(ctx->stream, true, ctx->d_inst, s->d_xl, s->d_xct, s->n_xct, s->d_cs_e,
+                                (float*)ctx->pool, s->d_cspin);
+        for (int smp = 0; smp < nsamples && e == hipSuccess; ++smp) {
+            for (int k = 0; k < samp_sweeps && e == hipSuccess; ++k)
+                for (int col = 0; col < 2 && e == hipSuccess; ++col) {
+                    const int off = col ? s->n_xt[0] : 0;
+                    e = launch_sweep_cstream(ctx->stream, ctx->dim3, ctx->dm, ctx->cone, true, ctx->d_inst, s->d_xl,
+                                             s->d_xt + off, s->n_xt[col], s->d_cs_p, s->d_cs_nb, s->d_cspin,
+                                             ctx->d_sweeps, ctx->st, col, (uint32_t)(smp * samp_sweeps + k), slo,
+                                             shi, s->d_tacc + xt_acc + off);
+                }
+            if (e == hipSuccess)
+                e = launch_measure_cs(ctx->stream, ctx->dim3, ctx->dm_meas, ctx->d_inst, s->d_xl, s->d_xmt,
+                                      s->n_xmt, s->d_cs_e, s->d_cs_nb, s->d_cs_mslot, s->d_cspin, ctx->st,
+                                      ctx->d_partials, pstride_of(ctx));
+            if (e == hipSuccess)
+                e = launch_measure_finalize(ctx->stream, ctx->d_inst, s->d_xlist, s->n_xl, s->d_xtb, s->d_xte,
+                                            ctx->d_partials, pstride_of(ctx), ctx->a2max, ctx->dim3, false, ctx->nsl,
+                                            ctx->d_series, nullptr, nullptr, true, ctx->series_n, smp, 2, true);
+            if (e == hipSuccess && d_avgflag)
+                e = launch_accum_avg_cs(ctx->stream, ctx->d_inst, s->d_xl, s->d_xct, s->n_xct, s->d_cs_e,
+                                        s->d_cspin, ctx->d_avgptr, d_avgflag);
+        }
+        if (e == hipSuccess)
+            e = launch_cstream_copy(ctx->stream, false, ctx->d_inst, s->d_xl, s->d_xct, s->n_xct, s->d_cs_e,
+                                    (float*)ctx->pool, s->d_cspin);
+    }
+    if (side) {
+        const hipError_t j = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+        if (e == hipSuccess) e = j;
+    }
+    if (e == hipSuccess && nsamples * samp_sweeps > 0)
+        e = launch_bump_sweeps(ctx->stream, ctx->d_sweeps, s->d_list, (int)s->list.size(),
+                               (uint64_t)nsamples * (uint64_t)samp_sweeps);
+    if (e != hipSuccess) return fail(NMC_E_HIP, std::string("fused sample loop: ") + hipGetErrorString(e));
+    return NMC_OK;
+}
+}  // namespace
+
 int nmc_sample(nmc_ctx* ctx, const int* inst, int n, int slot) {
     if (!ctx) return fail(NMC_E_INVALID, "null context");
     if (slot < 0 || slot >= ctx->series_n) return fail(NMC_E_INVALID, "sample slot outside the series buffer");
@@ -1675,22 +1814,23 @@ int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp
     Set* s;
     if (int rc = get_set(ctx, inst, n, &s)) return rc;
     if (int rc = ensure_partials(ctx, (size_t)s->n_mt * pstride_of(ctx))) return rc;
-    if (!ctx->graphs_on) {
+    const bool fused = fused_ok(ctx, s);
+    if (fused)
+        if (int rc = fused_prepare(s)) return rc;
+    auto loop = [&]() -> int {
+        if (fused) return fused_sample_launches(ctx, s, nsamples, samp_sweeps, nullptr);
         for (int slot = 0; slot < nsamples; ++slot) {
             if (int rc = sweep_launches(ctx, s, samp_sweeps)) return rc;
             if (int rc = sample_launches(ctx, s, slot)) return rc;
         }
         return NMC_OK;
-    }
+    };
+    if (!ctx->graphs_on) return loop();
     auto key = std::make_tuple(s->list, nsamples, samp_sweeps);
     auto it = ctx->graphs.find(key);
     if (it == ctx->graphs.end()) {
         NMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = NMC_OK;
-        for (int slot = 0; slot < nsamples && rc == NMC_OK; ++slot) {
-            rc = sweep_launches(ctx, s, samp_sweeps);
-            if (rc == NMC_OK) rc = sample_launches(ctx, s, slot);
-        }
+        const int rc = loop();
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc != NMC_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -1741,6 +1881,19 @@ int nmc_run_samples_avg(nmc_ctx* ctx, const int* inst, int n, int nsamples, int 
         if (int rc = upload(ctx, &dt, tiles)) return rc;
         at = ctx->avg_tiles.emplace(alist, std::make_pair(dt, (int)tiles.size())).first;
     }
+    const bool fused = fused_ok(ctx, s);
+    int* d_flag = nullptr;   // fused loops: the averaged instances as per-instance flags
+    if (fused) {
+        if (int rc = fused_prepare(s)) return rc;
+        auto af = ctx->avg_flags.find(alist);
+        if (af == ctx->avg_flags.end()) {
+            std::vector<int> flags(ctx->din.size(), 0);
+            for (int i : alist) flags[i] = 1;
+            if (int rc = upload(ctx, &d_flag, flags)) return rc;
+            af = ctx->avg_flags.emplace(alist, d_flag).first;
+        }
+        d_flag = af->second;
+    }
     NMC_HIP(hipStreamSynchronize(ctx->stream));   // the pointer table is current before capture
     auto step = [&](int slot) -> int {
         if (int rc = sweep_launches(ctx, s, samp_sweeps)) return rc;
@@ -1749,11 +1902,13 @@ int nmc_run_samples_avg(nmc_ctx* ctx, const int* inst, int n, int nsamples, int 
                                        ctx->pool, ctx->d_avgptr));
         return NMC_OK;
     };
-    if (!ctx->graphs_on) {
+    auto loop = [&]() -> int {
+        if (fused) return fused_sample_launches(ctx, s, nsamples, samp_sweeps, d_flag);
         for (int slot = 0; slot < nsamples; ++slot)
             if (int rc = step(slot)) return rc;
         return NMC_OK;
-    }
+    };
+    if (!ctx->graphs_on) return loop();
     // the key carries the averaged list after a -1 separator
     std::vector<int> gkey(s->list);
     gkey.push_back(-1);
@@ -1762,8 +1917,7 @@ int nmc_run_samples_avg(nmc_ctx* ctx, const int* inst, int n, int nsamples, int 
     auto it = ctx->graphs.find(key);
     if (it == ctx->graphs.end()) {
         NMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = NMC_OK;
-        for (int slot = 0; slot < nsamples && rc == NMC_OK; ++slot) rc = step(slot);
+        const int rc = loop();
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc != NMC_OK) { if (g) (void)hipGraphDestroy(g); return rc; }

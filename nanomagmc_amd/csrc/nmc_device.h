@@ -159,6 +159,20 @@ struct CInst {
     int64_t spin0;    // streamed: first float of its compacted spin copy (x, y, z per slot, then a zero slot)
 };
 
+// Tile of the measurement of a streamed compacted instance: entries q0 ..
+// q0 + n - 1 (n <= kMeasureSites) of its present sites in element order, its
+// partials at tile index pt of the set's measurement tiles.
+struct CMTile {
+    int x, q0, n, pt;
+};
+
+// Fused sample loops of LDS-resident compacted grains: measurement tiles per
+// grain (N <= kMaxCTiles x kMeasureSites) and the doubles of HBM scratch per
+// workgroup (cross-wave sums of up to four tiles at once, the tile partials,
+// the instance's sums).
+constexpr int kMaxCTiles = 16;
+constexpr int kSampleScratch = 4 * 4 * 15 + kMaxCTiles * 15 + 16;
+
 // Tile of a streamed compacted pass or copy: items [item0, item0 + n) of
 // compacted instance x (pairs of one colour for a pass, slots for a copy).
 struct XTile {
@@ -226,6 +240,111 @@ __device__ __forceinline__ SitePos site_of(const DevInst& in, uint32_t e, int c,
     p.q = (c + p.i0 + p.i1) & 1;
     p.i2 = 2 * p.k + p.q;
     return p;
+}
+
+// v of lane l moved through the data-parallel-primitive crossbar (`ctrl` one
+// of the DPP patterns below); lanes outside row_mask read 0.0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the wave, complete in lane 63, in a fixed order: pairs and quads
+// (quad permutations), half rows (row_half_mirror), rows of 16
+// (row_mirror), then row 0 into row 1 and row 2 into row 3 (row_bcast15),
+// then rows 0-1 into rows 2-3 (row_bcast31).  Register moves only, no LDS.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma clang fp contract(off)
+    v += dpp_f64<0xB1, 0xF>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E, 0xF>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141, 0xF>(v);   // row_half_mirror
+    v += dpp_f64<0x140, 0xF>(v);   // row_mirror
+    v += dpp_f64<0x142, 0xA>(v);   // row_bcast15
+    v += dpp_f64<0x143, 0xC>(v);   // row_bcast31
+    return v;
+}
+
+// C remainder semantics (truncating), as the reference's % on int.
+__device__ __forceinline__ int crem(int a, int b) { return a % b; }
+
+// The bond-form observable sums of one Heisenberg site (the k_measure_sites
+// pass for tables with J equal and D opposite for a bond and its reverse),
+// from values: spin s, the spins n[m] of its neighbours in the + stencil
+// slots g = 2m + 1 (3-D: m = 0, 1, 2; 2-D: m = 1, 2; an absent neighbour as
+// the zero spin).  acc 0-2 M, 3-5 subM(0), 6-8 sub4M, 9 the bond sum with
+// +D, 10 with -D; the same expressions, in the same order, as the
+// measurement kernel's site_sums.
+template <bool DIM3, int DM>
+__device__ __forceinline__ void bond_site_sums(const DevInst& in, const Stencil& st, const SitePos& p, double s0,
+                                               double s1, double s2, const double (*n)[3], double* acc) {
+#pragma clang fp contract(off)
+    const int i0 = p.i0, i1 = p.i1, i2 = p.i2;
+    acc[0] += s0; acc[1] += s1; acc[2] += s2;
+    const int rx = DIM3 ? i0 + in.o0 : i1 + in.o1;
+    const int ry = DIM3 ? i1 + in.o1 : i2 + in.o2;
+    const int rz = DIM3 ? i2 + in.o2 : 0;
+    if (((i0 + in.o0 + i1 + in.o1 + i2 + in.o2) % 2) == 0) {   // calc_subM(0)
+        acc[3] += s0; acc[4] += s1; acc[5] += s2;
+    }
+    {   // calc_sub4M
+        const int possum = rx + ry;
+        const int posdiff = -rx + ry + (in.edge - in.edge % 4);
+        const int kz = rz;
+        if ((crem(possum, 4) == 0 && crem(posdiff, 4) == 0 && crem(kz, 4) == 0) ||
+            (crem(possum, 4) == 2 && crem(posdiff, 4) == 2 && crem(kz, 4) == 2)) {
+            acc[6] += s0; acc[7] += s1; acc[8] += s2;
+        }
+    }
+    double bE = 0.0, bS = 0.0;
+#pragma unroll
+    for (int g = DIM3 ? 1 : 3; g < 6; g += 2) {
+        const double n0 = n[g >> 1][0], n1 = n[g >> 1][1], n2 = n[g >> 1][2];
+        const double x = st.Jd[g] * ((s0 * n0 + s1 * n1) + s2 * n2);
+        double y = 0.0;
+        if (DM != 0) {
+            const double* D = st.Dd[g];
+            const double y0 = D[1] * n2 - D[2] * n1, y1 = D[2] * n0 - D[0] * n2, y2 = D[0] * n1 - D[1] * n0;
+            y = (s0 * y0 + s1 * y1) + s2 * y2;
+        }
+        bE += x + y;
+        bS += x - y;
+    }
+    acc[9] += bE;
+    acc[10] += bS;
+}
+
+// One observable row from an instance's 15 summed partials (the
+// measurement finalize's bond-form case): M, subM, sub4M, E_r, E_s.
+__device__ __forceinline__ void bond_obs_row(const DevInst& in, int hidx, const double* sum, double* o) {
+#pragma clang fp contract(off)
+    for (int i = 0; i < 9; ++i) o[i] = sum[i];
+    double H[3] = {0.0, 0.0, 0.0};
+    H[hidx] = in.h_d;
+    const double hm = (-H[0] * sum[0] + -H[1] * sum[1]) + -H[2] * sum[2];
+    o[9] = hm - sum[9];
+    o[10] = hm - sum[10];
+}
+
+// The measurement finalize's sum of one scalar over an instance's tile
+// partials p[0], p[stride], ..., p[(nt - 1) stride]: eight running sums
+// (tiles 8m + r), added in a fixed order.
+__device__ __forceinline__ double tile_sum8(const double* p, int nt, int64_t stride) {
+#pragma clang fp contract(off)
+    double v8[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v8[r] = 0.0;
+    int t = 0;
+    for (; t + 8 <= nt; t += 8) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v8[r] += p[(int64_t)(t + r) * stride];
+    }
+#pragma unroll
+    for (int r = 0; r < 7; ++r)
+        if (t + r < nt) v8[r] += p[(int64_t)(t + r) * stride];
+    return ((v8[0] + v8[1]) + (v8[2] + v8[3])) + ((v8[4] + v8[5]) + (v8[6] + v8[7]));
 }
 
 // The one spin axis along which the axial DMI form carries D for bond slot g:

@@ -70,33 +70,6 @@ __device__ __forceinline__ void load_spin(const void* pool, const DevInst& in, i
     }
 }
 
-// v of lane l moved through the data-parallel-primitive crossbar (`ctrl` one
-// of the DPP patterns below); lanes outside row_mask read 0.0.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROW_MASK, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROW_MASK, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// Sum over the wave, complete in lane 63, in a fixed order: pairs and quads
-// (quad permutations), half rows (row_half_mirror), rows of 16
-// (row_mirror), then row 0 into row 1 and row 2 into row 3 (row_bcast15),
-// then rows 0-1 into rows 2-3 (row_bcast31).  Register moves only, no LDS.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma clang fp contract(off)
-    v += dpp_f64<0xB1, 0xF>(v);    // quad_perm [1, 0, 3, 2]
-    v += dpp_f64<0x4E, 0xF>(v);    // quad_perm [2, 3, 0, 1]
-    v += dpp_f64<0x141, 0xF>(v);   // row_half_mirror
-    v += dpp_f64<0x140, 0xF>(v);   // row_mirror
-    v += dpp_f64<0x142, 0xA>(v);   // row_bcast15
-    v += dpp_f64<0x143, 0xC>(v);   // row_bcast31
-    return v;
-}
-
-// C remainder semantics (truncating), as the reference's % on int.
-__device__ __forceinline__ int crem(int a, int b) { return a % b; }
 
 // Whether stencil slot g's neighbour lies inside the box without periodic
 // wrap (the reference's adj table of an open lattice).

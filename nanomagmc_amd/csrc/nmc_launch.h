@@ -65,6 +65,29 @@ hipError_t launch_sweep_cstream(hipStream_t stream, bool dim3, int dm, bool cone
 hipError_t launch_cstream_copy(hipStream_t stream, bool to_copy, const DevInst* insts, const CInst* xl,
                                const XTile* tiles, int ntiles, const uint32_t* cs_e, float* pool, float* cspin);
 
+// nsamples x (samp_sweeps sweeps + bond-form measurement into series rows
+// inst x series_n + sample, + add_to_av where avgflag[inst]) of LDS-resident
+// compacted grains in one launch (block a multiple of 256; scratch holds
+// kSampleScratch doubles per workgroup of the grid).
+hipError_t launch_samples_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,
+                                const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
+                                const uint32_t* cs_p, const uint16_t* cs_nb, const uint16_t* cs_mslot,
+                                const uint64_t* sweeps, const Stencil& st, int nsamples, int samp_sweeps,
+                                uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc, double* series,
+                                int series_n, int hidx, double* scratch, double* const* avgptr, const int* avgflag);
+
+// Bond-form measurement partials of streamed compacted instances from their
+// copies (the tiles and order of the measurement over site lists).
+hipError_t launch_measure_cs(hipStream_t stream, bool dim3, int dm, const DevInst* insts, const CInst* xl,
+                             const CMTile* tiles, int ntiles, const uint32_t* cs_e, const uint16_t* cs_nb,
+                             const uint16_t* cs_mslot, const float* cspin, const Stencil& st, double* partials,
+                             int pstride);
+
+// add_to_av of flagged streamed compacted instances from their copies.
+hipError_t launch_accum_avg_cs(hipStream_t stream, const DevInst* insts, const CInst* xl, const XTile* tiles,
+                               int ntiles, const uint32_t* cs_e, const float* cspin, double* const* avgptr,
+                               const int* avgflag);
+
 // Compacted LDS-resident sweeps of shaped Heisenberg lattices (see CInst).
 hipError_t launch_sweep_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,
                               const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,

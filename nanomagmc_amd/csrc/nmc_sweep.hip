@@ -822,6 +822,234 @@ __global__ __launch_bounds__(256) void k_sweep_cstream(const DevInst* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused sample loops of compacted grains (nmc_run_samples without charges,
+// bond-form measurement): `nsamples` x (`samp_sweeps` sweeps + one
+// measurement) in one launch.
+//
+// LDS-resident (k_samples_lds_c): the workgroup keeps its grain in LDS for
+// the whole loop and measures it there after every sample's sweeps, writing
+// the series row itself, so the spins cross HBM once per loop instead of
+// once per sample.  The measurement reproduces k_measure_sites and
+// k_measure_finalize operation for operation: the instance's present sites
+// in element order (colour 0, then 1; MS maps that order to slots), cut into
+// the same tiles of at most kMeasureSites, each summed by 256 work-items
+// (work-item j takes entries j, j + 256, ...), wave sums through DPP, the
+// four waves in order, and the tiles in the finalize's order; so the
+// observables equal the separate measurement's bit for bit.  The cross-wave
+// sums go through a per-workgroup scratch in HBM (the whole LDS may hold
+// spins).  Needs blockDim a multiple of 256.
+// ---------------------------------------------------------------------------
+template <bool DIM3, int DM, bool CONE>
+__global__ __launch_bounds__(1024) void k_samples_lds_c(const DevInst* __restrict__ insts,
+                                                        const CInst* __restrict__ cl, int n,
+                                                        float* __restrict__ pool,
+                                                        const uint32_t* __restrict__ cs_e,
+                                                        const uint32_t* __restrict__ cs_p,
+                                                        const uint32_t* __restrict__ cs_nb,
+                                                        const uint16_t* __restrict__ cs_mslot,
+                                                        const uint64_t* __restrict__ sweeps, Stencil st,
+                                                        int nsamples, int samp_sweeps, uint32_t seed_lo,
+                                                        uint32_t seed_hi, unsigned long long* __restrict__ acc,
+                                                        double* __restrict__ series, int series_n, int hidx,
+                                                        double* __restrict__ scratch,
+                                                        double* const* __restrict__ avgptr,
+                                                        const int* __restrict__ avgflag) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float* ls = (float*)lds_raw;
+    const int G = (int)(blockDim.x >> 8);
+    const int grp = (int)(threadIdx.x >> 8), vt = (int)(threadIdx.x & 255), lane = (int)(threadIdx.x & 63);
+    double* scr = scratch + (int64_t)blockIdx.x * kSampleScratch;
+    double* red = scr;                      // [group][wave][15]
+    double* part = scr + 4 * 4 * 15;        // [tile][15]
+    double* sums = part + kMaxCTiles * 15;  // [15]
+    for (int li = blockIdx.x; li < n; li += gridDim.x) {
+        const CInst ci = cl[li];
+        const DevInst& in = insts[ci.inst];
+        const int N = ci.n0 + ci.n1;
+        const uint32_t* E = cs_e + ci.site0;
+        const uint32_t* NB = cs_nb + 3 * ci.site0;
+        const uint16_t* MS = cs_mslot + ci.site0;
+        const uint32_t* P0 = cs_p + ci.pair0;
+        const int64_t Sp = in.Sp;
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            const float* p = pool + in.base + (j < ci.n0 ? 0 : 3 * Sp);
+            const uint32_t e = E[j];
+            ls[3 * j] = ld32(p, e); ls[3 * j + 1] = ld32(p + Sp, e); ls[3 * j + 2] = ld32(p + 2 * Sp, e);
+        }
+        if (threadIdx.x < 3) ls[3 * N + threadIdx.x] = 0.f;
+        __syncthreads();
+        const uint64_t sweep0 = sweeps[ci.inst];
+        const uint32_t k1 = seed_hi + in.inst_id;
+        const bool avg_on = avgflag != nullptr && avgflag[ci.inst] != 0;
+        const int nt = (N + kMeasureSites - 1) / kMeasureSites;
+        int nacc = 0;
+        for (int smp = 0; smp < nsamples; ++smp) {
+            for (int k = 0; k < samp_sweeps; ++k) {
+                const uint64_t sw = sweep0 + (uint64_t)smp * (uint64_t)samp_sweeps + (uint64_t)k;
+                const uint32_t slo = (uint32_t)sw, shi = (uint32_t)(sw >> 32);
+                for (int c = 0; c < 2; ++c) {
+                    const int nd = c ? ci.nd1 : ci.nd0, np = nd + (c ? ci.ns1 : ci.ns0);
+                    const uint32_t* P = P0 + (c ? ci.nd0 + ci.ns0 : 0);
+                    const uint32_t base = c ? (uint32_t)ci.n0 : 0u;
+                    for (int t = threadIdx.x; t < np; t += blockDim.x) {
+                        const uint32_t pe = P[t];
+                        const bool dbl = t < nd;
+                        uint32_t w[4];
+                        philox4x32_10(pe & 0x7FFFFFFFu, (uint32_t)c, slo, shi, seed_lo, k1, w);
+                        const bool hi = !dbl && (pe >> 31);
+                        nacc += csite_update<DIM3, DM, CONE>(base + (dbl ? t : nd + t), NB, ls, in, st,
+                                                             hi ? w[2] : w[0], hi ? w[3] : w[1]);
+                        if (dbl)
+                            nacc += csite_update<DIM3, DM, CONE>(base + nd + t, NB, ls, in, st, w[2], w[3]);
+                    }
+                    __syncthreads();
+                }
+            }
+            // the measurement, tile by tile (G tiles at once)
+            for (int t0 = 0; t0 < nt; t0 += G) {
+                const int t = t0 + grp;
+                double a[15];
+#pragma unroll
+                for (int i = 0; i < 15; ++i) a[i] = 0.0;
+                if (t < nt) {
+                    const int q0 = (int)((int64_t)N * t / nt), q1 = (int)((int64_t)N * (t + 1) / nt);
+                    for (int j = q0 + vt; j < q1; j += 256) {
+                        const uint32_t slot = MS[j];
+                        const int c = slot < (uint32_t)ci.n0 ? 0 : 1;
+                        const SitePos p = site_of(in, E[slot], c, DIM3);
+                        const uint32_t* nbw = NB + 3u * slot;
+                        double nb3[3][3];
+#pragma unroll
+                        for (int g = DIM3 ? 1 : 3; g < 6; g += 2) {
+                            const uint32_t o = (nbw[g >> 1] >> 16) & 0xFFFFu;   // slot g = 2m + 1: the high half
+                            nb3[g >> 1][0] = ls[o]; nb3[g >> 1][1] = ls[o + 1]; nb3[g >> 1][2] = ls[o + 2];
+                        }
+                        bond_site_sums<DIM3, DM>(in, st, p, (double)ls[3 * slot], (double)ls[3 * slot + 1],
+                                                 (double)ls[3 * slot + 2], nb3, a);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 15; ++i) a[i] = wave_sum(a[i]);
+                if (lane == 63 && t < nt)
+                    for (int i = 0; i < 15; ++i) red[(grp * 4 + (vt >> 6)) * 15 + i] = a[i];
+                __syncthreads();
+                if (vt < 15 && t < nt) {
+                    const double* r = red + grp * 4 * 15 + vt;
+                    part[t * 15 + vt] = ((r[0] + r[15]) + r[30]) + r[45];
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x < 15) sums[threadIdx.x] = tile_sum8(part + threadIdx.x, nt, 15);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                bond_obs_row(in, hidx, sums, series + ((int64_t)ci.inst * series_n + smp) * 11);
+            if (avg_on) {   // add_to_av: each present site once, absent ones add zero
+                double* av = avgptr[ci.inst];
+                for (int j = threadIdx.x; j < N; j += blockDim.x) {
+                    const int c = j < ci.n0 ? 0 : 1;
+                    const int64_t e = E[j];
+                    av[(c * 3 + 0) * Sp + e] += (double)ls[3 * j];
+                    av[(c * 3 + 1) * Sp + e] += (double)ls[3 * j + 1];
+                    av[(c * 3 + 2) * Sp + e] += (double)ls[3 * j + 2];
+                }
+            }
+            __syncthreads();
+        }
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            float* p = pool + in.base + (j < ci.n0 ? 0 : 3 * Sp);
+            const uint32_t e = E[j];
+            st32(p, e, ls[3 * j]); st32(p + Sp, e, ls[3 * j + 1]); st32(p + 2 * Sp, e, ls[3 * j + 2]);
+        }
+        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        __syncthreads();
+        int* pa = (int*)lds_raw;
+        if ((threadIdx.x & 63) == 0) pa[threadIdx.x >> 6] = nacc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += pa[w];
+            if (t) acc[ci.inst] += (unsigned long long)t;
+        }
+        __syncthreads();
+    }
+}
+
+// k_measure_sites' bond-form pass on streamed compacted copies: the same
+// tiles, order and reductions, the spins read from the copy (x, y, z of slot
+// i at 3i), so the partials equal k_measure_sites' on the written-back
+// lattice bit for bit.
+template <bool DIM3, int DM>
+__global__ __launch_bounds__(256) void k_measure_cs(const DevInst* __restrict__ insts, const CInst* __restrict__ xl,
+                                                    const CMTile* __restrict__ tiles,
+                                                    const uint32_t* __restrict__ cs_e,
+                                                    const uint32_t* __restrict__ cs_nb,
+                                                    const uint16_t* __restrict__ cs_mslot,
+                                                    const float* __restrict__ cspin, Stencil st,
+                                                    double* __restrict__ partials, int pstride) {
+#pragma clang fp contract(off)
+    __shared__ double red[4 * 15];
+    const CMTile tl = tiles[blockIdx.x];
+    const CInst ci = xl[tl.x];
+    const DevInst& in = insts[ci.inst];
+    const uint32_t* E = cs_e + ci.site0;
+    const uint32_t* NB = cs_nb + 3 * ci.site0;
+    const uint16_t* MS = cs_mslot + ci.site0;
+    const float* sp = cspin + ci.spin0;
+    double a[15];
+#pragma unroll
+    for (int i = 0; i < 15; ++i) a[i] = 0.0;
+    for (int j = threadIdx.x; j < tl.n; j += 256) {
+        const uint32_t slot = MS[tl.q0 + j];
+        const int c = slot < (uint32_t)ci.n0 ? 0 : 1;
+        const SitePos p = site_of(in, E[slot], c, DIM3);
+        const uint32_t* nbw = NB + 3u * slot;
+        double nb3[3][3];
+#pragma unroll
+        for (int g = DIM3 ? 1 : 3; g < 6; g += 2) {
+            const uint32_t o = 3u * ((nbw[g >> 1] >> 16) & 0xFFFFu);
+            nb3[g >> 1][0] = sp[o]; nb3[g >> 1][1] = sp[o + 1]; nb3[g >> 1][2] = sp[o + 2];
+        }
+        bond_site_sums<DIM3, DM>(in, st, p, (double)sp[3 * slot], (double)sp[3 * slot + 1], (double)sp[3 * slot + 2],
+                                 nb3, a);
+    }
+    const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int i = 0; i < 15; ++i) a[i] = wave_sum(a[i]);
+    if (lane == 63)
+        for (int i = 0; i < 15; ++i) red[wv * 15 + i] = a[i];
+    __syncthreads();
+    if (threadIdx.x < 15) {
+        const int i = (int)threadIdx.x;
+        partials[(int64_t)tl.pt * pstride + i] = ((red[i] + red[15 + i]) + red[30 + i]) + red[45 + i];
+    }
+}
+
+// add_to_av of streamed compacted instances from their copies: the slots of
+// the listed copy tiles whose instance is flagged.
+__global__ __launch_bounds__(256) void k_accum_avg_cs(const DevInst* __restrict__ insts,
+                                                      const CInst* __restrict__ xl,
+                                                      const XTile* __restrict__ tiles,
+                                                      const uint32_t* __restrict__ cs_e,
+                                                      const float* __restrict__ cspin,
+                                                      double* const* __restrict__ avgptr,
+                                                      const int* __restrict__ avgflag) {
+    const XTile tl = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tl.n) return;
+    const CInst ci = xl[tl.x];
+    if (!avgflag[ci.inst]) return;
+    const DevInst& in = insts[ci.inst];
+    const int j = tl.item0 + (int)threadIdx.x;
+    const int c = j < ci.n0 ? 0 : 1;
+    const int64_t e = cs_e[ci.site0 + j];
+    const float* v = cspin + ci.spin0 + 3 * (int64_t)j;
+    double* av = avgptr[ci.inst];
+    av[(c * 3 + 0) * in.Sp + e] += (double)v[0];
+    av[(c * 3 + 1) * in.Sp + e] += (double)v[1];
+    av[(c * 3 + 2) * in.Sp + e] += (double)v[2];
+}
+
 // Copies slots [item0, item0 + n) of compacted instance x between the
 // colour-split arrays and its compacted copy (TO_COPY: into the copy).
 template <bool TO_COPY>
@@ -1008,7 +1236,8 @@ hipError_t set_lds_limits(int shm, int* lds_lim, int* ldsc_lim) {
     *lds_lim = *ldsc_lim = shm;
 #define NMC_ATTR(K) \
     if (e == hipSuccess) e = set_one_limit((const void*)(K), shm, *lim);
-#define NMC_ATTR_C(D3, UD) NMC_ATTR((k_sweep_lds_c<D3, UD, false>)) NMC_ATTR((k_sweep_lds_c<D3, UD, true>))
+#define NMC_ATTR_C(D3, UD) NMC_ATTR((k_sweep_lds_c<D3, UD, false>)) NMC_ATTR((k_sweep_lds_c<D3, UD, true>)) \
+    NMC_ATTR((k_samples_lds_c<D3, UD, false>)) NMC_ATTR((k_samples_lds_c<D3, UD, true>))
     NMC_ATTR_C(true, 0) NMC_ATTR_C(true, 1) NMC_ATTR_C(true, 2)
     NMC_ATTR_C(false, 0) NMC_ATTR_C(false, 1) NMC_ATTR_C(false, 2)
     lim = lds_lim;
@@ -1073,6 +1302,56 @@ hipError_t launch_cstream_copy(hipStream_t stream, bool to_copy, const DevInst* 
     if (ntiles <= 0) return hipSuccess;
     if (to_copy) k_cstream_copy<true><<<ntiles, 256, 0, stream>>>(insts, xl, tiles, cs_e, pool, cspin);
     else k_cstream_copy<false><<<ntiles, 256, 0, stream>>>(insts, xl, tiles, cs_e, pool, cspin);
+    return hipGetLastError();
+}
+
+hipError_t launch_samples_lds_c(hipStream_t stream, bool dim3, int dm, bool cone, int grid, int block, size_t shm,
+                                const DevInst* insts, const CInst* cl, int n, float* pool, const uint32_t* cs_e,
+                                const uint32_t* cs_p, const uint16_t* cs_nb, const uint16_t* cs_mslot,
+                                const uint64_t* sweeps, const Stencil& st, int nsamples, int samp_sweeps,
+                                uint32_t seed_lo, uint32_t seed_hi, unsigned long long* acc, double* series,
+                                int series_n, int hidx, double* scratch, double* const* avgptr, const int* avgflag) {
+    if (n <= 0 || nsamples <= 0) return hipSuccess;
+    if (block % 256 != 0 || block > 1024) return hipErrorInvalidValue;
+    const uint32_t* nb = (const uint32_t*)cs_nb;
+#define NMC_SLC(D3, UD, CN)                                                                                      \
+    if (dim3 == D3 && dm == UD && cone == CN) {                                                                  \
+        k_samples_lds_c<D3, UD, CN><<<grid, block, shm, stream>>>(insts, cl, n, pool, cs_e, cs_p, nb, cs_mslot,   \
+                                                                  sweeps, st, nsamples, samp_sweeps, seed_lo,     \
+                                                                  seed_hi, acc, series, series_n, hidx, scratch,  \
+                                                                  avgptr, avgflag);                               \
+        return hipGetLastError();                                                                                 \
+    }
+#define NMC_SLC_D(D3, UD) NMC_SLC(D3, UD, false) NMC_SLC(D3, UD, true)
+    NMC_SLC_D(true, 0) NMC_SLC_D(true, 1) NMC_SLC_D(true, 2)
+    NMC_SLC_D(false, 0) NMC_SLC_D(false, 1) NMC_SLC_D(false, 2)
+#undef NMC_SLC_D
+#undef NMC_SLC
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_measure_cs(hipStream_t stream, bool dim3, int dm, const DevInst* insts, const CInst* xl,
+                             const CMTile* tiles, int ntiles, const uint32_t* cs_e, const uint16_t* cs_nb,
+                             const uint16_t* cs_mslot, const float* cspin, const Stencil& st, double* partials,
+                             int pstride) {
+    if (ntiles <= 0) return hipSuccess;
+    const uint32_t* nb = (const uint32_t*)cs_nb;
+#define NMC_MCS(D3, UD)                                                                                        \
+    if (dim3 == D3 && dm == UD) {                                                                              \
+        k_measure_cs<D3, UD><<<ntiles, 256, 0, stream>>>(insts, xl, tiles, cs_e, nb, cs_mslot, cspin, st,        \
+                                                         partials, pstride);                                   \
+        return hipGetLastError();                                                                              \
+    }
+    NMC_MCS(true, 0) NMC_MCS(true, 1) NMC_MCS(true, 2) NMC_MCS(false, 0) NMC_MCS(false, 1) NMC_MCS(false, 2)
+#undef NMC_MCS
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_accum_avg_cs(hipStream_t stream, const DevInst* insts, const CInst* xl, const XTile* tiles,
+                               int ntiles, const uint32_t* cs_e, const float* cspin, double* const* avgptr,
+                               const int* avgflag) {
+    if (ntiles <= 0) return hipSuccess;
+    k_accum_avg_cs<<<ntiles, 256, 0, stream>>>(insts, xl, tiles, cs_e, cspin, avgptr, avgflag);
     return hipGetLastError();
 }
 

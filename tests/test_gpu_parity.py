@@ -738,12 +738,15 @@ def test_invalid_arguments_raise(orc):
     assert heis.equil_adapt(0, 10).shape == (1,)
 
 
-@pytest.mark.parametrize("graphs", ["0", "1"])
-def test_sample_loop_with_averages_equals_the_calls(orc, bloch3d, monkeypatch, graphs):
+@pytest.mark.parametrize("graphs,charge", [("0", True), ("1", True), ("0", False), ("1", False)])
+def test_sample_loop_with_averages_equals_the_calls(orc, bloch3d, monkeypatch, graphs, charge):
     # nmc_run_samples_avg (the drop-in driver's whole sample loop, one graph)
     # against nmc_sweep + nmc_sample + nmc_accumulate_avgs per slot, on open
-    # grains (compacted in LDS and streamed from compacted copies) and a
-    # periodic lattice set
+    # grains (compacted in LDS and streamed from compacted copies).  Without
+    # charges the loop runs fused: the LDS-resident grains sweep and measure
+    # in one launch, the streamed one is measured from its compacted copy;
+    # series, averages, lattices and accept counts must still be the calls'
+    # bit for bit.
     from nanomagmc_amd import Bond, Lattices
     monkeypatch.setenv("NMC_NO_GRAPHS", "1" if graphs == "0" else "0")
     bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
@@ -756,22 +759,34 @@ def test_sample_loop_with_averages_equals_the_calls(orc, bloch3d, monkeypatch, g
         lat.set_params([0.5, 1.5, 3.0], [0.0, 0.1, 0.05])
         lat.randomize()
         lat.equil(20)
-        lat.series_reset(7, with_charge=True)
-        for i in (0, 2):
+        lat.series_reset(7, with_charge=charge)
+        avg = [0, 2] if charge else [0, 1, 2]
+        for i in avg:
             lat.zero_avg(i)
+        lat.reset_accepts()
         if way == "loop":
-            lat.run_samples_avg(7, 3, [0, 2])
+            lat.run_samples_avg(7, 3, avg)
         else:
             for slot in range(7):
                 lat.equil(3)
                 lat.sample(slot)
-                lat.accumulate_avgs([0, 2])
+                lat.accumulate_avgs(avg)
         lats.append(lat)
     a, b = lats
+    from nanomagmc_amd._lib import NMC_KIND_COMPACT_STREAM, NMC_KIND_LDS_COMPACT
+    assert a.sweep_kinds().tolist() == [NMC_KIND_LDS_COMPACT, NMC_KIND_COMPACT_STREAM, NMC_KIND_LDS_COMPACT]
     for x, y in zip(a.get_series(), b.get_series()):
         assert np.array_equal(x, y)
+    assert np.abs(a.get_series()[0][:, :, 10]).min() > 0
     for i in range(3):
         assert np.array_equal(a.get_spins(i), b.get_spins(i)), i
-    for i in (0, 2):
+    for i in avg:
         assert np.array_equal(a.get_avg(i), b.get_avg(i)), i
-    assert np.abs(a.get_avg(0)).sum() > 0
+        assert np.abs(a.get_avg(i)).sum() > 0
+    assert np.array_equal(a.accepts(), b.accepts())
+    # the loop continues the same random streams: one more sweep of each
+    # lattice stays equal
+    a.equil(2)
+    b.equil(2)
+    for i in range(3):
+        assert np.array_equal(a.get_spins(i), b.get_spins(i)), i
