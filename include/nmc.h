@@ -237,8 +237,11 @@ int nmc_get_series(nmc_ctx* ctx, const int* inst, int n, double* obs,
  * 0..nsamples-1, `samp_sweeps` sweeps then a measurement into that slot,
  * for the listed instances.  Recorded once per (instance list, nsamples,
  * samp_sweeps) as a HIP graph and replayed as one launch; the random stream
- * is the one nmc_sweep + nmc_sample would draw.  nsamples must not exceed
- * the series size of nmc_series_reset. */
+ * is the one nmc_sweep + nmc_sample would draw.  When every listed instance
+ * is a compacted shaped lattice and the series holds no charges, the loop
+ * runs fused (a grain resident in LDS sweeps and measures itself for the
+ * whole loop), with the same series and lattices bit for bit.  nsamples
+ * must not exceed the series size of nmc_series_reset. */
 int nmc_run_samples(nmc_ctx* ctx, const int* inst, int n, int nsamples, int samp_sweeps);
 
 /* The sample loop of one path point with the lattice average: for slot
