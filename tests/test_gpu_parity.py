@@ -738,6 +738,44 @@ def test_invalid_arguments_raise(orc):
     assert heis.equil_adapt(0, 10).shape == (1,)
 
 
+@pytest.mark.parametrize("table,trial", [("neel", "sphere"), ("general", "cone")])
+def test_fused_sample_loop_2d_equals_the_calls(table, trial):
+    # the fused sample loop on 2-D open grains (one compacted in LDS, one
+    # streamed from its compacted copy), axial and general DMI, sphere and
+    # cone trials: series and lattices equal the step-by-step loop's
+    from nanomagmc_amd import Bond, Lattices
+    from nanomagmc_amd._lib import NMC_KIND_COMPACT_STREAM, NMC_KIND_LDS_COMPACT
+    d, J, D = neel2d(0.3)
+    D = np.asarray(D)
+    if table == "general":
+        D = D + 0.1 * np.sign(D.sum(axis=1, keepdims=True))
+    bonds = [Bond(d[k][:2], J[k], D[k]) for k in range(4)]
+    rng = np.random.default_rng(17)
+    insts = [{"n": (n, n), "edge": n, "mask": (rng.random(n * n) < 0.8).astype(np.uint8)} for n in (41, 180)]
+    lats = []
+    for way in ("loop", "calls"):
+        lat = Lattices(2, False, False, bonds, insts, use_d=True, trial=trial, seed=23)
+        lat.set_params([0.8, 2.5], [0.06, 0.0])
+        if trial == "cone":
+            lat.set_cone([0.9, 0.6])
+        lat.randomize()
+        lat.equil(4)
+        lat.series_reset(5, with_charge=False)
+        if way == "loop":
+            lat.run_samples(5, 2)
+        else:
+            for slot in range(5):
+                lat.equil(2)
+                lat.sample(slot)
+        lats.append(lat)
+    a, b = lats
+    assert a.sweep_kinds().tolist() == [NMC_KIND_LDS_COMPACT, NMC_KIND_COMPACT_STREAM]
+    for x, y in zip(a.get_series(), b.get_series()):
+        assert np.array_equal(x, y)
+    for i in range(2):
+        assert np.array_equal(a.get_spins(i), b.get_spins(i)), i
+
+
 @pytest.mark.parametrize("graphs,charge", [("0", True), ("1", True), ("0", False), ("1", False)])
 def test_sample_loop_with_averages_equals_the_calls(orc, bloch3d, monkeypatch, graphs, charge):
     # nmc_run_samples_avg (the drop-in driver's whole sample loop, one graph)
