@@ -103,7 +103,7 @@ struct Set {
     CInst* d_cl = nullptr;         // shaped Heisenberg lattices swept compacted in LDS
     int n_cl = 0;
     uint32_t* d_cs_e = nullptr;    // their present sites (element index) ...
-    uint16_t* d_cs_nb = nullptr;   // ... and six neighbour float indices per site
+    uint16_t* d_cs_nb = nullptr;   // ... and six neighbour slots per site
     uint32_t* d_cs_p = nullptr;    // their pairs of colour elements with a present site
     CInst* d_xl = nullptr;         // open shaped lattices streamed from compacted copies (tables as above) ...
     int n_xl = 0;
@@ -704,7 +704,6 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                 PD += col ? ci.nd1 + ci.ns1 : ci.nd0 + ci.ns0;
             }
             const int zero = ci.n0 + ci.n1;
-            const int nbmul = q < ncl ? 3 : 1;   // LDS: float indices; streamed: slots
             for (int j = 0; j < zero; ++j) {
                 const int col = j < ci.n0 ? 0 : 1;
                 const int64_t e = E[j];
@@ -727,7 +726,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                         const int t = slot[col ^ 1][((int64_t)n0 * d.a1 + n1) * d.half + kk];
                         if (t >= 0) sv = t;
                     }
-                    NB[6 * j + g] = (uint16_t)(nbmul * sv);
+                    NB[6 * j + g] = (uint16_t)sv;
                 }
             }
         }

@@ -141,8 +141,7 @@ struct STile {
 // streamed sweeps of a compacted copy in HBM: its present sites, colour 0
 // first (n0 of them) then colour 1 (n1), stored from site0 on in the per-set
 // site tables (element index; six neighbours, each a site of the other
-// colour or n0 + n1 = the zero slot: its float index 3 x slot for the LDS
-// kernel, its slot for the streamed one), and its pairs of colour elements
+// colour, or n0 + n1 = the zero slot), and its pairs of colour elements
 // (2k, 2k + 1) with a present site,
 // colour 0 first, from pair0 on in the per-set pair table: per colour the
 // nd pairs with both elements present, then the ns with one, each entry

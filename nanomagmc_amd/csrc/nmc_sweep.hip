@@ -119,16 +119,18 @@ __device__ __forceinline__ bool heis_decide(const DevInst& in, float sx, float s
         // z = 2 ua - 1 with ua = (ka + 1/2) 2^-20, as one fused multiply-add
         // of exact operands (2^-20 - 1 is exact in fp32): one rounding of the
         // same real number
+        // |tz| <= 1 - 2^-20, so 1 - tz^2 (one rounding) is positive
         tz = __builtin_fmaf(ka, 1.0f / 524288.0f, 1.0f / 1048576.0f - 1.0f);
-        const float r = __builtin_amdgcn_sqrtf(fmaxf(0.f, __builtin_fmaf(-tz, tz, 1.f)));
+        const float r = __builtin_amdgcn_sqrtf(__builtin_fmaf(-tz, tz, 1.f));
         tx = r * cphi;
         ty = r * sphi;
     } else {
         // uniform on the cap of half-angle theta around s
         // (orthonormal frame: Duff et al., JCGT 6(1), 2017)
         const float ua = (ka + 0.5f) * (1.0f / 1048576.0f);   // (0, 1), centred on the 2^-20 grid
+        // ua < 1 and cone_w in [0, 2], so ca in [-1, 1] and 1 - ca^2 >= 0
         const float ca = __builtin_fmaf(-ua, in.cone_w, 1.f);
-        const float sa = __builtin_amdgcn_sqrtf(fmaxf(0.f, __builtin_fmaf(-ca, ca, 1.f)));
+        const float sa = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ca, ca, 1.f));
         const float sgn = sz >= 0.f ? 1.f : -1.f;
         const float a = -__builtin_amdgcn_rcpf(sgn + sz);   // |sgn + sz| >= 1
         const float b = sx * sy * a;
@@ -668,37 +670,49 @@ struct F3 {
     float x, y, z;
 };
 
+// LDS float at a byte address.  The compacted LDS-resident kernels keep no
+// other LDS than their dynamic area, which therefore starts at address 0.
+typedef __attribute__((address_space(3))) float lds_f32;
+
+template <bool HBM>
+__device__ __forceinline__ F3 ld_slot(const float* sp, uint32_t b) {
+    if (HBM) return *(const F3*)((const char*)sp + b);   // one 12-byte load
+    const lds_f32* q = (const lds_f32*)(size_t)b;
+    return F3{q[0], q[1], q[2]};
+}
+
+template <bool HBM>
+__device__ __forceinline__ void st_slot(float* sp, uint32_t b, float x, float y, float z) {
+    if (HBM) { *(F3*)((char*)sp + b) = F3{x, y, z}; return; }   // one 12-byte store
+    lds_f32* q = (lds_f32*)(size_t)b;
+    q[0] = x; q[1] = y; q[2] = z;
+}
+
 // One site of the other-colour-reading update: slot j of the compacted spins
-// ls (x, y, z of slot i at floats 3i .. 3i + 2; in LDS or in HBM), Philox
-// words wa, wb.  NB holds per slot its six neighbours, 16 bits each in three
-// aligned words, as float indices (NBMUL = 1: 3 x slot) or as slots
-// (NBMUL = 3).  Returns 1 if the trial was accepted.
-template <bool DIM3, int DM, bool CONE, uint32_t NBMUL = 1>
-__device__ __forceinline__ int csite_update(uint32_t j, const uint32_t* __restrict__ NB, float* ls,
-                                            const DevInst& in, const Stencil& st, uint32_t wa, uint32_t wb) {
-    const uint32_t* nbw = NB + 3u * j;
+// (x, y, z of slot i at floats 3i .. 3i + 2: in HBM at sp, or in LDS from
+// address 0), Philox words wa, wb.  NB holds per slot its six neighbour
+// slots, 16 bits each in three aligned words.  Every address is a byte
+// offset, 12 x a slot (one 24-bit multiply that also takes the 16-bit half).
+// Counts an accepted trial in nacc.
+template <bool DIM3, int DM, bool CONE, bool HBM = false>
+__device__ __forceinline__ void csite_update(uint32_t j, const uint32_t* __restrict__ NB, float* sp,
+                                             const DevInst& in, const Stencil& st, uint32_t wa, uint32_t wb,
+                                             int& nacc) {
+    const uint32_t jb = __umul24(j, 12u);   // byte offset of slot j and of its table words
+    const uint32_t* nbw = (const uint32_t*)((const char*)NB + jb);
     const uint32_t nw[3] = {nbw[0], nbw[1], nbw[2]};
-    const uint32_t b = 3u * j;
-    float sx, sy, sz;
-    if (NBMUL == 3) {   // HBM: one 12-byte load
-        const F3 v = *(const F3*)(ls + b);
-        sx = v.x; sy = v.y; sz = v.z;
-    } else {
-        sx = ls[b]; sy = ls[b + 1]; sz = ls[b + 2];
-    }
+    const F3 s = ld_slot<HBM>(sp, jb);
     float vx = 0.f, vy = 0.f, vz = 0.f;
 #pragma unroll
     for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-        const uint32_t o = NBMUL * ((nw[g >> 1] >> (16 * (g & 1))) & 0xFFFFu);
-        add_bond<DM, DIM3>(st, g, ls[o], ls[o + 1], ls[o + 2], vx, vy, vz);
+        const F3 n = ld_slot<HBM>(sp, __umul24((nw[g >> 1] >> (16 * (g & 1))) & 0xFFFFu, 12u));
+        add_bond<DM, DIM3>(st, g, n.x, n.y, n.z, vx, vy, vz);
     }
     float tx, ty, tz;
-    if (heis_decide<CONE>(in, sx, sy, sz, vx, vy, vz, wa, wb, tx, ty, tz)) {
-        if (NBMUL == 3) *(F3*)(ls + b) = F3{tx, ty, tz};   // HBM: one 12-byte store
-        else { ls[b] = tx; ls[b + 1] = ty; ls[b + 2] = tz; }
-        return 1;
+    if (heis_decide<CONE>(in, s.x, s.y, s.z, vx, vy, vz, wa, wb, tx, ty, tz)) {
+        st_slot<HBM>(sp, jb, tx, ty, tz);
+        ++nacc;
     }
-    return 0;
 }
 
 template <bool DIM3, int DM, bool CONE>
@@ -711,7 +725,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
                                                       const uint64_t* __restrict__ sweeps, Stencil st, int nsweeps,
                                                       uint32_t s_off, uint32_t seed_lo, uint32_t seed_hi,
                                                       unsigned long long* __restrict__ acc) {
-    // the only LDS of this kernel, so float index i is byte address 4 i
+    // the only LDS of this kernel: it starts at address 0 (csite_update)
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float* ls = (float*)lds_raw;
     for (int li = blockIdx.x; li < n; li += gridDim.x) {
@@ -745,9 +759,9 @@ __global__ __launch_bounds__(1024) void k_sweep_lds_c(const DevInst* __restrict_
                     philox4x32_10(pe & 0x7FFFFFFFu, (uint32_t)c, slo, shi, seed_lo, k1, w);
                     // doubles: element 2k at base + t; singles: at base + nd + t
                     const bool hi = !dbl && (pe >> 31);
-                    nacc += csite_update<DIM3, DM, CONE>(base + (dbl ? t : nd + t), NB, ls, in, st,
-                                                         hi ? w[2] : w[0], hi ? w[3] : w[1]);
-                    if (dbl) nacc += csite_update<DIM3, DM, CONE>(base + nd + t, NB, ls, in, st, w[2], w[3]);
+                    csite_update<DIM3, DM, CONE>(base + (dbl ? t : nd + t), NB, ls, in, st, hi ? w[2] : w[0],
+                                                 hi ? w[3] : w[1], nacc);
+                    if (dbl) csite_update<DIM3, DM, CONE>(base + nd + t, NB, ls, in, st, w[2], w[3], nacc);
                 }
                 __syncthreads();
             }
@@ -812,14 +826,12 @@ __global__ __launch_bounds__(256) void k_sweep_cstream(const DevInst* __restrict
         const bool hi = !dbl && (pe >> 31);
         const uint32_t* NB = cs_nb + 3 * ci.site0;
         float* sp = cspin + ci.spin0;
-        nacc = csite_update<DIM3, DM, CONE, 3>(base + (uint32_t)(dbl ? item : nd + item), NB, sp, in, st,
-                                               hi ? w[2] : w[0], hi ? w[3] : w[1]);
-        if (dbl) nacc += csite_update<DIM3, DM, CONE, 3>(base + (uint32_t)(nd + item), NB, sp, in, st, w[2], w[3]);
+        csite_update<DIM3, DM, CONE, true>(base + (uint32_t)(dbl ? item : nd + item), NB, sp, in, st,
+                                           hi ? w[2] : w[0], hi ? w[3] : w[1], nacc);
+        if (dbl) csite_update<DIM3, DM, CONE, true>(base + (uint32_t)(nd + item), NB, sp, in, st, w[2], w[3], nacc);
     }
-    if (COUNT) {
-        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
-        tile_count(nacc, acc, t);
-    }
+    // the wavefront's count from two lane masks (nacc is 0, 1 or 2 here)
+    if (COUNT) tile_count(__popcll(__ballot(nacc & 1)) + 2 * __popcll(__ballot(nacc & 2)), acc, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -856,6 +868,7 @@ __global__ __launch_bounds__(1024) void k_samples_lds_c(const DevInst* __restric
                                                         double* const* __restrict__ avgptr,
                                                         const int* __restrict__ avgflag) {
 #pragma clang fp contract(off)
+    // the only LDS of this kernel: it starts at address 0 (csite_update)
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float* ls = (float*)lds_raw;
     const int G = (int)(blockDim.x >> 8);
@@ -899,10 +912,9 @@ __global__ __launch_bounds__(1024) void k_samples_lds_c(const DevInst* __restric
                         uint32_t w[4];
                         philox4x32_10(pe & 0x7FFFFFFFu, (uint32_t)c, slo, shi, seed_lo, k1, w);
                         const bool hi = !dbl && (pe >> 31);
-                        nacc += csite_update<DIM3, DM, CONE>(base + (dbl ? t : nd + t), NB, ls, in, st,
-                                                             hi ? w[2] : w[0], hi ? w[3] : w[1]);
-                        if (dbl)
-                            nacc += csite_update<DIM3, DM, CONE>(base + nd + t, NB, ls, in, st, w[2], w[3]);
+                        csite_update<DIM3, DM, CONE>(base + (dbl ? t : nd + t), NB, ls, in, st, hi ? w[2] : w[0],
+                                                     hi ? w[3] : w[1], nacc);
+                        if (dbl) csite_update<DIM3, DM, CONE>(base + nd + t, NB, ls, in, st, w[2], w[3], nacc);
                     }
                     __syncthreads();
                 }
@@ -923,7 +935,7 @@ __global__ __launch_bounds__(1024) void k_samples_lds_c(const DevInst* __restric
                         double nb3[3][3];
 #pragma unroll
                         for (int g = DIM3 ? 1 : 3; g < 6; g += 2) {
-                            const uint32_t o = (nbw[g >> 1] >> 16) & 0xFFFFu;   // slot g = 2m + 1: the high half
+                            const uint32_t o = 3u * ((nbw[g >> 1] >> 16) & 0xFFFFu);   // slot g = 2m + 1: the high half
                             nb3[g >> 1][0] = ls[o]; nb3[g >> 1][1] = ls[o + 1]; nb3[g >> 1][2] = ls[o + 2];
                         }
                         bond_site_sums<DIM3, DM>(in, st, p, (double)ls[3 * slot], (double)ls[3 * slot + 1],
