@@ -17,7 +17,7 @@ The JSON line also carries:
 * roofline: the kernel family that takes the most time (each family timed
   alone on its share of the batch with HIP events on the context's stream),
   bounded by VALU issue for the LDS-resident kernels and by HBM for the
-  streaming ones, with its HBM traffic and VALU instruction counts from
+  streaming ones, with its HBM traffic and VALU issue slots from
   rocprofv3 counter runs of the same workload (child processes started
   before this process touches the GPU, calibrated by a copy kernel of known
   size); every family under `parts`, the whole step under `step`;
@@ -45,8 +45,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-# wave64 VALU instructions per second: 256 CUs x 4 SIMDs, each issuing one
-# wave64 instruction per 4 cycles (16 lanes) at 2.4 GHz
+# wave64 VALU issue slots per second: 256 CUs x 4 SIMDs, each issuing one
+# wave64 instruction per 4 cycles (16 lanes) at 2.4 GHz.  An fp32
+# transcendental (v_sin, v_cos, v_sqrt, v_exp, ...) takes two slots: on this
+# GPU a stream of them issues at half the rate of v_fma_f32 or v_mad_u64_u32
 VALU_PEAK = 256 * 4 * 2.4e9 / 4
 # algorithmic bytes per update: a spin is read as a target, read as a
 # neighbour and written once per sweep (3 x 12 B fp32), plus one presence bit
@@ -119,7 +121,8 @@ def workload_args(args):
 
 def measure_counters(args, launch, steps=6, warmup=2):
     """HBM bytes (FETCH_SIZE + WRITE_SIZE, each scaled by what it reads for
-    device copies of 256 MiB) and VALU instructions per dispatch of every
+    device copies of 256 MiB) and VALU issue slots (instructions, one more
+    per fp32 transcendental) per dispatch of every
     sweep kernel family, from counter runs of this workload with `steps`
     timed sweeps after `warmup`.  For the LDS-resident kernels (one dispatch
     per call) the per-sweep slope and the per-call intercept come from the
@@ -136,7 +139,7 @@ def measure_counters(args, launch, steps=6, warmup=2):
     try:
         runs = {}
         for name, ctrs, ch in (("fetch", ["FETCH_SIZE"], child), ("write", ["WRITE_SIZE"], child),
-                               ("valu", ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "GRBM_GUI_ACTIVE"], child),
+                               ("valu", ["SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "GRBM_GUI_ACTIVE"], child),
                                ("cfetch", ["FETCH_SIZE"], ["--copy-calibration"]),
                                ("cwrite", ["WRITE_SIZE"], ["--copy-calibration"])):
             runs[name] = _run_pmc(ctrs, ch, os.path.join(tmp, name))
@@ -157,6 +160,10 @@ def measure_counters(args, launch, steps=6, warmup=2):
         f = _per_kernel(runs["fetch"], prefix, "FETCH_SIZE")
         w = _per_kernel(runs["write"], prefix, "WRITE_SIZE")
         v = _per_kernel(runs["valu"], prefix, "SQ_INSTS_VALU")
+        tr = _per_kernel(runs["valu"], prefix, "SQ_INSTS_VALU_TRANS_F32")
+        # issue slots: every instruction one, a transcendental one more
+        if len(tr) == len(v):
+            v = [a + b for a, b in zip(v, tr)]
         if not f:
             continue
         hbm = [1024.0 * (a / ffac + b / wfac) for a, b in zip(f, w)]
@@ -535,8 +542,8 @@ def main():
                 part["traffic"] = c["hbm_bytes_timed_call"] + c["hbm_bytes_per_sweep"] * (sweeps_per_launch - steps_c)
             if "valu_per_sweep" in c:
                 part["valu_per_launch"] = c["valu_per_call"] + c["valu_per_sweep"] * sweeps_per_launch
-                part["valu_Ginst_per_s"] = part["valu_per_launch"] / (part["avg_launch_us"] * 1e-6) / 1e9
-                part["valu_frac"] = part["valu_Ginst_per_s"] * 1e9 / VALU_PEAK
+                part["valu_Gslots_per_s"] = part["valu_per_launch"] / (part["avg_launch_us"] * 1e-6) / 1e9
+                part["valu_frac"] = part["valu_Gslots_per_s"] * 1e9 / VALU_PEAK
         else:
             # one colour pass per launch; the time includes, spread over the
             # sweeps of the call, any per-call copies of the family
@@ -548,8 +555,8 @@ def main():
                 part["traffic"] = c["hbm_bytes_per_dispatch"]
             if "valu_per_dispatch" in c:
                 part["valu_per_launch"] = c["valu_per_dispatch"]
-                part["valu_Ginst_per_s"] = part["valu_per_launch"] / (part["avg_launch_us"] * 1e-6) / 1e9
-                part["valu_frac"] = part["valu_Ginst_per_s"] * 1e9 / VALU_PEAK
+                part["valu_Gslots_per_s"] = part["valu_per_launch"] / (part["avg_launch_us"] * 1e-6) / 1e9
+                part["valu_frac"] = part["valu_Gslots_per_s"] * 1e9 / VALU_PEAK
             # a streaming pass is bound by whichever of HBM and VALU issue it
             # uses the larger share of
             if part.get("valu_frac", 0.0) > part["hbm_frac"]:
@@ -563,7 +570,7 @@ def main():
             "counters": counters if counters else {"error": counters_err}}
     if dom is not None:
         if dom["bound"] == "valu":
-            ach = dom.get("valu_Ginst_per_s")
+            ach = dom.get("valu_Gslots_per_s")
             if dom["family"] in ("lds", "lds_compact"):
                 what = ("%s: %d instances, %d sites, %d sweeps per launch (%s dispatches, one per size class, back "
                         "to back)" % (dom["kernel"], dom["instances"], dom["sites"],
@@ -571,11 +578,12 @@ def main():
             else:
                 what = ("%s: one colour pass of %d instances (%d updates)"
                         % (dom["kernel"], dom["instances"], dom["sites"] // 2))
-            roof.update({"bound": "valu", "achieved": ach, "peak": VALU_PEAK / 1e9, "unit": "Ginst/s",
+            roof.update({"bound": "valu", "achieved": ach, "peak": VALU_PEAK / 1e9, "unit": "G issue slots/s",
                          "frac": None if ach is None else ach * 1e9 / VALU_PEAK,
                          "traffic": dom.get("traffic"),
-                         "kernel": what + "; achieved = wave64 VALU instructions per launch (counters, the family "
-                                          "swept alone) / launch time (events, the family alone)",
+                         "kernel": what + "; achieved = wave64 VALU issue slots per launch (instructions, one more "
+                                          "per fp32 transcendental; counters, the family swept alone) / launch "
+                                          "time (events, the family alone)",
                          "avg_launch_us": dom["avg_launch_us"]})
         else:
             ach = dom["alg_bytes_per_launch"] / (dom["avg_launch_us"] * 1e-6)
