@@ -437,7 +437,7 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             const uint32_t* mw = c->h_mask.data() + d.mask_base;
             for (int64_t w = 0; w < 2 * d.mask_stride; ++w) np += __builtin_popcount(mw[w]);
             fits_compact = c->path == NMC_PATH_AUTO &&
-                           (size_t)(np + 1) * 12 + kStreamLdsReserve <= c->max_lds_c && 3 * (np + 1) <= 65535;
+                           (size_t)(np + 1) * 12 + kStreamLdsReserve <= c->max_lds_c && np + 1 <= 65536;
             // too large for LDS: a compacted copy in HBM, neighbours as
             // 16-bit slots (the zero slot np included)
             fits_xstream = np + 1 <= 65536;

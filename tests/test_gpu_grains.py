@@ -222,3 +222,82 @@ def test_full_tile_ising_passes_match_oracle_sweeps(orc):
     assert np.array_equal(lat.get_spins(0), s)
     obs, _, _ = lat.measure()
     assert obs[0, 9] == f.energy(s, [h, 0, 0])
+
+
+def test_lds_capacity_boundary_and_degenerate_grains(orc, bloch3d):
+    # open 3-D grains in a 32^3 box at the compacted kernels' limits:
+    # 13,567 present sites (the most one workgroup's LDS holds on MI355X:
+    # 12 B a site plus the zero slot, with 1 KB to spare) and one more
+    # (streamed from a compacted copy instead), a single site, 40 sites of
+    # one colour only, and no site at all.  AUTO, STREAM (packed row spans)
+    # and CSTREAM give the same lattices and accept counts bit for bit, the
+    # fused sample loop the step-by-step loop's series; the lone site's
+    # energy is its Zeeman term alone, the empty grain's observables zero.
+    from nanomagmc_amd import Bond, Lattices
+    from nanomagmc_amd._lib import NMC_KIND_COMPACT_STREAM, NMC_KIND_LDS_COMPACT, NMC_KIND_SPANS
+    n = 32
+    rng = np.random.default_rng(41)
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
+    bonds = [Bond(bd[k][:3], J[k], D[k]) for k in range(len(J))]
+
+    def mask_of(sites):
+        m = np.zeros(n ** 3, np.uint8)
+        m[sites] = 1
+        return m
+
+    idx = np.arange(n ** 3)                          # box-scan order, last coordinate fastest
+    even = np.flatnonzero((idx // (n * n) + (idx // n) % n + idx % n) % 2 == 0)
+    perm = rng.permutation(n ** 3)
+    masks = [mask_of(perm[:13567]), mask_of(perm[:13568]), mask_of(perm[:1]),
+             mask_of(rng.choice(even, 40, replace=False)), np.zeros(n ** 3, np.uint8)]
+    insts = [{"n": (n, n, n), "edge": n, "mask": m} for m in masks]
+    beta, h = [1.1, 1.0, 2.0, 0.7, 1.5], [0.05, 0.0, 0.1, -0.05, 0.02]
+    runs = []
+    for path in ("auto", "stream", "cstream"):
+        lat = Lattices(3, False, False, bonds, insts, use_d=True, trial="sphere", seed=57)
+        lat.set_path(path)
+        lat.set_params(beta, h)
+        lat.randomize()
+        lat.equil(3)
+        lat.equil(30)
+        runs.append(lat)
+    a = runs[0]
+    assert [a.num_sites(i) for i in range(5)] == [13567, 13568, 1, 40, 0]
+    assert a.sweep_kinds().tolist() == [NMC_KIND_LDS_COMPACT, NMC_KIND_COMPACT_STREAM] + [NMC_KIND_LDS_COMPACT] * 3
+    assert runs[1].sweep_kinds().tolist() == [NMC_KIND_SPANS] * 5
+    assert runs[2].sweep_kinds().tolist() == [NMC_KIND_COMPACT_STREAM] * 5
+    for r in runs[1:]:
+        for i in range(4):
+            assert np.array_equal(a.get_spins(i), r.get_spins(i)), i
+        assert np.array_equal(a.accepts(), r.accepts())
+    acc = a.accepts()
+    assert acc[4] == 0 and acc[2] > 0
+    for i in range(4):
+        s = a.get_spins(i)
+        assert np.allclose(np.linalg.norm(s, axis=1), 1.0, atol=1e-6)
+    obs, _, _ = a.measure()
+    s2 = a.get_spins(2)[0]
+    assert obs[2, 10] == pytest.approx(-h[2] * s2[2], abs=1e-12)
+    assert obs[2, 9] == pytest.approx(-h[2] * s2[2], abs=1e-12)
+    assert np.all(obs[4] == 0.0)
+    # the fused sample loop (every grain compacted, no charges) against the
+    # step-by-step loop, from the same state
+    ways = []
+    for way in ("loop", "calls"):
+        lat = Lattices(3, False, False, bonds, insts, use_d=True, trial="sphere", seed=57)
+        lat.set_params(beta, h)
+        lat.randomize()
+        lat.equil(3)
+        lat.series_reset(4, with_charge=False)
+        if way == "loop":
+            lat.run_samples(4, 3)
+        else:
+            for slot in range(4):
+                lat.equil(3)
+                lat.sample(slot)
+        ways.append(lat)
+    for x, y in zip(ways[0].get_series(), ways[1].get_series()):
+        assert np.array_equal(x, y)
+    assert np.all(ways[0].get_series()[0][4] == 0.0)
+    for i in range(4):
+        assert np.array_equal(ways[0].get_spins(i), ways[1].get_spins(i)), i
