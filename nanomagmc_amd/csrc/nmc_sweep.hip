@@ -1230,11 +1230,14 @@ static hipError_t lds_c_one(hipStream_t stream, int grid, int block, size_t shm,
 // is not a stream operation, so it is set once per device when a context is
 // created, never from a launcher: launchers also run while the context's
 // stream is being captured into a graph.  Returns the smallest limit over the
-// instances of each kernel.
-static hipError_t set_one_limit(const void* k, int shm, int& lim) {
+// instances of each kernel.  The compacted kernels address their dynamic
+// LDS from byte 0 (lds_f32), which holds only while they have no static LDS:
+// for them (no_static) any static LDS is an error.
+static hipError_t set_one_limit(const void* k, int shm, int& lim, bool no_static) {
     hipFuncAttributes a{};
     hipError_t e = hipFuncGetAttributes(&a, k);
     if (e != hipSuccess) return e;
+    if (no_static && a.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
     const int dyn = shm - (int)a.sharedSizeBytes;
     if (dyn <= 0) return hipErrorInvalidValue;
     e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
@@ -1245,14 +1248,16 @@ static hipError_t set_one_limit(const void* k, int shm, int& lim) {
 hipError_t set_lds_limits(int shm, int* lds_lim, int* ldsc_lim) {
     hipError_t e = hipSuccess;
     int* lim = ldsc_lim;
+    bool no_static = true;
     *lds_lim = *ldsc_lim = shm;
 #define NMC_ATTR(K) \
-    if (e == hipSuccess) e = set_one_limit((const void*)(K), shm, *lim);
+    if (e == hipSuccess) e = set_one_limit((const void*)(K), shm, *lim, no_static);
 #define NMC_ATTR_C(D3, UD) NMC_ATTR((k_sweep_lds_c<D3, UD, false>)) NMC_ATTR((k_sweep_lds_c<D3, UD, true>)) \
     NMC_ATTR((k_samples_lds_c<D3, UD, false>)) NMC_ATTR((k_samples_lds_c<D3, UD, true>))
     NMC_ATTR_C(true, 0) NMC_ATTR_C(true, 1) NMC_ATTR_C(true, 2)
     NMC_ATTR_C(false, 0) NMC_ATTR_C(false, 1) NMC_ATTR_C(false, 2)
     lim = lds_lim;
+    no_static = false;
 #define NMC_ATTR_L(D3, PER, MSK)                                                                     \
     NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 0, false>)) NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 0, true>)) \
     NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 1, false>)) NMC_ATTR((k_sweep_lds<false, D3, PER, MSK, 1, true>)) \
