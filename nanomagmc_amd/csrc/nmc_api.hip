@@ -442,9 +442,11 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
             // 16-bit slots (the zero slot np included)
             fits_xstream = np + 1 <= 65536;
         }
-        // both colour arrays, then (masked lattices) both colours' presence bits
+        // both colour arrays, then (masked lattices) both colours' presence
+        // bits, then (Ising) the acceptance table
         const size_t bytes = (size_t)(c->ising ? 2 * d.Sp : 6 * d.Sp * (int64_t)sizeof(float)) +
-                             (c->masked ? (size_t)(2 * d.mask_stride * 4 + 15) / 16 * 16 : 0);
+                             (c->masked ? (size_t)(2 * d.mask_stride * 4 + 15) / 16 * 16 : 0) +
+                             (c->ising ? (size_t)c->tabsize * sizeof(uint32_t) : 0);
         if (fits_compact) {
             cinst.push_back(i);
             s->kind[li] = NMC_KIND_LDS_COMPACT;

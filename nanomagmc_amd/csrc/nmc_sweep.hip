@@ -285,46 +285,95 @@ __device__ __forceinline__ int heis_quad(const DevInst& in, const Stencil& st, c
     return nacc;
 }
 
+// Entries of an instance's Ising acceptance table: 2 x 3^z neighbourhoods
+// (z = 6 in 3-D, 4 in 2-D; absent neighbours count as 0).
+template <bool DIM3>
+constexpr int kIsingTab = DIM3 ? 2 * 729 : 2 * 81;
+
+// The fixed geometry of the Ising pair (elements 2P, 2P + 1 of one colour):
+// which elements are present, and the elements of their stencil neighbours
+// in the other colour (-1: outside an open box).  A work-item that keeps its
+// pair for a whole launch computes it once.
+template <bool DIM3>
+struct IsingPair {
+    bool live[2];
+    int ne[2][6];
+};
+
+template <bool DIM3, bool PERIODIC, bool MASKED>
+__device__ __forceinline__ IsingPair<DIM3> ising_pair(const DevInst& in, const uint32_t* mw, uint32_t P, int colour) {
+    IsingPair<DIM3> g;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const uint32_t e = 2 * P + j;
+        g.live[j] = false;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) g.ne[j][d] = -1;
+        if (e >= (uint32_t)in.S) continue;
+        const SitePos p = site_of(in, e, colour, DIM3);
+        bool present = p.i2 < in.n2;
+        if (MASKED && present) present = mask_bit_s(mw, in, colour, e);
+        g.live[j] = present;
+#pragma unroll
+        for (int d = DIM3 ? 0 : 2; d < 6; ++d) g.ne[j][d] = nbr_elem<PERIODIC>(in, p, d);
+    }
+    return g;
+}
+
 // Two Ising updates (elements 2P and 2P+1 of one colour) from one Philox
 // block.  The move is the reference's fresh +-1 trial; a trial equal to the
 // current spin is the dE = 0 no-op the reference accepts.  A flip is
-// accepted iff the 32-bit uniform k satisfies k <= T[idx], where T is the
-// host-built threshold of -dE*beta > log(k * 2^-32) for the neighbourhood
-// idx -- the reference's double-precision decision, exactly
-// (state.cpp:190-201).  Returns the number of accepted attempts.
+// accepted iff the 32-bit uniform k satisfies k <= T[idx], where T (`tab`,
+// the instance's table) is the host-built threshold of
+// -dE*beta > log(k * 2^-32) for the neighbourhood idx -- the reference's
+// double-precision decision, exactly (state.cpp:190-201).  Returns the
+// number of accepted attempts.
+template <bool DIM3, bool PERIODIC>
+__device__ __forceinline__ int ising_pair_update(const IsingPair<DIM3>& g, const DevInst& in, const Stencil& st,
+                                                 int8_t* own, const int8_t* oth, const uint32_t* __restrict__ tab,
+                                                 uint32_t P, int colour, uint64_t sweep, uint32_t seed_lo,
+                                                 uint32_t seed_hi) {
+    uint32_t w[4];
+    philox4x32_10(P, (uint32_t)colour, (uint32_t)sweep, (uint32_t)(sweep >> 32), seed_lo, seed_hi + in.inst_id, w);
+    // both elements' reads before either write: they are distinct elements
+    // of one colour, and a write changes only its own element, so the two
+    // table lookups are in flight together
+    int nacc = 0;
+    bool flip[2] = {false, false};
+    int trial[2] = {0, 0}, idx[2] = {0, 0};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (!g.live[j]) continue;
+        const int s = ld32(own, 2 * P + j);
+        trial[j] = ((w[2] >> j) & 1u) ? 1 : -1;
+        if (trial[j] == s) { ++nacc; continue; }   // the dE = 0 no-op, accepted
+        int k = 0;
+#pragma unroll
+        for (int d = DIM3 ? 0 : 2; d < 6; ++d) {
+            const int ne = g.ne[j][d];
+            const int sn = (PERIODIC || ne >= 0) ? (int)ld32(oth, (uint32_t)ne) : 0;
+            k += (sn + 1) * st.pow3[d];
+        }
+        idx[j] = 2 * k + (s > 0 ? 1 : 0);
+        flip[j] = true;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (flip[j] && w[j] <= tab[idx[j]]) {
+            st32(own, 2 * P + j, (int8_t)trial[j]);
+            ++nacc;
+        }
+    }
+    return nacc;
+}
+
 template <bool DIM3, bool PERIODIC, bool MASKED>
 __device__ __forceinline__ int ising_update2(const DevInst& in, const Stencil& st, int8_t* own, const int8_t* oth,
                                              const uint32_t* __restrict__ tab, const uint32_t* mw,
                                              uint32_t P, int colour, uint64_t sweep, uint32_t seed_lo,
                                              uint32_t seed_hi) {
-    uint32_t w[4];
-    philox4x32_10(P, (uint32_t)colour, (uint32_t)sweep, (uint32_t)(sweep >> 32), seed_lo, seed_hi + in.inst_id, w);
-    int nacc = 0;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const uint32_t e = 2 * P + j;
-        if (e >= (uint32_t)in.S) break;
-        const SitePos p = site_of(in, e, colour, DIM3);
-        bool present = p.i2 < in.n2;
-        if (MASKED && present) present = mask_bit_s(mw, in, colour, e);
-        if (!present) continue;
-        const int s = ld32(own, e);
-        const int trial = ((w[2] >> j) & 1u) ? 1 : -1;
-        if (trial == s) { ++nacc; continue; }
-        int idx = 0;
-#pragma unroll
-        for (int g = DIM3 ? 0 : 2; g < 6; ++g) {
-            const int ne = nbr_elem<PERIODIC>(in, p, g);
-            const int sn = (PERIODIC || ne >= 0) ? (int)ld32(oth, (uint32_t)ne) : 0;
-            idx += (sn + 1) * st.pow3[g];
-        }
-        idx = 2 * idx + (s > 0 ? 1 : 0);
-        if (w[j] <= tab[in.tab_base + idx]) {
-            st32(own, e, (int8_t)trial);
-            ++nacc;
-        }
-    }
-    return nacc;
+    return ising_pair_update<DIM3, PERIODIC>(ising_pair<DIM3, PERIODIC, MASKED>(in, mw, P, colour), in, st, own, oth,
+                                             tab, P, colour, sweep, seed_lo, seed_hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -565,8 +614,8 @@ __global__ __launch_bounds__(256) void k_sweep_ising(const DevInst* __restrict__
     const int8_t* oth = pool + in.base + (int64_t)(colour ^ 1) * in.Sp;
     int nacc = 0;
     if (2 * P < (uint32_t)in.S)
-        nacc = ising_update2<DIM3, PERIODIC, MASKED>(in, st, own, oth, tab, mpool + in.mask_base, P, colour,
-                                                     sweeps[tl.inst] + s_off, seed_lo, seed_hi);
+        nacc = ising_update2<DIM3, PERIODIC, MASKED>(in, st, own, oth, tab + in.tab_base, mpool + in.mask_base, P,
+                                                     colour, sweeps[tl.inst] + s_off, seed_lo, seed_hi);
     if (COUNT) {
         int v = nacc;   // per-wave sum of accepted attempts (0..2 per lane)
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -597,21 +646,42 @@ __global__ __launch_bounds__(1024) void k_sweep_lds(const DevInst* __restrict__ 
         const uint4* src = (const uint4*)((const unsigned char*)pool + in.base * (ISING ? 1 : 4));
         uint4* dst = (uint4*)lds_raw;
         for (int64_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) dst[i] = src[i];
-        // the presence bits of both colours follow the spins in LDS
+        // the presence bits of both colours follow the spins in LDS, and
+        // (Ising) the instance's acceptance table follows them: a lookup
+        // waits for LDS instead of L2
         uint32_t* lmask = (uint32_t*)(lds_raw + bytes);
         if (MASKED)
             for (int64_t i = threadIdx.x; i < 2 * in.mask_stride; i += blockDim.x) lmask[i] = mpool[in.mask_base + i];
+        uint32_t* ltab = (uint32_t*)(lds_raw + bytes + (MASKED ? (2 * in.mask_stride * 4 + 15) / 16 * 16 : 0));
+        if (ISING)
+            for (int i = threadIdx.x; i < kIsingTab<DIM3>; i += blockDim.x) ltab[i] = tab[in.tab_base + i];
         __syncthreads();
         const uint64_t sweep0 = sweeps[inst] + s_off;
         int nacc = 0;
+        // Ising with at most one pair per work-item and colour (the whole
+        // lattice in one pass of the workgroup): the pairs' geometry once
+        const uint32_t P1 = threadIdx.x;
+        const bool one = ISING && 2u * blockDim.x >= (uint32_t)in.S;
+        IsingPair<DIM3> g1[2];
+        if (one && 2 * P1 < (uint32_t)in.S) {
+            g1[0] = ising_pair<DIM3, PERIODIC, MASKED>(in, lmask, P1, 0);
+            g1[1] = ising_pair<DIM3, PERIODIC, MASKED>(in, lmask, P1, 1);
+        }
         for (int s = 0; s < nsweeps; ++s) {
+#pragma unroll
             for (int c = 0; c < 2; ++c) {
                 if (ISING) {
                     int8_t* own = (int8_t*)lds_raw + c * Sp;
                     const int8_t* oth = (const int8_t*)lds_raw + (c ^ 1) * Sp;
-                    for (uint32_t P = threadIdx.x; 2 * P < (uint32_t)in.S; P += blockDim.x)
-                        nacc += ising_update2<DIM3, PERIODIC, MASKED>(in, st, own, oth, tab, lmask, P, c,
-                                                                      sweep0 + s, seed_lo, seed_hi);
+                    if (one) {
+                        if (2 * P1 < (uint32_t)in.S)
+                            nacc += ising_pair_update<DIM3, PERIODIC>(g1[c], in, st, own, oth, ltab, P1, c,
+                                                                       sweep0 + s, seed_lo, seed_hi);
+                    } else {
+                        for (uint32_t P = threadIdx.x; 2 * P < (uint32_t)in.S; P += blockDim.x)
+                            nacc += ising_update2<DIM3, PERIODIC, MASKED>(in, st, own, oth, ltab, lmask, P, c,
+                                                                          sweep0 + s, seed_lo, seed_hi);
+                    }
                 } else {
                     float* own = (float*)lds_raw + c * 3 * Sp;
                     const float* oth = (const float*)lds_raw + (c ^ 1) * 3 * Sp;
