@@ -269,6 +269,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 // C remainder semantics (truncating), as the reference's % on int.
 __device__ __forceinline__ int crem(int a, int b) { return a % b; }
 
+// Sums the bond form uses of the measurement's 15: M (3), subM (3), sub4M
+// (3), the bond sums with +D and -D; the other four stay exact zeros.
+constexpr int kBondSums = 11;
+
 // The bond-form observable sums of one Heisenberg site (the k_measure_sites
 // pass for tables with J equal and D opposite for a bond and its reverse),
 // from values: spin s, the spins n[m] of its neighbours in the + stencil

@@ -207,18 +207,21 @@ __device__ __forceinline__ void site_charge(const DevInst& in, const void* __res
 }
 
 // Fixed-order workgroup reduction of the 15 scalars: wave sums, then the four
-// waves in order, into out[0..14].  Ends with the workgroup synchronised.
+// waves in order, into out[0..14].  Only the first NS are reduced; the rest
+// are exact zeros (the bond form leaves acc 11-14 at 0) and are written as
+// such.  Ends with the workgroup synchronised.
+template <int NS = 15>
 __device__ __forceinline__ void block_reduce_out(double* acc, double* red, int wv, int lane,
                                                  double* __restrict__ out) {
 #pragma clang fp contract(off)
 #pragma unroll
-    for (int i = 0; i < 15; ++i) acc[i] = wave_sum(acc[i]);
+    for (int i = 0; i < NS; ++i) acc[i] = wave_sum(acc[i]);
     if (lane == 63)
-        for (int i = 0; i < 15; ++i) red[wv * 17 + i] = acc[i];
+        for (int i = 0; i < NS; ++i) red[wv * 17 + i] = acc[i];
     __syncthreads();
     if (threadIdx.x < 15) {
         const int i = threadIdx.x;
-        out[i] = ((red[i] + red[17 + i]) + red[34 + i]) + red[51 + i];
+        out[i] = i < NS ? ((red[i] + red[17 + i]) + red[34 + i]) + red[51 + i] : 0.0;
     }
 }
 
@@ -292,7 +295,7 @@ __global__ __launch_bounds__(256) void k_measure(const DevInst* __restrict__ ins
             site_sums<DIM3, PERIODIC, ISING, DM, BOND>(in, pool, st, c, e, p, acc);
         });
         if (!BOND && DM == 0) { acc[12] = acc[9]; acc[13] = acc[10]; acc[14] = acc[11]; }
-        block_reduce_out(acc, red, wv, lane, out);
+        block_reduce_out<BOND ? kBondSums : 15>(acc, red, wv, lane, out);
     }
     if (!TC) return;
     // pass 2: the charges (per slice in LDS for 3-D)
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(256) void k_measure_sites(const DevInst* __restrict
             site_sums<DIM3, PERIODIC, ISING, DM, BOND>(in, pool, st, c, e, site_of(in, e, c, DIM3), acc);
         }
         if (!BOND && DM == 0) { acc[12] = acc[9]; acc[13] = acc[10]; acc[14] = acc[11]; }
-        block_reduce_out(acc, red, wv, lane, out);
+        block_reduce_out<BOND ? kBondSums : 15>(acc, red, wv, lane, out);
     }
     if (!TC) return;
     // pass 2: the charges (3-D: per site into LDS at its slice rank)

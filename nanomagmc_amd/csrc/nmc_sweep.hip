@@ -1013,17 +1013,17 @@ __global__ __launch_bounds__(1024) void k_samples_lds_c(const DevInst* __restric
                     }
                 }
 #pragma unroll
-                for (int i = 0; i < 15; ++i) a[i] = wave_sum(a[i]);
+                for (int i = 0; i < kBondSums; ++i) a[i] = wave_sum(a[i]);
                 if (lane == 63 && t < nt)
-                    for (int i = 0; i < 15; ++i) red[(grp * 4 + (vt >> 6)) * 15 + i] = a[i];
+                    for (int i = 0; i < kBondSums; ++i) red[(grp * 4 + (vt >> 6)) * 15 + i] = a[i];
                 __syncthreads();
-                if (vt < 15 && t < nt) {
+                if (vt < kBondSums && t < nt) {
                     const double* r = red + grp * 4 * 15 + vt;
                     part[t * 15 + vt] = ((r[0] + r[15]) + r[30]) + r[45];
                 }
                 __syncthreads();
             }
-            if (threadIdx.x < 15) sums[threadIdx.x] = tile_sum8(part + threadIdx.x, nt, 15);
+            if (threadIdx.x < kBondSums) sums[threadIdx.x] = tile_sum8(part + threadIdx.x, nt, 15);
             __syncthreads();
             if (threadIdx.x == 0)
                 bond_obs_row(in, hidx, sums, series + ((int64_t)ci.inst * series_n + smp) * 11);
@@ -1098,13 +1098,14 @@ __global__ __launch_bounds__(256) void k_measure_cs(const DevInst* __restrict__ 
     }
     const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
 #pragma unroll
-    for (int i = 0; i < 15; ++i) a[i] = wave_sum(a[i]);
+    for (int i = 0; i < kBondSums; ++i) a[i] = wave_sum(a[i]);
     if (lane == 63)
-        for (int i = 0; i < 15; ++i) red[wv * 15 + i] = a[i];
+        for (int i = 0; i < kBondSums; ++i) red[wv * 15 + i] = a[i];
     __syncthreads();
-    if (threadIdx.x < 15) {
+    if (threadIdx.x < 15) {   // the bond form's other four sums are exact zeros
         const int i = (int)threadIdx.x;
-        partials[(int64_t)tl.pt * pstride + i] = ((red[i] + red[15 + i]) + red[30 + i]) + red[45 + i];
+        partials[(int64_t)tl.pt * pstride + i] =
+            i < kBondSums ? ((red[i] + red[15 + i]) + red[30 + i]) + red[45 + i] : 0.0;
     }
 }
 
