@@ -363,6 +363,37 @@ def test_lds_resident_and_streaming_paths_agree(orc, bloch3d, ising, dim, period
     assert np.array_equal(lats[0].accepts(), lats[1].accepts())
 
 
+@pytest.mark.parametrize("dim,periodic,masked", [(2, 1, False), (3, 0, True)])
+def test_lds_ising_with_several_pairs_per_work_item(orc, bloch3d, dim, periodic, masked):
+    # a 64-work-item LDS workgroup takes several pairs per colour (the loop
+    # form, not the geometry kept per launch): still the streaming lattice
+    rng = np.random.default_rng(47 + dim)
+    edge = 10 if dim == 3 else 24
+    bd, J, D = (orc.parse_bonds(bloch3d, 3, 1.0, 0.2) if dim == 3 else neel2d(0.2))
+    locs = None
+    if masked:
+        locs = box_locs(dim, edge)
+        locs = locs[rng.random(len(locs)) < 0.8]
+    lats = []
+    for path, block in (("auto", 64), ("auto", 0), ("stream", 0)):
+        lat, f = make_pair(orc, dim, edge, periodic, bd, J, D, True, use_d=False, locs=locs, trial="sphere",
+                           ninst=3, seed=78)
+        lat.set_path(path)
+        if block:
+            lat.set_lds_block(block)
+        lat.set_params([0.4, 1.0, 2.5], [0.0, 0.05, -0.1])
+        lat.randomize()
+        lat.equil(7)
+        lat.equil(300, inst=[1])
+        lats.append(lat)
+    from nanomagmc_amd._lib import NMC_KIND_LDS
+    assert lats[0].sweep_kinds().tolist() == [NMC_KIND_LDS] * 3
+    for r in lats[1:]:
+        for i in range(3):
+            assert np.array_equal(lats[0].get_spins(i), r.get_spins(i)), i
+        assert np.array_equal(lats[0].accepts(), r.accepts())
+
+
 @pytest.mark.parametrize("kind", ["stream", "lds", "grains"])
 def test_device_cone_adaptation_matches_host_steps(orc, bloch3d, kind):
     # nmc_sweep_adapt adapts the half-angles on the device after every chunk;
