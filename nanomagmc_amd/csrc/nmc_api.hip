@@ -125,9 +125,10 @@ struct Set {
     double* d_sscr = nullptr;
     bool fusable = false;
     // size classes of the compacted lattices: d_cl[off .. off + n) get
-    // workgroups of `block` work-items and `lds` bytes of dynamic LDS
+    // workgroups of `block` work-items (`sblock` in the fused sample loop)
+    // and `lds` bytes of dynamic LDS
     struct CClass {
-        int off, n, block;
+        int off, n, block, sblock;
         size_t lds;
     };
     std::vector<CClass> cclass;
@@ -167,6 +168,7 @@ struct Set {
 };
 
 constexpr int kLdsSweepChunk = 256;   // sweeps per LDS-resident launch
+constexpr int kSplitSweeps = 32;      // shorter compacted sweeps keep full-cover workgroups
 constexpr int kGraphSweeps = 64;      // sweeps per captured graph of a small set
 constexpr size_t kStreamLdsReserve = 1024;   // LDS a compacted lattice leaves free on its CU
 constexpr int kMeasureTile = 1024;    // colour elements per measurement tile (one workgroup)
@@ -786,21 +788,33 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         }
     }
     // size classes: the compacted lattices sorted by their larger colour's
-    // pair count into (0, 128], (128, 256], (256, 512], (512, inf), each
-    // class one launch whose workgroup covers its largest lattice's pairs
-    // (up to 1024 work-items) and whose LDS fits its largest lattice, so a
-    // small grain does not hold the largest grain's waves and LDS on its CU.
-    // A forced LDS workgroup size keeps one class.
+    // pair count m into (0, 128], (128, 256], (256, 512], (512, 1024] and
+    // (1024, inf), each class one launch whose LDS fits its largest lattice,
+    // so a small grain does not hold the largest grain's waves and LDS on
+    // its CU.  A sweep's workgroup covers the class's largest m up to 512;
+    // in (512, 1024] it covers half of it (two pairs per work-item), so that
+    // three grains of up to 4096 sites (49 KB of LDS) share a CU with 24
+    // waves instead of one grain with 16; above, 1024 work-items.  The fused
+    // sample loop (more registers, five waves per SIMD) and sweep calls of
+    // few sweeps keep a workgroup that covers the class's largest m, up to
+    // 1024 (`sblock`, class_groups).  A forced LDS workgroup size keeps one
+    // class.
     {
         auto cls = [&](const CInst& ci) {
             if (c->lds_block > 0) return 0;
             const int m = std::max(ci.nd0 + ci.ns0, ci.nd1 + ci.ns1);
-            return m <= 128 ? 0 : m <= 256 ? 1 : m <= 512 ? 2 : 3;
+            return m <= 128 ? 0 : m <= 256 ? 1 : m <= 512 ? 2 : m <= 1024 ? 3 : 4;
         };
-        std::stable_sort(cl.begin(), cl.end(), [&](const CInst& a, const CInst& b) { return cls(a) < cls(b); });
+        // the two upper classes in descending order, so that where they share
+        // one launch (class_groups) the largest grains are dispatched first
+        auto order = [&](const CInst& ci) { const int k = cls(ci); return k < 3 ? k : 7 - k; };
+        std::stable_sort(cl.begin(), cl.end(), [&](const CInst& a, const CInst& b) { return order(a) < order(b); });
+        auto round64 = [](int w) { return (int)std::min(1024, std::max(64, (w + 63) / 64 * 64)); };
+        int mbig = 0;   // the largest m above 512: the full-cover workgroup of both upper classes
+        for (const CInst& ci : cl) mbig = std::max(mbig, std::max(ci.nd0 + ci.ns0, ci.nd1 + ci.ns1));
         for (size_t q = 0; q < cl.size();) {
             const int k = cls(cl[q]);
-            Set::CClass cc{(int)q, 0, 0, 64};
+            Set::CClass cc{(int)q, 0, 0, 0, 64};
             int mq = 0;
             for (; q < cl.size() && cls(cl[q]) == k; ++q) {
                 const CInst& ci = cl[q];
@@ -808,7 +822,9 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
                 cc.lds = std::max(cc.lds, (size_t)(ci.n0 + ci.n1 + 1) * 3 * sizeof(float));
                 ++cc.n;
             }
-            cc.block = c->lds_block > 0 ? c->lds_block : (int)std::min(1024, std::max(64, (mq + 63) / 64 * 64));
+            const int cover = mq > 512 && mq <= 1024 ? (mq + 1) / 2 : mq;
+            cc.block = c->lds_block > 0 ? c->lds_block : round64(cover);
+            cc.sblock = c->lds_block > 0 ? c->lds_block : round64(mq > 512 ? mbig : mq);
             s->cclass.push_back(cc);
         }
     }
@@ -1284,6 +1300,26 @@ int release_sets(nmc_ctx* ctx) {
     return NMC_OK;
 }
 
+// The size classes merged into fewer launches: consecutive classes whose
+// full-cover workgroups (rounded up to 256 work-items, at least 256, for the
+// fused sample loop) are equal share one launch, with the LDS of the largest.
+// Sweep calls shorter than kSplitSweeps use them: there a grain's load into
+// and store from LDS weigh more than the sweeps, and a workgroup covering all
+// of a grain's pairs does them faster.
+std::vector<Set::CClass> class_groups(const Set* s, bool fused) {
+    std::vector<Set::CClass> g;
+    for (const Set::CClass& cc : s->cclass) {
+        const int block = fused ? (std::max(256, cc.sblock) + 255) / 256 * 256 : cc.sblock;
+        if (!g.empty() && g.back().block == block && g.back().off + g.back().n == cc.off) {
+            g.back().n += cc.n;
+            g.back().lds = std::max(g.back().lds, cc.lds);
+        } else {
+            g.push_back(Set::CClass{cc.off, cc.n, block, block, cc.lds});
+        }
+    }
+    return g;
+}
+
 // Enqueue `nsweeps` sweeps of a prepared set (launches only: no allocation,
 // no synchronisation, so it can be recorded into a graph).
 // The launches of `nsweeps` sweeps of one set.  Compacted grains
@@ -1294,9 +1330,10 @@ int release_sets(nmc_ctx* ctx) {
 hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream, bool side) {
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
     hipError_t e = hipSuccess;
+    const std::vector<Set::CClass> groups = nsweeps >= kSplitSweeps ? s->cclass : class_groups(s, false);
     for (int k = 0; k < nsweeps && s->n_cl > 0 && e == hipSuccess; k += kLdsSweepChunk)
-        for (size_t q = 0; q < s->cclass.size() && e == hipSuccess; ++q) {
-            const Set::CClass& cc = s->cclass[q];
+        for (size_t q = 0; q < groups.size() && e == hipSuccess; ++q) {
+            const Set::CClass& cc = groups[q];
             e = launch_sweep_lds_c(cstream, ctx->dim3, ctx->dm, ctx->cone, std::min(cc.n, 65535), cc.block, cc.lds,
                                    ctx->d_inst, s->d_cl + cc.off, cc.n, (float*)ctx->pool, s->d_cs_e, s->d_cs_p,
                                    s->d_cs_nb, ctx->d_sweeps, ctx->st, std::min(kLdsSweepChunk, nsweeps - k),
@@ -1730,7 +1767,7 @@ bool fused_ok(const nmc_ctx* ctx, const Set* s) {
 int fused_prepare(Set* s) {
     if (s->n_cl > 0 && !s->d_sscr) {
         int grid = 0;
-        for (const Set::CClass& cc : s->cclass) grid = std::max(grid, std::min(cc.n, 65535));
+        for (const Set::CClass& cc : class_groups(s, true)) grid = std::max(grid, std::min(cc.n, 65535));
         if (int rc = dalloc(&s->d_sscr, (size_t)grid * kSampleScratch)) return rc;
     }
     return NMC_OK;
@@ -1745,10 +1782,10 @@ int fused_sample_launches(nmc_ctx* ctx, Set* s, int nsamples, int samp_sweeps, c
     }
     const hipStream_t ls = side ? ctx->aux : ctx->stream;
     hipError_t e = hipSuccess;
-    for (size_t q = 0; q < s->cclass.size() && e == hipSuccess; ++q) {
-        const Set::CClass& cc = s->cclass[q];
-        const int block = (std::max(256, cc.block) + 255) / 256 * 256;
-        e = launch_samples_lds_c(ls, ctx->dim3, ctx->dm, ctx->cone, std::min(cc.n, 65535), block, cc.lds,
+    const std::vector<Set::CClass> groups = class_groups(s, true);
+    for (size_t q = 0; q < groups.size() && e == hipSuccess; ++q) {
+        const Set::CClass& cc = groups[q];
+        e = launch_samples_lds_c(ls, ctx->dim3, ctx->dm, ctx->cone, std::min(cc.n, 65535), cc.block, cc.lds,
                                  ctx->d_inst, s->d_cl + cc.off, cc.n, (float*)ctx->pool, s->d_cs_e, s->d_cs_p,
                                  s->d_cs_nb, s->d_cs_mslot, ctx->d_sweeps, ctx->st, nsamples, samp_sweeps, slo, shi,
                                  ctx->d_acc, ctx->d_series, ctx->series_n, 2, s->d_sscr, ctx->d_avgptr, d_avgflag);
