@@ -119,14 +119,16 @@ def workload_args(args):
     return a
 
 
-def measure_counters(args, launch, steps=6, warmup=2):
+def measure_counters(args, launch, steps=64, warmup=32):
     """HBM bytes (FETCH_SIZE + WRITE_SIZE, each scaled by what it reads for
     device copies of 256 MiB) and VALU issue slots (instructions, one more
     per fp32 transcendental) per dispatch of every
     sweep kernel family, from counter runs of this workload with `steps`
     timed sweeps after `warmup`.  For the LDS-resident kernels (one dispatch
-    per call) the per-sweep slope and the per-call intercept come from the
-    warm-up and timed dispatches."""
+    per size class and call) the per-sweep slope and the per-call intercept
+    come from the warm-up and timed calls.  Both calls are long enough (32
+    sweeps and more) to launch the size classes of the timed call; shorter
+    calls merge the upper classes into one launch."""
     if shutil.which("rocprofv3") is None:
         return None, "rocprofv3 not found"
     # --families-apart: the counters are sampled per dispatch over the whole

@@ -301,3 +301,57 @@ def test_lds_capacity_boundary_and_degenerate_grains(orc, bloch3d):
     assert np.all(ways[0].get_series()[0][4] == 0.0)
     for i in range(4):
         assert np.array_equal(ways[0].get_spins(i), ways[1].get_spins(i)), i
+
+
+def test_size_classes_long_and_short_sweep_calls_agree(orc, bloch3d):
+    # open 3-D grains whose larger colour's pair count m falls in every size
+    # class of the compacted kernel: (0, 128], (128, 256], (256, 512],
+    # (512, 1024] and above.  A call of 40 sweeps launches the upper two
+    # classes apart (two pairs per work-item in (512, 1024]); calls of 5
+    # sweeps launch them together with full-cover workgroups, and a forced
+    # workgroup size launches one class.  The lattices and accept counts
+    # agree bit for bit: a site's random numbers do not depend on the launch.
+    from nanomagmc_amd import Bond, Lattices
+    from nanomagmc_amd._lib import NMC_KIND_LDS_COMPACT
+    n = 32
+    rng = np.random.default_rng(43)
+    bd, J, D = orc.parse_bonds(bloch3d, 3, 1.0, 0.2)
+    bonds = [Bond(bd[k][:3], J[k], D[k]) for k in range(len(J))]
+    idx = np.arange(n ** 3)
+    colour = (idx // (n * n) + (idx // n) % n + idx % n) % 2
+    masks, ms = [], []
+    for count in (200, 400, 600, 1500, 2400, 4000, 9000):
+        m = np.zeros(n ** 3, np.uint8)
+        m[rng.permutation(n ** 3)[:count]] = 1
+        # pair k of a colour holds its colour elements 2k and 2k + 1 (box-scan order)
+        mc = [int(np.count_nonzero(m[colour == c].reshape(-1, 2).any(axis=1))) for c in (0, 1)]
+        masks.append(m)
+        ms.append(max(mc))
+    cls = [0 if x <= 128 else 1 if x <= 256 else 2 if x <= 512 else 3 if x <= 1024 else 4 for x in ms]
+    assert sorted(set(cls)) == [0, 1, 2, 3, 4], ms
+    insts, beta, h = [], [], []
+    for k, m in enumerate(masks):
+        for j in range(3):
+            insts.append({"n": (n, n, n), "edge": n, "mask": m})
+            beta.append(0.6 + 0.4 * j)
+            h.append(0.05 * (k % 3))
+    runs = []
+    for way in ("long", "short", "forced"):
+        lat = Lattices(3, False, False, bonds, insts, use_d=True, trial="sphere", seed=61)
+        if way == "forced":
+            lat.set_lds_block(1024)
+        lat.set_params(beta, h)
+        lat.randomize()
+        if way == "short":
+            for _ in range(8):
+                lat.equil(5)
+        else:
+            lat.equil(40)
+        runs.append(lat)
+    a = runs[0]
+    assert a.sweep_kinds().tolist() == [NMC_KIND_LDS_COMPACT] * len(insts)
+    for r in runs[1:]:
+        for i in range(len(insts)):
+            assert np.array_equal(a.get_spins(i), r.get_spins(i)), i
+        assert np.array_equal(a.accepts(), r.accepts())
+    assert np.all(a.accepts() > 0)
