@@ -810,7 +810,9 @@ int get_set(nmc_ctx* c, const int* inst, int n, Set** out) {
         auto order = [&](const CInst& ci) { const int k = cls(ci); return k < 3 ? k : 7 - k; };
         std::stable_sort(cl.begin(), cl.end(), [&](const CInst& a, const CInst& b) { return order(a) < order(b); });
         auto round64 = [](int w) { return (int)std::min(1024, std::max(64, (w + 63) / 64 * 64)); };
-        int mbig = 0;   // the largest m above 512: the full-cover workgroup of both upper classes
+        // the largest m of all: where it exceeds 512, the full-cover workgroup
+        // of both upper classes (one launch of them covers every grain)
+        int mbig = 0;
         for (const CInst& ci : cl) mbig = std::max(mbig, std::max(ci.nd0 + ci.ns0, ci.nd1 + ci.ns1));
         for (size_t q = 0; q < cl.size();) {
             const int k = cls(cl[q]);
