@@ -1308,8 +1308,8 @@ int release_sets(nmc_ctx* ctx) {
 // Sweep calls shorter than kSplitSweeps use them: there a grain's load into
 // and store from LDS weigh more than the sweeps, and a workgroup covering all
 // of a grain's pairs does them faster.
-std::vector<Set::CClass> class_groups(const Set* s, bool fused) {
-    std::vector<Set::CClass> g;
+void class_groups(const Set* s, bool fused, std::vector<Set::CClass>& g) {
+    g.clear();
     for (const Set::CClass& cc : s->cclass) {
         const int block = fused ? (std::max(256, cc.sblock) + 255) / 256 * 256 : cc.sblock;
         if (!g.empty() && g.back().block == block && g.back().off + g.back().n == cc.off) {
@@ -1319,7 +1319,6 @@ std::vector<Set::CClass> class_groups(const Set* s, bool fused) {
             g.push_back(Set::CClass{cc.off, cc.n, block, block, cc.lds});
         }
     }
-    return g;
 }
 
 // Enqueue `nsweeps` sweeps of a prepared set (launches only: no allocation,
@@ -1332,7 +1331,9 @@ std::vector<Set::CClass> class_groups(const Set* s, bool fused) {
 hipError_t sweep_kernels(nmc_ctx* ctx, Set* s, int nsweeps, hipStream_t cstream, bool side) {
     const uint32_t slo = (uint32_t)ctx->cfg.seed, shi = (uint32_t)(ctx->cfg.seed >> 32);
     hipError_t e = hipSuccess;
-    const std::vector<Set::CClass> groups = nsweeps >= kSplitSweeps ? s->cclass : class_groups(s, false);
+    std::vector<Set::CClass> groups;
+    if (nsweeps >= kSplitSweeps) groups = s->cclass;
+    else class_groups(s, false, groups);
     for (int k = 0; k < nsweeps && s->n_cl > 0 && e == hipSuccess; k += kLdsSweepChunk)
         for (size_t q = 0; q < groups.size() && e == hipSuccess; ++q) {
             const Set::CClass& cc = groups[q];
@@ -1769,7 +1770,9 @@ bool fused_ok(const nmc_ctx* ctx, const Set* s) {
 int fused_prepare(Set* s) {
     if (s->n_cl > 0 && !s->d_sscr) {
         int grid = 0;
-        for (const Set::CClass& cc : class_groups(s, true)) grid = std::max(grid, std::min(cc.n, 65535));
+        std::vector<Set::CClass> groups;
+        class_groups(s, true, groups);
+        for (const Set::CClass& cc : groups) grid = std::max(grid, std::min(cc.n, 65535));
         if (int rc = dalloc(&s->d_sscr, (size_t)grid * kSampleScratch)) return rc;
     }
     return NMC_OK;
@@ -1784,7 +1787,8 @@ int fused_sample_launches(nmc_ctx* ctx, Set* s, int nsamples, int samp_sweeps, c
     }
     const hipStream_t ls = side ? ctx->aux : ctx->stream;
     hipError_t e = hipSuccess;
-    const std::vector<Set::CClass> groups = class_groups(s, true);
+    std::vector<Set::CClass> groups;
+    class_groups(s, true, groups);
     for (size_t q = 0; q < groups.size() && e == hipSuccess; ++q) {
         const Set::CClass& cc = groups[q];
         e = launch_samples_lds_c(ls, ctx->dim3, ctx->dm, ctx->cone, std::min(cc.n, 65535), cc.block, cc.lds,
